@@ -1,0 +1,581 @@
+// C ABI (include/shci_gpu.h) and the variational-step driver.
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "../../include/shci_gpu.h"
+#include "engine.h"
+
+using namespace shci;
+
+struct shci_handle {
+  Engine eng;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_createError;
+
+shci_status fail(shci_handle* h, shci_status st, const std::string& msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_createError = msg;
+  return st;
+}
+
+template <typename F>
+shci_status guarded(shci_handle* h, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc& e) {
+    return fail(h, SHCI_OOM, e.what());
+  } catch (const HipError& e) {
+    std::string m = e.what();
+    return fail(h, m.find("out of memory") != std::string::npos ? SHCI_OOM : SHCI_HIP_ERROR, m);
+  } catch (const std::invalid_argument& e) {
+    return fail(h, SHCI_BAD_INPUT, e.what());
+  } catch (const std::exception& e) {
+    return fail(h, SHCI_HIP_ERROR, e.what());
+  }
+}
+
+double wallTime() {
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+bool getocc(const int64_t* d, int k) { return (((uint64_t)d[k / 64]) >> (k % 64)) & 1ull; }
+
+std::string detString(const int64_t* d, int nso) {  // SHCI/Determinants.h:459-477
+  std::string s;
+  for (int i = 0; i < nso / 2; i++) {
+    bool a = getocc(d, 2 * i), b = getocc(d, 2 * i + 1);
+    s += (!a && !b) ? "0 " : (a && !b) ? "a " : (!a && b) ? "b " : "2 ";
+    if ((i + 1) % 5 == 0) s += "  ";
+  }
+  return s;
+}
+
+double parityDiceToAlphaBeta(const int64_t* d, int nso) {  // SHCI/Determinants.cpp:691-706
+  double par = 1.0;
+  int norbs = nso / 2, nalpha = 0;
+  for (int i = 0; i < norbs; i++) nalpha += getocc(d, 2 * i);
+  for (int i = 0; i < norbs; i++) {
+    if (getocc(d, 2 * (norbs - 1 - i) + 1)) {
+      int before = 0;
+      for (int j = 0; j < norbs - i - 1; j++) before += getocc(d, 2 * j);
+      int after = nalpha - before - (getocc(d, 2 * (norbs - 1 - i)) ? 1 : 0);
+      if (after % 2 == 1) par *= -1;
+    }
+  }
+  return par;
+}
+
+// The variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446 and
+// SHCIbasics::DoVariational, SHCI/SHCIbasics.cpp:783-1378) on the device.
+shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, int precision) {
+  Engine& E = h->eng;
+  const double startofCalc = wallTime();
+  HostSchedule schd;
+  std::string err, echo;
+  bool ok = readInputFile(inputPath, schd, err, &echo);
+  if (log) {
+    fprintf(log, "\n\n**************************************************************\n");
+    fprintf(log, "INPUT FILE\n");
+    fprintf(log, "**************************************************************\n%s", echo.c_str());
+  }
+  if (!ok) {
+    if (log) fprintf(log, "%s\n", err.c_str());
+    return fail(h, SHCI_BAD_INPUT, err);
+  }
+  if (schd.Trev != 0 || schd.davidsonType != 0 || schd.restart || schd.fullrestart || schd.doSOC ||
+      schd.searchLowest || schd.doLCC) {
+    std::string m =
+        "keyword outside the variational path implemented here (Treversal, direct/disk davidson, restart, "
+        "fullrestart, dosoc, searchForLowestEnergyDet, dolcc)";
+    if (log) fprintf(log, "%s\n", m.c_str());
+    return fail(h, SHCI_UNSUPPORTED, m);
+  }
+  std::srand(schd.randomSeed + 0);
+  E.load(schd.integralFile);
+  E.precision = precision;
+  E.ncore = schd.ncore;
+  E.nact = schd.nact;
+  E.restrictions = schd.restrictions;
+  E.uploadRestrictions();
+  const HostSystem& hs = E.hs;
+  const int W = hs.W, nso = hs.nso;
+  if ((int)schd.occupied[0].size() != hs.nelec) {
+    std::string m = "The number of electrons given in the FCIDUMP should be equal to the nocc given in the shci input file.";
+    if (log) fprintf(log, "%s\n", m.c_str());
+    return fail(h, SHCI_BAD_INPUT, m);
+  }
+  if (log) {
+    fprintf(log, "\n\n**************************************************************\n");
+    fprintf(log, "SELECTING REFERENCE DETERMINANT(S)\n");
+    fprintf(log, "**************************************************************\n");
+  }
+  // reference determinant(s): SHCI/SHCI.cpp:286-316
+  const int64_t nref = (int64_t)schd.occupied.size();
+  std::vector<int64_t> hdets(nref * W, 0);
+  for (int64_t d = 0; d < nref; d++) {
+    for (int o : schd.occupied[d]) {
+      if (o < 0 || o >= nso || getocc(&hdets[d * W], o)) {
+        std::string m = "orbital " + std::to_string(o) + " appears twice in input determinant number " +
+                        std::to_string(d);
+        if (log) fprintf(log, "%s\n", m.c_str());
+        return fail(h, SHCI_BAD_INPUT, m);
+      }
+      hdets[d * W + o / 64] |= (int64_t)(1ull << (o % 64));
+    }
+    for (int64_t i = 0; i < d; i++)
+      if (std::equal(&hdets[d * W], &hdets[d * W] + W, &hdets[i * W])) {
+        std::string m = "Determinant appears twice in the input determinant list.";
+        if (log) fprintf(log, "%s\n", m.c_str());
+        return fail(h, SHCI_BAD_INPUT, m);
+      }
+  }
+  E.setDets(hdets.data(), nref);
+  E.buildHamiltonian();  // diagonal of the references (and their H)
+  std::vector<double> eref(nref);
+  HIP_CHECK(hipMemcpy(eref.data(), E.diag.p, nref * 8, hipMemcpyDeviceToHost));
+  int lowest = 0;
+  double lowestE = 1.e12;
+  for (int64_t d = 0; d < nref; d++) {
+    double e = eref[d] + hs.coreE;
+    if (log) fprintf(log, "%s Given Ref. Energy:    %18.10f\n", detString(&hdets[d * W], nso).c_str(), e);
+    if (e < lowestE) {
+      lowestE = e;
+      lowest = (int)d;
+    }
+  }
+  const int nroots = schd.nroots;
+  E.nroots = nroots;
+  // ci: N x nroots column-major on the device (ld = N)
+  std::vector<double> hci(nref * nroots, 0.0);
+  {
+    double s = 0.0;
+    for (int64_t j = 0; j < nref; j++) hci[j] = 1.0;
+    hci[lowest] += (double)nref;
+    for (int64_t j = 0; j < nref; j++) s += hci[j] * hci[j];
+    s = std::sqrt(s);
+    for (int64_t j = 0; j < nref; j++) hci[j] /= s;
+  }
+  E.ci.reserve(hci.size() * 8);
+  HIP_CHECK(hipMemcpy(E.ci.p, hci.data(), hci.size() * 8, hipMemcpyHostToDevice));
+  if (log) {
+    fprintf(log, "\n\n**************************************************************\n");
+    fprintf(log, "VARIATIONAL STEP  \n");
+    fprintf(log, "**************************************************************\n");
+  }
+  const double coreEbkp = hs.coreE;
+  std::vector<double> E0(nroots, std::fabs(eref[0]));
+  E.iters.clear();
+  if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
+  DevMem X0;
+  for (int iter = 0; iter < (int)schd.epsilon1.size(); iter++) {
+    const double eps1 = schd.epsilon1[iter];
+    const double t0 = wallTime();
+    const int64_t nold = E.n;
+    E.cmax.reserve((size_t)std::max<int64_t>(nold, 1) * 8);
+    launchCmax(E.ci.as<double>(), nroots, nold, nold, E.cmax.as<double>(), E.stream);
+    const int64_t nnew = E.screen(E.cmax.as<double>(), eps1);
+    E.appendNew();
+    const int64_t N = E.n;
+    // X0 = [ci; 0]
+    X0.reserve((size_t)N * nroots * 8);
+    HIP_CHECK(hipMemsetAsync(X0.p, 0, (size_t)N * nroots * 8, E.stream));
+    for (int r = 0; r < nroots; r++)
+      HIP_CHECK(hipMemcpyAsync(X0.as<double>() + (int64_t)r * N, E.ci.as<double>() + (int64_t)r * nold, nold * 8,
+                               hipMemcpyDeviceToDevice, E.stream));
+    (void)nnew;
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    const double t1 = wallTime();
+    E.buildHamiltonian();
+    const double t2 = wallTime();
+    const double prevE0 = iter == 0 ? -10.0 : E0[0];
+    int numIter = 0;
+    E0 = E.davidson(X0.as<double>(), nroots, nroots + 4, schd.davidsonTolLoose, numIter, false);
+    if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
+    const double t3 = wallTime();
+    const double t = t3 - startofCalc;
+    if (log) {
+      fprintf(log, "%4d %4d  %10.2e  %10lld", iter, 0, eps1, (long long)N);
+      fprintf(log, "   %18.10f  %9d  %10.2f\n", E0[0] + coreEbkp, numIter, t);
+      for (int i = 1; i < nroots; i++)
+        fprintf(log, "%4d %4d  %10.2e  %10lld   %18.10f  %9d  %10.2f\n", iter, i, eps1, (long long)N,
+                E0[i] + coreEbkp, numIter, t);
+      if (nroots > 1) fprintf(log, "\n");
+      fflush(log);
+    }
+    IterRecord rec{iter, eps1, N, {}, numIter, t};
+    rec.tScreen = t1 - t0;
+    rec.tHam = t2 - t1;
+    rec.tDavidson = t3 - t2;
+    rec.tIter = t3 - t0;
+    rec.nnzLower = E.lowerNnz();
+    if (std::getenv("SHCI_TIMING"))
+      fprintf(stderr, "# iter %d: n=%lld nnz_lower=%lld screen %.3fs hamiltonian %.3fs davidson %.3fs (%d it)\n", iter,
+              (long long)N, (long long)rec.nnzLower, rec.tScreen, rec.tHam, rec.tDavidson, numIter);
+    for (double e : E0) rec.e0.push_back(e + coreEbkp);
+    E.iters.push_back(rec);
+    // ci = X0
+    E.ci.reserve((size_t)N * nroots * 8);
+    HIP_CHECK(hipMemcpyAsync(E.ci.p, X0.p, (size_t)N * nroots * 8, hipMemcpyDeviceToDevice, E.stream));
+    if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
+      if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
+      E0 = E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
+      if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
+      if (log) fprintf(log, "Exiting variational iterations\n");
+      break;
+    }
+  }
+  if (log) {
+    fprintf(log, "\nVariational calculation result\n");
+    fprintf(log, "%4s %18s  %10s\n", "Root", "Energy", "Time(s)");
+  }
+  for (size_t i = 0; i < E0.size(); i++) {
+    E0[i] += coreEbkp;
+    if (log) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
+  }
+  if (log) fprintf(log, "\n");
+  E.energies = E0;
+  {  // SHCI/SHCI.cpp:437-446
+    std::string efile = schd.prefix[0] + "/shci.e";
+    FILE* f = fopen(efile.c_str(), "wb");
+    if (f) {
+      for (double e : E0) fwrite(&e, 1, sizeof(double), f);
+      fclose(f);
+    }
+  }
+  if (log) {  // SHCI/SHCI.cpp:526-551
+    const int64_t N = E.n;
+    std::vector<int64_t> dd(N * W);
+    std::vector<double> cc(N * nroots);
+    HIP_CHECK(hipMemcpy(dd.data(), E.dets.p, N * W * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(cc.data(), E.ci.p, N * nroots * 8, hipMemcpyDeviceToHost));
+    fprintf(log, "Printing most important determinants\n");
+    fprintf(log, "%4s %10s  Determinant string\n", "Det", "weight");
+    for (int root = 0; root < nroots; root++) {
+      fprintf(log, "State : %3d\n", root);
+      std::vector<double> prev(cc.begin() + root * N, cc.begin() + (root + 1) * N);
+      int num = std::max(6, schd.printBestDeterminants);
+      for (int i = 0; i < std::min<int64_t>(num, N); i++) {
+        int64_t m = 0;
+        for (int64_t k = 1; k < N; k++)
+          if (std::fabs((float)prev[m]) < std::fabs((float)prev[k])) m = k;
+        double par = parityDiceToAlphaBeta(&dd[m * W], nso);
+        fprintf(log, "%4d %18.10f  %s\n", i, prev[m] * par, detString(&dd[m * W], nso).c_str());
+        prev[m] = 0.0;
+      }
+    }
+    fflush(log);
+  }
+  return SHCI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+shci_status shci_create(const char* fcidump_path, shci_handle** out) {
+  if (!out || !fcidump_path) return fail(nullptr, SHCI_BAD_INPUT, "null argument");
+  *out = nullptr;
+  std::unique_ptr<shci_handle> h;
+  try {
+    h.reset(new shci_handle());
+  } catch (const std::exception& e) {
+    return fail(nullptr, SHCI_HIP_ERROR, e.what());
+  }
+  shci_status st = guarded(h.get(), [&] {
+    h->eng.load(fcidump_path);
+    return SHCI_OK;
+  });
+  if (st != SHCI_OK) {
+    g_createError = h->err;
+    return st;
+  }
+  *out = h.release();
+  return SHCI_OK;
+}
+
+shci_status shci_destroy(shci_handle* h) {
+  delete h;
+  return SHCI_OK;
+}
+
+const char* shci_last_error(const shci_handle* h) { return h ? h->err.c_str() : g_createError.c_str(); }
+int shci_norb(const shci_handle* h) { return h->eng.hs.norb; }
+int shci_nelec(const shci_handle* h) { return h->eng.hs.nelec; }
+int shci_det_words(const shci_handle* h) { return h->eng.hs.W; }
+double shci_core_energy(const shci_handle* h) { return h->eng.hs.coreE; }
+
+shci_status shci_set_precision(shci_handle* h, int precision) {
+  if (precision != SHCI_PREC_FP64 && precision != SHCI_PREC_FP32)
+    return fail(h, SHCI_BAD_INPUT, "unknown precision");
+  h->eng.precision = precision;
+  h->eng.hamRows = -1;
+  return SHCI_OK;
+}
+
+shci_status shci_set_active_space(shci_handle* h, int ncore, int nact) {
+  h->eng.ncore = ncore;
+  h->eng.nact = nact;
+  return SHCI_OK;
+}
+
+shci_status shci_add_restriction(shci_handle* h, int min_elec, int max_elec, const int* spatial_orbs, int n) {
+  return guarded(h, [&] {
+    Restriction r;
+    r.minElec = min_elec;
+    r.maxElec = max_elec;
+    for (int i = 0; i < n; i++) {
+      r.orbs.push_back(2 * spatial_orbs[i]);
+      r.orbs.push_back(2 * spatial_orbs[i] + 1);
+    }
+    h->eng.restrictions.push_back(r);
+    h->eng.uploadRestrictions();
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_det_energy(shci_handle* h, const int64_t* dets, int64_t n, double* energies) {
+  return guarded(h, [&] {
+    if (n <= 0) return SHCI_OK;
+    Engine& E = h->eng;
+    const int W = E.hs.W;
+    DevMem d, o;
+    d.reserve(n * W * 8);
+    o.reserve(n * 8);
+    HIP_CHECK(hipMemcpyAsync(d.p, dets, n * W * 8, hipMemcpyHostToDevice, E.stream));
+    launchEnergy(E.S, d.as<uint64_t>(), n, o.as<double>(), E.stream);
+    HIP_CHECK(hipMemcpyAsync(energies, o.p, n * 8, hipMemcpyDeviceToHost, E.stream));
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_hij(shci_handle* h, const int64_t* bra, const int64_t* ket, int64_t n, double* out) {
+  return guarded(h, [&] {
+    if (n <= 0) return SHCI_OK;
+    Engine& E = h->eng;
+    const int W = E.hs.W;
+    DevMem b, k, o;
+    b.reserve(n * W * 8);
+    k.reserve(n * W * 8);
+    o.reserve(n * 8);
+    HIP_CHECK(hipMemcpyAsync(b.p, bra, n * W * 8, hipMemcpyHostToDevice, E.stream));
+    HIP_CHECK(hipMemcpyAsync(k.p, ket, n * W * 8, hipMemcpyHostToDevice, E.stream));
+    launchHijPairs(E.S, b.as<uint64_t>(), k.as<uint64_t>(), n, o.as<double>(), E.stream);
+    HIP_CHECK(hipMemcpyAsync(out, o.p, n * 8, hipMemcpyDeviceToHost, E.stream));
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_set_dets(shci_handle* h, const int64_t* dets, int64_t n) {
+  return guarded(h, [&] {
+    if (n < 0) throw std::invalid_argument("negative determinant count");
+    h->eng.setDets(dets, n);
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_num_dets(const shci_handle* h, int64_t* n) {
+  *n = h->eng.n;
+  return SHCI_OK;
+}
+
+shci_status shci_get_dets(shci_handle* h, int64_t* out, int64_t cap) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    int64_t m = std::min(cap, E.n);
+    if (m > 0) HIP_CHECK(hipMemcpy(out, E.dets.p, m * E.hs.W * 8, hipMemcpyDeviceToHost));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_screen(shci_handle* h, const double* cmax, double eps1, int64_t* n_new) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    E.cmax.reserve((size_t)std::max<int64_t>(E.n, 1) * 8);
+    if (E.n) HIP_CHECK(hipMemcpyAsync(E.cmax.p, cmax, E.n * 8, hipMemcpyHostToDevice, E.stream));
+    *n_new = E.screen(E.cmax.as<double>(), eps1);
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_get_new_dets(shci_handle* h, int64_t* out, int64_t cap) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    int64_t m = std::min(cap, E.nNew);
+    if (m > 0) HIP_CHECK(hipMemcpy(out, E.newDets.p, m * E.hs.W * 8, hipMemcpyDeviceToHost));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_append_new(shci_handle* h) {
+  return guarded(h, [&] {
+    h->eng.appendNew();
+    HIP_CHECK(hipStreamSynchronize(h->eng.stream));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_build_hamiltonian(shci_handle* h, int64_t* nnz_lower) {
+  return guarded(h, [&] {
+    h->eng.buildHamiltonian();
+    if (nnz_lower) *nnz_lower = h->eng.lowerNnz();
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t* rowptr, int32_t* cols,
+                                      double* vals, int64_t cap) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    const int64_t n = E.n;
+    if (row_begin < 0 || row_begin > n) throw std::invalid_argument("row_begin out of range");
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> c(E.nnzOff);
+    std::vector<double> v(E.nnzOff), dg(n);
+    HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    if (E.nnzOff) {
+      HIP_CHECK(hipMemcpy(c.data(), E.cols.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(v.data(), E.vals.p, E.nnzOff * 8, hipMemcpyDeviceToHost));
+    }
+    HIP_CHECK(hipMemcpy(dg.data(), E.diag.p, n * 8, hipMemcpyDeviceToHost));
+    int64_t k = 0;
+    rowptr[0] = 0;
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t I = row_begin; I < n; I++) {
+      row.clear();
+      for (int64_t p = rp[I]; p < rp[I + 1]; p++)
+        if (c[p] < I) row.emplace_back(c[p], v[p]);
+      std::sort(row.begin(), row.end(), [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+        return a.first < b.first;
+      });
+      if (k + 1 + (int64_t)row.size() > cap) throw std::invalid_argument("output capacity too small");
+      cols[k] = (int32_t)I;
+      vals[k] = dg[I];
+      k++;
+      for (auto& e : row) {
+        cols[k] = e.first;
+        vals[k] = e.second;
+        k++;
+      }
+      rowptr[I - row_begin + 1] = k;
+    }
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    E.sigma(d_x, d_y, k, ld, false);
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    const int64_t n = E.n;
+    DevMem dx, dy;
+    dx.reserve(std::max<int64_t>(n * k, 1) * 8);
+    dy.reserve(std::max<int64_t>(n * k, 1) * 8);
+    HIP_CHECK(hipMemcpyAsync(dx.p, x, n * k * 8, hipMemcpyHostToDevice, E.stream));
+    E.sigma(dx.as<double>(), dy.as<double>(), k, n, false);
+    HIP_CHECK(hipMemcpyAsync(y, dy.p, n * k * 8, hipMemcpyDeviceToHost, E.stream));
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies, double tol, double* energies,
+                          int* iterations) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    if (nroots < 1 || max_copies < nroots + 3 || max_copies > 12)
+      throw std::invalid_argument("need nroots >= 1 and nroots + 3 <= max_copies <= 12");
+    const int64_t n = E.n;
+    DevMem dx;
+    dx.reserve(n * nroots * 8);
+    HIP_CHECK(hipMemcpyAsync(dx.p, x0, n * nroots * 8, hipMemcpyHostToDevice, E.stream));
+    int it = 0;
+    std::vector<double> e = E.davidson(dx.as<double>(), nroots, max_copies, tol, it, false);
+    if ((int)e.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
+    HIP_CHECK(hipMemcpyAsync(x0, dx.p, n * nroots * 8, hipMemcpyDeviceToHost, E.stream));
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    for (int r = 0; r < nroots; r++) energies[r] = e[r];
+    if (iterations) *iterations = it;
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out) {
+  if (!out || !input_path) return fail(nullptr, SHCI_BAD_INPUT, "null argument");
+  *out = nullptr;
+  shci_handle* h = nullptr;
+  try {
+    h = new shci_handle();
+  } catch (const std::exception& e) {
+    return fail(nullptr, SHCI_HIP_ERROR, e.what());
+  }
+  FILE* log = stdout;
+  bool own = false;
+  if (log_path && log_path[0]) {
+    log = fopen(log_path, "w");
+    own = log != nullptr;
+    if (!log) log = stdout;
+  }
+  shci_status st = guarded(h, [&] { return runVariational(h, input_path, log, precision); });
+  if (own) fclose(log);
+  *out = h;
+  return st;
+}
+
+int shci_run_iterations(const shci_handle* h) { return (int)h->eng.iters.size(); }
+
+shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, double* energy0, int* davidson_iters,
+                               double* seconds) {
+  if (it < 0 || it >= (int)h->eng.iters.size()) return SHCI_BAD_INPUT;
+  const IterRecord& r = h->eng.iters[it];
+  if (ndets) *ndets = r.ndets;
+  if (energy0) *energy0 = r.e0[0];
+  if (davidson_iters) *davidson_iters = r.davidsonIters;
+  if (seconds) *seconds = r.seconds;
+  return SHCI_OK;
+}
+
+shci_status shci_run_energies(const shci_handle* h, double* energies, int* nroots) {
+  for (size_t i = 0; i < h->eng.energies.size(); i++) energies[i] = h->eng.energies[i];
+  if (nroots) *nroots = (int)h->eng.energies.size();
+  return SHCI_OK;
+}
+
+shci_status shci_run_ci(shci_handle* h, double* out, int64_t cap) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    int64_t m = std::min(cap, E.n * (int64_t)E.nroots);
+    if (m > 0) HIP_CHECK(hipMemcpy(out, E.ci.p, m * 8, hipMemcpyDeviceToHost));
+    return SHCI_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// Engine: device-resident SHCI variational space, Hamiltonian and Davidson.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace shci {
+
+namespace {
+template <typename T>
+void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
+  m.reserve(std::max<size_t>(v.size() * sizeof(T), 8));
+  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+}  // namespace
+
+Engine::Engine() {}
+
+Engine::~Engine() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::load(const std::string& fcidump) {
+  std::string e;
+  if (!readFcidump(fcidump, hs, e)) throw std::invalid_argument(e);
+  buildScreeningTables(hs);
+  if (!stream) HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  upload(dH1, hs.h1, stream);
+  upload(dI2, hs.I2, stream);
+  upload(dDirect, hs.direct, stream);
+  upload(dExch, hs.exch, stream);
+  upload(dTsame, hs.tsame, stream);
+  upload(dTopp, hs.topp, stream);
+  upload(dSingles, hs.singles, stream);
+  for (int k = 0; k < 2; k++) {
+    upload(dHbVal[k], hs.hbVal[k], stream);
+    upload(dHbPair[k], hs.hbPair[k], stream);
+    upload(dHbStart[k], hs.hbStart[k], stream);
+  }
+  S.norb = hs.norb;
+  S.nso = hs.nso;
+  S.W = hs.W;
+  S.nelec = hs.nelec;
+  S.h1 = dH1.as<double>();
+  S.I2 = dI2.as<double>();
+  S.direct = dDirect.as<double>();
+  S.exch = dExch.as<double>();
+  S.tsame = dTsame.as<double>();
+  S.topp = dTopp.as<double>();
+  S.singles = dSingles.as<double>();
+  for (int k = 0; k < 2; k++) {
+    S.hbVal[k] = dHbVal[k].as<float>();
+    S.hbPair[k] = dHbPair[k].as<uint32_t>();
+    S.hbStart[k] = dHbStart[k].as<int64_t>();
+  }
+  S.maxEntryAbs = std::fabs(hs.maxEntry);
+  uploadRestrictions();
+  counter.reserve(64);
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::uploadRestrictions() {
+  if (restrictions.size() > SHCI_MAX_RESTRICT) throw std::invalid_argument("too many restrict keywords (max 4)");
+  std::vector<int8_t> mult(std::max<size_t>(restrictions.size(), 1) * hs.nso, 0);
+  S.nrestrict = (int)restrictions.size();
+  for (size_t r = 0; r < restrictions.size(); r++) {
+    S.rmin[r] = restrictions[r].minElec;
+    S.rmax[r] = restrictions[r].maxElec;
+    for (int o : restrictions[r].orbs)
+      if (o >= 0 && o < hs.nso) mult[r * hs.nso + o]++;
+  }
+  upload(dRmult, mult, stream);
+  S.rmult = dRmult.as<int8_t>();
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::ensureTable(int64_t need) {
+  uint64_t want = 1024;
+  while (want < (uint64_t)need * 2) want <<= 1;
+  if (want <= slots) return;
+  slots = want;
+  table.reserve(slots * sizeof(int32_t));
+  hashClear(table.as<int32_t>(), slots, stream);
+  hashInsert(S, dets.as<uint64_t>(), 0, n, table.as<int32_t>(), slots, stream);
+}
+
+void Engine::setDets(const int64_t* h, int64_t cnt) {
+  const int W = hs.W;
+  dets.reserve((size_t)std::max<int64_t>(cnt, 1) * W * 8, stream);
+  if (cnt) HIP_CHECK(hipMemcpyAsync(dets.p, h, cnt * W * 8, hipMemcpyHostToDevice, stream));
+  n = cnt;
+  slots = 0;
+  ensureTable(std::max<int64_t>(n, 512));
+  nDiag = 0;
+  hamRows = -1;
+  nNew = 0;
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+int64_t Engine::screen(const double* dCmax, double eps1) {
+  const int W = hs.W;
+  if (candCap == 0) candCap = std::max<int64_t>(1 << 16, 8 * n);
+  while (true) {
+    cand.reserve((size_t)candCap * W * 8);
+    HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
+    launchScreen(S, dets.as<uint64_t>(), n, dCmax, eps1, ncore, nact, table.as<int32_t>(), slots,
+                 cand.as<uint64_t>(), counter.as<unsigned long long>(), candCap, stream);
+    unsigned long long found = 0;
+    HIP_CHECK(hipMemcpyAsync(&found, counter.p, 8, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if ((int64_t)found <= candCap) {
+      int64_t m = (int64_t)found;
+      radixSortKeys(cand.as<uint64_t>(), nullptr, m, W, hs.nso, true, ws, stream);
+      m = uniqueKeys(cand.as<uint64_t>(), m, W, ws, stream);
+      newDets.reserve((size_t)std::max<int64_t>(m, 1) * W * 8);
+      if (m) HIP_CHECK(hipMemcpyAsync(newDets.p, cand.p, m * W * 8, hipMemcpyDeviceToDevice, stream));
+      nNew = m;
+      return m;
+    }
+    candCap = (int64_t)found + found / 4 + 1024;
+    cand.release();
+  }
+}
+
+void Engine::appendNew() {
+  const int W = hs.W;
+  if (nNew == 0) return;
+  dets.reserve((size_t)(n + nNew) * W * 8, stream, true);
+  HIP_CHECK(hipMemcpyAsync(dets.as<uint64_t>() + n * W, newDets.p, nNew * W * 8, hipMemcpyDeviceToDevice, stream));
+  int64_t old = n;
+  n += nNew;
+  nNew = 0;
+  uint64_t oldSlots = slots;
+  ensureTable(n);
+  if (slots == oldSlots) hashInsert(S, dets.as<uint64_t>(), old, n, table.as<int32_t>(), slots, stream);
+}
+
+void Engine::buildHamiltonian() {
+  // diagonal for determinants without one
+  diag.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream, true);
+  if (nDiag < n) launchEnergy(S, dets.as<uint64_t>() + nDiag * hs.W, n - nDiag, diag.as<double>() + nDiag, stream);
+  nDiag = n;
+  if (hamRows == n) return;
+  buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
+  rowPtr.reserve((size_t)(n + 1) * 8);
+  int64_t* rp = rowPtr.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+  launchCsrCount(S, dets.as<uint64_t>(), T, rp, stream);
+  exclusiveScanI64(rp, n + 1, ws, stream);
+  HIP_CHECK(hipMemcpyAsync(&nnzOff, rp + n, 8, hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  cols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+  vals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
+  launchCsrFill(S, dets.as<uint64_t>(), T, rp, cols.as<int32_t>(), vals.as<double>(), stream);
+  if (precision == 1) {
+    vals32.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    launchCastValues(vals.as<double>(), vals32.as<float>(), nnzOff, stream);
+  }
+  hamRows = n;
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) {
+  if (precision == 1 && !exact)
+    launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), diag.as<double>(), n, dx, dy, k, ld,
+                   stream);
+  else
+    launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k, ld,
+                   stream);
+}
+
+// Davidson with the semantics of SHCI/Davidson.cpp:111-348 (see the oracle
+// for the line-by-line statement); vectors stay on the device, the host
+// keeps the control flow and the m x m subspace scalars.
+std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double tol, int& numIter, bool exactSigma) {
+  const int64_t N = n;
+  std::vector<double> eroots;
+  dvB.reserve((size_t)N * maxCopies * 8);
+  dvS.reserve((size_t)N * maxCopies * 8);
+  dvR.reserve((size_t)N * 8);
+  dvScal.reserve(64 * 8);
+  dvG.reserve(512 * 8);
+  dvW.reserve(32 * 8);
+  dvV.reserve(256 * 8);
+  double* B = dvB.as<double>();
+  double* Sg = dvS.as<double>();
+  double* r = dvR.as<double>();
+  double* sc = dvScal.as<double>();  // [0..1] dot2, [2] norm2, [3..] residual norms
+  double* G = dvG.as<double>();
+  double* Wv = dvW.as<double>();
+  double* V = dvV.as<double>();
+  hipStream_t s = stream;
+  auto col = [&](double* M, int c) { return M + (int64_t)c * N; };
+  auto readScalars = [&](double* dst, const double* src, int cnt) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  };
+  auto norm = [&](const double* x) {
+    double v;
+    launchNorm2(x, N, sc + 2, dvPart, s);
+    readScalars(&v, sc + 2, 1);
+    return std::sqrt(v);
+  };
+  auto setRandom = [&](double* x) {
+    std::vector<double> h(N);
+    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + (1.0 - (-1.0)) * double(std::rand()) / double(RAND_MAX);
+    HIP_CHECK(hipMemcpyAsync(x, h.data(), N * 8, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  };
+  auto dotHost = [&](const double* a, const double* b) {
+    double v;
+    launchGram(a, 1, b, 1, N, N, sc + 2, dvPart, s);
+    readScalars(&v, sc + 2, 1);
+    return v;
+  };
+  // initial vectors
+  for (int i = 0; i < nr; i++) {
+    HIP_CHECK(hipMemcpyAsync(col(B, i), dX0 + (int64_t)i * N, N * 8, hipMemcpyDeviceToDevice, s));
+    if (norm(dX0 + (int64_t)i * N) < 1.e-10) {
+      setRandom(col(B, i));
+      launchNorm2(col(B, i), N, sc + 2, dvPart, s);
+      launchScaleInto(col(B, i), col(B, i), sc + 2, N, s);
+    }
+  }
+  for (int i = 0; i < nr; i++) {
+    for (int j = 0; j < i; j++) launchAxpy(col(B, i), col(B, j), -dotHost(col(B, j), col(B, i)), N, s);
+    if (norm(col(B, i)) < 1e-8) setRandom(col(B, i));
+    for (int j = 0; j < i; j++) launchAxpy(col(B, i), col(B, j), -dotHost(col(B, j), col(B, i)), N, s);
+    launchNorm2(col(B, i), N, sc + 2, dvPart, s);
+    launchScaleInto(col(B, i), col(B, i), sc + 2, N, s);
+  }
+  int sigmaSize = 0, bsize = nr, convergedRoot = 0;
+  numIter = 0;
+  std::vector<double> w(16), hres(16);
+  while (true) {
+    for (int i = sigmaSize; i < bsize; i++) {
+      sigma(col(B, i), col(Sg, i), 1, N, exactSigma);
+      sigmaSize++;
+    }
+    launchGram(B, bsize, Sg, bsize, N, N, G, dvPart, s);
+    launchSymmetrizeUpper(G, bsize, G + 256, s);
+    launchJacobi(G + 256, bsize, Wv, V, s);
+    launchRotate(B, bsize, N, N, V, s);
+    launchRotate(Sg, bsize, N, N, V, s);
+    const int cr0 = convergedRoot;
+    // residual norms of the roots below convergedRoot, then of convergedRoot
+    for (int i = 0; i < cr0; i++) launchResidual(col(Sg, i), col(B, i), Wv + i, N, r, sc + 3 + i, dvPart, s);
+    launchResidual(col(Sg, cr0), col(B, cr0), Wv + cr0, N, r, sc + 3 + cr0, dvPart, s);
+    HIP_CHECK(hipMemcpyAsync(w.data(), Wv, bsize * 8, hipMemcpyDeviceToHost, s));
+    readScalars(hres.data(), sc + 3, cr0 + 1);
+    const double* ePtr = Wv + cr0;  // ei = eigenvalue of the convergedRoot on entry
+    for (int i = 0; i < cr0; i++) {
+      if (std::sqrt(hres[i]) > tol) {
+        convergedRoot = i;
+        break;
+      }
+    }
+    double error;
+    if (convergedRoot != cr0) {
+      launchResidual(col(Sg, convergedRoot), col(B, convergedRoot), ePtr, N, r, sc + 2, dvPart, s);
+      double v;
+      readScalars(&v, sc + 2, 1);
+      error = std::sqrt(v);
+    } else {
+      error = std::sqrt(hres[cr0]);
+    }
+    numIter++;
+    if (bsize == N) {
+      for (int i = 0; i < nr; i++) {
+        HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)i * N, col(B, i), N * 8, hipMemcpyDeviceToDevice, s));
+        eroots.push_back(w[i]);
+      }
+      HIP_CHECK(hipStreamSynchronize(s));
+      return eroots;
+    }
+    if (error < tol || numIter > 800 * nr) {
+      if (numIter > 2000 * nr) {
+        err = "Davidson calculation did not converge";
+        return eroots;
+      }
+      convergedRoot++;
+      if (convergedRoot == nr) {
+        for (int i = 0; i < nr; i++) {
+          HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)i * N, col(B, i), N * 8, hipMemcpyDeviceToDevice, s));
+          eroots.push_back(w[i]);
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+        return eroots;
+      }
+    }
+    launchPrecondition(r, diag.as<double>(), ePtr, N, s);
+    for (int i = 0; i < bsize; i++) {
+      launchDot2(col(B, i), r, N, sc, dvPart, s);
+      launchOrthoStep(r, col(B, i), sc, N, s);
+    }
+    launchNorm2(r, N, sc + 2, dvPart, s);
+    if (bsize < maxCopies) {
+      launchScaleInto(col(B, bsize), r, sc + 2, N, s);
+      bsize++;
+    } else {
+      bsize = nr + 3;
+      sigmaSize = nr + 2;
+      launchScaleInto(col(B, bsize - 1), r, sc + 2, N, s);
+    }
+  }
+}
+
+}  // namespace shci

@@ -1,0 +1,79 @@
+// Device-resident state of one SHCI variational calculation.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "host_io.h"
+#include "kernels.h"
+
+namespace shci {
+
+struct IterRecord {
+  int iter;
+  double eps1;
+  int64_t ndets;
+  std::vector<double> e0;  // coreE added
+  int davidsonIters;
+  double seconds;       // cumulative wall time at the end of the iteration
+  double tScreen = 0, tHam = 0, tDavidson = 0, tIter = 0;  // phase wall times (s)
+  int64_t nnzLower = 0;
+};
+
+struct Engine {
+  HostSystem hs;
+  DevSystem S{};
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  int precision = 0;
+  int ncore = 0, nact = 1000000;
+  std::vector<Restriction> restrictions;
+
+  // system tables
+  DevMem dH1, dI2, dDirect, dExch, dTsame, dTopp, dSingles, dHbVal[2], dHbPair[2], dHbStart[2], dRmult;
+
+  // variational space
+  int64_t n = 0;
+  DevMem dets;                 // n x W
+  DevMem table;                // membership slots (int32)
+  uint64_t slots = 0;
+  DevMem newDets;              // staged new determinants
+  int64_t nNew = 0;
+  DevMem cand;                 // screening candidates
+  int64_t candCap = 0;
+  DevMem counter;              // unsigned long long
+  DevMem cmax;
+
+  // Hamiltonian: diagonal + full symmetric off-diagonal CSR
+  StringTables T;
+  DevMem diag;
+  int64_t nDiag = 0;
+  DevMem rowPtr, cols, vals, vals32, valsLo;
+  int64_t nnzOff = 0, hamRows = -1;
+
+  // Davidson scratch
+  DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;
+
+  // last run
+  std::vector<IterRecord> iters;
+  std::vector<double> energies;
+  DevMem ci;  // n x nroots, column-major, ld = n
+  int nroots = 1;
+  std::string err;
+
+  explicit Engine();
+  ~Engine();
+  void load(const std::string& fcidump);  // throws
+  void uploadRestrictions();
+  void setDets(const int64_t* h, int64_t cnt);
+  void ensureTable(int64_t need);
+  int64_t screen(const double* dCmax, double eps1);  // returns nNew
+  void appendNew();
+  void buildHamiltonian();
+  void sigma(const double* dx, double* dy, int k, int64_t ld, bool exact);
+  std::vector<double> davidson(double* dX0, int nr, int maxCopies, double tol, int& numIter, bool exactSigma);
+  int64_t lowerNnz() const { return nnzOff / 2 + n; }
+};
+
+}  // namespace shci

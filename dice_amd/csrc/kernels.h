@@ -1,0 +1,92 @@
+// Host launchers for the SHCI kernels (implemented in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "prims.h"
+#include "shci_common.h"
+
+namespace shci {
+
+// --- determinant primitives ---------------------------------------------------
+void launchEnergy(const DevSystem& S, const uint64_t* dets, int64_t n, double* out, hipStream_t s);
+void launchHijPairs(const DevSystem& S, const uint64_t* bra, const uint64_t* ket, int64_t n, double* out,
+                    hipStream_t s);
+
+// --- membership table (open addressing; slots hold determinant indices) -----------
+void hashClear(int32_t* table, uint64_t slots, hipStream_t s);
+void hashInsert(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, int32_t* table,
+                uint64_t slots, hipStream_t s);
+
+// --- screening (heat-bath selection of new determinants) --------------------------
+// cmax = sqrt(sum_r c_r^2) for every determinant; ci column-major with
+// leading dimension ld.
+void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cmax, hipStream_t s);
+// Emits candidate determinants not present in the table into `out`
+// (capacity `cap` determinants); *counter receives the total number found
+// (which may exceed cap: the caller then grows `out` and runs again).
+void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* cmax, double eps1,
+                  int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
+                  unsigned long long* counter, int64_t cap, hipStream_t s);
+
+// --- string tables and the Hamiltonian -----------------------------------------------
+struct StringTables {
+  int64_t nA = 0, nB = 0, n = 0;
+  DevMem alpha, beta;           // per det: alpha / beta string (u64)
+  DevMem aIdx, bIdx;            // per det: index into sorted unique strings (u32)
+  DevMem sortedA, sortedB;      // unique strings
+  DevMem aStart, bStart;        // group offsets (int64, nA+1 / nB+1)
+  DevMem aMajorDet, bMajorDet;  // det index per group member (u32)
+  DevMem aMajorB, bMajorA;      // partner string per group member (u64)
+  DevMem aSinglesStart, aSingles;  // alpha strings one excitation apart (int64 / u32)
+  DevMem key, val;              // scratch
+};
+void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, StringTables& T, Workspace& ws,
+                       hipStream_t s);
+
+// Count (rowCount != null) or fill (cols/vals) the off-diagonal connections
+// of every row: all J != I one or two excitations apart.  Values are
+// Hij(bra = D_min(I,J), ket = D_max(I,J)).
+void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t* rowCount,
+                    hipStream_t s);
+void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, const int64_t* rowPtr,
+                   int32_t* cols, double* vals, hipStream_t s);
+void launchCastValues(const double* v, float* out, int64_t n, hipStream_t s);
+// sort the columns of every row (ascending), values carried along
+void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals, hipStream_t s);
+
+// --- sigma = H x ------------------------------------------------------------------------
+// y[:,r] = diag .* x[:,r] + offdiag(H) x[:,r] for r < k (column-major, ld)
+void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag,
+                    int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
+void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
+                    const double* x, double* y, int k, int64_t ld, hipStream_t s);
+
+// --- Davidson vector kernels ----------------------------------------------------------
+// G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)
+void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G,
+                DevMem& partial, hipStream_t s);
+// X[:, :m] = X[:, :m] * V (V row-major m x m on device)
+void launchRotate(double* X, int m, int64_t n, int64_t ld, const double* V, hipStream_t s);
+// r = s - e * b ; out[0] = ||r||^2
+void launchResidual(const double* sig, const double* b, const double* e, int64_t n, double* r, double* out,
+                    DevMem& partial, hipStream_t s);
+// r_k /= (e - diag_k) with the 1e-12 guard
+void launchPrecondition(double* r, const double* diag, const double* e, int64_t n, hipStream_t s);
+// out[0] = b.r, out[1] = b.b
+void launchDot2(const double* b, const double* r, int64_t n, double* out, DevMem& partial, hipStream_t s);
+// r -= (ov/nn) b  with ov = d[0], nn = d[1]  (r - ov*b/nn elementwise)
+void launchOrthoStep(double* r, const double* b, const double* d, int64_t n, hipStream_t s);
+// out[0] = ||x||^2
+void launchNorm2(const double* x, int64_t n, double* out, DevMem& partial, hipStream_t s);
+// dst = src / sqrt(nrm2[0])
+void launchScaleInto(double* dst, const double* src, const double* nrm2, int64_t n, hipStream_t s);
+// x += alpha * y
+void launchAxpy(double* x, const double* y, double alpha, int64_t n, hipStream_t s);
+// A[i*m+j] = G[min(i,j)*m + max(i,j)]  (upper triangle mirrored)
+void launchSymmetrizeUpper(const double* G, int m, double* A, hipStream_t s);
+// symmetric eigen-decomposition of the m x m matrix (row-major) by cyclic
+// Jacobi, eigenvalues ascending in w, eigenvectors in the columns of V
+void launchJacobi(const double* A, int m, double* w, double* V, hipStream_t s);
+
+}  // namespace shci

@@ -1,0 +1,1088 @@
+// SHCI variational-step kernels for CDNA4 (gfx950, wave64).
+//
+//   energy / Hij batches      SHCI/Determinants.cpp:252-300,406-505,564-631
+//   heat-bath screening       SHCI/SHCIgetdeterminants.cpp:586-739
+//   connected-pair CSR        SHCI/SHCImakeHamiltonian.cpp:486-715 (row content)
+//   sigma = H x               SHCI/Hmult.h:52-150
+//   Davidson vector algebra   SHCI/Davidson.cpp:85-348
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.h"
+
+namespace shci {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+inline unsigned gridFor(int64_t n, int per = kBlock, int64_t cap = 1 << 20) {
+  int64_t g = (n + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// determinant primitives
+
+template <int W>
+__global__ void kEnergy(DevSystem S, const uint64_t* dets, int64_t n, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[i * W + w];
+    out[i] = detEnergy<W>(S, d);
+  }
+}
+
+template <int W>
+__global__ void kHijPairs(DevSystem S, const uint64_t* bra, const uint64_t* ket, int64_t n, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t b[W], k[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      b[w] = bra[i * W + w];
+      k[w] = ket[i * W + w];
+    }
+    out[i] = hijElement<W>(S, b, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// membership table
+
+__global__ void kFillNeg(int32_t* t, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    t[i] = -1;
+}
+
+template <int W>
+__device__ __forceinline__ bool detEqual(const uint64_t* a, const uint64_t* b) {
+  bool e = true;
+#pragma unroll
+  for (int w = 0; w < W; w++) e &= (a[w] == b[w]);
+  return e;
+}
+
+template <int W>
+__global__ void kHashInsert(const uint64_t* dets, int64_t begin, int64_t end, int32_t* table, uint64_t mask) {
+  for (int64_t i = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[i * W + w];
+    uint64_t h = detHash<W>(d) & mask;
+    while (true) {
+      int32_t prev = atomicCAS(&table[h], -1, (int32_t)i);
+      if (prev == -1) break;
+      if (detEqual<W>(dets + (int64_t)prev * W, d)) break;
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ bool hashContains(const int32_t* table, uint64_t mask, const uint64_t* dets,
+                                             const uint64_t* key) {
+  uint64_t h = detHash<W>(key) & mask;
+  while (true) {
+    int32_t v = table[h];
+    if (v < 0) return false;
+    if (detEqual<W>(dets + (int64_t)v * W, key)) return true;
+    h = (h + 1) & mask;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// screening
+
+__global__ void kCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cmax) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < nroots; r++) {
+      double c = fabs(ci[r * ld + j]);
+      s = __dadd_rn(s, __dmul_rn(c, c));
+    }
+    cmax[j] = sqrt(s);
+  }
+}
+
+// emit one candidate per lane with `want`, skipping determinants already in
+// the variational space; wave-aggregated append
+template <int W>
+__device__ __forceinline__ void emitCandidate(bool want, const uint64_t* cand, const int32_t* table, uint64_t mask,
+                                              const uint64_t* dets, uint64_t* out, unsigned long long* counter,
+                                              int64_t cap) {
+  bool keep = want && !hashContains<W>(table, mask, dets, cand);
+  uint64_t m = __ballot(keep);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  unsigned long long base = 0;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (keep) {
+    unsigned long long pos = base + __popcll(m & lanemaskLt());
+    if ((int64_t)pos < cap) {
+#pragma unroll
+      for (int w = 0; w < W; w++) out[pos * W + w] = cand[w];
+    }
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* dets, int64_t ndets,
+                                                  const double* cmax, double eps1, int ncore, int nact,
+                                                  const int32_t* table, uint64_t mask, uint64_t* out,
+                                                  unsigned long long* counter, int64_t cap) {
+  __shared__ uint8_t sClosed[kWaves][128];
+  __shared__ uint8_t sOpen[kWaves][128];
+  __shared__ int sExcl[kWaves][65];
+  __shared__ int64_t sStart[kWaves][64];
+  __shared__ int sPair[kWaves][64];
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int nso = S.nso;
+  for (int64_t di = (int64_t)blockIdx.x * kWaves + wave; di < ndets; di += (int64_t)gridDim.x * kWaves) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[di * W + w];
+    const double eps = eps1 / fabs(cmax[di]);
+    // ascending closed / open spin-orbital lists
+    int nclosed = 0, nopen = 0;
+    for (int base = 0; base < nso; base += 64) {
+      int k = base + lane;
+      bool valid = k < nso;
+      bool o = valid && occ<W>(d, k);
+      uint64_t mc = __ballot(o), mo = __ballot(valid && !o);
+      if (o) sClosed[wave][nclosed + __popcll(mc & lanemaskLt())] = (uint8_t)k;
+      if (valid && !o) sOpen[wave][nopen + __popcll(mo & lanemaskLt())] = (uint8_t)k;
+      nclosed += __popcll(mc);
+      nopen += __popcll(mo);
+    }
+    __syncwarp();
+    int cur[SHCI_MAX_RESTRICT];
+    for (int r = 0; r < S.nrestrict; r++) {
+      int c = 0;
+      for (int k = 0; k < nclosed; k++) c += S.rmult[r * nso + sClosed[wave][k]];
+      cur[r] = c;
+    }
+    // ---- single excitations
+    const int nsingle = nclosed * nopen;
+    for (int t0 = 0; t0 < nsingle; t0 += 64) {
+      int t = t0 + lane;
+      bool want = false;
+      uint64_t cand[W];
+#pragma unroll
+      for (int w = 0; w < W; w++) cand[w] = d[w];
+      if (t < nsingle) {
+        int i = t / nopen, a = t - (t / nopen) * nopen;
+        int ci = sClosed[wave][i], oa = sOpen[wave][a];
+        bool ok = !((ci >> 1) < ncore || (oa >> 1) >= ncore + nact);
+        for (int r = 0; r < S.nrestrict && ok; r++) {
+          int e = cur[r] - S.rmult[r * nso + ci] + S.rmult[r * nso + oa];
+          if (e < S.rmin[r] || e > S.rmax[r]) ok = false;
+        }
+        if (ok) {
+          double integral = S.singles[oa * nso + ci];
+          if (fabs(integral) > eps)
+            if (((ci ^ oa) & 1) == 0) integral = hijSingle<W>(S, d, oa, ci);
+          if (fabs(integral) > eps) {
+            want = true;
+            cand[oa >> 6] |= (1ull << (oa & 63));
+            cand[ci >> 6] &= ~(1ull << (ci & 63));
+          }
+        }
+      }
+      emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
+    }
+    // ---- double excitations
+    if (S.maxEntryAbs < eps) continue;
+    const int npairs = nclosed * (nclosed - 1) / 2;
+    for (int p0 = 0; p0 < npairs; p0 += 64) {
+      int p = p0 + lane;
+      int len = 0;
+      int64_t st = 0;
+      if (p < npairs) {
+        // p = i*(i-1)/2 + j with i > j
+        int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+        while (i * (i - 1) / 2 > p) i--;
+        while ((i + 1) * i / 2 <= p) i++;
+        int j = p - i * (i - 1) / 2;
+        int ci = sClosed[wave][i], cj = sClosed[wave][j];
+        if (!((ci >> 1) < ncore || (cj >> 1) < ncore)) {
+          int I = ci >> 1, J = cj >> 1;
+          int X = I > J ? I : J, Y = I > J ? J : I;
+          int pairIndex = X * (X + 1) / 2 + Y;
+          int sameSpin = ((ci ^ cj) & 1) == 0 ? 1 : 0;
+          int64_t s0 = S.hbStart[sameSpin][pairIndex], s1 = S.hbStart[sameSpin][pairIndex + 1];
+          const float* v = S.hbVal[sameSpin];
+          // number of leading entries with |v| >= eps (list is |v|-descending)
+          int64_t lo = s0, hi = s1;
+          while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if ((double)fabsf(v[mid]) < eps)
+              hi = mid;
+            else
+              lo = mid + 1;
+          }
+          len = (int)(lo - s0);
+          st = s0;
+        }
+        sPair[wave][lane] = i | (j << 8) | ((((sClosed[wave][i] ^ sClosed[wave][j]) & 1) == 0 ? 1 : 0) << 16);
+      }
+      // exclusive prefix of len over lanes
+      int inc = len;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      sExcl[wave][lane] = inc - len;
+      sStart[wave][lane] = st;
+      int total = __shfl(inc, 63, 64);
+      __syncwarp();
+      for (int k0 = 0; k0 < total; k0 += 64) {
+        int k = k0 + lane;
+        bool want = false;
+        uint64_t cand[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) cand[w] = d[w];
+        if (k < total) {
+          // owner lane q: last q with sExcl[q] <= k
+          int lo = 0, hi = 63;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sExcl[wave][mid] <= k)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          int q = lo;
+          int pk = sPair[wave][q];
+          int i = pk & 0xFF, j = (pk >> 8) & 0xFF, same = (pk >> 16) & 1;
+          int ci = sClosed[wave][i], cj = sClosed[wave][j];
+          int64_t idx = sStart[wave][q] + (k - sExcl[wave][q]);
+          uint32_t ab = S.hbPair[same][idx];
+          int a = 2 * (int)(ab & 0xFFFF) + (ci & 1), b = 2 * (int)(ab >> 16) + (cj & 1);
+          bool ok = !((a >> 1) >= ncore + nact || (b >> 1) >= ncore + nact);
+          for (int r = 0; r < S.nrestrict && ok; r++) {
+            const int8_t* m = S.rmult + r * nso;
+            int e = cur[r] - m[ci] - m[cj] + m[a] + m[b];
+            if (e < S.rmin[r] || e > S.rmax[r]) ok = false;
+          }
+          if (ok && !(occ<W>(d, a) || occ<W>(d, b))) {
+            want = true;
+            cand[a >> 6] |= (1ull << (a & 63));
+            cand[b >> 6] |= (1ull << (b & 63));
+            cand[ci >> 6] &= ~(1ull << (ci & 63));
+            cand[cj >> 6] &= ~(1ull << (cj & 63));
+          }
+        }
+        emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
+      }
+      __syncwarp();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// string tables
+
+__device__ __forceinline__ uint64_t compressEven(uint64_t x) {
+  x &= 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return x;
+}
+
+template <int W>
+__global__ void kExtractStrings(const uint64_t* dets, int64_t n, uint64_t* alpha, uint64_t* beta) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t a = 0, b = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      uint64_t x = dets[i * W + w];
+      a |= compressEven(x) << (32 * w);
+      b |= compressEven(x >> 1) << (32 * w);
+    }
+    alpha[i] = a;
+    beta[i] = b;
+  }
+}
+
+__device__ __forceinline__ int64_t lowerBound(const uint64_t* arr, int64_t n, uint64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (arr[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void kIndexOf(const uint64_t* str, int64_t n, const uint64_t* sorted, int64_t ns, uint32_t* idx) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    idx[i] = (uint32_t)lowerBound(sorted, ns, str[i]);
+}
+
+__global__ void kGroupKeys(const uint32_t* major, const uint32_t* minor, int64_t n, uint64_t nminor, uint64_t* key,
+                           uint32_t* val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    key[i] = (uint64_t)major[i] * nminor + minor[i];
+    val[i] = (uint32_t)i;
+  }
+}
+
+__global__ void kGroupStarts(const uint64_t* key, int64_t n, uint64_t nminor, int64_t ngroups, int64_t* start,
+                             const uint32_t* val, const uint64_t* partnerStr, uint64_t* memberPartner) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m <= n; m += (int64_t)gridDim.x * blockDim.x) {
+    if (m == n) {
+      start[ngroups] = n;
+      continue;
+    }
+    uint64_t g = key[m] / nminor;
+    if (m == 0 || key[m - 1] / nminor != g) start[g] = m;
+    memberPartner[m] = partnerStr[val[m]];
+  }
+}
+
+// alpha strings one excitation away from each unique alpha string
+template <bool FILL>
+__global__ void kStringSingles(const uint64_t* sorted, int64_t ns, int norb, const int64_t* start, int64_t* count,
+                               uint32_t* out) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t str = sorted[s];
+    int64_t c = 0, pos = FILL ? start[s] : 0;
+    uint64_t occd = str;
+    while (occd) {
+      int j = __ffsll((unsigned long long)occd) - 1;
+      occd &= occd - 1;
+      uint64_t emp = ~str & ((norb >= 64) ? ~0ull : ((1ull << norb) - 1ull));
+      while (emp) {
+        int k = __ffsll((unsigned long long)emp) - 1;
+        emp &= emp - 1;
+        uint64_t t = (str & ~(1ull << j)) | (1ull << k);
+        int64_t f = lowerBound(sorted, ns, t);
+        if (f < ns && sorted[f] == t) {
+          if (FILL) out[pos++] = (uint32_t)f;
+          c++;
+        }
+      }
+    }
+    if (!FILL) count[s] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// connected-pair rows: one wavefront per row
+
+struct RowTables {
+  const uint64_t* alpha;
+  const uint64_t* beta;
+  const uint32_t* aIdx;
+  const uint32_t* bIdx;
+  const int64_t* aStart;
+  const int64_t* bStart;
+  const uint32_t* aMajorDet;
+  const uint32_t* bMajorDet;
+  const uint64_t* aMajorB;
+  const uint64_t* bMajorA;
+  const int64_t* aSinglesStart;
+  const uint32_t* aSingles;
+  int64_t n;
+};
+
+template <int W, bool FILL>
+struct RowSink {
+  const DevSystem* S;
+  const uint64_t* dets;
+  const uint64_t* dI;
+  int64_t I;
+  int32_t* cols;
+  double* vals;
+  int64_t pos;  // wave-uniform write cursor
+  int64_t cnt;  // per-lane count (count pass)
+  __device__ __forceinline__ void hit(bool h, uint32_t J) {
+    if (!FILL) {
+      cnt += h ? 1 : 0;
+      return;
+    }
+    uint64_t m = __ballot(h);
+    if (h) {
+      int64_t p = pos + __popcll(m & lanemaskLt());
+      uint64_t dJ[W];
+#pragma unroll
+      for (int w = 0; w < W; w++) dJ[w] = dets[(int64_t)J * W + w];
+      double v = ((int64_t)J < I) ? hijElement<W>(*S, dJ, dI) : hijElement<W>(*S, dI, dJ);
+      cols[p] = (int32_t)J;
+      vals[p] = v;
+    }
+    pos += __popcll(m);
+  }
+};
+
+template <int W, bool FILL>
+__global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t* rowCount,
+                                                   const int64_t* rowPtr, int32_t* cols, double* vals) {
+  __shared__ int sExcl[kWaves][65];
+  __shared__ int64_t sBeg[kWaves][64];
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < T.n; I += (int64_t)gridDim.x * kWaves) {
+    uint64_t dI[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
+    const uint64_t astr = T.alpha[I], bstr = T.beta[I];
+    const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
+    RowSink<W, FILL> sink{&S, dets, dI, I, cols, vals, FILL ? rowPtr[I] : 0, 0};
+    // (a) same beta string: alpha singles and alpha-alpha doubles
+    {
+      int64_t b0 = T.bStart[bi], b1 = T.bStart[bi + 1];
+      for (int64_t m0 = b0; m0 < b1; m0 += 64) {
+        int64_t m = m0 + lane;
+        bool h = false;
+        uint32_t J = 0;
+        if (m < b1) {
+          int dd = __popcll(T.bMajorA[m] ^ astr);
+          h = (dd == 2 || dd == 4);
+          J = T.bMajorDet[m];
+        }
+        sink.hit(h, J);
+      }
+    }
+    // (b) same alpha string: beta singles and beta-beta doubles
+    {
+      int64_t a0 = T.aStart[ai], a1 = T.aStart[ai + 1];
+      for (int64_t m0 = a0; m0 < a1; m0 += 64) {
+        int64_t m = m0 + lane;
+        bool h = false;
+        uint32_t J = 0;
+        if (m < a1) {
+          int dd = __popcll(T.aMajorB[m] ^ bstr);
+          h = (dd == 2 || dd == 4);
+          J = T.aMajorDet[m];
+        }
+        sink.hit(h, J);
+      }
+    }
+    // (c) alpha single x beta single: members of the groups of alpha strings
+    // one excitation away whose beta string is one excitation away
+    {
+      int64_t s0 = T.aSinglesStart[ai], s1 = T.aSinglesStart[ai + 1];
+      for (int64_t c0 = s0; c0 < s1; c0 += 64) {
+        int64_t c = c0 + lane;
+        int len = 0;
+        int64_t beg = 0;
+        if (c < s1) {
+          uint32_t sa = T.aSingles[c];
+          beg = T.aStart[sa];
+          len = (int)(T.aStart[sa + 1] - beg);
+        }
+        int inc = len;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          int t = __shfl_up(inc, o, 64);
+          if (lane >= o) inc += t;
+        }
+        sExcl[wave][lane] = inc - len;
+        sBeg[wave][lane] = beg;
+        int total = __shfl(inc, 63, 64);
+        __syncwarp();
+        for (int k0 = 0; k0 < total; k0 += 64) {
+          int k = k0 + lane;
+          bool h = false;
+          uint32_t J = 0;
+          if (k < total) {
+            int lo = 0, hi = 63;
+            while (lo < hi) {
+              int mid = (lo + hi + 1) >> 1;
+              if (sExcl[wave][mid] <= k)
+                lo = mid;
+              else
+                hi = mid - 1;
+            }
+            int64_t m = sBeg[wave][lo] + (k - sExcl[wave][lo]);
+            h = __popcll(T.aMajorB[m] ^ bstr) == 2;
+            J = T.aMajorDet[m];
+          }
+          sink.hit(h, J);
+        }
+        __syncwarp();
+      }
+    }
+    if (!FILL) {
+      int64_t c = sink.cnt;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+      if (lane == 0) rowCount[I] = c;
+    }
+  }
+}
+
+__global__ void kCast(const double* v, float* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)v[i];
+}
+
+// per-row insertion/shell sort of columns (rows are short relative to n)
+__global__ void kSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = rowPtr[r], e = rowPtr[r + 1];
+    int64_t len = e - b;
+    for (int64_t gap = len / 2; gap > 0; gap /= 2) {
+      for (int64_t i = b + gap; i < e; i++) {
+        int32_t c = cols[i];
+        double v = vals[i];
+        int64_t j = i;
+        while (j - gap >= b && cols[j - gap] > c) {
+          cols[j] = cols[j - gap];
+          vals[j] = vals[j - gap];
+          j -= gap;
+        }
+        cols[j] = c;
+        vals[j] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sigma: one wavefront per row, lanes stride the row; FP64 accumulation
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) kSigma(const int64_t* __restrict__ rowPtr, const int32_t* __restrict__ cols,
+                                                 const VT* __restrict__ vals, const double* __restrict__ diag,
+                                                 int64_t n, const double* __restrict__ x, double* __restrict__ y,
+                                                 int k, int64_t ld) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
+    const int64_t b = rowPtr[I], e = rowPtr[I + 1];
+    for (int r = 0; r < k; r++) {
+      const double* xr = x + r * ld;
+      double acc0 = 0.0, acc1 = 0.0;
+      int64_t p = b + lane;
+      for (; p + 64 < e; p += 128) {
+        int32_t c0 = __builtin_nontemporal_load(cols + p);
+        int32_t c1 = __builtin_nontemporal_load(cols + p + 64);
+        double v0 = (double)__builtin_nontemporal_load(vals + p);
+        double v1 = (double)__builtin_nontemporal_load(vals + p + 64);
+        acc0 = fma(v0, xr[c0], acc0);
+        acc1 = fma(v1, xr[c1], acc1);
+      }
+      if (p < e) {
+        int32_t c0 = __builtin_nontemporal_load(cols + p);
+        double v0 = (double)__builtin_nontemporal_load(vals + p);
+        acc0 = fma(v0, xr[c0], acc0);
+      }
+      double acc = acc0 + acc1;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) y[r * ld + I] = fma(diag[I], xr[I], acc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Davidson vector kernels
+
+constexpr int kRedBlocks = 512;
+
+// per-block partial Gram entries, then a fixed-order final sum
+template <int MA, int MB>
+__global__ void __launch_bounds__(kBlock) kGramPartial(const double* A, const double* B, int ma, int mb, int64_t n,
+                                                       int64_t ld, double* partial) {
+  __shared__ double red[kWaves][MA * MB];
+  double acc[MA * MB];
+#pragma unroll
+  for (int q = 0; q < MA * MB; q++) acc[q] = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double a[MA], bb[MB];
+#pragma unroll
+    for (int i = 0; i < MA; i++) a[i] = i < ma ? A[i * ld + k] : 0.0;
+#pragma unroll
+    for (int j = 0; j < MB; j++) bb[j] = j < mb ? B[j * ld + k] : 0.0;
+#pragma unroll
+    for (int i = 0; i < MA; i++)
+#pragma unroll
+      for (int j = 0; j < MB; j++) acc[i * MB + j] = fma(a[i], bb[j], acc[i * MB + j]);
+  }
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+#pragma unroll
+  for (int q = 0; q < MA * MB; q++) {
+    double v = acc[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < MA * MB; q += blockDim.x) {
+    double v = 0.0;
+    for (int w = 0; w < kWaves; w++) v += red[w][q];
+    partial[(int64_t)blockIdx.x * MA * MB + q] = v;
+  }
+}
+
+__global__ void kGramFinal(const double* partial, int nblocks, int MA, int MB, int ma, int mb, double* G) {
+  int q = threadIdx.x;
+  if (q >= ma * mb) return;
+  int i = q / mb, j = q % mb;
+  double v = 0.0;
+  for (int b = 0; b < nblocks; b++) v += partial[(int64_t)b * MA * MB + i * MB + j];
+  G[i * mb + j] = v;
+}
+
+__global__ void kRotate(double* X, int m, int64_t n, int64_t ld, const double* V) {
+  __shared__ double sV[16 * 16];
+  for (int q = threadIdx.x; q < m * m; q += blockDim.x) sV[q] = V[q];
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double xin[16], xo[16];
+    for (int l = 0; l < m; l++) xin[l] = X[l * ld + k];
+    for (int j = 0; j < m; j++) {
+      double s = 0.0;
+      for (int l = 0; l < m; l++) s = fma(xin[l], sV[l * m + j], s);
+      xo[j] = s;
+    }
+    for (int j = 0; j < m; j++) X[j * ld + k] = xo[j];
+  }
+}
+
+__device__ __forceinline__ double blockSum(double v) {
+  __shared__ double red[kWaves];
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kWaves; w++) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(kBlock) kResidual(const double* sig, const double* b, const double* e, int64_t n,
+                                                    double* r, double* partial) {
+  const double ev = e[0];
+  double acc = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double v = sig[k] - ev * b[k];
+    r[k] = v;
+    acc = fma(v, v, acc);
+  }
+  double t = blockSum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void kFinalSum(const double* partial, int nb, int stride, int nvals, double* out) {
+  int q = threadIdx.x;
+  if (q >= nvals) return;
+  double v = 0.0;
+  for (int b = 0; b < nb; b++) v += partial[(int64_t)b * stride + q];
+  out[q] = v;
+}
+
+__global__ void kPrecondition(double* r, const double* diag, const double* e, int64_t n) {
+  const double ev = e[0];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double den = ev - diag[k];
+    if (fabs(den) > 1e-12)
+      r[k] = r[k] / den;
+    else
+      r[k] = r[k] / (den - 1.e-12);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kDot2(const double* b, const double* r, int64_t n, double* partial) {
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double bv = b[k];
+    s0 = fma(bv, r[k], s0);
+    s1 = fma(bv, bv, s1);
+  }
+  double t0 = blockSum(s0);
+  double t1 = blockSum(s1);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = t0;
+    partial[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+__global__ void kOrthoStep(double* r, const double* b, const double* d, int64_t n) {
+  const double ov = d[0], nn = d[1];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    r[k] = r[k] - ov * b[k] / nn;
+}
+
+__global__ void __launch_bounds__(kBlock) kNorm2(const double* x, int64_t n, double* partial) {
+  double s = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    s = fma(x[k], x[k], s);
+  double t = blockSum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void kScaleInto(double* dst, const double* src, const double* nrm2, int64_t n) {
+  const double nr = sqrt(nrm2[0]);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    dst[k] = src[k] / nr;
+}
+
+__global__ void kAxpy(double* x, const double* y, double alpha, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    x[k] = x[k] + alpha * y[k];
+}
+
+__global__ void kSymUpper(const double* G, int m, double* A) {
+  int q = threadIdx.x;
+  if (q >= m * m) return;
+  int i = q / m, j = q % m;
+  A[q] = i <= j ? G[i * m + j] : G[j * m + i];
+}
+
+// cyclic Jacobi (one thread), m <= 16
+__global__ void kJacobi(const double* A, int m, double* w, double* V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[16 * 16], v[16 * 16];
+  for (int q = 0; q < m * m; q++) {
+    a[q] = A[q];
+    v[q] = 0.0;
+  }
+  for (int i = 0; i < m; i++) v[i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) off += fabs(a[p * m + q]);
+    if (off == 0.0) break;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) {
+        double apq = a[p * m + q], app = a[p * m + p], aqq = a[q * m + q];
+        double g = 100.0 * fabs(apq);
+        if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
+          a[p * m + q] = 0.0;
+          a[q * m + p] = 0.0;
+          continue;
+        }
+        if (apq == 0.0) continue;
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < m; k++) {
+          double akp = a[k * m + p], akq = a[k * m + q];
+          a[k * m + p] = c * akp - sn * akq;
+          a[k * m + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < m; k++) {
+          double apk = a[p * m + k], aqk = a[q * m + k];
+          a[p * m + k] = c * apk - sn * aqk;
+          a[q * m + k] = sn * apk + c * aqk;
+        }
+        a[p * m + q] = 0.0;
+        a[q * m + p] = 0.0;
+        for (int k = 0; k < m; k++) {
+          double vkp = v[k * m + p], vkq = v[k * m + q];
+          v[k * m + p] = c * vkp - sn * vkq;
+          v[k * m + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  int order[16];
+  for (int i = 0; i < m; i++) order[i] = i;
+  for (int i = 1; i < m; i++) {  // stable insertion sort by eigenvalue
+    int o = order[i];
+    int j = i;
+    while (j > 0 && a[order[j - 1] * m + order[j - 1]] > a[o * m + o]) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = o;
+  }
+  for (int j = 0; j < m; j++) {
+    w[j] = a[order[j] * m + order[j]];
+    for (int k = 0; k < m; k++) V[k * m + j] = v[k * m + order[j]];
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+
+#define DISPATCH_W(W_, ...)                  \
+  do {                                       \
+    if ((W_) == 1) {                         \
+      constexpr int WW = 1;                  \
+      __VA_ARGS__;                           \
+    } else if ((W_) == 2) {                  \
+      constexpr int WW = 2;                  \
+      __VA_ARGS__;                           \
+    } else {                                 \
+      throw HipError("unsupported det width"); \
+    }                                        \
+  } while (0)
+
+void launchEnergy(const DevSystem& S, const uint64_t* dets, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  DISPATCH_W(S.W, kEnergy<WW><<<gridFor(n), kBlock, 0, s>>>(S, dets, n, out));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchHijPairs(const DevSystem& S, const uint64_t* bra, const uint64_t* ket, int64_t n, double* out,
+                    hipStream_t s) {
+  if (n <= 0) return;
+  DISPATCH_W(S.W, kHijPairs<WW><<<gridFor(n), kBlock, 0, s>>>(S, bra, ket, n, out));
+  HIP_CHECK(hipGetLastError());
+}
+
+void hashClear(int32_t* table, uint64_t slots, hipStream_t s) {
+  kFillNeg<<<gridFor((int64_t)slots), kBlock, 0, s>>>(table, slots);
+  HIP_CHECK(hipGetLastError());
+}
+
+void hashInsert(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, int32_t* table,
+                uint64_t slots, hipStream_t s) {
+  if (end <= begin) return;
+  DISPATCH_W(S.W, kHashInsert<WW><<<gridFor(end - begin), kBlock, 0, s>>>(dets, begin, end, table, slots - 1));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cmax, hipStream_t s) {
+  if (n <= 0) return;
+  kCmax<<<gridFor(n), kBlock, 0, s>>>(ci, nroots, ld, n, cmax);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* cmax, double eps1, int ncore,
+                  int nact, const int32_t* table, uint64_t slots, uint64_t* out, unsigned long long* counter,
+                  int64_t cap, hipStream_t s) {
+  if (n <= 0) return;
+  unsigned g = gridFor(n, kWaves, 1 << 16);
+  DISPATCH_W(S.W, kScreen<WW><<<g, kBlock, 0, s>>>(S, dets, n, cmax, eps1, ncore, nact, table, slots - 1, out,
+                                                      counter, cap));
+  HIP_CHECK(hipGetLastError());
+}
+
+void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, StringTables& T, Workspace& ws,
+                       hipStream_t s) {
+  T.n = n;
+  const int norb = S.norb;
+  T.alpha.reserve(n * 8);
+  T.beta.reserve(n * 8);
+  T.aIdx.reserve(n * 4);
+  T.bIdx.reserve(n * 4);
+  T.sortedA.reserve(n * 8);
+  T.sortedB.reserve(n * 8);
+  T.key.reserve(n * 8);
+  T.val.reserve(n * 4);
+  T.aMajorDet.reserve(n * 4);
+  T.bMajorDet.reserve(n * 4);
+  T.aMajorB.reserve(n * 8);
+  T.bMajorA.reserve(n * 8);
+  DISPATCH_W(S.W, kExtractStrings<WW><<<gridFor(n), kBlock, 0, s>>>(dets, n, T.alpha.as<uint64_t>(),
+                                                                        T.beta.as<uint64_t>()));
+  HIP_CHECK(hipMemcpyAsync(T.sortedA.p, T.alpha.p, n * 8, hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(T.sortedB.p, T.beta.p, n * 8, hipMemcpyDeviceToDevice, s));
+  radixSortKeys(T.sortedA.as<uint64_t>(), nullptr, n, 1, norb, false, ws, s);
+  T.nA = uniqueKeys(T.sortedA.as<uint64_t>(), n, 1, ws, s);
+  radixSortKeys(T.sortedB.as<uint64_t>(), nullptr, n, 1, norb, false, ws, s);
+  T.nB = uniqueKeys(T.sortedB.as<uint64_t>(), n, 1, ws, s);
+  kIndexOf<<<gridFor(n), kBlock, 0, s>>>(T.alpha.as<uint64_t>(), n, T.sortedA.as<uint64_t>(), T.nA,
+                                          T.aIdx.as<uint32_t>());
+  kIndexOf<<<gridFor(n), kBlock, 0, s>>>(T.beta.as<uint64_t>(), n, T.sortedB.as<uint64_t>(), T.nB,
+                                          T.bIdx.as<uint32_t>());
+  auto bitsFor = [](uint64_t v) {
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) b++;
+    return b;
+  };
+  T.aStart.reserve((T.nA + 1) * 8);
+  T.bStart.reserve((T.nB + 1) * 8);
+  // alpha-major groups: key = aIdx * nB + bIdx
+  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.aIdx.as<uint32_t>(), T.bIdx.as<uint32_t>(), n, (uint64_t)T.nB,
+                                            T.key.as<uint64_t>(), T.val.as<uint32_t>());
+  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * T.nB), false, ws, s);
+  HIP_CHECK(hipMemcpyAsync(T.aMajorDet.p, T.val.p, n * 4, hipMemcpyDeviceToDevice, s));
+  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)T.nB, T.nA,
+                                                  T.aStart.as<int64_t>(), T.val.as<uint32_t>(),
+                                                  T.beta.as<uint64_t>(), T.aMajorB.as<uint64_t>());
+  // beta-major groups: key = bIdx * nA + aIdx
+  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.bIdx.as<uint32_t>(), T.aIdx.as<uint32_t>(), n, (uint64_t)T.nA,
+                                            T.key.as<uint64_t>(), T.val.as<uint32_t>());
+  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * T.nB), false, ws, s);
+  HIP_CHECK(hipMemcpyAsync(T.bMajorDet.p, T.val.p, n * 4, hipMemcpyDeviceToDevice, s));
+  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)T.nA, T.nB,
+                                                  T.bStart.as<int64_t>(), T.val.as<uint32_t>(),
+                                                  T.alpha.as<uint64_t>(), T.bMajorA.as<uint64_t>());
+  // alpha single-excitation adjacency
+  T.aSinglesStart.reserve((T.nA + 1) * 8);
+  int64_t* cnt = T.aSinglesStart.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(cnt + T.nA, 0, 8, s));
+  kStringSingles<false><<<gridFor(T.nA), kBlock, 0, s>>>(T.sortedA.as<uint64_t>(), T.nA, norb, nullptr, cnt,
+                                                          nullptr);
+  exclusiveScanI64(cnt, T.nA + 1, ws, s);
+  int64_t tot = 0;
+  HIP_CHECK(hipMemcpyAsync(&tot, cnt + T.nA, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  T.aSingles.reserve(std::max<int64_t>(tot, 1) * 4);
+  kStringSingles<true><<<gridFor(T.nA), kBlock, 0, s>>>(T.sortedA.as<uint64_t>(), T.nA, norb, cnt, nullptr,
+                                                         T.aSingles.as<uint32_t>());
+  HIP_CHECK(hipGetLastError());
+}
+
+static RowTables rowTables(const StringTables& T) {
+  RowTables r;
+  r.alpha = T.alpha.as<uint64_t>();
+  r.beta = T.beta.as<uint64_t>();
+  r.aIdx = T.aIdx.as<uint32_t>();
+  r.bIdx = T.bIdx.as<uint32_t>();
+  r.aStart = T.aStart.as<int64_t>();
+  r.bStart = T.bStart.as<int64_t>();
+  r.aMajorDet = T.aMajorDet.as<uint32_t>();
+  r.bMajorDet = T.bMajorDet.as<uint32_t>();
+  r.aMajorB = T.aMajorB.as<uint64_t>();
+  r.bMajorA = T.bMajorA.as<uint64_t>();
+  r.aSinglesStart = T.aSinglesStart.as<int64_t>();
+  r.aSingles = T.aSingles.as<uint32_t>();
+  r.n = T.n;
+  return r;
+}
+
+void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t* rowCount,
+                    hipStream_t s) {
+  if (T.n <= 0) return;
+  unsigned g = gridFor(T.n, kWaves, 1 << 16);
+  RowTables r = rowTables(T);
+  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, rowCount, nullptr, nullptr, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, const int64_t* rowPtr,
+                   int32_t* cols, double* vals, hipStream_t s) {
+  if (T.n <= 0) return;
+  unsigned g = gridFor(T.n, kWaves, 1 << 16);
+  RowTables r = rowTables(T);
+  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nullptr, rowPtr, cols, vals));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCastValues(const double* v, float* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  kCast<<<gridFor(n), kBlock, 0, s>>>(v, out, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals, hipStream_t s) {
+  if (n <= 0) return;
+  kSortRows<<<gridFor(n, 64), 64, 0, s>>>(rowPtr, n, cols, vals);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
+                    const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  if (n <= 0) return;
+  kSigma<double><<<gridFor(n, kWaves, 1 << 18), kBlock, 0, s>>>(rowPtr, cols, vals, diag, n, x, y, k, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
+                    const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  if (n <= 0) return;
+  kSigma<float><<<gridFor(n, kWaves, 1 << 18), kBlock, 0, s>>>(rowPtr, cols, vals, diag, n, x, y, k, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,
+                hipStream_t s) {
+  const int nb = (int)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
+  const int nbb = nb < 1 ? 1 : nb;
+  if (ma <= 1 && mb <= 1) {
+    partial.reserve((size_t)nbb * 1 * sizeof(double));
+    kGramPartial<1, 1><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
+    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 1, 1, ma, mb, G);
+  } else if (ma <= 1 && mb <= 8) {
+    partial.reserve((size_t)nbb * 8 * sizeof(double));
+    kGramPartial<1, 8><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
+    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 1, 8, ma, mb, G);
+  } else if (ma <= 8 && mb <= 8) {
+    partial.reserve((size_t)nbb * 64 * sizeof(double));
+    kGramPartial<8, 8><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
+    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 8, 8, ma, mb, G);
+  } else if (ma <= 12 && mb <= 12) {
+    partial.reserve((size_t)nbb * 144 * sizeof(double));
+    kGramPartial<12, 12><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
+    kGramFinal<<<1, 256, 0, s>>>(partial.as<double>(), nbb, 12, 12, ma, mb, G);
+  } else {
+    throw HipError("launchGram: subspace too large");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchRotate(double* X, int m, int64_t n, int64_t ld, const double* V, hipStream_t s) {
+  if (m > 16) throw HipError("launchRotate: subspace too large");
+  kRotate<<<gridFor(n, kBlock, 4096), kBlock, 0, s>>>(X, m, n, ld, V);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchResidual(const double* sig, const double* b, const double* e, int64_t n, double* r, double* out,
+                    DevMem& partial, hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
+  partial.reserve((size_t)nb * sizeof(double));
+  kResidual<<<nb, kBlock, 0, s>>>(sig, b, e, n, r, partial.as<double>());
+  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPrecondition(double* r, const double* diag, const double* e, int64_t n, hipStream_t s) {
+  kPrecondition<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(r, diag, e, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchDot2(const double* b, const double* r, int64_t n, double* out, DevMem& partial, hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
+  partial.reserve((size_t)nb * 2 * sizeof(double));
+  kDot2<<<nb, kBlock, 0, s>>>(b, r, n, partial.as<double>());
+  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchOrthoStep(double* r, const double* b, const double* d, int64_t n, hipStream_t s) {
+  kOrthoStep<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(r, b, d, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchNorm2(const double* x, int64_t n, double* out, DevMem& partial, hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
+  partial.reserve((size_t)nb * sizeof(double));
+  kNorm2<<<nb, kBlock, 0, s>>>(x, n, partial.as<double>());
+  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchScaleInto(double* dst, const double* src, const double* nrm2, int64_t n, hipStream_t s) {
+  kScaleInto<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(dst, src, nrm2, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchAxpy(double* x, const double* y, double alpha, int64_t n, hipStream_t s) {
+  kAxpy<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(x, y, alpha, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSymmetrizeUpper(const double* G, int m, double* A, hipStream_t s) {
+  kSymUpper<<<1, 256, 0, s>>>(G, m, A);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchJacobi(const double* A, int m, double* w, double* V, hipStream_t s) {
+  if (m > 16) throw HipError("launchJacobi: subspace too large");
+  kJacobi<<<1, 64, 0, s>>>(A, m, w, V);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace shci

@@ -1,0 +1,278 @@
+"""Python mirror of the reference's SHCI variational-step interface, bound to
+the HIP library (dice_amd/lib/libshci_gpu.so) through its C ABI.
+
+Names follow the reference's C++ entry points:
+
+=====================================  =================================================
+this module                            reference (sanshar/Dice)
+=====================================  =================================================
+``System.from_fcidump``                readIntegrals + twoIntHeatBath(SHM)::constructClass
+                                       (SHCI/integral.cpp:43,228; SHCI/integral.h:125)
+``System.energy``                      Determinant::Energy (SHCI/Determinants.cpp:252)
+``System.hij``                         Hij (SHCI/Determinants.cpp:564)
+``System.get_determinants_variational``  getDeterminantsVariationalApprox + sort/unique
+                                       (SHCI/SHCIgetdeterminants.cpp:586; SHCI/SHCIbasics.cpp:984-1055)
+``System.make_hamiltonian``            PopulateHelpers + MakeHfromSMHelpers2
+                                       (SHCI/SHCImakeHamiltonian.cpp:290,486)
+``System.hmult``                       Hmult2::operator() (SHCI/Hmult.h:52)
+``System.davidson``                    davidson (SHCI/Davidson.cpp:111)
+``do_variational``                     SHCIbasics::DoVariational via ``Dice input.dat``
+                                       (SHCI/SHCIbasics.cpp:783; SHCI/SHCI.cpp:157-446)
+=====================================  =================================================
+
+Determinants are ``numpy.int64`` arrays of shape (n, words), the reference's
+``Determinant::repr`` layout.  There is no CPU fallback: every call runs the
+HIP kernels, and loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libshci_gpu.so")
+EXE_PATH = os.path.join(_HERE, "bin", "shci_var")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_STATUS = {0: "OK", 1: "BAD_INPUT", 2: "OOM", 3: "NO_CONVERGENCE", 4: "HIP_ERROR", 5: "UNSUPPORTED"}
+
+_lib = None
+
+
+class ShciError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (_STATUS.get(status, status), message))
+        self.status = status
+
+
+def library():
+    """Load libshci_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("HIP library %s is not built; run `python -m dice_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    pI64, pI32, pD = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "shci_create": ([ctypes.c_char_p, ctypes.POINTER(P)], I32),
+        "shci_destroy": ([P], I32),
+        "shci_last_error": ([P], ctypes.c_char_p),
+        "shci_norb": ([P], I32),
+        "shci_nelec": ([P], I32),
+        "shci_det_words": ([P], I32),
+        "shci_core_energy": ([P], D),
+        "shci_set_precision": ([P, I32], I32),
+        "shci_set_active_space": ([P, I32, I32], I32),
+        "shci_add_restriction": ([P, I32, I32, pI32, I32], I32),
+        "shci_det_energy": ([P, pI64, I64, pD], I32),
+        "shci_hij": ([P, pI64, pI64, I64, pD], I32),
+        "shci_set_dets": ([P, pI64, I64], I32),
+        "shci_num_dets": ([P, pI64], I32),
+        "shci_get_dets": ([P, pI64, I64], I32),
+        "shci_screen": ([P, pD, D, pI64], I32),
+        "shci_get_new_dets": ([P, pI64, I64], I32),
+        "shci_append_new": ([P], I32),
+        "shci_build_hamiltonian": ([P, pI64], I32),
+        "shci_get_hamiltonian_rows": ([P, I64, pI64, pI32, pD, I64], I32),
+        "shci_sigma": ([P, pD, pD, I32], I32),
+        "shci_sigma_device": ([P, P, P, I32, I64], I32),
+        "shci_davidson": ([P, pD, I32, I32, D, pD, ctypes.POINTER(ctypes.c_int)], I32),
+        "shci_run": ([ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(P)], I32),
+        "shci_run_iterations": ([P], I32),
+        "shci_run_iteration": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int), pD], I32),
+        "shci_run_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
+        "shci_run_ci": ([P, pD, I64], I32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class System:
+    """One FCIDUMP on the device plus a variational space and its Hamiltonian."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = library()
+
+    @classmethod
+    def from_fcidump(cls, path):
+        L = library()
+        h = ctypes.c_void_p()
+        st = L.shci_create(os.fsencode(path), ctypes.byref(h))
+        if st != 0:
+            raise ShciError(st, L.shci_last_error(None).decode())
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            self._lib.shci_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise ShciError(st, self._lib.shci_last_error(self._h).decode())
+
+    norb = property(lambda self: self._lib.shci_norb(self._h))
+    nelec = property(lambda self: self._lib.shci_nelec(self._h))
+    words = property(lambda self: self._lib.shci_det_words(self._h))
+    core_energy = property(lambda self: self._lib.shci_core_energy(self._h))
+
+    def _dets(self, dets):
+        d = np.ascontiguousarray(dets, dtype=np.int64)
+        if d.ndim == 1:
+            d = d.reshape(-1, self.words)
+        if d.shape[1] != self.words:
+            raise ValueError("determinants must have %d words" % self.words)
+        return d
+
+    def set_precision(self, prec):
+        self._check(self._lib.shci_set_precision(self._h, prec))
+
+    def set_active_space(self, ncore, nact):
+        self._check(self._lib.shci_set_active_space(self._h, ncore, nact))
+
+    def add_restriction(self, min_elec, max_elec, spatial_orbs):
+        o = np.ascontiguousarray(spatial_orbs, dtype=np.int32)
+        self._check(self._lib.shci_add_restriction(self._h, min_elec, max_elec, _p(o, ctypes.c_int32), len(o)))
+
+    def energy(self, dets):
+        d = self._dets(dets)
+        out = np.zeros(len(d))
+        self._check(self._lib.shci_det_energy(self._h, _p(d, ctypes.c_int64), len(d), _p(out, ctypes.c_double)))
+        return out
+
+    def hij(self, bra, ket):
+        b, k = self._dets(bra), self._dets(ket)
+        out = np.zeros(len(b))
+        self._check(self._lib.shci_hij(self._h, _p(b, ctypes.c_int64), _p(k, ctypes.c_int64), len(b),
+                                       _p(out, ctypes.c_double)))
+        return out
+
+    def set_dets(self, dets):
+        d = self._dets(dets)
+        self._check(self._lib.shci_set_dets(self._h, _p(d, ctypes.c_int64), len(d)))
+
+    def num_dets(self):
+        n = ctypes.c_int64()
+        self._check(self._lib.shci_num_dets(self._h, ctypes.byref(n)))
+        return n.value
+
+    def dets(self):
+        n = self.num_dets()
+        out = np.zeros((n, self.words), dtype=np.int64)
+        self._check(self._lib.shci_get_dets(self._h, _p(out, ctypes.c_int64), n))
+        return out
+
+    def get_determinants_variational(self, cmax, eps1):
+        """New determinants (sorted, unique, not yet held) selected by heat-bath screening."""
+        c = np.ascontiguousarray(cmax, dtype=np.float64)
+        m = ctypes.c_int64()
+        self._check(self._lib.shci_screen(self._h, _p(c, ctypes.c_double), eps1, ctypes.byref(m)))
+        out = np.zeros((m.value, self.words), dtype=np.int64)
+        self._check(self._lib.shci_get_new_dets(self._h, _p(out, ctypes.c_int64), m.value))
+        return out
+
+    def append_new(self):
+        self._check(self._lib.shci_append_new(self._h))
+
+    def make_hamiltonian(self):
+        """Build H; returns the element count of the reference's lower-triangular rows."""
+        nnz = ctypes.c_int64()
+        self._check(self._lib.shci_build_hamiltonian(self._h, ctypes.byref(nnz)))
+        return nnz.value
+
+    def hamiltonian_rows(self, row_begin=0):
+        """(rowptr, cols, vals) in the reference layout: diagonal first, then J < I ascending."""
+        n = self.num_dets()
+        nnz = self.make_hamiltonian()
+        rp = np.zeros(n - row_begin + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz)
+        self._check(self._lib.shci_get_hamiltonian_rows(self._h, row_begin, _p(rp, ctypes.c_int64),
+                                                        _p(cols, ctypes.c_int32), _p(vals, ctypes.c_double), nnz))
+        return rp, cols[:rp[-1]], vals[:rp[-1]]
+
+    def hmult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        k = 1 if x.ndim == 1 else x.shape[1]
+        xf = np.asfortranarray(x.reshape(len(x), k))
+        y = np.zeros_like(xf, order="F")
+        self._check(self._lib.shci_sigma(self._h, _p(xf, ctypes.c_double), _p(y, ctypes.c_double), k))
+        return y.reshape(x.shape)
+
+    def davidson(self, x0, max_copies=None, tol=5e-5):
+        x = np.asfortranarray(np.array(x0, dtype=np.float64, ndmin=2).T if np.ndim(x0) == 1 else x0, dtype=np.float64)
+        nroots = x.shape[1]
+        e = np.zeros(nroots)
+        it = ctypes.c_int()
+        mc = max_copies or nroots + 4
+        self._check(self._lib.shci_davidson(self._h, _p(x, ctypes.c_double), nroots, mc, tol,
+                                            _p(e, ctypes.c_double), ctypes.byref(it)))
+        return e, x, it.value
+
+
+class VariationalResult:
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = library()
+
+    def __del__(self):
+        try:
+            self._lib.shci_destroy(self._h)
+        except Exception:
+            pass
+
+    def iterations(self):
+        out = []
+        for i in range(self._lib.shci_run_iterations(self._h)):
+            nd, e, it, t = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+            self._lib.shci_run_iteration(self._h, i, ctypes.byref(nd), ctypes.byref(e), ctypes.byref(it),
+                                         ctypes.byref(t))
+            out.append({"ndets": nd.value, "energy": e.value, "davidson": it.value, "seconds": t.value})
+        return out
+
+    def energies(self):
+        e = np.zeros(64)
+        n = ctypes.c_int()
+        self._lib.shci_run_energies(self._h, _p(e, ctypes.c_double), ctypes.byref(n))
+        return e[:n.value].copy()
+
+    def dets(self):
+        n = ctypes.c_int64()
+        self._lib.shci_num_dets(self._h, ctypes.byref(n))
+        w = self._lib.shci_det_words(self._h)
+        out = np.zeros((n.value, w), dtype=np.int64)
+        self._lib.shci_get_dets(self._h, _p(out, ctypes.c_int64), n.value)
+        return out
+
+
+def do_variational(input_path, log_path=None, precision=PREC_FP64):
+    """Run the variational step of ``Dice input.dat``; returns a VariationalResult."""
+    L = library()
+    h = ctypes.c_void_p()
+    st = L.shci_run(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, precision,
+                    ctypes.byref(h))
+    if st != 0:
+        msg = L.shci_last_error(h).decode() if h else L.shci_last_error(None).decode()
+        if h:
+            L.shci_destroy(h)
+        raise ShciError(st, msg)
+    return VariationalResult(h)

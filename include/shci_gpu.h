@@ -1,0 +1,138 @@
+/*
+ * shci_gpu.h — C ABI of the MI355X-native SHCI variational step.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Determinants are
+ * passed as `words` 64-bit integers each (words = 2*norb/64 + 1, the
+ * reference's EffDetLen), bit k = spin-orbital k, even bits alpha, odd bits
+ * beta, word 0 first — the in-memory layout of the reference's
+ * `Determinant::repr` (SHCI/Determinants.h:166-200).  Every call is ordered
+ * on the handle's HIP stream and returns when its results are in the caller's
+ * buffers.  A handle is not re-entrant: one host thread per handle.
+ *
+ * Reference interfaces replaced (sanshar/Dice):
+ *   readIntegrals + twoIntHeatBath(SHM)::constructClass  SHCI/integral.cpp:43,228; SHCI/integral.h:125
+ *   Determinant::Energy / Hij                           SHCI/Determinants.cpp:252,564
+ *   getDeterminantsVariationalApprox + sort/unique      SHCI/SHCIgetdeterminants.cpp:586; SHCI/SHCIbasics.cpp:969-1077
+ *   PopulateHelpers + MakeHfromSMHelpers2               SHCI/SHCImakeHamiltonian.cpp:290,486
+ *   Hmult2::operator()(x, y)                            SHCI/Hmult.h:52
+ *   davidson(Hmult2&, x0, diag, maxCopies, tol, ...)    SHCI/Davidson.cpp:111
+ *   SHCIbasics::DoVariational                           SHCI/SHCIbasics.cpp:783
+ *   main (`Dice input.dat`, variational part)           SHCI/SHCI.cpp:157-446
+ */
+#ifndef SHCI_GPU_H
+#define SHCI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SHCI_OK = 0,
+  SHCI_BAD_INPUT = 1,      /* malformed file / argument (reference: message + exit) */
+  SHCI_OOM = 2,            /* device allocation failed */
+  SHCI_NO_CONVERGENCE = 3, /* Davidson iteration cap (reference: Davidson.cpp:305-309) */
+  SHCI_HIP_ERROR = 4,      /* HIP runtime error */
+  SHCI_UNSUPPORTED = 5     /* keyword / size outside this implementation */
+} shci_status;
+
+/* value precision of the stored Hamiltonian */
+typedef enum {
+  SHCI_PREC_FP64 = 0, /* FP64 values: the plain drop-in */
+  SHCI_PREC_FP32 = 1  /* FP32 values, FP64 accumulation, FP64 residual refinement */
+} shci_precision;
+
+typedef struct shci_handle shci_handle;
+
+/* Read an FCIDUMP and build the screening tables on the current device.
+ * Replaces readIntegrals (SHCI/integral.cpp:43-225) and
+ * twoIntHeatBath(SHM)::constructClass (SHCI/integral.h:125-153,
+ * SHCI/integral.cpp:228-389), as called from SHCI/SHCI.cpp:213,248-252. */
+shci_status shci_create(const char* fcidump_path, shci_handle** out);
+shci_status shci_destroy(shci_handle* h);
+/* last error message of the handle (or of the last failed shci_create when h is NULL) */
+const char* shci_last_error(const shci_handle* h);
+
+int shci_norb(const shci_handle* h);
+int shci_nelec(const shci_handle* h);
+int shci_det_words(const shci_handle* h);
+double shci_core_energy(const shci_handle* h);
+
+/* Hamiltonian value precision for subsequent shci_build_hamiltonian calls. */
+shci_status shci_set_precision(shci_handle* h, int precision);
+/* Restrict excitations as the ncore / nact keywords do (SHCI/SHCIgetdeterminants.cpp:649,694,711)
+ * and add RAS restrictions (`restrict min max orbs...`, SHCI/OccRestrictions.cpp). */
+shci_status shci_set_active_space(shci_handle* h, int ncore, int nact);
+shci_status shci_add_restriction(shci_handle* h, int min_elec, int max_elec, const int* spatial_orbs, int n);
+
+/* ---- determinant primitives ---------------------------------------------------------- */
+/* Determinant::Energy with coreE = 0 (SHCI/Determinants.cpp:252-300). */
+shci_status shci_det_energy(shci_handle* h, const int64_t* dets, int64_t n, double* energies);
+/* Hij(bra[k], ket[k]) (SHCI/Determinants.cpp:564-631); 0 beyond two excitations. */
+shci_status shci_hij(shci_handle* h, const int64_t* bra, const int64_t* ket, int64_t n, double* out);
+
+/* ---- variational space --------------------------------------------------------------- */
+/* Replace the handle's determinant list (order kept; duplicates are an error). */
+shci_status shci_set_dets(shci_handle* h, const int64_t* dets, int64_t n);
+shci_status shci_num_dets(const shci_handle* h, int64_t* n);
+/* Copy up to cap determinants out (in variational-space order). */
+shci_status shci_get_dets(shci_handle* h, int64_t* out, int64_t cap);
+
+/* Heat-bath screening of every held determinant with coefficient bound
+ * cmax[i] against eps1 (getDeterminantsVariationalApprox, SHCI/SHCIgetdeterminants.cpp:586-739),
+ * followed by sort + unique of the candidates not already held
+ * (SHCI/SHCIbasics.cpp:984-1055).  The new determinants, in the reference's
+ * signed-word order, are staged; *n_new receives their count. */
+shci_status shci_screen(shci_handle* h, const double* cmax, double eps1, int64_t* n_new);
+/* Copy the staged new determinants out (up to cap). */
+shci_status shci_get_new_dets(shci_handle* h, int64_t* out, int64_t cap);
+/* Append the staged determinants after the held ones (SHCI/SHCIbasics.cpp:1062-1077). */
+shci_status shci_append_new(shci_handle* h);
+
+/* ---- Hamiltonian ----------------------------------------------------------------------- */
+/* Build H over the held determinants: the diagonal (Determinant::Energy) and
+ * every pair one or two excitations apart, explicit zeros included
+ * (SHCImakeHamiltonian::PopulateHelpers + MakeHfromSMHelpers2,
+ * SHCI/SHCImakeHamiltonian.cpp:290-408,486-715).  *nnz_lower receives the
+ * number of stored elements of the reference's lower-triangular rows
+ * (diagonal + J < I). */
+shci_status shci_build_hamiltonian(shci_handle* h, int64_t* nnz_lower);
+/* The reference's row layout for rows [row_begin, n): row I = diagonal, then
+ * every J < I, columns ascending after the diagonal.  rowptr has n-row_begin+1
+ * entries relative to the first requested row. */
+shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t* rowptr, int32_t* cols,
+                                      double* vals, int64_t cap);
+
+/* ---- sigma and Davidson --------------------------------------------------------------- */
+/* y[:, r] = H x[:, r] for r < k; x, y host, column-major n x k
+ * (Hmult2::operator(), SHCI/Hmult.h:52-150). */
+shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k);
+/* Device-pointer variant on the handle's stream (stream = NULL) for callers
+ * that keep vectors resident; returns after enqueueing. */
+shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld);
+/* Davidson with the reference's semantics (SHCI/Davidson.cpp:111-348):
+ * x0 host n x nroots column-major guess in, eigenvectors out; energies out
+ * (coreE not added); *iterations = Davidson iteration count. */
+shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies, double tol, double* energies,
+                          int* iterations);
+
+/* ---- whole variational step ----------------------------------------------------------- */
+/* Run the variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446,
+ * SHCIbasics::DoVariational SHCI/SHCIbasics.cpp:783-1378): reads the input
+ * file and the FCIDUMP it names, writes <prefix>/shci.e, and prints the
+ * reference's tables to log_path (stdout when NULL or empty).  precision as
+ * shci_precision.  The result handle holds the final space for inspection. */
+shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);
+/* per-iteration records of the last shci_run on this handle */
+int shci_run_iterations(const shci_handle* h);
+shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, double* energy0, int* davidson_iters,
+                               double* seconds);
+shci_status shci_run_energies(const shci_handle* h, double* energies, int* nroots);
+/* final coefficients, n x nroots column-major */
+shci_status shci_run_ci(shci_handle* h, double* out, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHCI_GPU_H */

@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+    sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: CPU test taking tens of seconds")
+
+
+@pytest.fixture(scope="session")
+def golden_dir():
+    return os.path.join(TESTS, "golden")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle_bind
+    return oracle_bind.load()

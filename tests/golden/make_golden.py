@@ -1,0 +1,63 @@
+"""Copy the reference's own SHCI test data into tests/golden/ as fixtures.
+
+Run once in a container that has the reference mounted at /root/reference:
+
+    python tests/golden/make_golden.py
+
+It copies data files only (FCIDUMPs, the golden determinant list, trusted
+energies) and extracts the numbers of the golden variational log into
+golden.json.  Nothing under the reference is executed or imported.
+"""
+import json
+import os
+import re
+import shutil
+import struct
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+COPIES = {
+    "h8_631g/FCIDUMP": "examples/h8_631g/FCIDUMP",
+    "h8_631g/dets": "examples/h8_631g/dets",
+    "restart/FCIDUMP": "tests/SHCI/restart/FCIDUMP",
+    "n2_RAS/FCIDUMP": "tests/SHCI/n2_RAS/FCIDUMP",
+    "o2/FCIDUMP": "tests/SHCI/o2_det/FCIDUMP",
+    "h2o/FCIDUMP": "tests/SHCI/integrals/h2o_FCIDUMP_symm",
+    "h2h2/FCIDUMP": "tests/SHCI/integrals/H2H2_FCIDUMP_nosym",
+}
+ENERGIES = {
+    "restart": "tests/SHCI/restart/trusted_hci.e",
+    "n2_RAS": "tests/SHCI/n2_RAS/trusted_hci.e",
+    "o2_det_var_plus_pt": "tests/SHCI/o2_det/trusted_hci.e",
+}
+
+
+def main():
+    for dst, src in COPIES.items():
+        d = os.path.join(HERE, dst)
+        os.makedirs(os.path.dirname(d), exist_ok=True)
+        shutil.copyfile(os.path.join(REF, src), d)
+    gold = {"trusted_energy": {}}
+    for k, src in ENERGIES.items():
+        with open(os.path.join(REF, src), "rb") as f:
+            gold["trusted_energy"][k] = struct.unpack("<d", f.read(8))[0]
+    # golden variational table of examples/h8_631g/dice.out
+    rows = []
+    with open(os.path.join(REF, "examples/h8_631g/dice.out")) as f:
+        text = f.read()
+    for m in re.finditer(r"^\s+(\d+)\s+0\s+1\.00e-06\s+(\d+)\s+(-?\d+\.\d+)\s+(\d+)\s+[\d.]+$", text, re.M):
+        rows.append({"iter": int(m.group(1)), "ndets": int(m.group(2)), "energy": float(m.group(3)),
+                     "davidson": int(m.group(4))})
+    final = re.search(r"^\s+0\s+(-?\d+\.\d+)\s+[\d.]+$", text.split("VARIATIONAL CALCULATION RESULT")[1], re.M)
+    gold["h8_631g"] = {"iterations": rows, "final_energy": float(final.group(1)),
+                       "input": open(os.path.join(REF, "examples/h8_631g/dice.dat")).read()}
+    # inputs of the reference tests, kept as text data
+    gold["restart_input3"] = open(os.path.join(REF, "tests/SHCI/restart/input3.dat")).read()
+    gold["n2_RAS_input"] = open(os.path.join(REF, "tests/SHCI/n2_RAS/input.dat")).read()
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(gold, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

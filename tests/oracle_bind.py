@@ -1,0 +1,207 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so) for the tests.
+
+The oracle is the checker: tests compare the HIP path against it.  It is
+built from oracle/ with make when missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "build", "liboracle.so")
+EXE = os.path.join(ORACLE_DIR, "build", "shci_ref")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not (os.path.exists(LIB) and os.path.exists(EXE)):
+        build()
+    L = ctypes.CDLL(LIB)
+    P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    pI64, pD = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "oref_last_error": ([], ctypes.c_char_p),
+        "oref_create": ([ctypes.c_char_p], P),
+        "oref_destroy": ([P], None),
+        "oref_norb": ([P], I32),
+        "oref_nelec": ([P], I32),
+        "oref_det_words": ([P], I32),
+        "oref_core_energy": ([P], D),
+        "oref_max_entry": ([P], D),
+        "oref_i2_size": ([P], I64),
+        "oref_get_integrals": ([P, pD, pD, pD, pD], None),
+        "oref_hb_size": ([P, I32], I64),
+        "oref_hb_get": ([P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int16), pI64], None),
+        "oref_singles": ([P, pD], None),
+        "oref_set_active": ([P, I32, I32], None),
+        "oref_energy": ([P, pI64], D),
+        "oref_hij": ([P, pI64, pI64], D),
+        "oref_screen": ([P, pI64, I64, pD, D, pI64, I64, pI64, I64], I64),
+        "oref_csr_append": ([P, pI64, I64], I64),
+        "oref_csr_get": ([P, pI64, ctypes.POINTER(ctypes.c_int32), pD], None),
+        "oref_sigma": ([P, pD, pD], None),
+        "oref_davidson": ([P, pD, I32, pD, I32, D, pD], I32),
+        "oref_sym_eig": ([I32, pD, pD, pD], None),
+        "oref_run": ([ctypes.c_char_p, ctypes.c_char_p, I32], P),
+        "oref_run_status": ([P], I32),
+        "oref_run_error": ([P], ctypes.c_char_p),
+        "oref_run_niter": ([P], I32),
+        "oref_run_nroots": ([P], I32),
+        "oref_run_iter": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int)], None),
+        "oref_run_energies": ([P, pD], None),
+        "oref_run_ndets": ([P], I64),
+        "oref_run_iter_ndets": ([P, I32], I64),
+        "oref_run_dets": ([P, I32, pI64, I32], None),
+        "oref_run_ci": ([P, pD], None),
+        "oref_run_free": ([P], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class OracleSystem:
+    def __init__(self, path):
+        self.L = load()
+        self.h = self.L.oref_create(os.fsencode(path))
+        if not self.h:
+            raise RuntimeError(self.L.oref_last_error().decode())
+        self.words = self.L.oref_det_words(self.h)
+        self.norb = self.L.oref_norb(self.h)
+        self.nelec = self.L.oref_nelec(self.h)
+        self._n = 0
+
+    def __del__(self):
+        try:
+            self.L.oref_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_active(self, ncore, nact):
+        self.L.oref_set_active(self.h, ncore, nact)
+
+    def energy(self, dets):
+        d = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.words)
+        return np.array([self.L.oref_energy(self.h, _p(d[i], ctypes.c_int64)) for i in range(len(d))])
+
+    def hij(self, bra, ket):
+        b = np.ascontiguousarray(bra, dtype=np.int64).reshape(-1, self.words)
+        k = np.ascontiguousarray(ket, dtype=np.int64).reshape(-1, self.words)
+        return np.array([self.L.oref_hij(self.h, _p(b[i], ctypes.c_int64), _p(k[i], ctypes.c_int64))
+                         for i in range(len(b))])
+
+    def screen(self, dets, cmax, eps1, sorted_dets=None):
+        d = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.words)
+        if sorted_dets is None:
+            sorted_dets = sort_dets(d)
+        s = np.ascontiguousarray(sorted_dets, dtype=np.int64).reshape(-1, self.words)
+        c = np.ascontiguousarray(cmax, dtype=np.float64)
+        cap = 1 << 16
+        while True:
+            out = np.zeros((cap, self.words), dtype=np.int64)
+            m = self.L.oref_screen(self.h, _p(d, ctypes.c_int64), len(d), _p(c, ctypes.c_double), eps1,
+                                   _p(s, ctypes.c_int64), len(s), _p(out, ctypes.c_int64), cap)
+            if m <= cap:
+                return out[:m]
+            cap = m
+
+    def csr_append(self, dets):
+        d = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.words)
+        nnz = self.L.oref_csr_append(self.h, _p(d, ctypes.c_int64), len(d))
+        self._n = len(d)
+        self._nnz = nnz
+        return nnz
+
+    def csr(self):
+        rp = np.zeros(self._n + 1, dtype=np.int64)
+        cols = np.zeros(self._nnz, dtype=np.int32)
+        vals = np.zeros(self._nnz)
+        self.L.oref_csr_get(self.h, _p(rp, ctypes.c_int64), _p(cols, ctypes.c_int32), _p(vals, ctypes.c_double))
+        return rp, cols, vals
+
+    def sigma(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.L.oref_sigma(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+        return y
+
+    def davidson(self, x0, diag, max_copies, tol):
+        x = np.array(np.atleast_2d(np.asarray(x0, dtype=np.float64).T).T, order="F")
+        if x.ndim == 1:
+            x = x[:, None]
+        nroots = x.shape[1]
+        xx = np.ascontiguousarray(x.T)  # root-major = column-major n x nroots
+        d = np.ascontiguousarray(diag, dtype=np.float64)
+        e = np.zeros(nroots)
+        it = self.L.oref_davidson(self.h, _p(xx, ctypes.c_double), nroots, _p(d, ctypes.c_double), max_copies, tol,
+                                  _p(e, ctypes.c_double))
+        return e, xx.T, it
+
+
+class OracleRun:
+    def __init__(self, input_path, log_path=None, record=False):
+        self.L = load()
+        self.h = self.L.oref_run(os.fsencode(input_path), os.fsencode(log_path) if log_path else None,
+                                 1 if record else 0)
+        st = self.L.oref_run_status(self.h)
+        if st != 0:
+            msg = self.L.oref_run_error(self.h).decode()
+            raise RuntimeError("oracle run failed (%d): %s" % (st, msg))
+
+    def __del__(self):
+        try:
+            self.L.oref_run_free(self.h)
+        except Exception:
+            pass
+
+    def iterations(self):
+        out = []
+        nr = max(self.L.oref_run_nroots(self.h), 1)
+        for i in range(self.L.oref_run_niter(self.h)):
+            nd, e, it = ctypes.c_int64(), np.zeros(nr), ctypes.c_int()
+            self.L.oref_run_iter(self.h, i, ctypes.byref(nd), _p(e, ctypes.c_double), ctypes.byref(it))
+            out.append({"ndets": nd.value, "energy": float(e[0]), "energies": e.copy(), "davidson": it.value})
+        return out
+
+    def energies(self):
+        e = np.zeros(self.L.oref_run_nroots(self.h))
+        self.L.oref_run_energies(self.h, _p(e, ctypes.c_double))
+        return e
+
+    def dets(self, words, it=-1):
+        n = self.L.oref_run_ndets(self.h) if it < 0 else self.L.oref_run_iter_ndets(self.h, it)
+        out = np.zeros((n, words), dtype=np.int64)
+        self.L.oref_run_dets(self.h, it, _p(out, ctypes.c_int64), words)
+        return out
+
+    def ci(self):
+        n = self.L.oref_run_ndets(self.h)
+        nr = self.L.oref_run_nroots(self.h)
+        out = np.zeros(n * nr)
+        self.L.oref_run_ci(self.h, _p(out, ctypes.c_double))
+        return out.reshape(nr, n).T
+
+
+def sort_dets(d):
+    """Sort determinants in the reference's signed word order (highest word first)."""
+    d = np.asarray(d, dtype=np.int64)
+    keys = [d[:, w] for w in range(d.shape[1])]  # lexsort: last key primary
+    return d[np.lexsort(keys)]

@@ -1,0 +1,73 @@
+"""The C-ABI library on a machine without a GPU: it loads, exports every
+entry point include/shci_gpu.h declares, and its host-side input handling
+fails the way the reference does (SHCI/input.cpp:373-382) before any device
+call."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from dice_amd import shci
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "shci_gpu.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(shci_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = shci.library()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def _run_expect(tmp_path, text, status, needle):
+    p = tmp_path / "input.dat"
+    p.write_text(text)
+    with pytest.raises(shci.ShciError) as e:
+        shci.do_variational(str(p), str(tmp_path / "log"))
+    assert e.value.status == status
+    assert needle in str(e.value)
+
+
+def test_unknown_keyword_fails_like_reference(tmp_path):
+    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\nbogusword 3\n", 1,
+                "cannot read option bogusword")
+
+
+def test_maxiter_below_schedule(tmp_path):
+    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\n5 1e-4\nend\nmaxiter 2\n", 1,
+                "maxiter should be greater than last entry of sweep_iter")
+
+
+def test_missing_nocc(tmp_path):
+    _run_expect(tmp_path, "schedule\n0 1e-3\nend\nmaxiter 2\n", 1, "nocc keyword has to be included")
+
+
+def test_nocc_count_mismatch(tmp_path):
+    _run_expect(tmp_path, "nocc 3\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\n", 1, "nocc: 3 neq 2")
+
+
+def test_outside_path_keywords_refused(tmp_path):
+    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\nTreversal 1\n", 5,
+                "outside the variational path")
+
+
+def test_synthetic_fcidump_format(tmp_path):
+    """The generator writes the reference writer's layout (lib/shcitools.c:165-234)."""
+    from dice_amd import synth
+    p = tmp_path / "FCIDUMP"
+    synth.write_molecular_fcidump(str(p), 0, 6, 3, 3, 8)
+    lines = p.read_text().splitlines()
+    assert lines[0].startswith(" &FCI NORB=6 ,NELEC=6 ,MS2=0,")
+    assert lines[3].strip() == "&END"
+    first = lines[4]
+    assert len(first.split()[0]) == 18 and first[:20].strip() == first.split()[0]
+    assert lines[-1].split()[1:] == ["0", "0", "0", "0"]

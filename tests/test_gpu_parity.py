@@ -1,0 +1,247 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: bit-exact for determinants, CSR structure and Hamiltonian values;
+1e-12 relative for sigma (summation order differs); 1e-10 Ha for energies.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+from oracle_bind import OracleRun, OracleSystem, sort_dets
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def systems(tmp_path_factory):
+    from dice_amd import synth
+    d = tmp_path_factory.mktemp("fcidumps")
+    out = {"h8": (os.path.join(GOLD, "h8_631g", "FCIDUMP"), 16, 4, 4)}
+    p = str(d / "c0")
+    synth.write_molecular_fcidump(p, 0, 16, 8, 8, 64)
+    out["c0"] = (p, 16, 8, 8)
+    p = str(d / "w2")  # 34 orbitals: two 64-bit words
+    synth.write_molecular_fcidump(p, 5, 34, 6, 6, 12)
+    out["w2"] = (p, 34, 6, 6)
+    p = str(d / "hub")
+    synth.write_hubbard_fcidump(p, 6, 6, 1.0, 4.0, 18, 18)
+    out["hub"] = (p, 36, 18, 18)
+    return out
+
+
+def _gpu(path):
+    from dice_amd import shci
+    return shci.System.from_fcidump(path)
+
+
+def _sample(name, systems, n, seed):
+    path, norb, na, nb = systems[name]
+    rng = np.random.default_rng(seed)
+    W = helpers.words_for(norb)
+    base = helpers.random_dets(norb, na, nb, n, rng, W)
+    # add near neighbours so singles / doubles are well represented
+    near = []
+    for d in base[: n // 2]:
+        occ = helpers.det_to_occ(d)
+        near.append(helpers.occ_to_det(helpers.excite(occ, norb, rng, 1 + (len(near) % 2)), W))
+    return base, np.array(near)
+
+
+@pytest.mark.parametrize("name", ["h8", "c0", "w2", "hub"])
+def test_energy_and_hij_bitwise(name, systems):
+    path = systems[name][0]
+    g, o = _gpu(path), OracleSystem(path)
+    base, near = _sample(name, systems, 400, 11)
+    np.testing.assert_array_equal(g.energy(base), o.energy(base))
+    np.testing.assert_array_equal(g.hij(base[: len(near)], near), o.hij(base[: len(near)], near))
+    np.testing.assert_array_equal(g.hij(near, base[: len(near)]), o.hij(near, base[: len(near)]))
+
+
+@pytest.mark.parametrize("name,eps", [("h8", 1e-4), ("c0", 1e-3), ("w2", 1e-3), ("hub", 2e-4)])
+def test_screening_det_set(name, eps, systems):
+    path = systems[name][0]
+    g, o = _gpu(path), OracleSystem(path)
+    base, near = _sample(name, systems, 300, 5)
+    dets = np.concatenate([base, near])
+    dets = dets[np.unique(dets, axis=0, return_index=True)[1]]
+    rng = np.random.default_rng(3)
+    cmax = np.abs(rng.normal(0, 0.2, len(dets))) + 1e-3
+    g.set_dets(dets)
+    got = g.get_determinants_variational(cmax, eps)
+    ref = o.screen(dets, cmax, eps)
+    assert len(got) == len(ref) and len(ref) > 0
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(got, sort_dets(got))
+
+
+def _space(name, systems, n, seed):
+    """A connected determinant space: repeated screening from the reference."""
+    path, norb, na, nb = systems[name]
+    o = OracleSystem(path)
+    W = o.words
+    if name == "hub":
+        from dice_amd import synth
+        ref = synth.neel_occupation(6, 6)
+    else:
+        from dice_amd import synth
+        ref = synth.hf_occupation(na, nb)
+    dets = helpers.occ_to_det(ref, W)[None, :]
+    eps = 1e-2
+    while len(dets) < n:
+        new = o.screen(dets, np.ones(len(dets)), eps)
+        if len(new):
+            dets = np.concatenate([dets, new])
+        eps *= 0.5
+    return dets[:n]
+
+
+@pytest.mark.parametrize("name,n", [("h8", 1500), ("c0", 3000), ("w2", 2500), ("hub", 2500)])
+def test_hamiltonian_rows_and_sigma(name, n, systems):
+    path = systems[name][0]
+    dets = _space(name, systems, n, 1)
+    g, o = _gpu(path), OracleSystem(path)
+    g.set_dets(dets)
+    rp, cols, vals = g.hamiltonian_rows()
+    o.csr_append(dets)
+    orp, ocols, ovals = o.csr()
+    np.testing.assert_array_equal(np.diff(rp), np.diff(orp))
+    for i in range(len(dets)):
+        a, b = rp[i], rp[i + 1]
+        # diagonal first, then J < I; compare after sorting the oracle's row
+        assert cols[a] == ocols[orp[i]] == i
+        assert vals[a] == ovals[orp[i]]
+        order = np.argsort(ocols[orp[i] + 1:orp[i + 1]], kind="stable")
+        np.testing.assert_array_equal(cols[a + 1:b], ocols[orp[i] + 1:orp[i + 1]][order])
+        np.testing.assert_array_equal(vals[a + 1:b], ovals[orp[i] + 1:orp[i + 1]][order])
+    x = np.random.default_rng(2).normal(size=len(dets))
+    y = g.hmult(x)
+    yo = o.sigma(x)
+    assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("name,nroots", [("c0", 1), ("hub", 2)])
+def test_davidson_energies(name, nroots, systems):
+    path = systems[name][0]
+    dets = _space(name, systems, 2000, 1)
+    g, o = _gpu(path), OracleSystem(path)
+    g.set_dets(dets)
+    g.make_hamiltonian()
+    o.csr_append(dets)
+    x0 = np.zeros((len(dets), nroots))
+    x0[0, 0] = 1.0
+    if nroots > 1:
+        x0[1:, 1] = np.random.default_rng(4).normal(size=len(dets) - 1)
+    e, x, it = g.davidson(x0.copy(), tol=1e-8)
+    diag = o.energy(dets)
+    eo, xo, ito = o.davidson(x0.copy(), diag, nroots + 4, 1e-8)
+    np.testing.assert_allclose(e, eo, rtol=0, atol=1e-10)
+    assert abs(it - ito) <= 2
+
+
+def _write_input(tmp_path, text):
+    p = tmp_path / "input.dat"
+    p.write_text(text)
+    return str(p)
+
+
+def _compare_runs(inp, tmp_path, words, nroots=1):
+    from dice_amd import shci
+    o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
+    oi, gi = o.iterations(), r.iterations()
+    assert [x["ndets"] for x in oi] == [x["ndets"] for x in gi]
+    for a, b in zip(oi, gi):
+        assert abs(a["energy"] - b["energy"]) < 1e-10
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(r.dets(), o.dets(words))
+    return o, r
+
+
+def test_run_h8_golden(tmp_path):
+    import json
+    with open(os.path.join(GOLD, "golden.json")) as f:
+        g = json.load(f)
+    inp = _write_input(tmp_path, g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\n" % (
+        os.path.join(GOLD, "h8_631g", "FCIDUMP"), tmp_path))
+    o, r = _compare_runs(inp, tmp_path, 1)
+    its = r.iterations()
+    for mine, ref in zip(its, g["h8_631g"]["iterations"]):
+        assert mine["ndets"] == ref["ndets"]
+        assert abs(mine["energy"] - ref["energy"]) < 6e-11
+        assert mine["davidson"] == ref["davidson"]
+    assert abs(r.energies()[0] - g["h8_631g"]["final_energy"]) < 6e-11
+
+
+def test_run_n2_ras(tmp_path):
+    import json
+    with open(os.path.join(GOLD, "golden.json")) as f:
+        g = json.load(f)
+    text = g["n2_RAS_input"].replace("orbitals ./FCIDUMP", "orbitals %s" % os.path.join(GOLD, "n2_RAS", "FCIDUMP"))
+    inp = _write_input(tmp_path, text + "\nprefix %s\n" % tmp_path)
+    o, r = _compare_runs(inp, tmp_path, 1)
+    assert abs(r.energies()[0] - g["trusted_energy"]["n2_RAS"]) < 1e-9
+
+
+def test_run_restart_trusted(tmp_path):
+    """524k determinants; the reference's trusted energy to 1e-9 (no oracle run)."""
+    import json
+    from dice_amd import shci
+    with open(os.path.join(GOLD, "golden.json")) as f:
+        g = json.load(f)
+    text = "\n".join(l for l in g["restart_input3"].splitlines() if l.strip() != "restart")
+    text += "\norbitals %s\nprefix %s\n" % (os.path.join(GOLD, "restart", "FCIDUMP"), tmp_path)
+    inp = _write_input(tmp_path, text)
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
+    assert [x["ndets"] for x in r.iterations()][-1] == 524307
+    assert abs(r.energies()[0] - g["trusted_energy"]["restart"]) < 1e-9
+
+
+def test_run_c0_first_iteration(tmp_path, systems):
+    """The 16-orbital workload's first iteration (5,695 determinants; the next
+    one selects 650k and takes the CPU oracle about ten minutes)."""
+    from dice_amd import synth
+    path = systems["c0"][0]
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(8, 8), [(0, 1e-3)], 1)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    o, r = _compare_runs(inp, tmp_path, 1)
+    assert r.iterations()[-1]["ndets"] == 5695
+
+
+def test_run_small_molecular_per_iteration_det_sets(tmp_path):
+    """A 12-orbital system of the same construction through a full schedule:
+    the determinant set after every iteration is identical to the oracle's."""
+    from dice_amd import synth, shci
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 9, 12, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 3e-3), (2, 1e-3)], 5)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
+    oi, gi = o.iterations(), r.iterations()
+    assert [x["ndets"] for x in oi] == [x["ndets"] for x in gi]
+    for a, b in zip(oi, gi):
+        assert abs(a["energy"] - b["energy"]) < 1e-10
+    np.testing.assert_array_equal(r.dets(), o.dets(1))
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-10)
+
+
+def test_run_hubbard_two_roots(tmp_path):
+    from dice_amd import synth
+    p = str(tmp_path / "FCIDUMP")
+    synth.write_hubbard_fcidump(p, 4, 4, 1.0, 4.0, 8, 8)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, p, synth.neel_occupation(4, 4), [(0, 1e-3)], 5, nroots=2)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    from dice_amd import shci
+    o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-9)
