@@ -82,8 +82,11 @@ double parityDiceToAlphaBeta(const int64_t* d, int nso) {  // SHCI/Determinants.
 
 // The variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446 and
 // SHCIbasics::DoVariational, SHCI/SHCIbasics.cpp:783-1378) on the device.
-shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, int precision) {
+shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, const shci_run_options& opt) {
   Engine& E = h->eng;
+  const int precision = opt.precision;
+  E.profileSigma = opt.profile_sigma != 0;
+  E.resetSigmaProfile();
   const double startofCalc = wallTime();
   HostSchedule schd;
   std::string err, echo;
@@ -189,6 +192,8 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, int
     E.cmax.reserve((size_t)std::max<int64_t>(nold, 1) * 8);
     launchCmax(E.ci.as<double>(), nroots, nold, nold, E.cmax.as<double>(), E.stream);
     const int64_t nnew = E.screen(E.cmax.as<double>(), eps1);
+    if (std::getenv("SHCI_TIMING"))
+      fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);
     E.appendNew();
     const int64_t N = E.n;
     // X0 = [ci; 0]
@@ -233,7 +238,9 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, int
     HIP_CHECK(hipMemcpyAsync(E.ci.p, X0.p, (size_t)N * nroots * 8, hipMemcpyDeviceToDevice, E.stream));
     if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
       if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
+      const double tf0 = wallTime();
       E0 = E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
+      E.tFinalDavidson = wallTime() - tf0;
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");
       break;
@@ -447,10 +454,11 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
     if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
     const int64_t n = E.n;
     if (row_begin < 0 || row_begin > n) throw std::invalid_argument("row_begin out of range");
-    std::vector<int64_t> rp(n + 1);
+    const int64_t nseg = (int64_t)SHCI_COL_BLOCKS * n;
+    std::vector<int64_t> rp(nseg + 1);
     std::vector<int32_t> c(E.nnzOff);
     std::vector<double> v(E.nnzOff), dg(n);
-    HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (nseg + 1) * 8, hipMemcpyDeviceToHost));
     if (E.nnzOff) {
       HIP_CHECK(hipMemcpy(c.data(), E.cols.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
       HIP_CHECK(hipMemcpy(v.data(), E.vals.p, E.nnzOff * 8, hipMemcpyDeviceToHost));
@@ -461,8 +469,9 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
     std::vector<std::pair<int32_t, double>> row;
     for (int64_t I = row_begin; I < n; I++) {
       row.clear();
-      for (int64_t p = rp[I]; p < rp[I + 1]; p++)
-        if (c[p] < I) row.emplace_back(c[p], v[p]);
+      for (int b = 0; b < SHCI_COL_BLOCKS; b++)
+        for (int64_t p = rp[b * n + I]; p < rp[b * n + I + 1]; p++)
+          if (c[p] < I) row.emplace_back(c[p], v[p]);
       std::sort(row.begin(), row.end(), [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
         return a.first < b.first;
       });
@@ -529,7 +538,16 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
 }
 
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out) {
-  if (!out || !input_path) return fail(nullptr, SHCI_BAD_INPUT, "null argument");
+  shci_run_options opt;
+  std::memset(&opt, 0, sizeof(opt));
+  opt.precision = precision;
+  return shci_run_ex(input_path, log_path, &opt, out);
+}
+
+shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,
+                        shci_handle** out) {
+  if (!out || !input_path || !options) return fail(nullptr, SHCI_BAD_INPUT, "null argument");
+  const shci_run_options opt = *options;
   *out = nullptr;
   shci_handle* h = nullptr;
   try {
@@ -544,13 +562,38 @@ shci_status shci_run(const char* input_path, const char* log_path, int precision
     own = log != nullptr;
     if (!log) log = stdout;
   }
-  shci_status st = guarded(h, [&] { return runVariational(h, input_path, log, precision); });
+  shci_status st = guarded(h, [&] { return runVariational(h, input_path, log, opt); });
   if (own) fclose(log);
   *out = h;
   return st;
 }
 
 int shci_run_iterations(const shci_handle* h) { return (int)h->eng.iters.size(); }
+
+shci_status shci_run_iteration_phases(const shci_handle* h, int it, double* screen_s, double* hamiltonian_s,
+                                      double* davidson_s, int64_t* nnz_lower) {
+  if (it < 0 || it >= (int)h->eng.iters.size()) return SHCI_BAD_INPUT;
+  const IterRecord& r = h->eng.iters[it];
+  if (screen_s) *screen_s = r.tScreen;
+  if (hamiltonian_s) *hamiltonian_s = r.tHam;
+  if (davidson_s) *davidson_s = r.tDavidson;
+  if (nnz_lower) *nnz_lower = r.nnzLower;
+  return SHCI_OK;
+}
+
+double shci_run_final_davidson_seconds(const shci_handle* h) { return h->eng.tFinalDavidson; }
+
+shci_status shci_profile_sigma_read(shci_handle* h, int64_t* calls, double* milliseconds, double* bytes) {
+  return guarded(h, [&] {
+    int64_t c = 0;
+    double ms = 0, b = 0;
+    if (h->eng.stream) h->eng.readSigmaProfile(c, ms, b);
+    if (calls) *calls = c;
+    if (milliseconds) *milliseconds = ms;
+    if (bytes) *bytes = b;
+    return SHCI_OK;
+  });
+}
 
 shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, double* energy0, int* davidson_iters,
                                double* seconds) {

@@ -19,6 +19,10 @@ void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
 Engine::Engine() {}
 
 Engine::~Engine() {
+  for (SigmaEvent& e : sigmaEvents) {
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.stop);
+  }
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -99,29 +103,52 @@ void Engine::setDets(const int64_t* h, int64_t cnt) {
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+// Screening of every held determinant, in chunks of source determinants so
+// the candidate buffer stays within a fixed budget: each chunk's candidates
+// are sorted and made unique, then merged into the running unique set.
 int64_t Engine::screen(const double* dCmax, double eps1) {
   const int W = hs.W;
-  if (candCap == 0) candCap = std::max<int64_t>(1 << 16, 8 * n);
-  while (true) {
+  const int64_t budget = (int64_t)1 << 27;  // candidates held at once
+  if (candCap == 0) candCap = std::min<int64_t>(budget, std::max<int64_t>(1 << 16, 8 * n));
+  nNew = 0;
+  int64_t begin = 0, chunk = n;
+  while (begin < n) {
+    const int64_t end = std::min(n, begin + chunk);
     cand.reserve((size_t)candCap * W * 8);
     HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
-    launchScreen(S, dets.as<uint64_t>(), n, dCmax, eps1, ncore, nact, table.as<int32_t>(), slots,
+    launchScreen(S, dets.as<uint64_t>(), begin, end, dCmax, eps1, ncore, nact, table.as<int32_t>(), slots,
                  cand.as<uint64_t>(), counter.as<unsigned long long>(), candCap, stream);
     unsigned long long found = 0;
     HIP_CHECK(hipMemcpyAsync(&found, counter.p, 8, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
-    if ((int64_t)found <= candCap) {
-      int64_t m = (int64_t)found;
-      radixSortKeys(cand.as<uint64_t>(), nullptr, m, W, hs.nso, true, ws, stream);
-      m = uniqueKeys(cand.as<uint64_t>(), m, W, ws, stream);
+    if ((int64_t)found > candCap) {
+      if (candCap < budget) {  // grow the buffer first
+        candCap = std::min<int64_t>(budget, (int64_t)found + (int64_t)found / 4 + 1024);
+        cand.release();
+      } else {  // then shrink the chunk
+        if (chunk == 1) throw std::invalid_argument("one determinant yields more candidates than the buffer");
+        chunk = std::max<int64_t>(1, (int64_t)((double)(end - begin) * candCap / (double)found * 0.8));
+      }
+      continue;
+    }
+    int64_t m = (int64_t)found;
+    radixSortKeys(cand.as<uint64_t>(), nullptr, m, W, hs.nso, true, ws, stream);
+    m = uniqueKeys(cand.as<uint64_t>(), m, W, ws, stream);
+    if (nNew == 0) {
       newDets.reserve((size_t)std::max<int64_t>(m, 1) * W * 8);
       if (m) HIP_CHECK(hipMemcpyAsync(newDets.p, cand.p, m * W * 8, hipMemcpyDeviceToDevice, stream));
       nNew = m;
-      return m;
+    } else if (m) {
+      newDets.reserve((size_t)(nNew + m) * W * 8, stream, true);
+      HIP_CHECK(hipMemcpyAsync(newDets.as<uint64_t>() + nNew * W, cand.p, m * W * 8, hipMemcpyDeviceToDevice,
+                               stream));
+      radixSortKeys(newDets.as<uint64_t>(), nullptr, nNew + m, W, hs.nso, true, ws, stream);
+      nNew = uniqueKeys(newDets.as<uint64_t>(), nNew + m, W, ws, stream);
     }
-    candCap = (int64_t)found + found / 4 + 1024;
-    cand.release();
+    begin = end;
   }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  return nNew;
 }
 
 void Engine::appendNew() {
@@ -144,13 +171,23 @@ void Engine::buildHamiltonian() {
   nDiag = n;
   if (hamRows == n) return;
   buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
-  rowPtr.reserve((size_t)(n + 1) * 8);
+  const int64_t nseg = (int64_t)SHCI_COL_BLOCKS * n;
+  rowPtr.reserve((size_t)(nseg + 1) * 8);
   int64_t* rp = rowPtr.as<int64_t>();
-  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+  HIP_CHECK(hipMemsetAsync(rp + nseg, 0, 8, stream));
   launchCsrCount(S, dets.as<uint64_t>(), T, rp, stream);
-  exclusiveScanI64(rp, n + 1, ws, stream);
-  HIP_CHECK(hipMemcpyAsync(&nnzOff, rp + n, 8, hipMemcpyDeviceToHost, stream));
+  exclusiveScanI64(rp, nseg + 1, ws, stream);
+  HIP_CHECK(hipMemcpyAsync(&nnzOff, rp + nseg, 8, hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
+  {
+    size_t freeB = 0, totalB = 0;
+    HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    size_t need = (size_t)nnzOff * (4 + 8 + (precision == 1 ? 4 : 0));
+    if (need > freeB + cols.bytes + vals.bytes + vals32.bytes)
+      throw HipError("out of memory: the Hamiltonian of " + std::to_string(n) + " determinants has " +
+                     std::to_string(nnzOff) + " off-diagonal elements (" + std::to_string(need >> 30) +
+                     " GiB), " + std::to_string(freeB >> 30) + " GiB free");
+  }
   cols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
   vals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
   launchCsrFill(S, dets.as<uint64_t>(), T, rp, cols.as<int32_t>(), vals.as<double>(), stream);
@@ -163,12 +200,41 @@ void Engine::buildHamiltonian() {
 }
 
 void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) {
-  if (precision == 1 && !exact)
+  const bool f32 = precision == 1 && !exact;
+  SigmaEvent* ev = nullptr;
+  if (profileSigma) {
+    if (sigmaEventsUsed == sigmaEvents.size()) {
+      SigmaEvent e{};
+      HIP_CHECK(hipEventCreate(&e.start));
+      HIP_CHECK(hipEventCreate(&e.stop));
+      sigmaEvents.push_back(e);
+    }
+    ev = &sigmaEvents[sigmaEventsUsed++];
+    // algorithmic bytes: stored values + column indices, row offsets,
+    // diagonal, x read once and y written once per vector
+    ev->bytes = (double)nnzOff * ((f32 ? 4 : 8) + 4) + (double)(n + 1) * 8 + (double)n * 8 + 2.0 * k * n * 8;
+    HIP_CHECK(hipEventRecord(ev->start, stream));
+  }
+  if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), diag.as<double>(), n, dx, dy, k, ld,
-                   stream);
+                   sigmaPart, stream);
   else
     launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k, ld,
-                   stream);
+                   sigmaPart, stream);
+  if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+}
+
+void Engine::readSigmaProfile(int64_t& calls, double& ms, double& bytes) {
+  HIP_CHECK(hipStreamSynchronize(stream));
+  calls = (int64_t)sigmaEventsUsed;
+  ms = 0;
+  bytes = 0;
+  for (size_t i = 0; i < sigmaEventsUsed; i++) {
+    float t = 0;
+    HIP_CHECK(hipEventElapsedTime(&t, sigmaEvents[i].start, sigmaEvents[i].stop));
+    ms += t;
+    bytes += sigmaEvents[i].bytes;
+  }
 }
 
 // Davidson with the semantics of SHCI/Davidson.cpp:111-348 (see the oracle

@@ -49,7 +49,8 @@ struct Engine {
   StringTables T;
   DevMem diag;
   int64_t nDiag = 0;
-  DevMem rowPtr, cols, vals, vals32, valsLo;
+  DevMem rowPtr, cols, vals, vals32, valsLo;  // column-blocked CSR (see launchCsrCount)
+  DevMem sigmaPart;                           // per-column-block partial sums of sigma
   int64_t nnzOff = 0, hamRows = -1;
 
   // Davidson scratch
@@ -61,6 +62,19 @@ struct Engine {
   DevMem ci;  // n x nroots, column-major, ld = n
   int nroots = 1;
   std::string err;
+  double tFinalDavidson = 0;  // wall time of the last run's final tight Davidson (s)
+
+  // optional timing of every sigma launch with HIP events on `stream`
+  struct SigmaEvent {
+    hipEvent_t start, stop;
+    double bytes;
+  };
+  bool profileSigma = false;
+  std::vector<SigmaEvent> sigmaEvents;
+  size_t sigmaEventsUsed = 0;
+  void resetSigmaProfile() { sigmaEventsUsed = 0; }
+  // sum over recorded launches: calls, kernel milliseconds, algorithmic bytes
+  void readSigmaProfile(int64_t& calls, double& ms, double& bytes);
 
   explicit Engine();
   ~Engine();

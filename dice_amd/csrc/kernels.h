@@ -25,8 +25,10 @@ void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cma
 // Emits candidate determinants not present in the table into `out`
 // (capacity `cap` determinants); *counter receives the total number found
 // (which may exceed cap: the caller then grows `out` and runs again).
-void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* cmax, double eps1,
-                  int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
+// Screens the source determinants [begin, end) of `dets` (the whole
+// variational space is probed for membership).
+void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                  double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
                   unsigned long long* counter, int64_t cap, hipStream_t s);
 
 // --- string tables and the Hamiltonian -----------------------------------------------
@@ -44,9 +46,12 @@ struct StringTables {
 void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, StringTables& T, Workspace& ws,
                        hipStream_t s);
 
-// Count (rowCount != null) or fill (cols/vals) the off-diagonal connections
-// of every row: all J != I one or two excitations apart.  Values are
-// Hij(bra = D_min(I,J), ket = D_max(I,J)).
+// Count (rowCount) or fill (cols/vals) the off-diagonal connections of every
+// row: all J != I one or two excitations apart.  Values are
+// Hij(bra = D_min(I,J), ket = D_max(I,J)).  Storage is column-blocked: the
+// entries of row I with J in block c = J / ceil(n / SHCI_COL_BLOCKS) occupy
+// [rowPtr[c*n + I], rowPtr[c*n + I + 1]); rowCount / rowPtr hold
+// SHCI_COL_BLOCKS * n (+1) entries.
 void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t* rowCount,
                     hipStream_t s);
 void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, const int64_t* rowPtr,
@@ -56,11 +61,12 @@ void launchCastValues(const double* v, float* out, int64_t n, hipStream_t s);
 void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals, hipStream_t s);
 
 // --- sigma = H x ------------------------------------------------------------------------
-// y[:,r] = diag .* x[:,r] + offdiag(H) x[:,r] for r < k (column-major, ld)
+// y[:,r] = diag .* x[:,r] + offdiag(H) x[:,r] for r < k (column-major, ld),
+// column-blocked storage as above; `part` holds the per-block partial sums
 void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag,
-                    int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
+                    int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s);
 void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, hipStream_t s);
+                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -133,9 +133,9 @@ __device__ __forceinline__ void emitCandidate(bool want, const uint64_t* cand, c
 }
 
 template <int W>
-__global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* dets, int64_t ndets,
-                                                  const double* cmax, double eps1, int ncore, int nact,
-                                                  const int32_t* table, uint64_t mask, uint64_t* out,
+__global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* dets, int64_t srcBegin,
+                                                  int64_t ndets, const double* cmax, double eps1, int ncore,
+                                                  int nact, const int32_t* table, uint64_t mask, uint64_t* out,
                                                   unsigned long long* counter, int64_t cap) {
   __shared__ uint8_t sClosed[kWaves][128];
   __shared__ uint8_t sOpen[kWaves][128];
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
   __shared__ int sPair[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   const int nso = S.nso;
-  for (int64_t di = (int64_t)blockIdx.x * kWaves + wave; di < ndets; di += (int64_t)gridDim.x * kWaves) {
+  for (int64_t di = srcBegin + (int64_t)blockIdx.x * kWaves + wave; di < ndets; di += (int64_t)gridDim.x * kWaves) {
     uint64_t d[W];
 #pragma unroll
     for (int w = 0; w < W; w++) d[w] = dets[di * W + w];
@@ -399,24 +399,35 @@ struct RowTables {
   int64_t n;
 };
 
+// Collects the connections of row I, split by column block c = J / blockCols
+// (SHCI_COL_BLOCKS blocks, one per XCD).  Count pass: per-lane counters per
+// block; fill pass: wave-uniform cursors per block, positions by ballot.
 template <int W, bool FILL>
 struct RowSink {
   const DevSystem* S;
   const uint64_t* dets;
   const uint64_t* dI;
   int64_t I;
+  uint32_t blockCols;
   int32_t* cols;
   double* vals;
-  int64_t pos;  // wave-uniform write cursor
-  int64_t cnt;  // per-lane count (count pass)
+  int64_t pos[SHCI_COL_BLOCKS];  // wave-uniform write cursors (fill)
+  int cnt[SHCI_COL_BLOCKS];      // per-lane counts (count)
   __device__ __forceinline__ void hit(bool h, uint32_t J) {
+    const int c = h ? (int)(J / blockCols) : -1;
     if (!FILL) {
-      cnt += h ? 1 : 0;
+#pragma unroll
+      for (int q = 0; q < SHCI_COL_BLOCKS; q++) cnt[q] += (c == q) ? 1 : 0;
       return;
     }
-    uint64_t m = __ballot(h);
+    int64_t p = 0;
+#pragma unroll
+    for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
+      uint64_t m = __ballot(c == q);
+      if (c == q) p = pos[q] + __popcll(m & lanemaskLt());
+      pos[q] += __popcll(m);
+    }
     if (h) {
-      int64_t p = pos + __popcll(m & lanemaskLt());
       uint64_t dJ[W];
 #pragma unroll
       for (int w = 0; w < W; w++) dJ[w] = dets[(int64_t)J * W + w];
@@ -424,13 +435,14 @@ struct RowSink {
       cols[p] = (int32_t)J;
       vals[p] = v;
     }
-    pos += __popcll(m);
   }
 };
 
+// rowPtr / rowCount are indexed [c * n + I] for column block c
 template <int W, bool FILL>
 __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t* rowCount,
-                                                   const int64_t* rowPtr, int32_t* cols, double* vals) {
+                                                   const int64_t* rowPtr, int32_t* cols, double* vals,
+                                                   uint32_t blockCols) {
   __shared__ int sExcl[kWaves][65];
   __shared__ int64_t sBeg[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
@@ -440,7 +452,19 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
     const uint64_t astr = T.alpha[I], bstr = T.beta[I];
     const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
-    RowSink<W, FILL> sink{&S, dets, dI, I, cols, vals, FILL ? rowPtr[I] : 0, 0};
+    RowSink<W, FILL> sink;
+    sink.S = &S;
+    sink.dets = dets;
+    sink.dI = dI;
+    sink.I = I;
+    sink.blockCols = blockCols;
+    sink.cols = cols;
+    sink.vals = vals;
+#pragma unroll
+    for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
+      sink.pos[q] = FILL ? rowPtr[(int64_t)q * T.n + I] : 0;
+      sink.cnt[q] = 0;
+    }
     // (a) same beta string: alpha singles and alpha-alpha doubles
     {
       int64_t b0 = T.bStart[bi], b1 = T.bStart[bi + 1];
@@ -517,10 +541,13 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
       }
     }
     if (!FILL) {
-      int64_t c = sink.cnt;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-      if (lane == 0) rowCount[I] = c;
+      for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
+        int c = sink.cnt[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) rowCount[(int64_t)q * T.n + I] = c;
+      }
     }
   }
 }
@@ -553,37 +580,64 @@ __global__ void kSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, doubl
 }
 
 // ---------------------------------------------------------------------------
-// sigma: one wavefront per row, lanes stride the row; FP64 accumulation
-
+// sigma, column-blocked: block c of the columns (determinant indices
+// [c*blockCols, (c+1)*blockCols)) is handled only by workgroups with
+// blockIdx.x % SHCI_COL_BLOCKS == c.  Workgroups are dispatched round-robin
+// over the 8 XCDs, so each XCD gathers from one slice of x that stays
+// resident in its own L2 while the matrix streams through.  One wavefront per
+// (row, block) segment, FP64 accumulation; partial sums per block are
+// combined in fixed order by kSigmaCombine.
 template <typename VT>
-__global__ void __launch_bounds__(kBlock) kSigma(const int64_t* __restrict__ rowPtr, const int32_t* __restrict__ cols,
-                                                 const VT* __restrict__ vals, const double* __restrict__ diag,
-                                                 int64_t n, const double* __restrict__ x, double* __restrict__ y,
-                                                 int k, int64_t ld) {
+__global__ void __launch_bounds__(kBlock) kSigmaBlocks(const int64_t* __restrict__ rowPtr,
+                                                       const int32_t* __restrict__ cols,
+                                                       const VT* __restrict__ vals, int64_t n,
+                                                       const double* __restrict__ x, double* __restrict__ part,
+                                                       int k, int64_t ld) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
-  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
-    const int64_t b = rowPtr[I], e = rowPtr[I + 1];
+  const int c = blockIdx.x % SHCI_COL_BLOCKS;
+  const int64_t g = blockIdx.x / SHCI_COL_BLOCKS, ng = gridDim.x / SHCI_COL_BLOCKS;
+  const int64_t* rp = rowPtr + (int64_t)c * n;
+  constexpr int U = 8;  // loads in flight per lane
+  for (int64_t I = g * kWaves + wave; I < n; I += ng * kWaves) {
+    const int64_t b = rp[I], e = rp[I + 1];
     for (int r = 0; r < k; r++) {
       const double* xr = x + r * ld;
-      double acc0 = 0.0, acc1 = 0.0;
-      int64_t p = b + lane;
-      for (; p + 64 < e; p += 128) {
-        int32_t c0 = __builtin_nontemporal_load(cols + p);
-        int32_t c1 = __builtin_nontemporal_load(cols + p + 64);
-        double v0 = (double)__builtin_nontemporal_load(vals + p);
-        double v1 = (double)__builtin_nontemporal_load(vals + p + 64);
-        acc0 = fma(v0, xr[c0], acc0);
-        acc1 = fma(v1, xr[c1], acc1);
-      }
-      if (p < e) {
-        int32_t c0 = __builtin_nontemporal_load(cols + p);
-        double v0 = (double)__builtin_nontemporal_load(vals + p);
-        acc0 = fma(v0, xr[c0], acc0);
-      }
-      double acc = acc0 + acc1;
+      double acc[U];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      if (lane == 0) y[r * ld + I] = fma(diag[I], xr[I], acc);
+      for (int u = 0; u < U; u++) acc[u] = 0.0;
+      int64_t p = b + lane;
+      for (; p + 64 * (U - 1) < e; p += 64 * U) {
+        int32_t c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
+          v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[u] = fma(v[u], xr[c[u]], acc[u]);
+      }
+      for (; p < e; p += 64) acc[0] = fma((double)vals[p], xr[cols[p]], acc[0]);
+      double sum = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; u++) sum += acc[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (lane == 0) part[((int64_t)c * k + r) * n + I] = sum;
+    }
+  }
+}
+
+// y[:, r] = diag .* x[:, r] + sum_c part_c[:, r]
+__global__ void kSigmaCombine(const double* __restrict__ part, const double* __restrict__ diag,
+                              const double* __restrict__ x, double* __restrict__ y, int64_t n, int k, int64_t ld) {
+  for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
+    const double d = diag[I];
+    for (int r = 0; r < k; r++) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < SHCI_COL_BLOCKS; c++) s += part[((int64_t)c * k + r) * n + I];
+      y[r * ld + I] = fma(d, x[r * ld + I], s);
     }
   }
 }
@@ -628,13 +682,23 @@ __global__ void __launch_bounds__(kBlock) kGramPartial(const double* A, const do
   }
 }
 
-__global__ void kGramFinal(const double* partial, int nblocks, int MA, int MB, int ma, int mb, double* G) {
-  int q = threadIdx.x;
-  if (q >= ma * mb) return;
-  int i = q / mb, j = q % mb;
+// one block per output: fixed-order strided partial sums, then a block tree
+__global__ void __launch_bounds__(kBlock) kGramFinal(const double* partial, int nblocks, int MA, int MB, int ma,
+                                                     int mb, double* G) {
+  __shared__ double red[kWaves];
+  const int q = blockIdx.x;
+  const int i = q / mb, j = q % mb;
   double v = 0.0;
-  for (int b = 0; b < nblocks; b++) v += partial[(int64_t)b * MA * MB + i * MB + j];
-  G[i * mb + j] = v;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v += partial[(int64_t)b * MA * MB + i * MB + j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kWaves; w++) t += red[w];
+    G[i * mb + j] = t;
+  }
 }
 
 __global__ void kRotate(double* X, int m, int64_t n, int64_t ld, const double* V) {
@@ -680,12 +744,22 @@ __global__ void __launch_bounds__(kBlock) kResidual(const double* sig, const dou
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ void kFinalSum(const double* partial, int nb, int stride, int nvals, double* out) {
-  int q = threadIdx.x;
-  if (q >= nvals) return;
+// one block per output value (blockIdx.x = q < nvals)
+__global__ void __launch_bounds__(kBlock) kFinalSum(const double* partial, int nb, int stride, int nvals,
+                                                    double* out) {
+  __shared__ double red[kWaves];
+  const int q = blockIdx.x;
   double v = 0.0;
-  for (int b = 0; b < nb; b++) v += partial[(int64_t)b * stride + q];
-  out[q] = v;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partial[(int64_t)b * stride + q];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kWaves; w++) t += red[w];
+    out[q] = t;
+  }
 }
 
 __global__ void kPrecondition(double* r, const double* diag, const double* e, int64_t n) {
@@ -858,13 +932,13 @@ void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cma
   HIP_CHECK(hipGetLastError());
 }
 
-void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* cmax, double eps1, int ncore,
-                  int nact, const int32_t* table, uint64_t slots, uint64_t* out, unsigned long long* counter,
-                  int64_t cap, hipStream_t s) {
-  if (n <= 0) return;
-  unsigned g = gridFor(n, kWaves, 1 << 16);
-  DISPATCH_W(S.W, kScreen<WW><<<g, kBlock, 0, s>>>(S, dets, n, cmax, eps1, ncore, nact, table, slots - 1, out,
-                                                      counter, cap));
+void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                  double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
+                  unsigned long long* counter, int64_t cap, hipStream_t s) {
+  if (end <= begin) return;
+  unsigned g = gridFor(end - begin, kWaves, 1 << 16);
+  DISPATCH_W(S.W, kScreen<WW><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, table, slots - 1,
+                                                      out, counter, cap));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -958,7 +1032,8 @@ void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, rowCount, nullptr, nullptr, nullptr));
+  const uint32_t bc = (uint32_t)((T.n + SHCI_COL_BLOCKS - 1) / SHCI_COL_BLOCKS);
+  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, rowCount, nullptr, nullptr, nullptr, bc));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -967,7 +1042,8 @@ void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables&
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nullptr, rowPtr, cols, vals));
+  const uint32_t bc = (uint32_t)((T.n + SHCI_COL_BLOCKS - 1) / SHCI_COL_BLOCKS);
+  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nullptr, rowPtr, cols, vals, bc));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -983,18 +1059,26 @@ void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* val
   HIP_CHECK(hipGetLastError());
 }
 
-void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+template <typename VT>
+static void launchSigmaT(const int64_t* rowPtr, const int32_t* cols, const VT* vals, const double* diag, int64_t n,
+                         const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
   if (n <= 0) return;
-  kSigma<double><<<gridFor(n, kWaves, 1 << 18), kBlock, 0, s>>>(rowPtr, cols, vals, diag, n, x, y, k, ld);
+  part.reserve((size_t)SHCI_COL_BLOCKS * k * n * sizeof(double));
+  const int64_t perBlock = std::min<int64_t>((n + kWaves - 1) / kWaves, 1 << 15);
+  kSigmaBlocks<VT><<<(unsigned)(perBlock * SHCI_COL_BLOCKS), kBlock, 0, s>>>(rowPtr, cols, vals, n, x,
+                                                                              part.as<double>(), k, ld);
+  kSigmaCombine<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(part.as<double>(), diag, x, y, n, k, ld);
   HIP_CHECK(hipGetLastError());
 }
 
+void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
+                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
+  launchSigmaT<double>(rowPtr, cols, vals, diag, n, x, y, k, ld, part, s);
+}
+
 void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, hipStream_t s) {
-  if (n <= 0) return;
-  kSigma<float><<<gridFor(n, kWaves, 1 << 18), kBlock, 0, s>>>(rowPtr, cols, vals, diag, n, x, y, k, ld);
-  HIP_CHECK(hipGetLastError());
+                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
+  launchSigmaT<float>(rowPtr, cols, vals, diag, n, x, y, k, ld, part, s);
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,
@@ -1004,19 +1088,19 @@ void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int
   if (ma <= 1 && mb <= 1) {
     partial.reserve((size_t)nbb * 1 * sizeof(double));
     kGramPartial<1, 1><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
-    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 1, 1, ma, mb, G);
+    kGramFinal<<<ma * mb, kBlock, 0, s>>>(partial.as<double>(), nbb, 1, 1, ma, mb, G);
   } else if (ma <= 1 && mb <= 8) {
     partial.reserve((size_t)nbb * 8 * sizeof(double));
     kGramPartial<1, 8><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
-    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 1, 8, ma, mb, G);
+    kGramFinal<<<ma * mb, kBlock, 0, s>>>(partial.as<double>(), nbb, 1, 8, ma, mb, G);
   } else if (ma <= 8 && mb <= 8) {
     partial.reserve((size_t)nbb * 64 * sizeof(double));
     kGramPartial<8, 8><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
-    kGramFinal<<<1, 64, 0, s>>>(partial.as<double>(), nbb, 8, 8, ma, mb, G);
+    kGramFinal<<<ma * mb, kBlock, 0, s>>>(partial.as<double>(), nbb, 8, 8, ma, mb, G);
   } else if (ma <= 12 && mb <= 12) {
     partial.reserve((size_t)nbb * 144 * sizeof(double));
     kGramPartial<12, 12><<<nbb, kBlock, 0, s>>>(A, B, ma, mb, n, ld, partial.as<double>());
-    kGramFinal<<<1, 256, 0, s>>>(partial.as<double>(), nbb, 12, 12, ma, mb, G);
+    kGramFinal<<<ma * mb, kBlock, 0, s>>>(partial.as<double>(), nbb, 12, 12, ma, mb, G);
   } else {
     throw HipError("launchGram: subspace too large");
   }
@@ -1034,7 +1118,7 @@ void launchResidual(const double* sig, const double* b, const double* e, int64_t
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * sizeof(double));
   kResidual<<<nb, kBlock, 0, s>>>(sig, b, e, n, r, partial.as<double>());
-  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
+  kFinalSum<<<1, kBlock, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1047,7 +1131,7 @@ void launchDot2(const double* b, const double* r, int64_t n, double* out, DevMem
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * 2 * sizeof(double));
   kDot2<<<nb, kBlock, 0, s>>>(b, r, n, partial.as<double>());
-  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
+  kFinalSum<<<2, kBlock, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1060,7 +1144,7 @@ void launchNorm2(const double* x, int64_t n, double* out, DevMem& partial, hipSt
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * sizeof(double));
   kNorm2<<<nb, kBlock, 0, s>>>(x, n, partial.as<double>());
-  kFinalSum<<<1, 64, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
+  kFinalSum<<<1, kBlock, 0, s>>>(partial.as<double>(), nb, 1, 1, out);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -41,6 +41,11 @@ _STATUS = {0: "OK", 1: "BAD_INPUT", 2: "OOM", 3: "NO_CONVERGENCE", 4: "HIP_ERROR
 _lib = None
 
 
+class RunOptions(ctypes.Structure):
+    """shci_run_options of include/shci_gpu.h."""
+    _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
 class ShciError(RuntimeError):
     def __init__(self, status, message):
         super().__init__("%s: %s" % (_STATUS.get(status, status), message))
@@ -82,6 +87,10 @@ def library():
         "shci_sigma_device": ([P, P, P, I32, I64], I32),
         "shci_davidson": ([P, pD, I32, I32, D, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_run": ([ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(P)], I32),
+        "shci_run_ex": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RunOptions), ctypes.POINTER(P)], I32),
+        "shci_run_iteration_phases": ([P, I32, pD, pD, pD, pI64], I32),
+        "shci_run_final_davidson_seconds": ([P], D),
+        "shci_profile_sigma_read": ([P, pI64, pD, pD], I32),
         "shci_run_iterations": ([P], I32),
         "shci_run_iteration": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int), pD], I32),
         "shci_run_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
@@ -255,6 +264,25 @@ class VariationalResult:
         self._lib.shci_run_energies(self._h, _p(e, ctypes.c_double), ctypes.byref(n))
         return e[:n.value].copy()
 
+    def phases(self):
+        """Per iteration: (screen s, Hamiltonian s, Davidson s, lower-triangle element count)."""
+        out = []
+        for i in range(self._lib.shci_run_iterations(self._h)):
+            a, b, c, z = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+            self._lib.shci_run_iteration_phases(self._h, i, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                ctypes.byref(z))
+            out.append((a.value, b.value, c.value, z.value))
+        return out
+
+    def final_davidson_seconds(self):
+        return self._lib.shci_run_final_davidson_seconds(self._h)
+
+    def sigma_profile(self):
+        """(calls, kernel milliseconds, algorithmic bytes) of the timed sigma launches."""
+        c, ms, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self._lib.shci_profile_sigma_read(self._h, ctypes.byref(c), ctypes.byref(ms), ctypes.byref(b))
+        return c.value, ms.value, b.value
+
     def dets(self):
         n = ctypes.c_int64()
         self._lib.shci_num_dets(self._h, ctypes.byref(n))
@@ -264,12 +292,15 @@ class VariationalResult:
         return out
 
 
-def do_variational(input_path, log_path=None, precision=PREC_FP64):
+def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False):
     """Run the variational step of ``Dice input.dat``; returns a VariationalResult."""
     L = library()
     h = ctypes.c_void_p()
-    st = L.shci_run(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, precision,
-                    ctypes.byref(h))
+    opt = RunOptions()
+    opt.precision = precision
+    opt.profile_sigma = 1 if profile_sigma else 0
+    st = L.shci_run_ex(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, ctypes.byref(opt),
+                       ctypes.byref(h))
     if st != 0:
         msg = L.shci_last_error(h).decode() if h else L.shci_last_error(None).decode()
         if h:

@@ -124,10 +124,30 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
  * reference's tables to log_path (stdout when NULL or empty).  precision as
  * shci_precision.  The result handle holds the final space for inspection. */
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);
+
+/* Options of shci_run_ex (zero-initialise, then set fields). */
+typedef struct {
+  int precision;     /* shci_precision */
+  int profile_sigma; /* nonzero: time every sigma launch with HIP events on the handle's stream */
+  int reserved[6];
+} shci_run_options;
+/* shci_run with options; same files, log and result handle. */
+shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,
+                        shci_handle** out);
+
 /* per-iteration records of the last shci_run on this handle */
 int shci_run_iterations(const shci_handle* h);
 shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, double* energy0, int* davidson_iters,
                                double* seconds);
+/* wall time of each phase of iteration `it` (screening + merge, Hamiltonian build, Davidson) and the
+ * element count of the reference's lower-triangular rows after it */
+shci_status shci_run_iteration_phases(const shci_handle* h, int it, double* screen_s, double* hamiltonian_s,
+                                      double* davidson_s, int64_t* nnz_lower);
+/* wall time of the final tight Davidson of the last run (seconds) */
+double shci_run_final_davidson_seconds(const shci_handle* h);
+/* sigma launches timed since the run started (profile_sigma): count, summed kernel time (HIP
+ * events) and summed algorithmic bytes (stored values + indices, row offsets, diagonal, x and y once) */
+shci_status shci_profile_sigma_read(shci_handle* h, int64_t* calls, double* milliseconds, double* bytes);
 shci_status shci_run_energies(const shci_handle* h, double* energies, int* nroots);
 /* final coefficients, n x nroots column-major */
 shci_status shci_run_ci(shci_handle* h, double* out, int64_t cap);

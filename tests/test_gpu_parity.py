@@ -238,7 +238,9 @@ def test_run_hubbard_two_roots(tmp_path):
     p = str(tmp_path / "FCIDUMP")
     synth.write_hubbard_fcidump(p, 4, 4, 1.0, 4.0, 8, 8)
     inp = str(tmp_path / "input.dat")
-    synth.write_input(inp, p, synth.neel_occupation(4, 4), [(0, 1e-3)], 5, nroots=2)
+    # three iterations (29,937 determinants): the oracle needs ~1.5 s; a
+    # fourth would take it minutes
+    synth.write_input(inp, p, synth.neel_occupation(4, 4), [(0, 1e-3)], 3, nroots=2)
     with open(inp, "a") as f:
         f.write("prefix %s\n" % tmp_path)
     from dice_amd import shci
