@@ -51,6 +51,36 @@ def workload_inputs(directory):
     return full, sample
 
 
+def run_form(shci, full, log, prec, steps, warmup):
+    """Warm up, then time `steps` complete variational steps in one precision."""
+    tag = "fp32" if prec == shci.PREC_FP32 else "fp64"
+    for w in range(warmup):
+        print("# %s warmup %d/%d" % (tag, w + 1, warmup), file=sys.stderr, flush=True)
+        shci.do_variational(full, log, precision=prec)
+    R = {"iter_times": [], "step_times": [], "calls": 0, "ms": 0.0, "nbytes": 0.0, "energies": [], "per_iter": None,
+         "sigma_loose": None}
+    for k in range(steps):
+        t0 = time.time()
+        r = shci.do_variational(full, log, precision=prec, profile_sigma=True)
+        wall = time.time() - t0
+        ph = r.phases()
+        its = r.iterations()
+        tt = [a + b + c for a, b, c, _ in ph]
+        R["iter_times"] += tt
+        R["step_times"].append(sum(tt) + r.final_davidson_seconds())
+        c, m, b = r.sigma_profile()
+        R["calls"] += c
+        R["ms"] += m
+        R["nbytes"] += b
+        R["energies"].append(float(r.energies()[0]))
+        R["per_iter"] = [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"],
+                          "screen_s": round(p[0], 4), "hamiltonian_s": round(p[1], 4), "davidson_s": round(p[2], 4)}
+                         for it, p in zip(its, ph)]
+        print("# %s step %d: %.3f s (wall %.3f s), E = %.10f" % (tag, k + 1, R["step_times"][-1], wall,
+                                                                  R["energies"][-1]), file=sys.stderr, flush=True)
+    return R
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,37 +95,17 @@ def main():
         sys.exit(2)
 
     from dice_amd import shci
-    prec = shci.PREC_FP32 if args.precision == "fp32" else shci.PREC_FP64
     work = os.path.join(tempfile.gettempdir(), "shci_bench_c1")
     full, sample = workload_inputs(work)
     log = os.path.join(work, "gpu.log")
-
-    for w in range(args.warmup):
-        print("# warmup %d/%d" % (w + 1, args.warmup), file=sys.stderr, flush=True)
-        shci.do_variational(full, log, precision=prec)
-
-    iter_times, step_times, calls, ms, nbytes, energies = [], [], 0, 0.0, 0.0, []
-    per_iter = None
-    for k in range(args.steps):
-        t0 = time.time()
-        r = shci.do_variational(full, log, precision=prec, profile_sigma=True)
-        wall = time.time() - t0
-        ph = r.phases()
-        its = r.iterations()
-        tt = [a + b + c for a, b, c, _ in ph]
-        iter_times += tt
-        step_times.append(sum(tt) + r.final_davidson_seconds())
-        c, m, b = r.sigma_profile()
-        calls += c
-        ms += m
-        nbytes += b
-        energies.append(float(r.energies()[0]))
-        per_iter = [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"],
-                     "screen_s": round(p[0], 4), "hamiltonian_s": round(p[1], 4), "davidson_s": round(p[2], 4)}
-                    for it, p in zip(its, ph)]
-        print("# step %d: %.3f s (wall %.3f s), E = %.10f" % (k + 1, step_times[-1], wall, energies[-1]),
-              file=sys.stderr, flush=True)
-
+    primary = shci.PREC_FP32 if args.precision == "fp32" else shci.PREC_FP64
+    results = {}
+    for prec in ([primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]):
+        results[prec] = run_form(shci, full, log, prec, args.steps, args.warmup)
+    prec = primary
+    R = results[prec]
+    iter_times, step_times, calls, ms, nbytes, energies, per_iter = (
+        R["iter_times"], R["step_times"], R["calls"], R["ms"], R["nbytes"], R["energies"], R["per_iter"])
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     value = statistics.mean(iter_times)
     out = {
@@ -132,6 +142,26 @@ def main():
         },
         "iterations": per_iter,
     }
+
+    other = [p for p in results if p != prec]
+    if other:
+        o = results[other[0]]
+        fp64 = results[shci.PREC_FP64]
+        mixed = results[shci.PREC_FP32]
+        out["forms"] = {
+            "mixed_fp32": {
+                "description": "FP32 Hamiltonian values with FP64 accumulation in the Davidson iterations; "
+                               "final tight Davidson on FP32 hi+lo values (FP64-accurate refinement)",
+                "value": round(statistics.mean(mixed["iter_times"]), 6),
+                "ms_per_step": round(1e3 * statistics.mean(mixed["step_times"]), 3),
+                "final_energy": mixed["energies"][-1],
+                "energy_error_vs_fp64_Ha": mixed["energies"][-1] - fp64["energies"][-1],
+                "final_ndets": mixed["per_iter"][-1]["ndets"],
+                "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(mixed["step_times"]),
+                                         3),
+                "sigma_GBps": round(mixed["nbytes"] / (mixed["ms"] * 1e-3) / 1e9, 1) if mixed["ms"] else None,
+            }
+        }
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

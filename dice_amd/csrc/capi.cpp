@@ -461,7 +461,14 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
     HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (nseg + 1) * 8, hipMemcpyDeviceToHost));
     if (E.nnzOff) {
       HIP_CHECK(hipMemcpy(c.data(), E.cols.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
-      HIP_CHECK(hipMemcpy(v.data(), E.vals.p, E.nnzOff * 8, hipMemcpyDeviceToHost));
+      if (E.precision == 1) {  // FP32 storage: report hi + lo
+        std::vector<float> hi(E.nnzOff), lo(E.nnzOff);
+        HIP_CHECK(hipMemcpy(hi.data(), E.vals32.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(lo.data(), E.valsLo.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < E.nnzOff; q++) v[q] = (double)hi[q] + (double)lo[q];
+      } else {
+        HIP_CHECK(hipMemcpy(v.data(), E.vals.p, E.nnzOff * 8, hipMemcpyDeviceToHost));
+      }
     }
     HIP_CHECK(hipMemcpy(dg.data(), E.diag.p, n * 8, hipMemcpyDeviceToHost));
     int64_t k = 0;

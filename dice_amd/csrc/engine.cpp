@@ -192,15 +192,21 @@ void Engine::buildHamiltonian() {
   vals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
   launchCsrFill(S, dets.as<uint64_t>(), T, rp, cols.as<int32_t>(), vals.as<double>(), stream);
   if (precision == 1) {
+    // mixed precision: FP32 values for the Davidson iterations plus the FP32
+    // remainder, so the refinement sigma reproduces the FP64 values; the FP64
+    // array itself is not kept
     vals32.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    launchCastValues(vals.as<double>(), vals32.as<float>(), nnzOff, stream);
+    valsLo.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    launchSplitHiLo(vals.as<double>(), vals32.as<float>(), valsLo.as<float>(), nnzOff, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    vals.release();
   }
   hamRows = n;
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) {
-  const bool f32 = precision == 1 && !exact;
+  const bool f32 = precision == 1;
   SigmaEvent* ev = nullptr;
   if (profileSigma) {
     if (sigmaEventsUsed == sigmaEvents.size()) {
@@ -212,12 +218,13 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     ev = &sigmaEvents[sigmaEventsUsed++];
     // algorithmic bytes: stored values + column indices, row offsets,
     // diagonal, x read once and y written once per vector
-    ev->bytes = (double)nnzOff * ((f32 ? 4 : 8) + 4) + (double)(n + 1) * 8 + (double)n * 8 + 2.0 * k * n * 8;
+    ev->bytes = (double)nnzOff * ((f32 ? (exact ? 8 : 4) : 8) + 4) + (double)(n + 1) * 8 + (double)n * 8 +
+                2.0 * k * n * 8;
     HIP_CHECK(hipEventRecord(ev->start, stream));
   }
   if (f32)
-    launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), diag.as<double>(), n, dx, dy, k, ld,
-                   sigmaPart, stream);
+    launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
+                   diag.as<double>(), n, dx, dy, k, ld, sigmaPart, stream);
   else
     launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k, ld,
                    sigmaPart, stream);

@@ -65,8 +65,12 @@ void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* val
 // column-blocked storage as above; `part` holds the per-block partial sums
 void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag,
                     int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s);
-void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s);
+// FP32 values (`lo` null) or hi + lo FP32 pairs; FP64 accumulation either way
+void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
+                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
+                    hipStream_t s);
+// hi = fp32(v), lo = fp32(v - hi)
+void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

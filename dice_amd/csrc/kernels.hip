@@ -583,16 +583,19 @@ __global__ void kSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, doubl
 // sigma, column-blocked: block c of the columns (determinant indices
 // [c*blockCols, (c+1)*blockCols)) is handled only by workgroups with
 // blockIdx.x % SHCI_COL_BLOCKS == c.  Workgroups are dispatched round-robin
-// over the 8 XCDs, so each XCD gathers from one slice of x that stays
-// resident in its own L2 while the matrix streams through.  One wavefront per
-// (row, block) segment, FP64 accumulation; partial sums per block are
-// combined in fixed order by kSigmaCombine.
-template <typename VT>
+// over the 8 XCDs, so with 8 blocks each XCD gathers from one slice of x.
+// (On the 1.2M-determinant benchmark matrix that split cost more in
+// per-segment overhead than it saved, so the build uses one block: plain
+// CSR.)  One wavefront per (row, block) segment, eight loads in flight per
+// lane, FP64 accumulation; partial sums per block are combined in fixed
+// order by kSigmaCombine.  Values are VT, optionally plus an FP32 low part
+// (hi + lo reproduces the FP64 value to ~2^-48 relative).
+template <typename VT, bool LO>
 __global__ void __launch_bounds__(kBlock) kSigmaBlocks(const int64_t* __restrict__ rowPtr,
                                                        const int32_t* __restrict__ cols,
-                                                       const VT* __restrict__ vals, int64_t n,
-                                                       const double* __restrict__ x, double* __restrict__ part,
-                                                       int k, int64_t ld) {
+                                                       const VT* __restrict__ vals, const float* __restrict__ lo,
+                                                       int64_t n, const double* __restrict__ x,
+                                                       double* __restrict__ part, int k, int64_t ld) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   const int c = blockIdx.x % SHCI_COL_BLOCKS;
   const int64_t g = blockIdx.x / SHCI_COL_BLOCKS, ng = gridDim.x / SHCI_COL_BLOCKS;
@@ -613,11 +616,12 @@ __global__ void __launch_bounds__(kBlock) kSigmaBlocks(const int64_t* __restrict
         for (int u = 0; u < U; u++) {
           c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
           v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
+          if (LO) v[u] += (double)__builtin_nontemporal_load(lo + p + 64 * u);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) acc[u] = fma(v[u], xr[c[u]], acc[u]);
       }
-      for (; p < e; p += 64) acc[0] = fma((double)vals[p], xr[cols[p]], acc[0]);
+      for (; p < e; p += 64) acc[0] = fma((double)vals[p] + (LO ? (double)lo[p] : 0.0), xr[cols[p]], acc[0]);
       double sum = 0.0;
 #pragma unroll
       for (int u = 0; u < U; u++) sum += acc[u];
@@ -1059,26 +1063,46 @@ void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* val
   HIP_CHECK(hipGetLastError());
 }
 
-template <typename VT>
-static void launchSigmaT(const int64_t* rowPtr, const int32_t* cols, const VT* vals, const double* diag, int64_t n,
-                         const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
+template <typename VT, bool LO>
+static void launchSigmaT(const int64_t* rowPtr, const int32_t* cols, const VT* vals, const float* lo,
+                         const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
+                         hipStream_t s) {
   if (n <= 0) return;
   part.reserve((size_t)SHCI_COL_BLOCKS * k * n * sizeof(double));
   const int64_t perBlock = std::min<int64_t>((n + kWaves - 1) / kWaves, 1 << 15);
-  kSigmaBlocks<VT><<<(unsigned)(perBlock * SHCI_COL_BLOCKS), kBlock, 0, s>>>(rowPtr, cols, vals, n, x,
-                                                                              part.as<double>(), k, ld);
+  kSigmaBlocks<VT, LO><<<(unsigned)(perBlock * SHCI_COL_BLOCKS), kBlock, 0, s>>>(rowPtr, cols, vals, lo, n, x,
+                                                                                  part.as<double>(), k, ld);
   kSigmaCombine<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(part.as<double>(), diag, x, y, n, k, ld);
   HIP_CHECK(hipGetLastError());
 }
 
 void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
                     const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
-  launchSigmaT<double>(rowPtr, cols, vals, diag, n, x, y, k, ld, part, s);
+  launchSigmaT<double, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, part, s);
 }
 
-void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
-  launchSigmaT<float>(rowPtr, cols, vals, diag, n, x, y, k, ld, part, s);
+void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
+                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
+                    hipStream_t s) {
+  if (lo)
+    launchSigmaT<float, true>(rowPtr, cols, vals, lo, diag, n, x, y, k, ld, part, s);
+  else
+    launchSigmaT<float, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, part, s);
+}
+
+__global__ void kSplitHiLo(const double* v, float* hi, float* lo, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = v[i];
+    const float h = (float)d;
+    hi[i] = h;
+    lo[i] = (float)(d - (double)h);
+  }
+}
+
+void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  kSplitHiLo<<<gridFor(n), kBlock, 0, s>>>(v, hi, lo, n);
+  HIP_CHECK(hipGetLastError());
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,

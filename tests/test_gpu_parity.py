@@ -247,3 +247,27 @@ def test_run_hubbard_two_roots(tmp_path):
     o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
     r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
     np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-9)
+
+
+def test_run_mixed_precision_energy_gap(tmp_path):
+    """FP32 Hamiltonian values with FP64 accumulation and the FP64-accurate
+    refinement: the variational energy stays within 0.1 mHa of the FP64 path
+    (the bar of the mixed-precision form); the FP64 path is the oracle-pinned
+    one."""
+    from dice_amd import synth, shci
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 9, 12, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 3e-3), (2, 1e-3)], 5)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    e64 = shci.do_variational(inp, str(tmp_path / "a.log")).energies()[0]
+    r32 = shci.do_variational(inp, str(tmp_path / "b.log"), precision=shci.PREC_FP32)
+    e32 = r32.energies()[0]
+    print("mixed-precision energy gap: %.3e Ha" % (e32 - e64))
+    assert abs(e32 - e64) < 1e-4
+    # the refinement makes the final energy FP64-accurate for its space
+    g = shci.System.from_fcidump(path)
+    g.set_dets(r32.dets())
+    g.make_hamiltonian()
+    assert abs(e32 - g.core_energy - g.davidson(np.eye(len(r32.dets()), 1), tol=1e-8)[0][0]) < 1e-7
