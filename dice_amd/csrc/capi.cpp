@@ -39,6 +39,9 @@ shci_status guarded(shci_handle* h, F&& f) {
     return fail(h, SHCI_OOM, e.what());
   } catch (const HipError& e) {
     std::string m = e.what();
+    // a failed allocation also sets HIP's per-thread last error; clear it so
+    // the next launch check on this thread does not report it again
+    (void)hipGetLastError();
     return fail(h, m.find("out of memory") != std::string::npos ? SHCI_OOM : SHCI_HIP_ERROR, m);
   } catch (const std::invalid_argument& e) {
     return fail(h, SHCI_BAD_INPUT, e.what());
@@ -454,11 +457,10 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
     if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
     const int64_t n = E.n;
     if (row_begin < 0 || row_begin > n) throw std::invalid_argument("row_begin out of range");
-    const int64_t nseg = (int64_t)SHCI_COL_BLOCKS * n;
-    std::vector<int64_t> rp(nseg + 1);
+    std::vector<int64_t> rp(n + 1);
     std::vector<int32_t> c(E.nnzOff);
     std::vector<double> v(E.nnzOff), dg(n);
-    HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (nseg + 1) * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(rp.data(), E.rowPtr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
     if (E.nnzOff) {
       HIP_CHECK(hipMemcpy(c.data(), E.cols.p, E.nnzOff * 4, hipMemcpyDeviceToHost));
       if (E.precision == 1) {  // FP32 storage: report hi + lo
@@ -476,9 +478,8 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
     std::vector<std::pair<int32_t, double>> row;
     for (int64_t I = row_begin; I < n; I++) {
       row.clear();
-      for (int b = 0; b < SHCI_COL_BLOCKS; b++)
-        for (int64_t p = rp[b * n + I]; p < rp[b * n + I + 1]; p++)
-          if (c[p] < I) row.emplace_back(c[p], v[p]);
+      for (int64_t p = rp[I]; p < rp[I + 1]; p++)
+        if (c[p] < I) row.emplace_back(c[p], v[p]);
       std::sort(row.begin(), row.end(), [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
         return a.first < b.first;
       });
@@ -501,6 +502,10 @@ shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, in
   return guarded(h, [&] {
     Engine& E = h->eng;
     if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    if (k < 1 || ld < E.n) throw std::invalid_argument("need k >= 1 and ld >= number of determinants");
+    const char *xb = (const char*)d_x, *xe = xb + ((k - 1) * ld + E.n) * 8;
+    const char *yb = (const char*)d_y, *ye = yb + ((k - 1) * ld + E.n) * 8;
+    if (xb < ye && yb < xe) throw std::invalid_argument("d_y overlaps d_x");
     E.sigma(d_x, d_y, k, ld, false);
     return SHCI_OK;
   });

@@ -164,43 +164,78 @@ void Engine::appendNew() {
   if (slots == oldSlots) hashInsert(S, dets.as<uint64_t>(), old, n, table.as<int32_t>(), slots, stream);
 }
 
+static void swapMem(DevMem& a, DevMem& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
+
+// Builds H over the held determinants.  When a Hamiltonian over the first
+// hamRows determinants exists, only the connections that involve the new
+// determinants are enumerated and evaluated (new rows in full, old rows only
+// their new columns); the old rows are copied forward.
 void Engine::buildHamiltonian() {
   // diagonal for determinants without one
   diag.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream, true);
   if (nDiag < n) launchEnergy(S, dets.as<uint64_t>() + nDiag * hs.W, n - nDiag, diag.as<double>() + nDiag, stream);
   nDiag = n;
   if (hamRows == n) return;
+  const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
   buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
-  const int64_t nseg = (int64_t)SHCI_COL_BLOCKS * n;
-  rowPtr.reserve((size_t)(nseg + 1) * 8);
-  int64_t* rp = rowPtr.as<int64_t>();
-  HIP_CHECK(hipMemsetAsync(rp + nseg, 0, 8, stream));
-  launchCsrCount(S, dets.as<uint64_t>(), T, rp, stream);
-  exclusiveScanI64(rp, nseg + 1, ws, stream);
-  HIP_CHECK(hipMemcpyAsync(&nnzOff, rp + nseg, 8, hipMemcpyDeviceToHost, stream));
+  rowPtrNew.reserve((size_t)(n + 1) * 8);
+  int64_t* rp = rowPtrNew.as<int64_t>();
+  const int64_t* orp = rowPtr.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+  launchCsrCount(S, dets.as<uint64_t>(), T, nOld, rp, stream);
+  launchNewRowLengths(orp, nOld, n, rp, stream);
+  exclusiveScanI64(rp, n + 1, ws, stream);
+  int64_t nnzNew = 0;
+  HIP_CHECK(hipMemcpyAsync(&nnzNew, rp + n, 8, hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   {
     size_t freeB = 0, totalB = 0;
     HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-    size_t need = (size_t)nnzOff * (4 + 8 + (precision == 1 ? 4 : 0));
-    if (need > freeB + cols.bytes + vals.bytes + vals32.bytes)
+    size_t need = (size_t)nnzNew * (4 + 8 + (precision == 1 ? 8 : 0));
+    if (need > freeB + colsNew.bytes + valsNew.bytes + vals32New.bytes + valsLoNew.bytes)
       throw HipError("out of memory: the Hamiltonian of " + std::to_string(n) + " determinants has " +
-                     std::to_string(nnzOff) + " off-diagonal elements (" + std::to_string(need >> 30) +
-                     " GiB), " + std::to_string(freeB >> 30) + " GiB free");
+                     std::to_string(nnzNew) + " off-diagonal elements (" + std::to_string(need >> 30) +
+                     " GiB needed beside the previous one), " + std::to_string(freeB >> 30) + " GiB free");
   }
-  cols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-  vals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
-  launchCsrFill(S, dets.as<uint64_t>(), T, rp, cols.as<int32_t>(), vals.as<double>(), stream);
+  fillStart.reserve((size_t)std::max<int64_t>(n, 1) * 8);
+  launchFillStart(orp, nOld, n, rp, fillStart.as<int64_t>(), stream);
+  colsNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
+  valsNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 8);
   if (precision == 1) {
     // mixed precision: FP32 values for the Davidson iterations plus the FP32
-    // remainder, so the refinement sigma reproduces the FP64 values; the FP64
-    // array itself is not kept
-    vals32.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    valsLo.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    launchSplitHiLo(vals.as<double>(), vals32.as<float>(), valsLo.as<float>(), nnzOff, stream);
+    // remainder, so the refinement sigma reproduces the FP64 values; FP64
+    // values exist only transiently for the new connections
+    vals32New.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
+    valsLoNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
+    launchCopyOldRowsF32(orp, cols.as<int32_t>(), vals32.as<float>(), valsLo.as<float>(), nOld, rp,
+                         colsNew.as<int32_t>(), vals32New.as<float>(), valsLoNew.as<float>(), stream);
+    launchCsrFill(S, dets.as<uint64_t>(), T, nOld, fillStart.as<int64_t>(), colsNew.as<int32_t>(),
+                  valsNew.as<double>(), stream);
+    launchSplitNewRows(fillStart.as<int64_t>(), rp, n, valsNew.as<double>(), vals32New.as<float>(),
+                       valsLoNew.as<float>(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
+    swapMem(vals32, vals32New);
+    swapMem(valsLo, valsLoNew);
+    vals32New.release();
+    valsLoNew.release();
+    valsNew.release();
     vals.release();
+  } else {
+    launchCopyOldRowsF64(orp, cols.as<int32_t>(), vals.as<double>(), nOld, rp, colsNew.as<int32_t>(),
+                         valsNew.as<double>(), stream);
+    launchCsrFill(S, dets.as<uint64_t>(), T, nOld, fillStart.as<int64_t>(), colsNew.as<int32_t>(),
+                  valsNew.as<double>(), stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    swapMem(vals, valsNew);
+    valsNew.release();
   }
+  swapMem(rowPtr, rowPtrNew);
+  swapMem(cols, colsNew);
+  colsNew.release();
+  nnzOff = nnzNew;
   hamRows = n;
   HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -224,10 +259,10 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   }
   if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
-                   diag.as<double>(), n, dx, dy, k, ld, sigmaPart, stream);
+                   diag.as<double>(), n, dx, dy, k, ld, stream);
   else
     launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k, ld,
-                   sigmaPart, stream);
+                   stream);
   if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
 }
 

@@ -49,8 +49,9 @@ struct Engine {
   StringTables T;
   DevMem diag;
   int64_t nDiag = 0;
-  DevMem rowPtr, cols, vals, vals32, valsLo;  // column-blocked CSR (see launchCsrCount)
-  DevMem sigmaPart;                           // per-column-block partial sums of sigma
+  DevMem rowPtr, cols, vals, vals32, valsLo;  // full symmetric off-diagonal CSR
+  DevMem rowPtrNew, colsNew, valsNew, vals32New, valsLoNew, fillStart;  // incremental rebuild
+  bool incremental = true;
   int64_t nnzOff = 0, hamRows = -1;
 
   // Davidson scratch

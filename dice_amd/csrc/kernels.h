@@ -46,29 +46,39 @@ struct StringTables {
 void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, StringTables& T, Workspace& ws,
                        hipStream_t s);
 
-// Count (rowCount) or fill (cols/vals) the off-diagonal connections of every
-// row: all J != I one or two excitations apart.  Values are
-// Hij(bra = D_min(I,J), ket = D_max(I,J)).  Storage is column-blocked: the
-// entries of row I with J in block c = J / ceil(n / SHCI_COL_BLOCKS) occupy
-// [rowPtr[c*n + I], rowPtr[c*n + I + 1]); rowCount / rowPtr hold
-// SHCI_COL_BLOCKS * n (+1) entries.
-void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t* rowCount,
-                    hipStream_t s);
-void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, const int64_t* rowPtr,
-                   int32_t* cols, double* vals, hipStream_t s);
+// Count (rowCount) or fill (cols/vals from fillStart[I]) the off-diagonal
+// connections of every row, all J != I one or two excitations apart, except
+// that rows I < nOld only take J >= nOld (the columns a Hamiltonian built over
+// the first nOld determinants lacks).  Values are
+// Hij(bra = D_min(I,J), ket = D_max(I,J)).
+void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
+                    int64_t* rowCount, hipStream_t s);
+void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
+                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s);
+// incremental rebuild helpers: cnt[I] += old row length (I < nOld);
+// fs[I] = rowPtr[I] + old row length; old rows copied to the front of the
+// new rows; FP64 values of the new connections split into FP32 hi + lo
+void launchNewRowLengths(const int64_t* oldRowPtr, int64_t nOld, int64_t n, int64_t* cnt, hipStream_t s);
+void launchFillStart(const int64_t* oldRowPtr, int64_t nOld, int64_t n, const int64_t* rowPtr, int64_t* fs,
+                     hipStream_t s);
+void launchCopyOldRowsF64(const int64_t* oldRowPtr, const int32_t* oldCols, const double* oldVals, int64_t nOld,
+                          const int64_t* rowPtr, int32_t* cols, double* vals, hipStream_t s);
+void launchCopyOldRowsF32(const int64_t* oldRowPtr, const int32_t* oldCols, const float* oldHi, const float* oldLo,
+                          int64_t nOld, const int64_t* rowPtr, int32_t* cols, float* hi, float* lo, hipStream_t s);
+void launchSplitNewRows(const int64_t* fillStart, const int64_t* rowPtr, int64_t n, const double* v, float* hi,
+                        float* lo, hipStream_t s);
 void launchCastValues(const double* v, float* out, int64_t n, hipStream_t s);
 // sort the columns of every row (ascending), values carried along
 void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals, hipStream_t s);
 
 // --- sigma = H x ------------------------------------------------------------------------
-// y[:,r] = diag .* x[:,r] + offdiag(H) x[:,r] for r < k (column-major, ld),
-// column-blocked storage as above; `part` holds the per-block partial sums
+// y[:,r] = diag .* x[:,r] + offdiag(H) x[:,r] for r < k (column-major, ld);
+// y must not overlap x
 void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag,
-                    int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s);
+                    int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // FP32 values (`lo` null) or hi + lo FP32 pairs; FP64 accumulation either way
 void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
-                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
-                    hipStream_t s);
+                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // hi = fp32(v), lo = fp32(v - hi)
 void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream_t s);
 

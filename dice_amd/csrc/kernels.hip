@@ -332,10 +332,9 @@ __global__ void kIndexOf(const uint64_t* str, int64_t n, const uint64_t* sorted,
     idx[i] = (uint32_t)lowerBound(sorted, ns, str[i]);
 }
 
-__global__ void kGroupKeys(const uint32_t* major, const uint32_t* minor, int64_t n, uint64_t nminor, uint64_t* key,
-                           uint32_t* val) {
+__global__ void kGroupKeys(const uint32_t* major, int64_t n, uint64_t* key, uint32_t* val) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    key[i] = (uint64_t)major[i] * nminor + minor[i];
+    key[i] = (uint64_t)major[i] * (uint64_t)n + (uint64_t)i;
     val[i] = (uint32_t)i;
   }
 }
@@ -399,35 +398,26 @@ struct RowTables {
   int64_t n;
 };
 
-// Collects the connections of row I, split by column block c = J / blockCols
-// (SHCI_COL_BLOCKS blocks, one per XCD).  Count pass: per-lane counters per
-// block; fill pass: wave-uniform cursors per block, positions by ballot.
+// Collects the connections of row I.  Count pass: per-lane counters; fill
+// pass: wave-uniform cursor, positions by ballot (deterministic order).
 template <int W, bool FILL>
 struct RowSink {
   const DevSystem* S;
   const uint64_t* dets;
   const uint64_t* dI;
   int64_t I;
-  uint32_t blockCols;
   int32_t* cols;
   double* vals;
-  int64_t pos[SHCI_COL_BLOCKS];  // wave-uniform write cursors (fill)
-  int cnt[SHCI_COL_BLOCKS];      // per-lane counts (count)
+  int64_t pos;  // wave-uniform write cursor (fill)
+  int cnt;      // per-lane count (count)
   __device__ __forceinline__ void hit(bool h, uint32_t J) {
-    const int c = h ? (int)(J / blockCols) : -1;
     if (!FILL) {
-#pragma unroll
-      for (int q = 0; q < SHCI_COL_BLOCKS; q++) cnt[q] += (c == q) ? 1 : 0;
+      cnt += h ? 1 : 0;
       return;
     }
-    int64_t p = 0;
-#pragma unroll
-    for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
-      uint64_t m = __ballot(c == q);
-      if (c == q) p = pos[q] + __popcll(m & lanemaskLt());
-      pos[q] += __popcll(m);
-    }
+    uint64_t m = __ballot(h);
     if (h) {
+      int64_t p = pos + __popcll(m & lanemaskLt());
       uint64_t dJ[W];
 #pragma unroll
       for (int w = 0; w < W; w++) dJ[w] = dets[(int64_t)J * W + w];
@@ -435,14 +425,29 @@ struct RowSink {
       cols[p] = (int32_t)J;
       vals[p] = v;
     }
+    pos += __popcll(m);
   }
 };
 
-// rowPtr / rowCount are indexed [c * n + I] for column block c
+// first position in [lo, hi) whose determinant index is >= x (members of a
+// group are stored in increasing determinant index)
+__device__ __forceinline__ int64_t firstDetAtLeast(const uint32_t* a, int64_t lo, int64_t hi, uint32_t x) {
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Rows I < nOld enumerate only connections to J >= nOld (the columns the
+// previous Hamiltonian lacks); rows I >= nOld enumerate all connections.
 template <int W, bool FILL>
-__global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t* rowCount,
-                                                   const int64_t* rowPtr, int32_t* cols, double* vals,
-                                                   uint32_t blockCols) {
+__global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t nOld,
+                                                   int64_t* rowCount, const int64_t* fillStart, int32_t* cols,
+                                                   double* vals) {
   __shared__ int sExcl[kWaves][65];
   __shared__ int64_t sBeg[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
@@ -452,22 +457,20 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
     const uint64_t astr = T.alpha[I], bstr = T.beta[I];
     const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
+    const uint32_t jmin = I >= nOld ? 0u : (uint32_t)nOld;
     RowSink<W, FILL> sink;
     sink.S = &S;
     sink.dets = dets;
     sink.dI = dI;
     sink.I = I;
-    sink.blockCols = blockCols;
     sink.cols = cols;
     sink.vals = vals;
-#pragma unroll
-    for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
-      sink.pos[q] = FILL ? rowPtr[(int64_t)q * T.n + I] : 0;
-      sink.cnt[q] = 0;
-    }
+    sink.pos = FILL ? fillStart[I] : 0;
+    sink.cnt = 0;
     // (a) same beta string: alpha singles and alpha-alpha doubles
     {
       int64_t b0 = T.bStart[bi], b1 = T.bStart[bi + 1];
+      if (jmin) b0 = firstDetAtLeast(T.bMajorDet, b0, b1, jmin);
       for (int64_t m0 = b0; m0 < b1; m0 += 64) {
         int64_t m = m0 + lane;
         bool h = false;
@@ -483,6 +486,7 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     // (b) same alpha string: beta singles and beta-beta doubles
     {
       int64_t a0 = T.aStart[ai], a1 = T.aStart[ai + 1];
+      if (jmin) a0 = firstDetAtLeast(T.aMajorDet, a0, a1, jmin);
       for (int64_t m0 = a0; m0 < a1; m0 += 64) {
         int64_t m = m0 + lane;
         bool h = false;
@@ -506,7 +510,9 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
         if (c < s1) {
           uint32_t sa = T.aSingles[c];
           beg = T.aStart[sa];
-          len = (int)(T.aStart[sa + 1] - beg);
+          int64_t end = T.aStart[sa + 1];
+          if (jmin) beg = firstDetAtLeast(T.aMajorDet, beg, end, jmin);
+          len = (int)(end - beg);
         }
         int inc = len;
 #pragma unroll
@@ -541,13 +547,51 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
       }
     }
     if (!FILL) {
+      int c = sink.cnt;
 #pragma unroll
-      for (int q = 0; q < SHCI_COL_BLOCKS; q++) {
-        int c = sink.cnt[q];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-        if (lane == 0) rowCount[(int64_t)q * T.n + I] = c;
-      }
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+      if (lane == 0) rowCount[I] = c;
+    }
+  }
+}
+
+// new row length = kept old entries + new connections
+__global__ void kNewRowLengths(const int64_t* oldRowPtr, int64_t nOld, int64_t n, int64_t* cnt) {
+  for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x)
+    if (I < nOld) cnt[I] += oldRowPtr[I + 1] - oldRowPtr[I];
+}
+
+__global__ void kFillStart(const int64_t* oldRowPtr, int64_t nOld, int64_t n, const int64_t* rowPtr, int64_t* fs) {
+  for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x)
+    fs[I] = rowPtr[I] + (I < nOld ? oldRowPtr[I + 1] - oldRowPtr[I] : 0);
+}
+
+// copy the old rows to the front of the new rows (one wavefront per row)
+template <typename T1>
+__global__ void __launch_bounds__(kBlock) kCopyOldRows(const int64_t* oldRowPtr, const int32_t* oldCols,
+                                                       const T1* oldA, const T1* oldB, int64_t nOld,
+                                                       const int64_t* rowPtr, int32_t* cols, T1* a, T1* b) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < nOld; I += (int64_t)gridDim.x * kWaves) {
+    const int64_t s = oldRowPtr[I], e = oldRowPtr[I + 1], d = rowPtr[I];
+    for (int64_t p = s + lane; p < e; p += 64) {
+      cols[d + (p - s)] = __builtin_nontemporal_load(oldCols + p);
+      a[d + (p - s)] = __builtin_nontemporal_load(oldA + p);
+      if (oldB) b[d + (p - s)] = __builtin_nontemporal_load(oldB + p);
+    }
+  }
+}
+
+// split the FP64 values of the new connections into FP32 hi + lo
+__global__ void __launch_bounds__(kBlock) kSplitNewRows(const int64_t* fs, const int64_t* rowPtr, int64_t n,
+                                                        const double* v, float* hi, float* lo) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
+    for (int64_t p = fs[I] + lane; p < rowPtr[I + 1]; p += 64) {
+      const double d = v[p];
+      const float h = (float)d;
+      hi[p] = h;
+      lo[p] = (float)(d - (double)h);
     }
   }
 }
@@ -580,29 +624,24 @@ __global__ void kSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, doubl
 }
 
 // ---------------------------------------------------------------------------
-// sigma, column-blocked: block c of the columns (determinant indices
-// [c*blockCols, (c+1)*blockCols)) is handled only by workgroups with
-// blockIdx.x % SHCI_COL_BLOCKS == c.  Workgroups are dispatched round-robin
-// over the 8 XCDs, so with 8 blocks each XCD gathers from one slice of x.
-// (On the 1.2M-determinant benchmark matrix that split cost more in
-// per-segment overhead than it saved, so the build uses one block: plain
-// CSR.)  One wavefront per (row, block) segment, eight loads in flight per
-// lane, FP64 accumulation; partial sums per block are combined in fixed
-// order by kSigmaCombine.  Values are VT, optionally plus an FP32 low part
-// (hi + lo reproduces the FP64 value to ~2^-48 relative).
+// sigma = H x over the full symmetric CSR (gather only, no atomics): one
+// wavefront per row, eight loads in flight per lane, FP64 accumulation, the
+// diagonal term fused into the row's single store.  Values are VT,
+// optionally plus an FP32 low part (hi + lo reproduces the FP64 value to
+// ~2^-48 relative).  (Splitting the columns into per-XCD blocks, LDS-tiled
+// gathers and 4-wide vector loads were all measured slower on a
+// 1.2M-determinant matrix: the gathers of x are L2-request bound.)
 template <typename VT, bool LO>
-__global__ void __launch_bounds__(kBlock) kSigmaBlocks(const int64_t* __restrict__ rowPtr,
-                                                       const int32_t* __restrict__ cols,
-                                                       const VT* __restrict__ vals, const float* __restrict__ lo,
-                                                       int64_t n, const double* __restrict__ x,
-                                                       double* __restrict__ part, int k, int64_t ld) {
+__global__ void __launch_bounds__(kBlock) kSigma(const int64_t* __restrict__ rowPtr, const int32_t* __restrict__ cols,
+                                                 const VT* __restrict__ vals, const float* __restrict__ lo,
+                                                 const double* __restrict__ diag, int64_t n,
+                                                 const double* __restrict__ x, double* __restrict__ y, int k,
+                                                 int64_t ld) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
-  const int c = blockIdx.x % SHCI_COL_BLOCKS;
-  const int64_t g = blockIdx.x / SHCI_COL_BLOCKS, ng = gridDim.x / SHCI_COL_BLOCKS;
-  const int64_t* rp = rowPtr + (int64_t)c * n;
   constexpr int U = 8;  // loads in flight per lane
-  for (int64_t I = g * kWaves + wave; I < n; I += ng * kWaves) {
-    const int64_t b = rp[I], e = rp[I + 1];
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
+    const int64_t b = rowPtr[I], e = rowPtr[I + 1];
+    const double d = diag[I];
     for (int r = 0; r < k; r++) {
       const double* xr = x + r * ld;
       double acc[U];
@@ -627,21 +666,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaBlocks(const int64_t* __restrict
       for (int u = 0; u < U; u++) sum += acc[u];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-      if (lane == 0) part[((int64_t)c * k + r) * n + I] = sum;
-    }
-  }
-}
-
-// y[:, r] = diag .* x[:, r] + sum_c part_c[:, r]
-__global__ void kSigmaCombine(const double* __restrict__ part, const double* __restrict__ diag,
-                              const double* __restrict__ x, double* __restrict__ y, int64_t n, int k, int64_t ld) {
-  for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
-    const double d = diag[I];
-    for (int r = 0; r < k; r++) {
-      double s = 0.0;
-#pragma unroll
-      for (int c = 0; c < SHCI_COL_BLOCKS; c++) s += part[((int64_t)c * k + r) * n + I];
-      y[r * ld + I] = fma(d, x[r * ld + I], s);
+      if (lane == 0) y[r * ld + I] = fma(d, xr[I], sum);
     }
   }
 }
@@ -981,20 +1006,19 @@ void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, Stri
   };
   T.aStart.reserve((T.nA + 1) * 8);
   T.bStart.reserve((T.nB + 1) * 8);
-  // alpha-major groups: key = aIdx * nB + bIdx
-  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.aIdx.as<uint32_t>(), T.bIdx.as<uint32_t>(), n, (uint64_t)T.nB,
-                                            T.key.as<uint64_t>(), T.val.as<uint32_t>());
-  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * T.nB), false, ws, s);
+  // alpha-major groups, members in increasing determinant index:
+  // key = aIdx * n + det
+  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.aIdx.as<uint32_t>(), n, T.key.as<uint64_t>(), T.val.as<uint32_t>());
+  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * n), false, ws, s);
   HIP_CHECK(hipMemcpyAsync(T.aMajorDet.p, T.val.p, n * 4, hipMemcpyDeviceToDevice, s));
-  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)T.nB, T.nA,
+  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)n, T.nA,
                                                   T.aStart.as<int64_t>(), T.val.as<uint32_t>(),
                                                   T.beta.as<uint64_t>(), T.aMajorB.as<uint64_t>());
-  // beta-major groups: key = bIdx * nA + aIdx
-  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.bIdx.as<uint32_t>(), T.aIdx.as<uint32_t>(), n, (uint64_t)T.nA,
-                                            T.key.as<uint64_t>(), T.val.as<uint32_t>());
-  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * T.nB), false, ws, s);
+  // beta-major groups: key = bIdx * n + det
+  kGroupKeys<<<gridFor(n), kBlock, 0, s>>>(T.bIdx.as<uint32_t>(), n, T.key.as<uint64_t>(), T.val.as<uint32_t>());
+  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nB * n), false, ws, s);
   HIP_CHECK(hipMemcpyAsync(T.bMajorDet.p, T.val.p, n * 4, hipMemcpyDeviceToDevice, s));
-  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)T.nA, T.nB,
+  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)n, T.nB,
                                                   T.bStart.as<int64_t>(), T.val.as<uint32_t>(),
                                                   T.alpha.as<uint64_t>(), T.bMajorA.as<uint64_t>());
   // alpha single-excitation adjacency
@@ -1031,23 +1055,56 @@ static RowTables rowTables(const StringTables& T) {
   return r;
 }
 
-void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t* rowCount,
+void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld, int64_t* rowCount,
                     hipStream_t s) {
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  const uint32_t bc = (uint32_t)((T.n + SHCI_COL_BLOCKS - 1) / SHCI_COL_BLOCKS);
-  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, rowCount, nullptr, nullptr, nullptr, bc));
+  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr, nullptr, nullptr));
   HIP_CHECK(hipGetLastError());
 }
 
-void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, const int64_t* rowPtr,
-                   int32_t* cols, double* vals, hipStream_t s) {
+void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
+                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s) {
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  const uint32_t bc = (uint32_t)((T.n + SHCI_COL_BLOCKS - 1) / SHCI_COL_BLOCKS);
-  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nullptr, rowPtr, cols, vals, bc));
+  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, vals));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchNewRowLengths(const int64_t* oldRowPtr, int64_t nOld, int64_t n, int64_t* cnt, hipStream_t s) {
+  if (nOld <= 0) return;
+  kNewRowLengths<<<gridFor(n), kBlock, 0, s>>>(oldRowPtr, nOld, n, cnt);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchFillStart(const int64_t* oldRowPtr, int64_t nOld, int64_t n, const int64_t* rowPtr, int64_t* fs,
+                     hipStream_t s) {
+  kFillStart<<<gridFor(n), kBlock, 0, s>>>(oldRowPtr, nOld, n, rowPtr, fs);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCopyOldRowsF64(const int64_t* oldRowPtr, const int32_t* oldCols, const double* oldVals, int64_t nOld,
+                          const int64_t* rowPtr, int32_t* cols, double* vals, hipStream_t s) {
+  if (nOld <= 0) return;
+  kCopyOldRows<double><<<gridFor(nOld, kWaves, 1 << 16), kBlock, 0, s>>>(oldRowPtr, oldCols, oldVals, nullptr, nOld,
+                                                                          rowPtr, cols, vals, nullptr);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCopyOldRowsF32(const int64_t* oldRowPtr, const int32_t* oldCols, const float* oldHi, const float* oldLo,
+                          int64_t nOld, const int64_t* rowPtr, int32_t* cols, float* hi, float* lo, hipStream_t s) {
+  if (nOld <= 0) return;
+  kCopyOldRows<float><<<gridFor(nOld, kWaves, 1 << 16), kBlock, 0, s>>>(oldRowPtr, oldCols, oldHi, oldLo, nOld, rowPtr,
+                                                                         cols, hi, lo);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSplitNewRows(const int64_t* fillStart, const int64_t* rowPtr, int64_t n, const double* v, float* hi,
+                        float* lo, hipStream_t s) {
+  if (n <= 0) return;
+  kSplitNewRows<<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(fillStart, rowPtr, n, v, hi, lo);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1065,29 +1122,24 @@ void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* val
 
 template <typename VT, bool LO>
 static void launchSigmaT(const int64_t* rowPtr, const int32_t* cols, const VT* vals, const float* lo,
-                         const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
-                         hipStream_t s) {
+                         const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s) {
   if (n <= 0) return;
-  part.reserve((size_t)SHCI_COL_BLOCKS * k * n * sizeof(double));
-  const int64_t perBlock = std::min<int64_t>((n + kWaves - 1) / kWaves, 1 << 15);
-  kSigmaBlocks<VT, LO><<<(unsigned)(perBlock * SHCI_COL_BLOCKS), kBlock, 0, s>>>(rowPtr, cols, vals, lo, n, x,
-                                                                                  part.as<double>(), k, ld);
-  kSigmaCombine<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(part.as<double>(), diag, x, y, n, k, ld);
+  const int64_t blocks = std::min<int64_t>((n + kWaves - 1) / kWaves, 1 << 15);
+  kSigma<VT, LO><<<(unsigned)blocks, kBlock, 0, s>>>(rowPtr, cols, vals, lo, diag, n, x, y, k, ld);
   HIP_CHECK(hipGetLastError());
 }
 
 void launchSigmaF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
-                    const double* x, double* y, int k, int64_t ld, DevMem& part, hipStream_t s) {
-  launchSigmaT<double, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, part, s);
+                    const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  launchSigmaT<double, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, s);
 }
 
 void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
-                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& part,
-                    hipStream_t s) {
+                    const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s) {
   if (lo)
-    launchSigmaT<float, true>(rowPtr, cols, vals, lo, diag, n, x, y, k, ld, part, s);
+    launchSigmaT<float, true>(rowPtr, cols, vals, lo, diag, n, x, y, k, ld, s);
   else
-    launchSigmaT<float, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, part, s);
+    launchSigmaT<float, false>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld, s);
 }
 
 __global__ void kSplitHiLo(const double* v, float* hi, float* lo, int64_t n) {

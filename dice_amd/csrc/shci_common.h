@@ -12,9 +12,6 @@
 #include <stdint.h>
 
 #define SHCI_MAX_RESTRICT 4
-// Column blocks of the stored Hamiltonian (workgroup b of the sigma kernel
-// serves block b % SHCI_COL_BLOCKS).
-#define SHCI_COL_BLOCKS 1
 #define SHCI_MAX_NSO 126  // 2 words, 63 spatial orbitals
 
 // Read-only system tables resident in HBM for the whole run.

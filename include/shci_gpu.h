@@ -109,7 +109,8 @@ shci_status shci_get_hamiltonian_rows(shci_handle* h, int64_t row_begin, int64_t
  * (Hmult2::operator(), SHCI/Hmult.h:52-150). */
 shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k);
 /* Device-pointer variant on the handle's stream (stream = NULL) for callers
- * that keep vectors resident; returns after enqueueing. */
+ * that keep vectors resident; returns after enqueueing.  ld >= n; d_y must
+ * not overlap d_x (SHCI_BAD_INPUT). */
 shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld);
 /* Davidson with the reference's semantics (SHCI/Davidson.cpp:111-348):
  * x0 host n x nroots column-major guess in, eigenvectors out; energies out

@@ -123,6 +123,40 @@ def test_hamiltonian_rows_and_sigma(name, n, systems):
     assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
+# (system, precision, screened sources, eps1 of the two rounds): a few
+# thousand to a few tens of thousands of new determinants per round
+@pytest.mark.parametrize("name,prec,nsrc,eps", [("h8", 0, 60, (3e-3, 1.5e-3)), ("w2", 0, 5, (1e-1, 5e-2)),
+                                                ("hub", 0, 60, (3e-3, 1.5e-3)), ("c0", 1, 20, (1e-2, 5e-3))])
+def test_incremental_hamiltonian(name, prec, nsrc, eps, systems):
+    """Two rounds of screen -> append -> build extend the Hamiltonian in place
+    (new rows, and the new columns of old rows); the rows equal those of a
+    build from scratch over the same determinants, element for element."""
+    path = systems[name][0]
+    dets = _space(name, systems, 800, 1)
+    g = _gpu(path)
+    g.set_precision(prec)
+    g.set_dets(dets)
+    g.make_hamiltonian()
+    rng = np.random.default_rng(8)
+    for e1 in eps:
+        cmax = np.zeros(g.num_dets())
+        cmax[:nsrc] = np.abs(rng.normal(0, 0.3, nsrc)) + 1e-3
+        new = g.get_determinants_variational(cmax, e1)
+        assert len(new) > 0
+        g.append_new()
+        rp, cols, vals = g.hamiltonian_rows()
+        f = _gpu(path)
+        f.set_precision(prec)
+        f.set_dets(g.dets())
+        frp, fcols, fvals = f.hamiltonian_rows()
+        np.testing.assert_array_equal(rp, frp)
+        np.testing.assert_array_equal(cols, fcols)
+        np.testing.assert_array_equal(vals, fvals)
+        x = rng.normal(size=g.num_dets())
+        y, yf = g.hmult(x), f.hmult(x)
+        assert np.max(np.abs(y - yf)) <= 1e-12 * np.max(np.abs(yf))
+
+
 @pytest.mark.parametrize("name,nroots", [("c0", 1), ("hub", 2)])
 def test_davidson_energies(name, nroots, systems):
     path = systems[name][0]
