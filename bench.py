@@ -11,10 +11,14 @@ merge, Hamiltonian build, Davidson) and the final tight Davidson, with the
 integrals resident on the GPU.  `value` is the mean wall time of one
 variational iteration over the timed steps.
 
-The roofline object is the sigma = H x kernel (the Davidson SpMV), timed with
-HIP events on the library's stream over the timed steps; achieved bandwidth
-uses algorithmic bytes per launch: stored values + column indices, row
-offsets, diagonal, x read once and y written once.
+The roofline object is the sigma = H x kernel (the Davidson SpMV over the
+alpha-major relabelled CSR, kSigmaPerm, with its x permutation kGatherPerm),
+timed with HIP events on the library's stream over the timed steps; achieved
+bandwidth uses algorithmic bytes per launch: stored values + column indices,
+row offsets, diagonal, x read once and y written once.  `traffic` is the HBM
+bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+this benchmark (scaled by a copy kernel of known size), read from
+profiles/sigma_traffic_c1_<precision>.json when that summary exists.
 
 cpu_baseline: the CPU oracle (a serial restatement of the reference
 algorithm) on a bounded sample of the same workload: the first variational
@@ -35,6 +39,15 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 EPS1 = 5e-3
 NITER = 6
+
+
+def traffic_bytes(precision):
+    """HBM bytes per sigma launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "sigma_traffic_c1_%s.json" % precision)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["traffic_bytes_per_dispatch"], 0)
 
 
 def workload_inputs(directory):
@@ -76,8 +89,10 @@ def run_form(shci, full, log, prec, steps, warmup):
         R["per_iter"] = [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"],
                           "screen_s": round(p[0], 4), "hamiltonian_s": round(p[1], 4), "davidson_s": round(p[2], 4)}
                          for it, p in zip(its, ph)]
-        print("# %s step %d: %.3f s (wall %.3f s), E = %.10f" % (tag, k + 1, R["step_times"][-1], wall,
-                                                                  R["energies"][-1]), file=sys.stderr, flush=True)
+        print("# %s step %d: %.3f s (wall %.3f s), E = %.10f; per iteration (screen, H, Davidson s / iters): %s"
+              % (tag, k + 1, R["step_times"][-1], wall, R["energies"][-1],
+                 " ".join("%.3f,%.3f,%.3f/%d" % (p[0], p[1], p[2], it["davidson"]) for it, p in zip(its, ph))),
+              file=sys.stderr, flush=True)
     return R
 
 
@@ -88,6 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-forms", action="store_true", help="time only the --precision form")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the variational step on a single MI355X "
@@ -100,7 +116,8 @@ def main():
     log = os.path.join(work, "gpu.log")
     primary = shci.PREC_FP32 if args.precision == "fp32" else shci.PREC_FP64
     results = {}
-    for prec in ([primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]):
+    forms = [primary] if args.no_forms else [primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]
+    for prec in forms:
         results[prec] = run_form(shci, full, log, prec, args.steps, args.warmup)
     prec = primary
     R = results[prec]
@@ -130,12 +147,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "kSigma (sigma = H x, CSR SpMV)",
+            "kernel": "kSigmaPerm (sigma = H x over the alpha-major relabelled CSR)",
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_bytes(args.precision),
             "launches": calls,
             "avg_launch_ms": round(ms / calls, 4) if calls else None,
             "bytes_per_launch": nbytes / calls if calls else None,
@@ -160,6 +177,7 @@ def main():
                 "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(mixed["step_times"]),
                                          3),
                 "sigma_GBps": round(mixed["nbytes"] / (mixed["ms"] * 1e-3) / 1e9, 1) if mixed["ms"] else None,
+                "iterations": mixed["per_iter"],
             }
         }
 

@@ -334,6 +334,7 @@ shci_status shci_set_precision(shci_handle* h, int precision) {
     return fail(h, SHCI_BAD_INPUT, "unknown precision");
   h->eng.precision = precision;
   h->eng.hamRows = -1;
+  h->eng.permRows = -1;
   return SHCI_OK;
 }
 

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <cstdio>
+#include <ctime>
 
 namespace shci {
 
@@ -60,6 +62,9 @@ void Engine::load(const std::string& fcidump) {
     S.hbStart[k] = dHbStart[k].as<int64_t>();
   }
   S.maxEntryAbs = std::fabs(hs.maxEntry);
+  // SHCI_SIGMA_LAYOUT=external keeps sigma on H in determinant order
+  const char* layout = std::getenv("SHCI_SIGMA_LAYOUT");
+  relabel = !(layout && std::strcmp(layout, "external") == 0);
   uploadRestrictions();
   counter.reserve(64);
   HIP_CHECK(hipStreamSynchronize(stream));
@@ -99,6 +104,7 @@ void Engine::setDets(const int64_t* h, int64_t cnt) {
   ensureTable(std::max<int64_t>(n, 512));
   nDiag = 0;
   hamRows = -1;
+  permRows = -1;
   nNew = 0;
   HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -164,6 +170,12 @@ void Engine::appendNew() {
   if (slots == oldSlots) hashInsert(S, dets.as<uint64_t>(), old, n, table.as<int32_t>(), slots, stream);
 }
 
+static double wallSeconds() {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
 static void swapMem(DevMem& a, DevMem& b) {
   std::swap(a.p, b.p);
   std::swap(a.bytes, b.bytes);
@@ -180,7 +192,18 @@ void Engine::buildHamiltonian() {
   nDiag = n;
   if (hamRows == n) return;
   const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
+  // SHCI_TIMING: print the sub-phase wall times of this build to stderr
+  const bool timing = std::getenv("SHCI_TIMING") != nullptr;
+  double t0 = timing ? wallSeconds() : 0.0;
+  auto mark = [&](const char* what) {
+    if (!timing) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    const double t = wallSeconds();
+    std::fprintf(stderr, "  build %-12s %8.2f ms\n", what, 1e3 * (t - t0));
+    t0 = t;
+  };
   buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
+  mark("strings");
   rowPtrNew.reserve((size_t)(n + 1) * 8);
   int64_t* rp = rowPtrNew.as<int64_t>();
   const int64_t* orp = rowPtr.as<int64_t>();
@@ -191,11 +214,17 @@ void Engine::buildHamiltonian() {
   int64_t nnzNew = 0;
   HIP_CHECK(hipMemcpyAsync(&nnzNew, rp + n, 8, hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
+  mark("count");
+  const bool f32 = precision == 1;
+  // FP64: cols + values; FP32 form: cols + hi + lo (the FP32 remainder lets
+  // the refinement sigma reproduce the FP64 values)
+  const size_t perEntry = 12;
   {
     size_t freeB = 0, totalB = 0;
     HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-    size_t need = (size_t)nnzNew * (4 + 8 + (precision == 1 ? 8 : 0));
-    if (need > freeB + colsNew.bytes + valsNew.bytes + vals32New.bytes + valsLoNew.bytes)
+    const size_t need = (size_t)nnzNew * perEntry;
+    const size_t reusable = colsNew.bytes + valsNew.bytes + vals32New.bytes + valsLoNew.bytes;
+    if (need > freeB + reusable)
       throw HipError("out of memory: the Hamiltonian of " + std::to_string(n) + " determinants has " +
                      std::to_string(nnzNew) + " off-diagonal elements (" + std::to_string(need >> 30) +
                      " GiB needed beside the previous one), " + std::to_string(freeB >> 30) + " GiB free");
@@ -203,41 +232,88 @@ void Engine::buildHamiltonian() {
   fillStart.reserve((size_t)std::max<int64_t>(n, 1) * 8);
   launchFillStart(orp, nOld, n, rp, fillStart.as<int64_t>(), stream);
   colsNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
-  valsNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 8);
-  if (precision == 1) {
-    // mixed precision: FP32 values for the Davidson iterations plus the FP32
-    // remainder, so the refinement sigma reproduces the FP64 values; FP64
-    // values exist only transiently for the new connections
+  if (f32) {
     vals32New.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
     valsLoNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 4);
+    mark("reserve");
     launchCopyOldRowsF32(orp, cols.as<int32_t>(), vals32.as<float>(), valsLo.as<float>(), nOld, rp,
                          colsNew.as<int32_t>(), vals32New.as<float>(), valsLoNew.as<float>(), stream);
-    launchCsrFill(S, dets.as<uint64_t>(), T, nOld, fillStart.as<int64_t>(), colsNew.as<int32_t>(),
-                  valsNew.as<double>(), stream);
-    launchSplitNewRows(fillStart.as<int64_t>(), rp, n, valsNew.as<double>(), vals32New.as<float>(),
-                       valsLoNew.as<float>(), stream);
-    HIP_CHECK(hipStreamSynchronize(stream));
+    mark("copy old");
+    launchCsrFillSplit(S, dets.as<uint64_t>(), T, nOld, fillStart.as<int64_t>(), colsNew.as<int32_t>(),
+                       vals32New.as<float>(), valsLoNew.as<float>(), stream);
+    mark("fill");
     swapMem(vals32, vals32New);
     swapMem(valsLo, valsLoNew);
-    vals32New.release();
-    valsLoNew.release();
-    valsNew.release();
     vals.release();
+    valsNew.release();
   } else {
+    valsNew.reserve((size_t)std::max<int64_t>(nnzNew, 1) * 8);
+    mark("reserve");
     launchCopyOldRowsF64(orp, cols.as<int32_t>(), vals.as<double>(), nOld, rp, colsNew.as<int32_t>(),
                          valsNew.as<double>(), stream);
+    mark("copy old");
     launchCsrFill(S, dets.as<uint64_t>(), T, nOld, fillStart.as<int64_t>(), colsNew.as<int32_t>(),
                   valsNew.as<double>(), stream);
-    HIP_CHECK(hipStreamSynchronize(stream));
+    mark("fill");
     swapMem(vals, valsNew);
-    valsNew.release();
+    vals32.release();
+    valsLo.release();
+    vals32New.release();
+    valsLoNew.release();
   }
   swapMem(rowPtr, rowPtrNew);
   swapMem(cols, colsNew);
-  colsNew.release();
   nnzOff = nnzNew;
   hamRows = n;
+  // the previous H's buffers are kept for the next build unless memory is short
+  {
+    size_t freeB = 0, totalB = 0;
+    HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    if (freeB < totalB / 4) {
+      colsNew.release();
+      valsNew.release();
+      vals32New.release();
+      valsLoNew.release();
+    }
+  }
+  buildPermuted();
+  mark("relabel");
   HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// Alpha-major relabelled copy of H (FP64 values, or the FP32 hi part) for
+// the sigma kernel; skipped when it does not fit beside H.
+void Engine::buildPermuted() {
+  permRows = -1;
+  if (!relabel || n == 0) return;
+  const bool f32 = precision == 1;
+  const size_t need = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4;
+  size_t freeB = 0, totalB = 0;
+  HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  if (need + (size_t)n * 64 > freeB + permCols.bytes + permVals.bytes) {
+    permCols.release();
+    permVals.release();
+    return;
+  }
+  const uint32_t* perm = T.aMajorDet.as<uint32_t>();
+  permPos.reserve((size_t)n * 4);
+  launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
+  permRp.reserve((size_t)(n + 1) * 8);
+  int64_t* rp = permRp.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+  launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
+  exclusiveScanI64(rp, n + 1, ws, stream);
+  permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+  if (f32) {
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(), n,
+                      rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
+  } else {
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
+    launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(), n,
+                      rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
+  }
+  permRows = n;
 }
 
 void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) {
@@ -257,7 +333,14 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
                 2.0 * k * n * 8;
     HIP_CHECK(hipEventRecord(ev->start, stream));
   }
-  if (f32)
+  const bool perm = permRows == n && !(f32 && exact);
+  if (perm && f32)
+    launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
+                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, stream);
+  else if (perm)
+    launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
+                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, stream);
+  else if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
                    diag.as<double>(), n, dx, dy, k, ld, stream);
   else

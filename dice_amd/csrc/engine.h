@@ -52,6 +52,11 @@ struct Engine {
   DevMem rowPtr, cols, vals, vals32, valsLo;  // full symmetric off-diagonal CSR
   DevMem rowPtrNew, colsNew, valsNew, vals32New, valsLoNew, fillStart;  // incremental rebuild
   bool incremental = true;
+  // alpha-major relabelled copy of H used by sigma (rows/columns permuted by
+  // T.aMajorDet); valid when permRows == n
+  bool relabel = true;
+  DevMem permPos, permRp, permCols, permVals, permX;
+  int64_t permRows = -1;
   int64_t nnzOff = 0, hamRows = -1;
 
   // Davidson scratch
@@ -86,6 +91,7 @@ struct Engine {
   int64_t screen(const double* dCmax, double eps1);  // returns nNew
   void appendNew();
   void buildHamiltonian();
+  void buildPermuted();
   void sigma(const double* dx, double* dy, int k, int64_t ld, bool exact);
   std::vector<double> davidson(double* dX0, int nr, int maxCopies, double tol, int& numIter, bool exactSigma);
   int64_t lowerNnz() const { return nnzOff / 2 + n; }

@@ -55,9 +55,12 @@ void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables
                     int64_t* rowCount, hipStream_t s);
 void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
                    const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s);
+// fill with each value stored as FP32 hi = fp32(v) and lo = fp32(v - hi)
+void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
+                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s);
 // incremental rebuild helpers: cnt[I] += old row length (I < nOld);
 // fs[I] = rowPtr[I] + old row length; old rows copied to the front of the
-// new rows; FP64 values of the new connections split into FP32 hi + lo
+// new rows
 void launchNewRowLengths(const int64_t* oldRowPtr, int64_t nOld, int64_t n, int64_t* cnt, hipStream_t s);
 void launchFillStart(const int64_t* oldRowPtr, int64_t nOld, int64_t n, const int64_t* rowPtr, int64_t* fs,
                      hipStream_t s);
@@ -65,8 +68,6 @@ void launchCopyOldRowsF64(const int64_t* oldRowPtr, const int32_t* oldCols, cons
                           const int64_t* rowPtr, int32_t* cols, double* vals, hipStream_t s);
 void launchCopyOldRowsF32(const int64_t* oldRowPtr, const int32_t* oldCols, const float* oldHi, const float* oldLo,
                           int64_t nOld, const int64_t* rowPtr, int32_t* cols, float* hi, float* lo, hipStream_t s);
-void launchSplitNewRows(const int64_t* fillStart, const int64_t* rowPtr, int64_t n, const double* v, float* hi,
-                        float* lo, hipStream_t s);
 void launchCastValues(const double* v, float* out, int64_t n, hipStream_t s);
 // sort the columns of every row (ascending), values carried along
 void launchSortRows(const int64_t* rowPtr, int64_t n, int32_t* cols, double* vals, hipStream_t s);
@@ -81,6 +82,29 @@ void launchSigmaF32(const int64_t* rowPtr, const int32_t* cols, const float* val
                     const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // hi = fp32(v), lo = fp32(v - hi)
 void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream_t s);
+
+// --- alpha-major relabelled copy of H for sigma -----------------------------------------
+// perm[m] = determinant at alpha-major position m (StringTables::aMajorDet);
+// pos = inverse of perm
+void launchInversePerm(const uint32_t* perm, int64_t n, uint32_t* pos, hipStream_t s);
+// len[m] = length of row perm[m]
+void launchPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int64_t n, int64_t* len, hipStream_t s);
+// row m of the copy = row perm[m] with columns pos[J]
+void launchPermRowsF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const uint32_t* perm,
+                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, double* valsP,
+                       hipStream_t s);
+void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const uint32_t* perm,
+                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, float* valsP,
+                       hipStream_t s);
+// y = H x through the copy; x, y in determinant order (column-major, ld),
+// y must not overlap x; xP is scratch for x in alpha-major order (FP64, or
+// FP32 for the FP32-value form)
+void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
+                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
+                        hipStream_t s);
+void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
+                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
+                        hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -400,7 +400,8 @@ struct RowTables {
 
 // Collects the connections of row I.  Count pass: per-lane counters; fill
 // pass: wave-uniform cursor, positions by ballot (deterministic order).
-template <int W, bool FILL>
+// SPLIT stores each FP64 value as FP32 hi = fp32(v) and lo = fp32(v - hi).
+template <int W, bool FILL, bool SPLIT>
 struct RowSink {
   const DevSystem* S;
   const uint64_t* dets;
@@ -408,6 +409,8 @@ struct RowSink {
   int64_t I;
   int32_t* cols;
   double* vals;
+  float* hi;
+  float* lo;
   int64_t pos;  // wave-uniform write cursor (fill)
   int cnt;      // per-lane count (count)
   __device__ __forceinline__ void hit(bool h, uint32_t J) {
@@ -423,7 +426,13 @@ struct RowSink {
       for (int w = 0; w < W; w++) dJ[w] = dets[(int64_t)J * W + w];
       double v = ((int64_t)J < I) ? hijElement<W>(*S, dJ, dI) : hijElement<W>(*S, dI, dJ);
       cols[p] = (int32_t)J;
-      vals[p] = v;
+      if (SPLIT) {
+        const float h = (float)v;
+        hi[p] = h;
+        lo[p] = (float)(v - (double)h);
+      } else {
+        vals[p] = v;
+      }
     }
     pos += __popcll(m);
   }
@@ -444,10 +453,10 @@ __device__ __forceinline__ int64_t firstDetAtLeast(const uint32_t* a, int64_t lo
 
 // Rows I < nOld enumerate only connections to J >= nOld (the columns the
 // previous Hamiltonian lacks); rows I >= nOld enumerate all connections.
-template <int W, bool FILL>
+template <int W, bool FILL, bool SPLIT>
 __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t nOld,
                                                    int64_t* rowCount, const int64_t* fillStart, int32_t* cols,
-                                                   double* vals) {
+                                                   double* vals, float* hi, float* lo) {
   __shared__ int sExcl[kWaves][65];
   __shared__ int64_t sBeg[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
@@ -458,13 +467,15 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     const uint64_t astr = T.alpha[I], bstr = T.beta[I];
     const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
     const uint32_t jmin = I >= nOld ? 0u : (uint32_t)nOld;
-    RowSink<W, FILL> sink;
+    RowSink<W, FILL, SPLIT> sink;
     sink.S = &S;
     sink.dets = dets;
     sink.dI = dI;
     sink.I = I;
     sink.cols = cols;
     sink.vals = vals;
+    sink.hi = hi;
+    sink.lo = lo;
     sink.pos = FILL ? fillStart[I] : 0;
     sink.cnt = 0;
     // (a) same beta string: alpha singles and alpha-alpha doubles
@@ -582,20 +593,6 @@ __global__ void __launch_bounds__(kBlock) kCopyOldRows(const int64_t* oldRowPtr,
   }
 }
 
-// split the FP64 values of the new connections into FP32 hi + lo
-__global__ void __launch_bounds__(kBlock) kSplitNewRows(const int64_t* fs, const int64_t* rowPtr, int64_t n,
-                                                        const double* v, float* hi, float* lo) {
-  const int wave = threadIdx.x >> 6, lane = __lane_id();
-  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
-    for (int64_t p = fs[I] + lane; p < rowPtr[I + 1]; p += 64) {
-      const double d = v[p];
-      const float h = (float)d;
-      hi[p] = h;
-      lo[p] = (float)(d - (double)h);
-    }
-  }
-}
-
 __global__ void kCast(const double* v, float* out, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = (float)v[i];
@@ -667,6 +664,97 @@ __global__ void __launch_bounds__(kBlock) kSigma(const int64_t* __restrict__ row
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
       if (lane == 0) y[r * ld + I] = fma(d, xr[I], sum);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sigma over the alpha-major relabelled copy of H.  Row m of the copy is
+// determinant perm[m] (members of one alpha string are consecutive, in
+// increasing determinant index) and its columns are alpha-major positions,
+// so the x values a row gathers (same-alpha group members, members of the
+// groups of alpha strings one excitation away) sit in a few contiguous
+// ranges, and consecutive rows gather from the same ranges.  A workgroup
+// takes kPermChunk consecutive rows so that reuse stays within one CU.
+
+constexpr int kPermChunk = 16;
+
+__global__ void kInversePerm(const uint32_t* perm, int64_t n, uint32_t* pos) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x)
+    pos[perm[m]] = (uint32_t)m;
+}
+
+__global__ void kPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int64_t n, int64_t* len) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t I = perm[m];
+    len[m] = rowPtr[I + 1] - rowPtr[I];
+  }
+}
+
+// row m of the copy = row perm[m] of H with columns mapped through pos
+template <typename T1>
+__global__ void __launch_bounds__(kBlock) kPermRows(const int64_t* rowPtr, const int32_t* cols, const T1* vals,
+                                                    const uint32_t* perm, const uint32_t* pos, int64_t n,
+                                                    const int64_t* rpP, int32_t* colsP, T1* valsP) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t m = (int64_t)blockIdx.x * kWaves + wave; m < n; m += (int64_t)gridDim.x * kWaves) {
+    const int64_t I = perm[m];
+    const int64_t b = rowPtr[I], e = rowPtr[I + 1], d = rpP[m];
+    for (int64_t p = b + lane; p < e; p += 64) {
+      colsP[d + (p - b)] = (int32_t)pos[__builtin_nontemporal_load(cols + p)];
+      valsP[d + (p - b)] = __builtin_nontemporal_load(vals + p);
+    }
+  }
+}
+
+// xP[r * n + m] = x[r * ld + perm[m]]
+template <typename XT>
+__global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, int64_t n, XT* xP) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t I = perm[m];
+    for (int r = 0; r < k; r++) xP[r * n + m] = (XT)x[r * ld + I];
+  }
+}
+
+// y[r*ld + perm[m]] = diag .* x + sum over row m of the copy (gathers from xP)
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                     const VT* __restrict__ vals, const double* __restrict__ diag,
+                                                     const uint32_t* __restrict__ perm, int64_t n,
+                                                     const XT* __restrict__ xP, const double* __restrict__ x,
+                                                     double* __restrict__ y, int k, int64_t ld) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  constexpr int U = 8;
+  const int64_t r0 = (int64_t)blockIdx.x * kPermChunk;
+  const int64_t r1 = r0 + kPermChunk < n ? r0 + kPermChunk : n;
+  for (int64_t m = r0 + wave; m < r1; m += kWaves) {
+    const int64_t b = rp[m], e = rp[m + 1];
+    const int64_t I = perm[m];
+    const double d = diag[I];
+    for (int r = 0; r < k; r++) {
+      const XT* xr = xP + r * n;
+      double acc[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) acc[u] = 0.0;
+      int64_t p = b + lane;
+      for (; p + 64 * (U - 1) < e; p += 64 * U) {
+        int32_t c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
+          v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[u] = fma(v[u], (double)xr[c[u]], acc[u]);
+      }
+      for (; p < e; p += 64) acc[0] = fma((double)vals[p], (double)xr[cols[p]], acc[0]);
+      double sum = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; u++) sum += acc[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (lane == 0) y[r * ld + I] = fma(d, x[r * ld + I], sum);
     }
   }
 }
@@ -1060,7 +1148,8 @@ void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr, nullptr, nullptr));
+  DISPATCH_W(S.W, kCsrRows<WW, false, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr, nullptr,
+                                                                    nullptr, nullptr, nullptr));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1069,7 +1158,18 @@ void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables&
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, vals));
+  DISPATCH_W(S.W, kCsrRows<WW, true, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, vals,
+                                                                   nullptr, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
+                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s) {
+  if (T.n <= 0) return;
+  unsigned g = gridFor(T.n, kWaves, 1 << 16);
+  RowTables r = rowTables(T);
+  DISPATCH_W(S.W, kCsrRows<WW, true, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, nullptr,
+                                                                  hi, lo));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1098,13 +1198,6 @@ void launchCopyOldRowsF32(const int64_t* oldRowPtr, const int32_t* oldCols, cons
   if (nOld <= 0) return;
   kCopyOldRows<float><<<gridFor(nOld, kWaves, 1 << 16), kBlock, 0, s>>>(oldRowPtr, oldCols, oldHi, oldLo, nOld, rowPtr,
                                                                          cols, hi, lo);
-  HIP_CHECK(hipGetLastError());
-}
-
-void launchSplitNewRows(const int64_t* fillStart, const int64_t* rowPtr, int64_t n, const double* v, float* hi,
-                        float* lo, hipStream_t s) {
-  if (n <= 0) return;
-  kSplitNewRows<<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(fillStart, rowPtr, n, v, hi, lo);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1155,6 +1248,61 @@ void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream
   if (n <= 0) return;
   kSplitHiLo<<<gridFor(n), kBlock, 0, s>>>(v, hi, lo, n);
   HIP_CHECK(hipGetLastError());
+}
+
+void launchInversePerm(const uint32_t* perm, int64_t n, uint32_t* pos, hipStream_t s) {
+  if (n <= 0) return;
+  kInversePerm<<<gridFor(n), kBlock, 0, s>>>(perm, n, pos);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int64_t n, int64_t* len, hipStream_t s) {
+  if (n <= 0) return;
+  kPermRowLengths<<<gridFor(n), kBlock, 0, s>>>(rowPtr, perm, n, len);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPermRowsF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const uint32_t* perm,
+                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, double* valsP,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  kPermRows<double><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, n, rpP, colsP,
+                                                                     valsP);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const uint32_t* perm,
+                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, float* valsP,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  kPermRows<float><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, n, rpP, colsP,
+                                                                    valsP);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename VT, typename XT>
+static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
+                             const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld,
+                             DevMem& xP, hipStream_t s) {
+  if (n <= 0) return;
+  xP.reserve((size_t)k * n * sizeof(XT));
+  kGatherPerm<XT><<<gridFor(n), kBlock, 0, s>>>(x, k, ld, perm, n, xP.as<XT>());
+  const int64_t blocks = (n + kPermChunk - 1) / kPermChunk;
+  if (blocks > 0x7fffffffLL) throw HipError("too many rows for the relabelled sigma");
+  kSigmaPerm<VT, XT><<<(unsigned)blocks, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), x, y, k, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
+                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
+                        hipStream_t s) {
+  launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, s);
+}
+
+void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
+                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
+                        hipStream_t s) {
+  launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, s);
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,

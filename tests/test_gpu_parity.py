@@ -157,6 +157,27 @@ def test_incremental_hamiltonian(name, prec, nsrc, eps, systems):
         assert np.max(np.abs(y - yf)) <= 1e-12 * np.max(np.abs(yf))
 
 
+@pytest.mark.parametrize("name,prec", [("c0", 0), ("hub", 0), ("w2", 0), ("c0", 1)])
+def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
+    """Sigma through the alpha-major relabelled copy of H (the default) equals
+    sigma over H in determinant order: to 1e-12 relative with FP64 values;
+    with FP32 values the relabelled path also rounds x to FP32 (1e-6)."""
+    path = systems[name][0]
+    dets = _space(name, systems, 3000, 1)
+    x = np.random.default_rng(5).normal(size=(len(dets), 2))
+    out = []
+    for layout in ("relabelled", "external"):
+        if layout == "external":
+            monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "external")
+        g = _gpu(path)
+        g.set_precision(prec)
+        g.set_dets(dets)
+        g.make_hamiltonian()
+        out.append(g.hmult(x))
+    tol = 1e-12 if prec == 0 else 1e-6
+    assert np.max(np.abs(out[0] - out[1])) <= tol * np.max(np.abs(out[1]))
+
+
 @pytest.mark.parametrize("name,nroots", [("c0", 1), ("hub", 2)])
 def test_davidson_energies(name, nroots, systems):
     path = systems[name][0]
