@@ -83,12 +83,24 @@ double parityDiceToAlphaBeta(const int64_t* d, int nso) {  // SHCI/Determinants.
   return par;
 }
 
+void applyMethodOptions(Engine& E, const shci_run_options& opt) {
+  E.profileSigma = opt.profile_sigma != 0;
+  E.davidsonMode = opt.davidson_block ? 1 : 0;
+  E.autotune = opt.autotune != 0;
+  E.useGraphs = opt.hip_graphs != 0;
+  if (!E.autotune) {
+    E.sigmaCfg[0] = SigmaConfig{};
+    E.sigmaCfg[1] = SigmaConfig{};
+    E.tunedRows = 0;
+  }
+}
+
 // The variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446 and
 // SHCIbasics::DoVariational, SHCI/SHCIbasics.cpp:783-1378) on the device.
 shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, const shci_run_options& opt) {
   Engine& E = h->eng;
   const int precision = opt.precision;
-  E.profileSigma = opt.profile_sigma != 0;
+  applyMethodOptions(E, opt);
   E.resetSigmaProfile();
   const double startofCalc = wallTime();
   HostSchedule schd;
@@ -212,7 +224,8 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     const double t2 = wallTime();
     const double prevE0 = iter == 0 ? -10.0 : E0[0];
     int numIter = 0;
-    E0 = E.davidson(X0.as<double>(), nroots, nroots + 4, schd.davidsonTolLoose, numIter, false);
+    E0 = E.davidsonMode == 1 ? E.davidsonBlock(X0.as<double>(), nroots, schd.davidsonTolLoose, numIter, false)
+                             : E.davidson(X0.as<double>(), nroots, nroots + 4, schd.davidsonTolLoose, numIter, false);
     if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
     const double t3 = wallTime();
     const double t = t3 - startofCalc;
@@ -242,7 +255,8 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
       if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
       const double tf0 = wallTime();
-      E0 = E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
+      E0 = E.davidsonMode == 1 ? E.davidsonBlock(E.ci.as<double>(), nroots, schd.davidsonTol, numIter, true)
+                               : E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
       E.tFinalDavidson = wallTime() - tf0;
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");
@@ -335,6 +349,16 @@ shci_status shci_set_precision(shci_handle* h, int precision) {
   h->eng.precision = precision;
   h->eng.hamRows = -1;
   h->eng.permRows = -1;
+  return SHCI_OK;
+}
+
+shci_status shci_set_options(shci_handle* h, const shci_run_options* options) {
+  if (!options) return fail(h, SHCI_BAD_INPUT, "options is NULL");
+  if (options->precision != h->eng.precision) {
+    shci_status st = shci_set_precision(h, options->precision);
+    if (st != SHCI_OK) return st;
+  }
+  applyMethodOptions(h->eng, *options);
   return SHCI_OK;
 }
 
@@ -540,7 +564,8 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
     dx.reserve(n * nroots * 8);
     HIP_CHECK(hipMemcpyAsync(dx.p, x0, n * nroots * 8, hipMemcpyHostToDevice, E.stream));
     int it = 0;
-    std::vector<double> e = E.davidson(dx.as<double>(), nroots, max_copies, tol, it, false);
+    std::vector<double> e = E.davidsonMode == 1 ? E.davidsonBlock(dx.as<double>(), nroots, tol, it, false)
+                                                : E.davidson(dx.as<double>(), nroots, max_copies, tol, it, false);
     if ((int)e.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
     HIP_CHECK(hipMemcpyAsync(x0, dx.p, n * nroots * 8, hipMemcpyDeviceToHost, E.stream));
     HIP_CHECK(hipStreamSynchronize(E.stream));

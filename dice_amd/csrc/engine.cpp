@@ -21,6 +21,8 @@ void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
 Engine::Engine() {}
 
 Engine::~Engine() {
+  clearGraphs();
+  if (bdHost) (void)hipHostFree(bdHost);
   for (SigmaEvent& e : sigmaEvents) {
     (void)hipEventDestroy(e.start);
     (void)hipEventDestroy(e.stop);
@@ -314,6 +316,262 @@ void Engine::buildPermuted() {
                       rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
   }
   permRows = n;
+  // re-tune when the space has doubled since the last tuning (small spaces
+  // keep the default shape: their sigma is too short to time reliably)
+  if (autotune && n >= 20000 && n >= 2 * tunedRows) tuneSigma();
+}
+
+// Times every relabelled-sigma launch shape (rows per work item 8 / 16 / 32,
+// static or persistent) on the current H, for one vector and, in the block
+// Davidson, for an SpMM of four, and keeps the fastest of each.
+void Engine::tuneSigma() {
+  const int ks[2] = {1, 4};
+  tuneX.reserve((size_t)n * 4 * 8);
+  tuneY.reserve((size_t)n * 4 * 8);
+  HIP_CHECK(hipMemsetAsync(tuneX.p, 0, (size_t)n * 4 * 8, stream));
+  hipEvent_t a, b;
+  HIP_CHECK(hipEventCreate(&a));
+  HIP_CHECK(hipEventCreate(&b));
+  const bool saveProfile = profileSigma;
+  profileSigma = false;
+  tuneLog.clear();
+  char line[160];
+  for (int which = 0; which < (davidsonMode == 1 ? 2 : 1); which++) {
+    const int k = ks[which];
+    SigmaConfig best = sigmaCfg[which];
+    float bestMs = 1e30f;
+    for (int chunk : {8, 16, 32})
+      for (bool pers : {false, true}) {
+        sigmaCfg[which].chunk = chunk;
+        sigmaCfg[which].persistent = pers;
+        sigma(tuneX.as<double>(), tuneY.as<double>(), k, n, false);
+        HIP_CHECK(hipEventRecord(a, stream));
+        for (int r = 0; r < 2; r++) sigma(tuneX.as<double>(), tuneY.as<double>(), k, n, false);
+        HIP_CHECK(hipEventRecord(b, stream));
+        HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        ms *= 0.5f;
+        std::snprintf(line, sizeof line, "k=%d chunk=%d %s %.3f ms; ", k, chunk, pers ? "persistent" : "static", ms);
+        tuneLog += line;
+        if (ms < bestMs) {
+          bestMs = ms;
+          best = sigmaCfg[which];
+        }
+      }
+    sigmaCfg[which] = best;
+    std::snprintf(line, sizeof line, "k=%d chosen chunk=%d %s. ", k, best.chunk, best.persistent ? "persistent" : "static");
+    tuneLog += line;
+  }
+  profileSigma = saveProfile;
+  HIP_CHECK(hipEventDestroy(a));
+  HIP_CHECK(hipEventDestroy(b));
+  tunedRows = n;
+  if (std::getenv("SHCI_TIMING")) std::fprintf(stderr, "# sigma autotune (n=%lld): %s\n", (long long)n, tuneLog.c_str());
+}
+
+void Engine::clearGraphs() {
+  for (auto& kv : graphs) {
+    (void)hipGraphExecDestroy(kv.second.exec);
+    (void)hipGraphDestroy(kv.second.graph);
+  }
+  graphs.clear();
+}
+
+// Block Davidson (the batching form).  Every iteration: Rayleigh-Ritz on the
+// subspace V (m <= maxM columns, S = H V), Ritz vectors and residuals of the
+// nr lowest roots; each root whose residual norm is >= tol contributes a
+// preconditioned correction (the reference's diagonal preconditioner,
+// Davidson.cpp:85-106); the corrections are orthogonalised against V (two
+// classical Gram-Schmidt passes) and among themselves, appended, and their
+// sigma is one SpMM.  When the subspace is full it restarts from the
+// 2 nr + 2 lowest Ritz vectors.  The iteration body after the host's choice
+// of corrections (SpMM, Gram matrix, eigensolve, Ritz vectors, residuals,
+// copy of Ritz values and residual norms to pinned host memory) is captured
+// once per (subspace size, block size) as a hipGraph and replayed.
+std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& numIter, bool exactSigma) {
+  const int64_t N = n;
+  const int maxM = (int)std::min<int64_t>(N, std::min(32, std::max(3 * nr, nr + 8)));
+  if (nr > 8 || nr > maxM) throw std::invalid_argument("block Davidson supports up to 8 roots");
+  hipStream_t s = stream;
+  if (graphN != N) {
+    clearGraphs();
+    graphN = N;
+  }
+  auto reserveKeep = [&](DevMem& m, size_t bytes) {
+    if (m.bytes < bytes) {
+      clearGraphs();  // captured graphs hold the old addresses
+      m.reserve(bytes);
+    }
+  };
+  for (int q = 0; q < 2; q++) {
+    reserveKeep(bdV[q], (size_t)N * maxM * 8);
+    reserveKeep(bdS[q], (size_t)N * maxM * 8);
+  }
+  reserveKeep(bdT, (size_t)N * nr * 8);
+  reserveKeep(bdX, (size_t)N * nr * 8);
+  reserveKeep(bdHX, (size_t)N * nr * 8);
+  reserveKeep(bdR, (size_t)N * nr * 8);
+  reserveKeep(bdSmall, (size_t)8 * 4096 * 8);
+  reserveKeep(dvPart, (size_t)16 * 256 * 64 * 8);  // largest Gram: 16 tiles x 256 blocks x 8 x 8
+  reserveKeep(dvScal, 64 * 8);
+  reserveKeep(permX, (size_t)N * 8 * (precision == 1 && !exactSigma ? 4 : 8));
+  reserveKeep(sigmaCounter, 256);
+  if (!bdHost) HIP_CHECK(hipHostMalloc((void**)&bdHost, 64 * 8, hipHostMallocDefault));
+  double* sm = bdSmall.as<double>();
+  double* G = sm;                 // m x m Gram matrix (upper triangle used)
+  double* A = sm + 4096;          // symmetrised
+  double* C = sm + 2 * 4096;      // eigenvectors (row-major, column j = vector j)
+  double* w = sm + 3 * 4096;      // eigenvalues
+  double* Cn = sm + 4 * 4096;     // first nr (or keep) columns of C
+  double* coef = sm + 5 * 4096;   // V^T T
+  double* nrm = sm + 6 * 4096;    // residual norms^2 (nr), then correction norms^2
+  double* sc = dvScal.as<double>();
+  int cur = 0;
+  double* V = bdV[0].as<double>();
+  double* S = bdS[0].as<double>();
+  double* Tm = bdT.as<double>();
+  double* X = bdX.as<double>();
+  double* HX = bdHX.as<double>();
+  double* R = bdR.as<double>();
+  auto col = [&](double* M, int c) { return M + (int64_t)c * N; };
+  auto sync = [&]() { HIP_CHECK(hipStreamSynchronize(s)); };
+  // orthonormalise T[:, 0:b] against V[:, 0:m] and among themselves; append
+  // the columns that keep a norm > 1e-10; returns how many were appended
+  auto appendBlock = [&](int m, int b) {
+    if (m > 0)
+      for (int pass = 0; pass < 2; pass++) {
+        launchGramTiled(V, m, Tm, b, N, N, coef, dvPart, s);
+        launchGemmTS(V, m, coef, b, N, N, Tm, N, -1.0, 1.0, s);
+      }
+    for (int j = 0; j < b; j++) {
+      for (int i = 0; i < j; i++) {
+        launchDot2(col(Tm, i), col(Tm, j), N, sc, dvPart, s);
+        launchOrthoStep(col(Tm, j), col(Tm, i), sc, N, s);
+      }
+      launchNorm2(col(Tm, j), N, nrm + j, dvPart, s);
+    }
+    std::vector<double> h(b);
+    HIP_CHECK(hipMemcpyAsync(h.data(), nrm, b * 8, hipMemcpyDeviceToHost, s));
+    sync();
+    int added = 0;
+    for (int j = 0; j < b; j++) {
+      if (!(std::sqrt(h[j]) > 1e-10)) continue;
+      launchScaleInto(col(V, m + added), col(Tm, j), nrm + j, N, s);
+      added++;
+    }
+    return added;
+  };
+  // sigma of the new columns, Rayleigh-Ritz, Ritz vectors and residuals;
+  // leaves Ritz values and residual norms^2 in bdHost[0:nr], bdHost[32:32+nr]
+  auto body = [&](int m0, int b) {
+    const int m = m0 + b;
+    sigma(col(V, m0), col(S, m0), b, N, exactSigma);
+    launchGramTiled(V, m, S, m, N, N, G, dvPart, s);
+    launchSymmetrizeUpper(G, m, A, s);
+    launchJacobi(A, m, w, C, s);
+    launchSubMatrix(C, m, m, nr, Cn, s);
+    launchGemmTS(V, m, Cn, nr, N, N, X, N, 1.0, 0.0, s);
+    launchGemmTS(S, m, Cn, nr, N, N, HX, N, 1.0, 0.0, s);
+    for (int j = 0; j < nr; j++) launchResidual(col(HX, j), col(X, j), w + j, N, col(R, j), nrm + j, dvPart, s);
+    HIP_CHECK(hipMemcpyAsync(bdHost, w, nr * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(bdHost + 32, nrm, nr * 8, hipMemcpyDeviceToHost, s));
+  };
+  auto runBody = [&](int m0, int b) {
+    blockMatvecs += b;
+    const bool graphOk = useGraphs && !profileSigma;  // per-launch events are not captured
+    if (!graphOk) {
+      body(m0, b);
+      sync();
+      return;
+    }
+    char key[96];
+    std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%d", m0, b, nr, cur, exactSigma ? 1 : 0, precision);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      GraphEntry e{};
+      HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      body(m0, b);
+      HIP_CHECK(hipStreamEndCapture(s, &e.graph));
+      HIP_CHECK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+      it = graphs.emplace(key, e).first;
+      graphCaptures++;
+    }
+    HIP_CHECK(hipGraphLaunch(it->second.exec, s));
+    graphReplays++;
+    sync();
+  };
+
+  // initial block: the given vectors (random where zero, as the reference's
+  // setRandom), orthonormalised
+  auto randomColumn = [&](double* x) {
+    std::vector<double> h(N);
+    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + 2.0 * double(std::rand()) / double(RAND_MAX);
+    HIP_CHECK(hipMemcpyAsync(x, h.data(), N * 8, hipMemcpyHostToDevice, s));
+    sync();
+  };
+  for (int i = 0; i < nr; i++) {
+    HIP_CHECK(hipMemcpyAsync(col(Tm, i), dX0 + (int64_t)i * N, N * 8, hipMemcpyDeviceToDevice, s));
+    launchNorm2(col(Tm, i), N, nrm + i, dvPart, s);
+  }
+  {
+    std::vector<double> h(nr);
+    HIP_CHECK(hipMemcpyAsync(h.data(), nrm, nr * 8, hipMemcpyDeviceToHost, s));
+    sync();
+    for (int i = 0; i < nr; i++)
+      if (!(std::sqrt(h[i]) > 1e-10)) randomColumn(col(Tm, i));
+  }
+  int m = 0;
+  {
+    const int added = appendBlock(0, nr);
+    if (added < nr) throw std::invalid_argument("block Davidson: linearly dependent initial vectors");
+    runBody(0, added);
+    m = added;
+  }
+  numIter = 0;
+  std::vector<double> theta(nr);
+  while (true) {
+    numIter++;
+    std::vector<int> unconv;
+    for (int j = 0; j < nr; j++) {
+      theta[j] = bdHost[j];
+      if (!(std::sqrt(std::max(bdHost[32 + j], 0.0)) < tol)) unconv.push_back(j);
+    }
+    if (unconv.empty() || m >= N || numIter > 800 * nr) {
+      for (int j = 0; j < nr; j++)
+        HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)j * N, col(X, j), N * 8, hipMemcpyDeviceToDevice, s));
+      sync();
+      return theta;
+    }
+    const int b = (int)unconv.size();
+    // corrections (before any restart: they use R and the Ritz values w)
+    for (int t = 0; t < b; t++) {
+      const int j = unconv[t];
+      HIP_CHECK(hipMemcpyAsync(col(Tm, t), col(R, j), N * 8, hipMemcpyDeviceToDevice, s));
+      launchPrecondition(col(Tm, t), diag.as<double>(), w + j, N, s);
+    }
+    if (m + b > maxM) {  // thick restart: keep the lowest Ritz vectors
+      const int keep = std::min(m, std::min(maxM - b, 2 * nr + 2));
+      launchSubMatrix(C, m, m, keep, Cn, s);
+      double* V2 = bdV[1 - cur].as<double>();
+      double* S2 = bdS[1 - cur].as<double>();
+      launchGemmTS(V, m, Cn, keep, N, N, V2, N, 1.0, 0.0, s);
+      launchGemmTS(S, m, Cn, keep, N, N, S2, N, 1.0, 0.0, s);
+      cur = 1 - cur;
+      V = V2;
+      S = S2;
+      m = keep;
+    }
+    const int added = appendBlock(m, b);
+    if (added == 0) {  // no new direction: the subspace is exhausted
+      for (int j = 0; j < nr; j++)
+        HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)j * N, col(X, j), N * 8, hipMemcpyDeviceToDevice, s));
+      sync();
+      return theta;
+    }
+    runBody(m, added);
+    m += added;
+  }
 }
 
 void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) {
@@ -334,12 +592,14 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     HIP_CHECK(hipEventRecord(ev->start, stream));
   }
   const bool perm = permRows == n && !(f32 && exact);
+  const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
+  if (perm) sigmaCounter.reserve(256);
   if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, stream);
+                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
   else if (perm)
     launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, stream);
+                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
   else if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
                    diag.as<double>(), n, dx, dy, k, ld, stream);

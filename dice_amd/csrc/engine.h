@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,29 @@ struct Engine {
 
   // Davidson scratch
   DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;
+
+  // batching form: block Davidson whose new block's sigma is one SpMM,
+  // iterations replayed from captured hipGraphs, sigma launch shape chosen
+  // by timing on the current H
+  int davidsonMode = 0;  // 0: the reference's single-vector Davidson, 1: block Davidson
+  bool autotune = false;
+  bool useGraphs = false;
+  SigmaConfig sigmaCfg[2];  // [0]: one vector, [1]: SpMM
+  int64_t tunedRows = 0;
+  std::string tuneLog;
+  DevMem sigmaCounter, tuneX, tuneY;
+  DevMem bdV[2], bdS[2], bdT, bdX, bdHX, bdR, bdSmall;
+  double* bdHost = nullptr;  // pinned: Ritz values and residual norms
+  struct GraphEntry {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::map<std::string, GraphEntry> graphs;
+  int64_t graphN = -1;
+  int64_t graphCaptures = 0, graphReplays = 0, blockMatvecs = 0;
+  void clearGraphs();
+  void tuneSigma();
+  std::vector<double> davidsonBlock(double* dX0, int nr, double tol, int& numIter, bool exactSigma);
 
   // last run
   std::vector<IterRecord> iters;

@@ -96,15 +96,24 @@ void launchPermRowsF64(const int64_t* rowPtr, const int32_t* cols, const double*
 void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const uint32_t* perm,
                        const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, float* valsP,
                        hipStream_t s);
+// Launch shape of the relabelled sigma: rows per work item (8, 16 or 32)
+// and static (one workgroup per work item) or persistent (workgroups pull
+// work items from an atomic counter).
+struct SigmaConfig {
+  int chunk = 16;
+  bool persistent = false;
+};
 // y = H x through the copy; x, y in determinant order (column-major, ld),
-// y must not overlap x; xP is scratch for x in alpha-major order (FP64, or
-// FP32 for the FP32-value form)
+// y must not overlap x; k > 1 runs as SpMM over up to 8 interleaved
+// vectors at a time.  xP is scratch for x in alpha-major order (FP64, or
+// FP32 for the FP32-value form); counter (device, one unsigned) is needed
+// by the persistent configuration.
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
                         const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        hipStream_t s);
+                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
                         const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        hipStream_t s);
+                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)
@@ -112,6 +121,15 @@ void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int
                 DevMem& partial, hipStream_t s);
 // X[:, :m] = X[:, :m] * V (V row-major m x m on device)
 void launchRotate(double* X, int m, int64_t n, int64_t ld, const double* V, hipStream_t s);
+// out[i*cols + j] = C[i*ldc + j] for i < rows, j < cols (small matrices)
+void launchSubMatrix(const double* C, int ldc, int rows, int cols, double* out, hipStream_t s);
+// G[i*mb + j] = A[:,i] . B[:,j] for ma, mb <= 64 (8 x 8 tiles; fixed-order sums)
+void launchGramTiled(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G,
+                     DevMem& partial, hipStream_t s);
+// out[:, j] = alpha * sum_l A[:, l] W[l*mb + j] + beta * out[:, j] for j < mb
+// (W row-major ma x mb on device, ma, mb <= 64); out must not overlap A
+void launchGemmTS(const double* A, int ma, const double* W, int mb, int64_t n, int64_t lda, double* out,
+                  int64_t ldo, double alpha, double beta, hipStream_t s);
 // r = s - e * b ; out[0] = ||r||^2
 void launchResidual(const double* sig, const double* b, const double* e, int64_t n, double* r, double* out,
                     DevMem& partial, hipStream_t s);

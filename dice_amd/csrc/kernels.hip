@@ -6,6 +6,7 @@
 //   sigma = H x               SHCI/Hmult.h:52-150
 //   Davidson vector algebra   SHCI/Davidson.cpp:85-348
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "kernels.h"
@@ -674,10 +675,8 @@ __global__ void __launch_bounds__(kBlock) kSigma(const int64_t* __restrict__ row
 // increasing determinant index) and its columns are alpha-major positions,
 // so the x values a row gathers (same-alpha group members, members of the
 // groups of alpha strings one excitation away) sit in a few contiguous
-// ranges, and consecutive rows gather from the same ranges.  A workgroup
-// takes kPermChunk consecutive rows so that reuse stays within one CU.
-
-constexpr int kPermChunk = 16;
+// ranges, and consecutive rows gather from the same ranges.  A work item is
+// a chunk of consecutive rows, so that reuse stays within one CU.
 
 __global__ void kInversePerm(const uint32_t* perm, int64_t n, uint32_t* pos) {
   for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x)
@@ -707,35 +706,124 @@ __global__ void __launch_bounds__(kBlock) kPermRows(const int64_t* rowPtr, const
   }
 }
 
-// xP[r * n + m] = x[r * ld + perm[m]]
-template <typename XT>
-__global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, int64_t n, XT* xP) {
+// x in alpha-major order for the sigma kernels: K == 1: xP[r * n + m] =
+// x[r * ld + perm[m]] for r < k; K > 1 (SpMM): interleaved, xP[m * K + r]
+// (zero for k <= r < K).  Also zeroes the work counter of a persistent sigma
+// launch that follows on the same stream.
+template <typename XT, int K>
+__global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, int64_t n, XT* xP,
+                            unsigned* counter) {
+  if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
   for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
     const int64_t I = perm[m];
-    for (int r = 0; r < k; r++) xP[r * n + m] = (XT)x[r * ld + I];
+    if (K == 1) {
+      for (int r = 0; r < k; r++) xP[r * n + m] = (XT)x[r * ld + I];
+    } else {
+#pragma unroll
+      for (int r = 0; r < K; r++) xP[m * K + r] = r < k ? (XT)x[r * ld + I] : (XT)0;
+    }
   }
 }
 
-// y[r*ld + perm[m]] = diag .* x + sum over row m of the copy (gathers from xP)
-template <typename VT, typename XT>
+// Row-chunk scheduler of the sigma kernels: CHUNK consecutive rows per work
+// item; static (work item = workgroup) or persistent (workgroups take work
+// items from an atomic counter until none is left, which evens out rows of
+// very different lengths).
+template <int CHUNK, bool PERSIST>
+struct ChunkSched {
+  int64_t nChunks;
+  unsigned* counter;
+  int64_t cur;
+  __device__ __forceinline__ bool next() {
+    if (!PERSIST) {
+      if (cur < 0) {
+        cur = blockIdx.x;
+        return cur < nChunks;
+      }
+      return false;
+    }
+    __shared__ unsigned sNext;
+    __syncthreads();
+    if (threadIdx.x == 0) sNext = atomicAdd(counter, 1u);
+    __syncthreads();
+    cur = sNext;
+    return cur < nChunks;
+  }
+};
+
+// y[r*ld + perm[m]] = diag .* x + sum over row m of the copy (gathers from
+// xP, one vector at a time)
+template <typename VT, typename XT, int CHUNK, bool PERSIST>
 __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
                                                      const VT* __restrict__ vals, const double* __restrict__ diag,
                                                      const uint32_t* __restrict__ perm, int64_t n,
                                                      const XT* __restrict__ xP, const double* __restrict__ x,
-                                                     double* __restrict__ y, int k, int64_t ld) {
+                                                     double* __restrict__ y, int k, int64_t ld, unsigned* counter) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   constexpr int U = 8;
-  const int64_t r0 = (int64_t)blockIdx.x * kPermChunk;
-  const int64_t r1 = r0 + kPermChunk < n ? r0 + kPermChunk : n;
-  for (int64_t m = r0 + wave; m < r1; m += kWaves) {
-    const int64_t b = rp[m], e = rp[m + 1];
-    const int64_t I = perm[m];
-    const double d = diag[I];
-    for (int r = 0; r < k; r++) {
-      const XT* xr = xP + r * n;
-      double acc[U];
+  ChunkSched<CHUNK, PERSIST> sched{(n + CHUNK - 1) / CHUNK, counter, -1};
+  while (sched.next()) {
+    const int64_t r0 = sched.cur * CHUNK;
+    const int64_t r1 = r0 + CHUNK < n ? r0 + CHUNK : n;
+    for (int64_t m = r0 + wave; m < r1; m += kWaves) {
+      const int64_t b = rp[m], e = rp[m + 1];
+      const int64_t I = perm[m];
+      const double d = diag[I];
+      for (int r = 0; r < k; r++) {
+        const XT* xr = xP + r * n;
+        double acc[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) acc[u] = 0.0;
+        for (int u = 0; u < U; u++) acc[u] = 0.0;
+        int64_t p = b + lane;
+        for (; p + 64 * (U - 1) < e; p += 64 * U) {
+          int32_t c[U];
+          double v[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
+            v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) acc[u] = fma(v[u], (double)xr[c[u]], acc[u]);
+        }
+        for (; p < e; p += 64) acc[0] = fma((double)vals[p], (double)xr[cols[p]], acc[0]);
+        double sum = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++) sum += acc[u];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if (lane == 0) y[r * ld + I] = fma(d, x[r * ld + I], sum);
+      }
+    }
+  }
+}
+
+// SpMM: K interleaved vectors per gather (one K*sizeof(XT)-byte load per
+// stored element serves all K products); rows read once.
+template <typename XT, int K>
+struct XVec {
+  typedef XT T __attribute__((ext_vector_type(K)));
+};
+
+template <typename VT, typename XT, int K, int CHUNK, bool PERSIST>
+__global__ void __launch_bounds__(kBlock) kSpmmPerm(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                                    const VT* __restrict__ vals, const double* __restrict__ diag,
+                                                    const uint32_t* __restrict__ perm, int64_t n,
+                                                    const XT* __restrict__ xP, const double* __restrict__ x,
+                                                    double* __restrict__ y, int k, int64_t ld, unsigned* counter) {
+  typedef typename XVec<XT, K>::T XV;
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  constexpr int U = K >= 8 ? 1 : 8 / K;
+  ChunkSched<CHUNK, PERSIST> sched{(n + CHUNK - 1) / CHUNK, counter, -1};
+  while (sched.next()) {
+    const int64_t r0 = sched.cur * CHUNK;
+    const int64_t r1 = r0 + CHUNK < n ? r0 + CHUNK : n;
+    for (int64_t m = r0 + wave; m < r1; m += kWaves) {
+      const int64_t b = rp[m], e = rp[m + 1];
+      const int64_t I = perm[m];
+      double acc[K];
+#pragma unroll
+      for (int r = 0; r < K; r++) acc[r] = 0.0;
       int64_t p = b + lane;
       for (; p + 64 * (U - 1) < e; p += 64 * U) {
         int32_t c[U];
@@ -746,15 +834,30 @@ __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__
           v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) acc[u] = fma(v[u], (double)xr[c[u]], acc[u]);
+        for (int u = 0; u < U; u++) {
+          const XV xv = *reinterpret_cast<const XV*>(xP + (int64_t)c[u] * K);
+#pragma unroll
+          for (int r = 0; r < K; r++) acc[r] = fma(v[u], (double)xv[r], acc[r]);
+        }
       }
-      for (; p < e; p += 64) acc[0] = fma((double)vals[p], (double)xr[cols[p]], acc[0]);
-      double sum = 0.0;
+      for (; p < e; p += 64) {
+        const XV xv = *reinterpret_cast<const XV*>(xP + (int64_t)cols[p] * K);
+        const double v = (double)vals[p];
 #pragma unroll
-      for (int u = 0; u < U; u++) sum += acc[u];
+        for (int r = 0; r < K; r++) acc[r] = fma(v, (double)xv[r], acc[r]);
+      }
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-      if (lane == 0) y[r * ld + I] = fma(d, x[r * ld + I], sum);
+      for (int r = 0; r < K; r++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[r] += __shfl_xor(acc[r], o, 64);
+      }
+      if (lane < k && lane < K) {
+        double sum = acc[0];
+#pragma unroll
+        for (int r = 1; r < K; r++)
+          if (lane == r) sum = acc[r];
+        y[lane * ld + I] = fma(diag[I], x[lane * ld + I], sum);
+      }
     }
   }
 }
@@ -831,6 +934,95 @@ __global__ void kRotate(double* X, int m, int64_t n, int64_t ld, const double* V
       xo[j] = s;
     }
     for (int j = 0; j < m; j++) X[j * ld + k] = xo[j];
+  }
+}
+
+// Gram tiles: blockIdx.y = tile (ta, tb) of 8 x 8 entries
+__global__ void __launch_bounds__(kBlock) kGramTile(const double* A, int ma, const double* B, int mb, int64_t n,
+                                                    int64_t ld, double* partial) {
+  constexpr int T = 8;
+  __shared__ double red[kWaves][T * T];
+  const int tilesB = (mb + T - 1) / T;
+  const int ta = blockIdx.y / tilesB, tb = blockIdx.y % tilesB;
+  const int na = min(T, ma - ta * T), nb = min(T, mb - tb * T);
+  const double* A0 = A + (int64_t)ta * T * ld;
+  const double* B0 = B + (int64_t)tb * T * ld;
+  double acc[T * T];
+#pragma unroll
+  for (int q = 0; q < T * T; q++) acc[q] = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double a[T], bb[T];
+#pragma unroll
+    for (int i = 0; i < T; i++) a[i] = i < na ? A0[i * ld + k] : 0.0;
+#pragma unroll
+    for (int j = 0; j < T; j++) bb[j] = j < nb ? B0[j * ld + k] : 0.0;
+#pragma unroll
+    for (int i = 0; i < T; i++)
+#pragma unroll
+      for (int j = 0; j < T; j++) acc[i * T + j] = fma(a[i], bb[j], acc[i * T + j]);
+  }
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+#pragma unroll
+  for (int q = 0; q < T * T; q++) {
+    double v = acc[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < T * T; q += blockDim.x) {
+    double v = 0.0;
+    for (int w = 0; w < kWaves; w++) v += red[w][q];
+    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * T * T + q] = v;
+  }
+}
+
+// one block per output entry (i, j): fixed-order sum over the tile's partials
+__global__ void __launch_bounds__(kBlock) kGramTileFinal(const double* partial, int nblocks, int ma, int mb,
+                                                         double* G) {
+  constexpr int T = 8;
+  __shared__ double red[kWaves];
+  const int i = blockIdx.x / mb, j = blockIdx.x % mb;
+  const int tilesB = (mb + T - 1) / T;
+  const int tile = (i / T) * tilesB + j / T;
+  const int q = (i % T) * T + (j % T);
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v += partial[((int64_t)tile * nblocks + b) * T * T + q];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kWaves; w++) t += red[w];
+    G[i * mb + j] = t;
+  }
+}
+
+// out = alpha * A W + beta * out, one row per thread, 8 output columns per pass
+__global__ void __launch_bounds__(kBlock) kGemmTS(const double* A, int ma, const double* W, int mb, int64_t n,
+                                                  int64_t lda, double* out, int64_t ldo, double alpha, double beta) {
+  __shared__ double sW[64 * 64];
+  for (int q = threadIdx.x; q < ma * mb; q += blockDim.x) sW[q] = W[q];
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    for (int j0 = 0; j0 < mb; j0 += 8) {
+      double acc[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; jj++) acc[jj] = 0.0;
+      for (int l = 0; l < ma; l++) {
+        const double a = A[l * lda + k];
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++)
+          if (j0 + jj < mb) acc[jj] = fma(a, sW[l * mb + j0 + jj], acc[jj]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 8; jj++)
+        if (j0 + jj < mb) {
+          double* o = out + (int64_t)(j0 + jj) * ldo + k;
+          *o = beta == 0.0 ? alpha * acc[jj] : fma(beta, *o, alpha * acc[jj]);
+        }
+    }
   }
 }
 
@@ -931,72 +1123,97 @@ __global__ void kAxpy(double* x, const double* y, double alpha, int64_t n) {
 }
 
 __global__ void kSymUpper(const double* G, int m, double* A) {
-  int q = threadIdx.x;
-  if (q >= m * m) return;
-  int i = q / m, j = q % m;
-  A[q] = i <= j ? G[i * m + j] : G[j * m + i];
+  for (int q = threadIdx.x; q < m * m; q += blockDim.x) {
+    const int i = q / m, j = q % m;
+    A[q] = i <= j ? G[i * m + j] : G[j * m + i];
+  }
 }
 
-// cyclic Jacobi (one thread), m <= 16
-__global__ void kJacobi(const double* A, int m, double* w, double* V) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double a[16 * 16], v[16 * 16];
-  for (int q = 0; q < m * m; q++) {
+// Cyclic Jacobi for m <= kJacobiMax on one wavefront with the matrices in
+// LDS: rotations are applied in the sequential (p, q) order, lane k updating
+// element k of the two rows / columns, so every element sees the same
+// operations in the same order as a one-thread sweep.
+constexpr int kJacobiMax = 32;
+
+__global__ void __launch_bounds__(64) kJacobi(const double* A, int m, double* w, double* V) {
+  __shared__ double a[kJacobiMax * kJacobiMax], v[kJacobiMax * kJacobiMax];
+  __shared__ double sOff;
+  __shared__ int order[kJacobiMax];
+  const int lane = threadIdx.x;
+  for (int q = lane; q < m * m; q += 64) {
     a[q] = A[q];
     v[q] = 0.0;
   }
-  for (int i = 0; i < m; i++) v[i * m + i] = 1.0;
+  __syncthreads();
+  if (lane < m) v[lane * m + lane] = 1.0;
+  __syncthreads();
   for (int sweep = 0; sweep < 100; sweep++) {
-    double off = 0.0;
-    for (int p = 0; p < m; p++)
-      for (int q = p + 1; q < m; q++) off += fabs(a[p * m + q]);
-    if (off == 0.0) break;
+    if (lane == 0) {
+      double off = 0.0;
+      for (int p = 0; p < m; p++)
+        for (int q = p + 1; q < m; q++) off += fabs(a[p * m + q]);
+      sOff = off;
+    }
+    __syncthreads();
+    if (sOff == 0.0) break;
     for (int p = 0; p < m; p++)
       for (int q = p + 1; q < m; q++) {
-        double apq = a[p * m + q], app = a[p * m + p], aqq = a[q * m + q];
-        double g = 100.0 * fabs(apq);
+        const double apq = a[p * m + q], app = a[p * m + p], aqq = a[q * m + q];
+        const double g = 100.0 * fabs(apq);
         if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
-          a[p * m + q] = 0.0;
-          a[q * m + p] = 0.0;
+          __syncthreads();
+          if (lane == 0) {
+            a[p * m + q] = 0.0;
+            a[q * m + p] = 0.0;
+          }
+          __syncthreads();
           continue;
         }
         if (apq == 0.0) continue;
-        double theta = (aqq - app) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < m; k++) {
-          double akp = a[k * m + p], akq = a[k * m + q];
-          a[k * m + p] = c * akp - sn * akq;
-          a[k * m + q] = sn * akp + c * akq;
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        __syncthreads();
+        if (lane < m) {
+          const double akp = a[lane * m + p], akq = a[lane * m + q];
+          a[lane * m + p] = c * akp - sn * akq;
+          a[lane * m + q] = sn * akp + c * akq;
         }
-        for (int k = 0; k < m; k++) {
-          double apk = a[p * m + k], aqk = a[q * m + k];
-          a[p * m + k] = c * apk - sn * aqk;
-          a[q * m + k] = sn * apk + c * aqk;
+        __syncthreads();
+        if (lane < m) {
+          const double apk = a[p * m + lane], aqk = a[q * m + lane];
+          a[p * m + lane] = c * apk - sn * aqk;
+          a[q * m + lane] = sn * apk + c * aqk;
+          const double vkp = v[lane * m + p], vkq = v[lane * m + q];
+          v[lane * m + p] = c * vkp - sn * vkq;
+          v[lane * m + q] = sn * vkp + c * vkq;
         }
-        a[p * m + q] = 0.0;
-        a[q * m + p] = 0.0;
-        for (int k = 0; k < m; k++) {
-          double vkp = v[k * m + p], vkq = v[k * m + q];
-          v[k * m + p] = c * vkp - sn * vkq;
-          v[k * m + q] = sn * vkp + c * vkq;
+        __syncthreads();
+        if (lane == 0) {
+          a[p * m + q] = 0.0;
+          a[q * m + p] = 0.0;
         }
+        __syncthreads();
       }
+    __syncthreads();
   }
-  int order[16];
-  for (int i = 0; i < m; i++) order[i] = i;
-  for (int i = 1; i < m; i++) {  // stable insertion sort by eigenvalue
-    int o = order[i];
-    int j = i;
-    while (j > 0 && a[order[j - 1] * m + order[j - 1]] > a[o * m + o]) {
-      order[j] = order[j - 1];
-      j--;
+  if (lane == 0) {
+    for (int i = 0; i < m; i++) order[i] = i;
+    for (int i = 1; i < m; i++) {  // stable insertion sort by eigenvalue
+      const int o = order[i];
+      int j = i;
+      while (j > 0 && a[order[j - 1] * m + order[j - 1]] > a[o * m + o]) {
+        order[j] = order[j - 1];
+        j--;
+      }
+      order[j] = o;
     }
-    order[j] = o;
   }
-  for (int j = 0; j < m; j++) {
-    w[j] = a[order[j] * m + order[j]];
-    for (int k = 0; k < m; k++) V[k * m + j] = v[k * m + order[j]];
+  __syncthreads();
+  if (lane < m) w[lane] = a[order[lane] * m + order[lane]];
+  for (int q = lane; q < m * m; q += 64) {
+    const int k = q / m, j = q % m;
+    V[k * m + j] = v[k * m + order[j]];
   }
 }
 
@@ -1280,29 +1497,82 @@ void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* 
   HIP_CHECK(hipGetLastError());
 }
 
+static int computeUnits() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+template <typename VT, typename XT, int CHUNK, bool PERSIST>
+static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
+                           const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
+                           unsigned* counter, hipStream_t s) {
+  const int64_t chunks = (n + CHUNK - 1) / CHUNK;
+  if (chunks > 0x7fffffffLL) throw HipError("too many rows for the relabelled sigma");
+  unsigned grid = (unsigned)chunks;
+  if (PERSIST) grid = (unsigned)std::min<int64_t>(chunks, (int64_t)computeUnits() * 8);
+  for (int c0 = 0; c0 < k;) {
+    const int kk = k - c0;
+    const double* xc = x + (int64_t)c0 * ld;
+    double* yc = y + (int64_t)c0 * ld;
+    auto spmm = [&](auto KT) {
+      constexpr int K = decltype(KT)::value;
+      const int take = kk < K ? kk : K;
+      xP.reserve((size_t)n * K * sizeof(XT));
+      kGatherPerm<XT, K><<<gridFor(n), kBlock, 0, s>>>(xc, take, ld, perm, n, xP.as<XT>(), counter);
+      kSpmmPerm<VT, XT, K, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), xc,
+                                                                     yc, take, ld, counter);
+      return take;
+    };
+    if (kk == 1) {
+      xP.reserve((size_t)n * sizeof(XT));
+      kGatherPerm<XT, 1><<<gridFor(n), kBlock, 0, s>>>(xc, 1, ld, perm, n, xP.as<XT>(), counter);
+      kSigmaPerm<VT, XT, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), xc, yc,
+                                                                   1, ld, counter);
+      c0 += 1;
+    } else if (kk == 2) {
+      c0 += spmm(std::integral_constant<int, 2>());
+    } else if (kk <= 4) {
+      c0 += spmm(std::integral_constant<int, 4>());
+    } else {
+      c0 += spmm(std::integral_constant<int, 8>());
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
 template <typename VT, typename XT>
 static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
                              const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld,
-                             DevMem& xP, hipStream_t s) {
-  if (n <= 0) return;
-  xP.reserve((size_t)k * n * sizeof(XT));
-  kGatherPerm<XT><<<gridFor(n), kBlock, 0, s>>>(x, k, ld, perm, n, xP.as<XT>());
-  const int64_t blocks = (n + kPermChunk - 1) / kPermChunk;
-  if (blocks > 0x7fffffffLL) throw HipError("too many rows for the relabelled sigma");
-  kSigmaPerm<VT, XT><<<(unsigned)blocks, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), x, y, k, ld);
-  HIP_CHECK(hipGetLastError());
+                             DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+  if (n <= 0 || k <= 0) return;
+  if (cfg.persistent && !counter) throw HipError("persistent sigma needs a work counter");
+  switch (cfg.chunk * 2 + (cfg.persistent ? 1 : 0)) {
+    case 16: sigmaPermChunk<VT, XT, 8, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 17: sigmaPermChunk<VT, XT, 8, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 32: sigmaPermChunk<VT, XT, 16, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 33: sigmaPermChunk<VT, XT, 16, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 64: sigmaPermChunk<VT, XT, 32, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 65: sigmaPermChunk<VT, XT, 32, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    default: throw HipError("unsupported sigma configuration");
+  }
 }
 
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
                         const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        hipStream_t s) {
-  launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, s);
+                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+  launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
                         const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        hipStream_t s) {
-  launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, s);
+                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+  launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,
@@ -1328,6 +1598,36 @@ void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int
   } else {
     throw HipError("launchGram: subspace too large");
   }
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchGramTiled(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G,
+                     DevMem& partial, hipStream_t s) {
+  if (ma > 64 || mb > 64) throw HipError("launchGramTiled: block too large");
+  if (ma <= 0 || mb <= 0) return;
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks / 2, (n + kBlock - 1) / kBlock));
+  const int tiles = ((ma + 7) / 8) * ((mb + 7) / 8);
+  partial.reserve((size_t)tiles * nb * 64 * sizeof(double));
+  kGramTile<<<dim3(nb, tiles), kBlock, 0, s>>>(A, ma, B, mb, n, ld, partial.as<double>());
+  kGramTileFinal<<<ma * mb, kBlock, 0, s>>>(partial.as<double>(), nb, ma, mb, G);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchGemmTS(const double* A, int ma, const double* W, int mb, int64_t n, int64_t lda, double* out,
+                  int64_t ldo, double alpha, double beta, hipStream_t s) {
+  if (ma > 64 || mb > 64) throw HipError("launchGemmTS: block too large");
+  if (n <= 0 || mb <= 0) return;
+  kGemmTS<<<gridFor(n, kBlock, 4096), kBlock, 0, s>>>(A, ma, W, mb, n, lda, out, ldo, alpha, beta);
+  HIP_CHECK(hipGetLastError());
+}
+
+__global__ void kSubMatrix(const double* C, int ldc, int rows, int cols, double* out) {
+  for (int q = threadIdx.x; q < rows * cols; q += blockDim.x) out[q] = C[(q / cols) * ldc + q % cols];
+}
+
+void launchSubMatrix(const double* C, int ldc, int rows, int cols, double* out, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  kSubMatrix<<<1, kBlock, 0, s>>>(C, ldc, rows, cols, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1388,7 +1688,7 @@ void launchSymmetrizeUpper(const double* G, int m, double* A, hipStream_t s) {
 }
 
 void launchJacobi(const double* A, int m, double* w, double* V, hipStream_t s) {
-  if (m > 16) throw HipError("launchJacobi: subspace too large");
+  if (m > kJacobiMax) throw HipError("launchJacobi: subspace too large");
   kJacobi<<<1, 64, 0, s>>>(A, m, w, V);
   HIP_CHECK(hipGetLastError());
 }

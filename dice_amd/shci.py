@@ -43,7 +43,8 @@ _lib = None
 
 class RunOptions(ctypes.Structure):
     """shci_run_options of include/shci_gpu.h."""
-    _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+    _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("davidson_block", ctypes.c_int),
+                ("autotune", ctypes.c_int), ("hip_graphs", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class ShciError(RuntimeError):
@@ -71,6 +72,7 @@ def library():
         "shci_det_words": ([P], I32),
         "shci_core_energy": ([P], D),
         "shci_set_precision": ([P, I32], I32),
+        "shci_set_options": ([P, ctypes.POINTER(RunOptions)], I32),
         "shci_set_active_space": ([P, I32, I32], I32),
         "shci_add_restriction": ([P, I32, I32, pI32, I32], I32),
         "shci_det_energy": ([P, pI64, I64, pD], I32),
@@ -154,6 +156,17 @@ class System:
 
     def set_precision(self, prec):
         self._check(self._lib.shci_set_precision(self._h, prec))
+
+    def set_options(self, precision=PREC_FP64, davidson_block=False, autotune=False, hip_graphs=False,
+                    profile_sigma=False):
+        """Method options for the piecewise calls (see shci_run_options)."""
+        o = RunOptions()
+        o.precision = precision
+        o.profile_sigma = int(bool(profile_sigma))
+        o.davidson_block = int(bool(davidson_block))
+        o.autotune = int(bool(autotune))
+        o.hip_graphs = int(bool(hip_graphs))
+        self._check(self._lib.shci_set_options(self._h, ctypes.byref(o)))
 
     def set_active_space(self, ncore, nact):
         self._check(self._lib.shci_set_active_space(self._h, ncore, nact))
@@ -292,13 +305,21 @@ class VariationalResult:
         return out
 
 
-def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False):
-    """Run the variational step of ``Dice input.dat``; returns a VariationalResult."""
+def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False, davidson_block=False,
+                   autotune=False, hip_graphs=False):
+    """Run the variational step of ``Dice input.dat``; returns a VariationalResult.
+
+    precision: PREC_FP64 (the drop-in) or PREC_FP32 (FP32 values, FP64
+    accumulation and refinement); davidson_block / autotune / hip_graphs
+    select the batching form (see shci_run_options)."""
     L = library()
     h = ctypes.c_void_p()
     opt = RunOptions()
     opt.precision = precision
     opt.profile_sigma = 1 if profile_sigma else 0
+    opt.davidson_block = 1 if davidson_block else 0
+    opt.autotune = 1 if autotune else 0
+    opt.hip_graphs = 1 if hip_graphs else 0
     st = L.shci_run_ex(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, ctypes.byref(opt),
                        ctypes.byref(h))
     if st != 0:

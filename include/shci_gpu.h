@@ -114,7 +114,9 @@ shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k);
 shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld);
 /* Davidson with the reference's semantics (SHCI/Davidson.cpp:111-348):
  * x0 host n x nroots column-major guess in, eigenvectors out; energies out
- * (coreE not added); *iterations = Davidson iteration count. */
+ * (coreE not added); *iterations = Davidson iteration count.  With the
+ * davidson_block option (shci_set_options) the block Davidson runs instead
+ * and max_copies is ignored. */
 shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies, double tol, double* energies,
                           int* iterations);
 
@@ -126,15 +128,28 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
  * shci_precision.  The result handle holds the final space for inspection. */
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);
 
-/* Options of shci_run_ex (zero-initialise, then set fields). */
+/* Options of shci_run_ex and shci_set_options (zero-initialise, then set
+ * fields; all zero = the reference's method in FP64). */
 typedef struct {
-  int precision;     /* shci_precision */
-  int profile_sigma; /* nonzero: time every sigma launch with HIP events on the handle's stream */
-  int reserved[6];
+  int precision;      /* shci_precision */
+  int profile_sigma;  /* nonzero: time every sigma launch with HIP events on the handle's stream */
+  int davidson_block; /* nonzero: block Davidson (all unconverged roots corrected per iteration, the new
+                         block's sigma as one SpMM) in place of the reference's single-vector Davidson
+                         (Davidson.cpp:111-348); up to 8 roots */
+  int autotune;       /* nonzero: choose the sigma launch shape (rows per work item, static or persistent
+                         workgroups) by timing on each Hamiltonian whose size has doubled */
+  int hip_graphs;     /* nonzero (with davidson_block): replay block-Davidson iterations from captured
+                         hipGraphs */
+  int reserved[3];
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,
                         shci_handle** out);
+/* Method options for the piecewise calls on h (precision, profile_sigma,
+ * davidson_block, autotune, hip_graphs); a precision change resets the
+ * Hamiltonian like shci_set_precision.  No reference counterpart: the
+ * reference has one method. */
+shci_status shci_set_options(shci_handle* h, const shci_run_options* options);
 
 /* per-iteration records of the last shci_run on this handle */
 int shci_run_iterations(const shci_handle* h);
