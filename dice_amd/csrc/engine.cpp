@@ -391,7 +391,7 @@ void Engine::clearGraphs() {
 // once per (subspace size, block size) as a hipGraph and replayed.
 std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& numIter, bool exactSigma) {
   const int64_t N = n;
-  const int maxM = (int)std::min<int64_t>(N, std::min(32, std::max(3 * nr, nr + 8)));
+  const int maxM = (int)std::min<int64_t>(N, std::min(32, std::max(6 * nr, nr + 16)));
   if (nr > 8 || nr > maxM) throw std::invalid_argument("block Davidson supports up to 8 roots");
   hipStream_t s = stream;
   if (graphN != N) {
@@ -437,8 +437,10 @@ std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& 
   auto col = [&](double* M, int c) { return M + (int64_t)c * N; };
   auto sync = [&]() { HIP_CHECK(hipStreamSynchronize(s)); };
   // orthonormalise T[:, 0:b] against V[:, 0:m] and among themselves; append
-  // the columns that keep a norm > 1e-10; returns how many were appended
+  // the columns that keep more than 1e-8 of their norm (and > 1e-10);
+  // returns how many were appended
   auto appendBlock = [&](int m, int b) {
+    for (int j = 0; j < b; j++) launchNorm2(col(Tm, j), N, nrm + 16 + j, dvPart, s);
     if (m > 0)
       for (int pass = 0; pass < 2; pass++) {
         launchGramTiled(V, m, Tm, b, N, N, coef, dvPart, s);
@@ -451,12 +453,13 @@ std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& 
       }
       launchNorm2(col(Tm, j), N, nrm + j, dvPart, s);
     }
-    std::vector<double> h(b);
+    std::vector<double> h(b), h0(b);
     HIP_CHECK(hipMemcpyAsync(h.data(), nrm, b * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h0.data(), nrm + 16, b * 8, hipMemcpyDeviceToHost, s));
     sync();
     int added = 0;
     for (int j = 0; j < b; j++) {
-      if (!(std::sqrt(h[j]) > 1e-10)) continue;
+      if (!(std::sqrt(h[j]) > 1e-10) || !(std::sqrt(h[j]) > 1e-8 * std::sqrt(h0[j]))) continue;
       launchScaleInto(col(V, m + added), col(Tm, j), nrm + j, N, s);
       added++;
     }
