@@ -11,6 +11,7 @@
 
 #include "../../include/shci_gpu.h"
 #include "engine.h"
+#include "mlp.h"
 
 using namespace shci;
 
@@ -660,3 +661,82 @@ shci_status shci_run_ci(shci_handle* h, double* out, int64_t cap) {
 }
 
 }  // extern "C"
+
+// ---- candidate-ranking MLP (handle-free) ----------------------------------
+
+namespace {
+struct MlpCall {
+  hipStream_t s = nullptr;
+  shci::MlpModel model;
+  shci::DevMem X, t, y;
+  MlpCall(int din, int hidden) {
+    if (din < 1 || hidden < 1 || hidden > 4096 || din > 65536) throw std::invalid_argument("bad MLP shape");
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    model.init(din, hidden, 0, s);
+  }
+  ~MlpCall() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void upload(shci::DevMem& m, const float* h, size_t count) {
+    m.reserve(std::max<size_t>(count, 1) * sizeof(float));
+    if (count) HIP_CHECK(hipMemcpyAsync(m.p, h, count * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+};
+}  // namespace
+
+int64_t shci_mlp_param_count(int din, int hidden) { return (int64_t)shci::mlpLayout(din, hidden).total; }
+
+shci_status shci_mlp_init(float* params, int din, int hidden, uint64_t seed) {
+  return guarded(nullptr, [&] {
+    MlpCall c(din, hidden);
+    c.model.init(din, hidden, seed, c.s);
+    c.model.getParams(params, c.s);
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_mlp_forward(const float* params, int din, int hidden, const float* X, int64_t n, float* y) {
+  return guarded(nullptr, [&] {
+    if (n < 0) throw std::invalid_argument("n < 0");
+    MlpCall c(din, hidden);
+    c.model.setParams(params, c.s);
+    c.upload(c.X, X, (size_t)n * din);
+    c.y.reserve(std::max<int64_t>(n, 1) * sizeof(float));
+    c.model.predict(c.X.as<float>(), n, c.y.as<float>(), c.s);
+    if (n) HIP_CHECK(hipMemcpyAsync(y, c.y.p, n * sizeof(float), hipMemcpyDeviceToHost, c.s));
+    HIP_CHECK(hipStreamSynchronize(c.s));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_mlp_loss_grad(const float* params, int din, int hidden, const float* X, const float* t, int64_t n,
+                               float* loss, float* grad) {
+  return guarded(nullptr, [&] {
+    if (n < 1) throw std::invalid_argument("need n >= 1");
+    MlpCall c(din, hidden);
+    c.model.setParams(params, c.s);
+    c.upload(c.X, X, (size_t)n * din);
+    c.upload(c.t, t, (size_t)n);
+    *loss = c.model.lossGrad(c.X.as<float>(), c.t.as<float>(), n, c.s);
+    HIP_CHECK(hipMemcpyAsync(grad, c.model.G.p, c.model.L.total * sizeof(float), hipMemcpyDeviceToHost, c.s));
+    HIP_CHECK(hipStreamSynchronize(c.s));
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_mlp_train(float* params, int din, int hidden, const float* X, const float* t, int64_t n, int epochs,
+                           int batch, float lr, uint64_t seed, float* first_loss, float* last_loss) {
+  return guarded(nullptr, [&] {
+    if (n < 1 || batch < 1 || epochs < 0) throw std::invalid_argument("need n >= 1, batch >= 1, epochs >= 0");
+    MlpCall c(din, hidden);
+    c.model.setParams(params, c.s);
+    c.upload(c.X, X, (size_t)n * din);
+    c.upload(c.t, t, (size_t)n);
+    float f = 0.f, l = 0.f;
+    c.model.train(c.X.as<float>(), c.t.as<float>(), n, epochs, batch, lr, seed, f, l, c.s);
+    c.model.getParams(params, c.s);
+    if (first_loss) *first_loss = f;
+    if (last_loss) *last_loss = l;
+    return SHCI_OK;
+  });
+}

@@ -31,6 +31,37 @@ void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64
                   double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
                   unsigned long long* counter, int64_t cap, hipStream_t s);
 
+// One double excitation of a source determinant proposed to the learned
+// ranking: spin-orbitals ci, cj (occupied) -> a, b (empty), heat-bath value v.
+struct CandRecord {
+  uint32_t src;
+  uint8_t ci, cj, a, b;
+  float v;
+};
+// singles only (the learned form's exact singles)
+void launchScreenSingles(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                         double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
+                         unsigned long long* counter, int64_t cap, hipStream_t s);
+// doubles whose |v| >= eps / loosen (eps = eps1 / cmax), as records; *counter
+// receives the total (may exceed cap)
+void launchScreenRecords(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                         double eps1, double loosen, int ncore, int nact, CandRecord* rec,
+                         unsigned long long* counter, int64_t cap, hipStream_t s);
+
+// features of records (row r: occupation bits, one-hot ci, cj, a, b, log10
+// |cmax[src]|; din >= 5 nso + 1, zero padded)
+void launchCandFeatures(const DevSystem& S, const CandRecord* rec, int64_t n, const uint64_t* dets,
+                        const double* cmax, float* X, int din, hipStream_t s);
+// t = log10(|cmax[src]| |v|)
+void launchCandTargets(const CandRecord* rec, int64_t n, const double* cmax, float* t, hipStream_t s);
+// exact test of the records with score >= tau; accepted determinants not in
+// the table are appended to out (cap, counter as launchScreen); counts[0] +=
+// evaluated, counts[1] += accepted
+void launchCandExact(const DevSystem& S, const CandRecord* rec, int64_t n, const float* score, float tau,
+                     const uint64_t* dets, const double* cmax, double eps1, const int32_t* table, uint64_t slots,
+                     uint64_t* out, unsigned long long* counter, int64_t cap, unsigned long long* counts,
+                     hipStream_t s);
+
 // --- string tables and the Hamiltonian -----------------------------------------------
 struct StringTables {
   int64_t nA = 0, nB = 0, n = 0;

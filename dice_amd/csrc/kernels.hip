@@ -133,11 +133,31 @@ __device__ __forceinline__ void emitCandidate(bool want, const uint64_t* cand, c
   }
 }
 
-template <int W>
+__device__ __forceinline__ void emitRecord(bool want, const CandRecord& r, CandRecord* out,
+                                           unsigned long long* counter, int64_t cap) {
+  uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  unsigned long long base = 0;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (want) {
+    unsigned long long pos = base + __popcll(m & lanemaskLt());
+    if ((int64_t)pos < cap) out[pos] = r;
+  }
+}
+
+// MODE 0: singles and doubles emitted as determinants (the reference's
+// screening).  MODE 1: singles only.  MODE 2: no singles; every double
+// excitation whose heat-bath value passes eps / loosen (a superset of the
+// accepted ones) is emitted as a CandRecord for the learned ranking.
+template <int W, int MODE>
 __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* dets, int64_t srcBegin,
                                                   int64_t ndets, const double* cmax, double eps1, int ncore,
                                                   int nact, const int32_t* table, uint64_t mask, uint64_t* out,
-                                                  unsigned long long* counter, int64_t cap) {
+                                                  unsigned long long* counter, int64_t cap, double loosen,
+                                                  CandRecord* rec) {
   __shared__ uint8_t sClosed[kWaves][128];
   __shared__ uint8_t sOpen[kWaves][128];
   __shared__ int sExcl[kWaves][65];
@@ -170,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
       cur[r] = c;
     }
     // ---- single excitations
-    const int nsingle = nclosed * nopen;
+    const int nsingle = MODE == 2 ? 0 : nclosed * nopen;
     for (int t0 = 0; t0 < nsingle; t0 += 64) {
       int t = t0 + lane;
       bool want = false;
@@ -199,7 +219,9 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
       emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
     }
     // ---- double excitations
-    if (S.maxEntryAbs < eps) continue;
+    if (MODE == 1) continue;
+    const double epsD = MODE == 2 ? eps / loosen : eps;
+    if (S.maxEntryAbs < epsD) continue;
     const int npairs = nclosed * (nclosed - 1) / 2;
     for (int p0 = 0; p0 < npairs; p0 += 64) {
       int p = p0 + lane;
@@ -223,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
           int64_t lo = s0, hi = s1;
           while (lo < hi) {
             int64_t mid = (lo + hi) >> 1;
-            if ((double)fabsf(v[mid]) < eps)
+            if ((double)fabsf(v[mid]) < epsD)
               hi = mid;
             else
               lo = mid + 1;
@@ -248,6 +270,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
         int k = k0 + lane;
         bool want = false;
         uint64_t cand[W];
+        CandRecord recTmp{};
 #pragma unroll
         for (int w = 0; w < W; w++) cand[w] = d[w];
         if (k < total) {
@@ -275,16 +298,99 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
           }
           if (ok && !(occ<W>(d, a) || occ<W>(d, b))) {
             want = true;
+            if (MODE == 2) {
+              recTmp.src = (uint32_t)di;
+              recTmp.ci = (uint8_t)ci;
+              recTmp.cj = (uint8_t)cj;
+              recTmp.a = (uint8_t)a;
+              recTmp.b = (uint8_t)b;
+              recTmp.v = S.hbVal[same][idx];
+            }
             cand[a >> 6] |= (1ull << (a & 63));
             cand[b >> 6] |= (1ull << (b & 63));
             cand[ci >> 6] &= ~(1ull << (ci & 63));
             cand[cj >> 6] &= ~(1ull << (cj & 63));
           }
         }
-        emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
+        if (MODE == 2)
+          emitRecord(want, recTmp, rec, counter, cap);
+        else
+          emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
       }
       __syncwarp();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// learned screening: features, targets and the exact test of ranked records
+
+// X[r * din + f]: f < nso occupation bit f of the source determinant; then
+// one-hot blocks (nso each) of ci, cj, a, b; then log10 |cmax[src]|; zero
+// padding up to din
+template <int W>
+__global__ void kCandFeatures(const CandRecord* rec, int64_t n, const uint64_t* dets, int nso, const double* cmax,
+                              float* X, int din) {
+  const int64_t total = n * din;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = q / din;
+    const int f = (int)(q - r * din);
+    const CandRecord c = rec[r];
+    float v = 0.f;
+    if (f < nso) {
+      v = (float)((dets[(int64_t)c.src * W + (f >> 6)] >> (f & 63)) & 1ull);
+    } else if (f < 5 * nso) {
+      const int blk = (f - nso) / nso, o = (f - nso) - blk * nso;
+      const int which = blk == 0 ? c.ci : blk == 1 ? c.cj : blk == 2 ? c.a : c.b;
+      v = which == o ? 1.f : 0.f;
+    } else if (f == 5 * nso) {
+      v = (float)log10(fabs(cmax[c.src]) + 1e-300);
+    }
+    X[q] = v;
+  }
+}
+
+// t[r] = log10(|cmax[src]| * |v|): the screening value of the record
+__global__ void kCandTargets(const CandRecord* rec, int64_t n, const double* cmax, float* t) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    t[r] = (float)log10(fabs(cmax[rec[r].src]) * (double)fabsf(rec[r].v) + 1e-300);
+}
+
+// records whose score >= tau are evaluated with the reference's double test
+// (|v| >= eps1 / |cmax|) and, when accepted, emitted as determinants not yet
+// held; counts[0] += evaluated, counts[1] += accepted
+template <int W>
+__global__ void __launch_bounds__(kBlock) kCandExact(const CandRecord* rec, int64_t n, const float* score, float tau,
+                                                     const uint64_t* dets, const double* cmax, double eps1,
+                                                     const int32_t* table, uint64_t mask, uint64_t* out,
+                                                     unsigned long long* counter, int64_t cap,
+                                                     unsigned long long* counts) {
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x; r0 < n; r0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = r0 + threadIdx.x;
+    bool sel = false, pass = false;
+    uint64_t cand[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) cand[w] = 0;
+    if (r < n && score[r] >= tau) {
+      sel = true;
+      const CandRecord c = rec[r];
+      const double eps = eps1 / fabs(cmax[c.src]);
+      pass = (double)fabsf(c.v) >= eps;
+      if (pass) {
+#pragma unroll
+        for (int w = 0; w < W; w++) cand[w] = dets[(int64_t)c.src * W + w];
+        cand[c.a >> 6] |= (1ull << (c.a & 63));
+        cand[c.b >> 6] |= (1ull << (c.b & 63));
+        cand[c.ci >> 6] &= ~(1ull << (c.ci & 63));
+        cand[c.cj >> 6] &= ~(1ull << (c.cj & 63));
+      }
+    }
+    const uint64_t ms = __ballot(sel), mp = __ballot(pass);
+    if (__lane_id() == 0 && ms) {
+      atomicAdd(&counts[0], (unsigned long long)__popcll(ms));
+      atomicAdd(&counts[1], (unsigned long long)__popcll(mp));
+    }
+    emitCandidate<W>(pass, cand, table, mask, dets, out, counter, cap);
   }
 }
 
@@ -1271,8 +1377,51 @@ void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64
                   unsigned long long* counter, int64_t cap, hipStream_t s) {
   if (end <= begin) return;
   unsigned g = gridFor(end - begin, kWaves, 1 << 16);
-  DISPATCH_W(S.W, kScreen<WW><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, table, slots - 1,
-                                                      out, counter, cap));
+  DISPATCH_W(S.W, kScreen<WW, 0><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, table,
+                                                         slots - 1, out, counter, cap, 1.0, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchScreenSingles(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                         double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
+                         unsigned long long* counter, int64_t cap, hipStream_t s) {
+  if (end <= begin) return;
+  unsigned g = gridFor(end - begin, kWaves, 1 << 16);
+  DISPATCH_W(S.W, kScreen<WW, 1><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, table,
+                                                         slots - 1, out, counter, cap, 1.0, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchScreenRecords(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
+                         double eps1, double loosen, int ncore, int nact, CandRecord* rec,
+                         unsigned long long* counter, int64_t cap, hipStream_t s) {
+  if (end <= begin) return;
+  unsigned g = gridFor(end - begin, kWaves, 1 << 16);
+  DISPATCH_W(S.W, kScreen<WW, 2><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, nullptr, 0,
+                                                         nullptr, counter, cap, loosen, rec));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCandFeatures(const DevSystem& S, const CandRecord* rec, int64_t n, const uint64_t* dets,
+                        const double* cmax, float* X, int din, hipStream_t s) {
+  if (n <= 0) return;
+  DISPATCH_W(S.W, kCandFeatures<WW><<<gridFor(n * din), kBlock, 0, s>>>(rec, n, dets, S.nso, cmax, X, din));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCandTargets(const CandRecord* rec, int64_t n, const double* cmax, float* t, hipStream_t s) {
+  if (n <= 0) return;
+  kCandTargets<<<gridFor(n), kBlock, 0, s>>>(rec, n, cmax, t);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCandExact(const DevSystem& S, const CandRecord* rec, int64_t n, const float* score, float tau,
+                     const uint64_t* dets, const double* cmax, double eps1, const int32_t* table, uint64_t slots,
+                     uint64_t* out, unsigned long long* counter, int64_t cap, unsigned long long* counts,
+                     hipStream_t s) {
+  if (n <= 0) return;
+  DISPATCH_W(S.W, kCandExact<WW><<<gridFor(n), kBlock, 0, s>>>(rec, n, score, tau, dets, cmax, eps1, table,
+                                                                  slots - 1, out, counter, cap, counts));
   HIP_CHECK(hipGetLastError());
 }
 

@@ -63,6 +63,7 @@ def library():
     L = ctypes.CDLL(LIB_PATH)
     P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
     pI64, pI32, pD = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    pF = ctypes.POINTER(ctypes.c_float)
     sig = {
         "shci_create": ([ctypes.c_char_p, ctypes.POINTER(P)], I32),
         "shci_destroy": ([P], I32),
@@ -97,6 +98,11 @@ def library():
         "shci_run_iteration": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int), pD], I32),
         "shci_run_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_run_ci": ([P, pD, I64], I32),
+        "shci_mlp_param_count": ([I32, I32], I64),
+        "shci_mlp_init": ([pF, I32, I32, ctypes.c_uint64], I32),
+        "shci_mlp_forward": ([pF, I32, I32, pF, I64, pF], I32),
+        "shci_mlp_loss_grad": ([pF, I32, I32, pF, pF, I64, pF, pF], I32),
+        "shci_mlp_train": ([pF, I32, I32, pF, pF, I64, I32, I32, ctypes.c_float, ctypes.c_uint64, pF, pF], I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -328,3 +334,56 @@ def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma
             L.shci_destroy(h)
         raise ShciError(st, msg)
     return VariationalResult(h)
+
+
+# ---- candidate-ranking MLP of the learned screening form (no reference counterpart) ----
+
+def _mlp_check(L, st):
+    if st != 0:
+        raise ShciError(st, L.shci_last_error(None).decode())
+
+
+def mlp_param_count(din, hidden):
+    return int(library().shci_mlp_param_count(din, hidden))
+
+
+def mlp_init(din, hidden, seed=0):
+    """Xavier-uniform parameters (float32, flat layout W1, b1, W2, b2, W3, b3)."""
+    L = library()
+    p = np.zeros(mlp_param_count(din, hidden), dtype=np.float32)
+    _mlp_check(L, L.shci_mlp_init(_p(p, ctypes.c_float), din, hidden, seed))
+    return p
+
+
+def mlp_forward(params, din, hidden, X):
+    L = library()
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, din)
+    y = np.zeros(len(Xc), dtype=np.float32)
+    _mlp_check(L, L.shci_mlp_forward(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float), len(Xc),
+                                     _p(y, ctypes.c_float)))
+    return y
+
+
+def mlp_loss_grad(params, din, hidden, X, t):
+    L = library()
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, din)
+    tc = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros_like(P)
+    loss = ctypes.c_float()
+    _mlp_check(L, L.shci_mlp_loss_grad(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float),
+                                       _p(tc, ctypes.c_float), len(Xc), ctypes.byref(loss), _p(g, ctypes.c_float)))
+    return loss.value, g
+
+
+def mlp_train(params, din, hidden, X, t, epochs, batch, lr=1e-3, seed=0):
+    """Adam training on the GPU; returns (new params, first minibatch loss, last minibatch loss)."""
+    L = library()
+    P = np.array(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, din)
+    tc = np.ascontiguousarray(t, dtype=np.float32)
+    f, l = ctypes.c_float(), ctypes.c_float()
+    _mlp_check(L, L.shci_mlp_train(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float), _p(tc, ctypes.c_float),
+                                   len(Xc), epochs, batch, lr, seed, ctypes.byref(f), ctypes.byref(l)))
+    return P, f.value, l.value

@@ -151,6 +151,24 @@ shci_status shci_run_ex(const char* input_path, const char* log_path, const shci
  * reference has one method. */
 shci_status shci_set_options(shci_handle* h, const shci_run_options* options);
 
+/* ---- candidate-ranking MLP of the learned screening form ------------------------------
+ * No reference counterpart (the reference screens with the heat-bath lists
+ * alone).  A network din -> hidden -> hidden -> 1 with GELU, FP32 parameters
+ * in the flat layout W1 [hidden x din], b1, W2 [hidden x hidden], b2,
+ * W3 [hidden], b3; matrix products on bf16 MFMA with FP32 accumulation.
+ * Inputs X are n x din row-major FP32, host buffers; these calls use their
+ * own stream on the current device (errors: shci_last_error(NULL)). */
+int64_t shci_mlp_param_count(int din, int hidden);
+/* Xavier-uniform weights, zero biases */
+shci_status shci_mlp_init(float* params, int din, int hidden, uint64_t seed);
+shci_status shci_mlp_forward(const float* params, int din, int hidden, const float* X, int64_t n, float* y);
+/* loss = 0.5 mean((y - t)^2) and its gradient (parameter layout) */
+shci_status shci_mlp_loss_grad(const float* params, int din, int hidden, const float* X, const float* t, int64_t n,
+                               float* loss, float* grad);
+/* Adam (lr, betas 0.9 / 0.999) over shuffled minibatches for `epochs` passes; params updated in place */
+shci_status shci_mlp_train(float* params, int din, int hidden, const float* X, const float* t, int64_t n, int epochs,
+                           int batch, float lr, uint64_t seed, float* first_loss, float* last_loss);
+
 /* per-iteration records of the last shci_run on this handle */
 int shci_run_iterations(const shci_handle* h);
 shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, double* energy0, int* davidson_iters,

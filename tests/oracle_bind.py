@@ -65,6 +65,10 @@ def load():
         "oref_run_dets": ([P, I32, pI64, I32], None),
         "oref_run_ci": ([P, pD], None),
         "oref_run_free": ([P], None),
+        "oref_mlp_forward": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float), I64, I32,
+                              pD], None),
+        "oref_mlp_loss_grad": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float), I64, I32, pD], D),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -205,3 +209,25 @@ def sort_dets(d):
     d = np.asarray(d, dtype=np.int64)
     keys = [d[:, w] for w in range(d.shape[1])]  # lexsort: last key primary
     return d[np.lexsort(keys)]
+
+
+def mlp_forward(params, din, hidden, X, bf16=True):
+    """The oracle's MLP forward pass (double; bf16-rounded matmul operands when bf16)."""
+    L = load()
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, din)
+    y = np.zeros(len(Xc))
+    L.oref_mlp_forward(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float), len(Xc), 1 if bf16 else 0,
+                       _p(y, ctypes.c_double))
+    return y
+
+
+def mlp_loss_grad(params, din, hidden, X, t, bf16=True):
+    L = load()
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, din)
+    tc = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(len(P))
+    loss = L.oref_mlp_loss_grad(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float), _p(tc, ctypes.c_float),
+                                len(Xc), 1 if bf16 else 0, _p(g, ctypes.c_double))
+    return loss, g
