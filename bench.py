@@ -64,17 +64,18 @@ def workload_inputs(directory):
     return full, sample
 
 
-def run_form(shci, full, log, prec, steps, warmup):
-    """Warm up, then time `steps` complete variational steps in one precision."""
-    tag = "fp32" if prec == shci.PREC_FP32 else "fp64"
+def run_form(shci, full, log, prec, steps, warmup, tag=None, profile=True, **method):
+    """Warm up, then time `steps` complete variational steps in one precision
+    (method: the batching / learned options of do_variational)."""
+    tag = tag or ("fp32" if prec == shci.PREC_FP32 else "fp64")
     for w in range(warmup):
         print("# %s warmup %d/%d" % (tag, w + 1, warmup), file=sys.stderr, flush=True)
-        shci.do_variational(full, log, precision=prec)
+        shci.do_variational(full, log, precision=prec, **method)
     R = {"iter_times": [], "step_times": [], "calls": 0, "ms": 0.0, "nbytes": 0.0, "energies": [], "per_iter": None,
          "sigma_loose": None}
     for k in range(steps):
         t0 = time.time()
-        r = shci.do_variational(full, log, precision=prec, profile_sigma=True)
+        r = shci.do_variational(full, log, precision=prec, profile_sigma=profile, **method)
         wall = time.time() - t0
         ph = r.phases()
         its = r.iterations()
@@ -86,6 +87,9 @@ def run_form(shci, full, log, prec, steps, warmup):
         R["ms"] += m
         R["nbytes"] += b
         R["energies"].append(float(r.energies()[0]))
+        R["all_energies"] = [float(e) for e in r.energies()]
+        if method.get("learned"):
+            R["learned"] = r.learned()
         R["per_iter"] = [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"],
                           "screen_s": round(p[0], 4), "hamiltonian_s": round(p[1], 4), "davidson_s": round(p[2], 4)}
                          for it, p in zip(its, ph)]
@@ -94,6 +98,40 @@ def run_form(shci, full, log, prec, steps, warmup):
                  " ".join("%.3f,%.3f,%.3f/%d" % (p[0], p[1], p[2], it["davidson"]) for it, p in zip(its, ph))),
               file=sys.stderr, flush=True)
     return R
+
+
+def batching_form(shci, args):
+    """Batching form on the 6x6 Hubbard lattice (18+18 electrons, t = 1,
+    U = 4, site basis, Neel reference), eps1 2e-4, four roots, three
+    variational iterations (428,641 determinants): the reference's Davidson
+    vs the block Davidson (one SpMM per block, hipGraph-replayed iterations,
+    autotuned sigma launch shape)."""
+    from dice_amd import synth
+    d = os.path.join(tempfile.gettempdir(), "shci_bench_hubbard6x6")
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, "FCIDUMP")
+    if not os.path.exists(p):
+        synth.write_hubbard_fcidump(p, 6, 6, 1.0, 4.0, 18, 18)
+    inp = os.path.join(d, "input.dat")
+    synth.write_input(inp, p, synth.neel_occupation(6, 6), [(0, 2e-4)], 3, nroots=4, extra=["prefix " + d])
+    log = os.path.join(d, "gpu.log")
+    res = {}
+    for name, method in (("reference_davidson", {}),
+                         ("block_davidson", dict(davidson_block=True, autotune=True, hip_graphs=True))):
+        # (per-launch sigma events would keep the block Davidson from replaying graphs)
+        R = run_form(shci, inp, log, shci.PREC_FP64, 1, 1, tag="hubbard " + name, profile=not method, **method)
+        res[name] = {
+            "s_per_iteration": round(statistics.mean(R["iter_times"]), 6),
+            "ms_per_step": round(1e3 * statistics.mean(R["step_times"]), 3),
+            "davidson_s": round(sum(it["davidson_s"] for it in R["per_iter"]), 4),
+            "energies": R["all_energies"],
+            "sigma_GBps": round(R["nbytes"] / (R["ms"] * 1e-3) / 1e9, 1) if R["ms"] else None,
+            "iterations": R["per_iter"],
+        }
+    res["workload"] = ("6x6 periodic Hubbard, t=1, U=4, 18a+18b, eps1 2e-4, 4 roots, 3 iterations "
+                       "(the fourth would select 12.6M more determinants)")
+    res["davidson_speedup"] = round(res["reference_davidson"]["davidson_s"] / res["block_davidson"]["davidson_s"], 3)
+    return res
 
 
 def main():
@@ -180,6 +218,25 @@ def main():
                 "iterations": mixed["per_iter"],
             }
         }
+        # learned screening + mixed precision (the combined fast form)
+        L = run_form(shci, full, log, shci.PREC_FP32, args.steps, args.warmup, tag="learned+fp32", learned=True)
+        recs, train = L["learned"]
+        out["forms"]["learned_mixed_fp32"] = {
+            "description": "learned candidate ranking (MLP trained on the first two iterations' screening values "
+                           "predicts log10(|c||v|); candidates predicted above log10(eps1) - 0.8 are tested "
+                           "exactly, full screening when a sample shows more than half would be) with the FP32 "
+                           "form",
+            "value": round(statistics.mean(L["iter_times"]), 6),
+            "ms_per_step": round(1e3 * statistics.mean(L["step_times"]), 3),
+            "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(L["step_times"]), 3),
+            "final_energy": L["energies"][-1],
+            "energy_error_vs_fp64_Ha": L["energies"][-1] - fp64["energies"][-1],
+            "final_ndets": L["per_iter"][-1]["ndets"],
+            "training": train,
+            "ranked_iterations": recs,
+            "iterations": L["per_iter"],
+        }
+        out["forms"]["batched_block"] = batching_form(shci, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "lib")
 BIN = os.path.join(HERE, "bin")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["prims.hip", "kernels.hip", "mlp.hip", "host_io.cpp", "engine.cpp", "capi.cpp"]
+SOURCES = ["prims.hip", "kernels.hip", "mlp.hip", "host_io.cpp", "engine.cpp", "learned.cpp", "capi.cpp"]
 ARCH = os.environ.get("SHCI_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

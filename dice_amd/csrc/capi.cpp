@@ -89,6 +89,11 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   E.davidsonMode = opt.davidson_block ? 1 : 0;
   E.autotune = opt.autotune != 0;
   E.useGraphs = opt.hip_graphs != 0;
+  E.learned = opt.learned != 0;
+  E.learnedControl = opt.learned == 2;
+  // SHCI_LEARNED_MARGIN: log10 margin below eps1 of the learned form's
+  // evaluation threshold (default 0.8)
+  if (const char* m = std::getenv("SHCI_LEARNED_MARGIN")) E.learnedMargin = std::atof(m);
   if (!E.autotune) {
     E.sigmaCfg[0] = SigmaConfig{};
     E.sigmaCfg[1] = SigmaConfig{};
@@ -102,6 +107,11 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
   Engine& E = h->eng;
   const int precision = opt.precision;
   applyMethodOptions(E, opt);
+  E.learnedRecs.clear();
+  E.sampX.clear();
+  E.sampT.clear();
+  E.trainSamples = 0;
+  E.mlpReady = false;
   E.resetSigmaProfile();
   const double startofCalc = wallTime();
   HostSchedule schd;
@@ -207,7 +217,18 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     const int64_t nold = E.n;
     E.cmax.reserve((size_t)std::max<int64_t>(nold, 1) * 8);
     launchCmax(E.ci.as<double>(), nroots, nold, nold, E.cmax.as<double>(), E.stream);
-    const int64_t nnew = E.screen(E.cmax.as<double>(), eps1);
+    int64_t nnew = -1;
+    if (E.learned && E.mlpReady && iter >= E.learnedLogIters) {
+      nnew = E.screenLearned(E.cmax.as<double>(), eps1, iter);
+      // once the ranking would not save evaluations, later iterations
+      // screen in full without trying it again
+      if (nnew < 0) E.mlpReady = false;
+    }
+    if (nnew < 0) nnew = E.screen(E.cmax.as<double>(), eps1);
+    if (E.learned && iter < E.learnedLogIters) {
+      E.learnedLog(E.cmax.as<double>(), eps1);
+      if (iter == E.learnedLogIters - 1) E.learnedTrain();
+    }
     if (std::getenv("SHCI_TIMING"))
       fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);
     E.appendNew();
@@ -739,4 +760,33 @@ shci_status shci_mlp_train(float* params, int din, int hidden, const float* X, c
     if (last_loss) *last_loss = l;
     return SHCI_OK;
   });
+}
+
+int shci_run_learned_count(const shci_handle* h) { return h ? (int)h->eng.learnedRecs.size() : 0; }
+
+shci_status shci_run_learned(const shci_handle* h, int k, int* iteration, int64_t* candidates, int64_t* evaluated,
+                             int64_t* accepted, double* sampled_recall, double* sampled_fraction, int* fallback,
+                             double* seconds) {
+  if (!h || k < 0 || k >= (int)h->eng.learnedRecs.size()) return SHCI_BAD_INPUT;
+  const Engine::LearnedRecord& r = h->eng.learnedRecs[k];
+  if (iteration) *iteration = r.iter;
+  if (candidates) *candidates = r.candidates;
+  if (evaluated) *evaluated = r.evaluated;
+  if (accepted) *accepted = r.accepted;
+  if (sampled_recall) *sampled_recall = r.samplePass ? (double)r.sampleHit / (double)r.samplePass : 1.0;
+  if (sampled_fraction) *sampled_fraction = r.sampleTotal ? (double)r.sampleSelected / (double)r.sampleTotal : 0.0;
+  if (fallback) *fallback = r.fallback ? 1 : 0;
+  if (seconds) *seconds = r.seconds;
+  return SHCI_OK;
+}
+
+shci_status shci_run_learned_training(const shci_handle* h, int64_t* samples, float* first_loss, float* last_loss,
+                                      double* seconds) {
+  if (!h) return SHCI_BAD_INPUT;
+  const Engine& E = h->eng;
+  if (samples) *samples = (int64_t)E.sampT.size();
+  if (first_loss) *first_loss = E.trainFirst;
+  if (last_loss) *last_loss = E.trainLast;
+  if (seconds) *seconds = E.trainSeconds;
+  return SHCI_OK;
 }

@@ -8,6 +8,7 @@
 
 #include "host_io.h"
 #include "kernels.h"
+#include "mlp.h"
 
 namespace shci {
 
@@ -85,6 +86,41 @@ struct Engine {
   void clearGraphs();
   void tuneSigma();
   std::vector<double> davidsonBlock(double* dX0, int nr, double tol, int& numIter, bool exactSigma);
+
+  // learned screening form: the plain screening of the first learnedLogIters
+  // iterations logs (determinant, double excitation, log10 |c| |v|) samples
+  // over the candidates whose heat-bath value passes eps / learnedLoosen;
+  // the MLP trained on them predicts log10(|c| |v|) for the candidates of
+  // later iterations and only those predicted above log10(eps1) -
+  // learnedMargin are evaluated with the exact eps1 test; when a sample shows
+  // that more than learnedMaxFraction would be, the iteration screens in
+  // full.
+  bool learned = false;
+  int learnedLogIters = 2;
+  double learnedLoosen = 10.0, learnedMargin = 0.8, learnedMaxFraction = 0.5;
+  bool learnedControl = false;  // keep the untrained network (control runs)
+  int learnedHidden = 256;
+  int64_t learnedMaxSamples = (int64_t)1 << 18;
+  MlpModel mlp;
+  bool mlpReady = false;
+  std::vector<float> sampX, sampT;
+  uint64_t learnedRng = 0x9e3779b97f4a7c15ull;
+  DevMem recBuf, recScore, lcounts, featX, lIdx, lFlags, lVals;
+  struct LearnedRecord {
+    int iter = 0;
+    bool fallback = false;
+    int64_t candidates = 0, evaluated = 0, accepted = 0, samplePass = 0, sampleHit = 0, sampleSelected = 0,
+            sampleTotal = 0;
+    double seconds = 0;
+  };
+  std::vector<LearnedRecord> learnedRecs;
+  float trainFirst = 0, trainLast = 0;
+  double trainSeconds = 0;
+  int64_t trainSamples = 0;
+  int mlpDin() const { return (5 * hs.nso + 1 + 15) / 16 * 16; }
+  void learnedLog(const double* dCmax, double eps1);
+  void learnedTrain();
+  int64_t screenLearned(const double* dCmax, double eps1, int iter);
 
   // last run
   std::vector<IterRecord> iters;

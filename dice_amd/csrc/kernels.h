@@ -62,6 +62,14 @@ void launchCandExact(const DevSystem& S, const CandRecord* rec, int64_t n, const
                      uint64_t* out, unsigned long long* counter, int64_t cap, unsigned long long* counts,
                      hipStream_t s);
 
+// for sampled records idx[0..m): flags bit 0 = passes the exact test, bit 1 =
+// score >= tau; vals = scores
+void launchCandSample(const CandRecord* rec, const float* score, const int64_t* idx, int m, const double* cmax,
+                      double eps1, float tau, int* flags, float* vals, hipStream_t s);
+
+// out[i] = rec[idx[i]]
+void launchGatherRecords(const CandRecord* rec, const int64_t* idx, int64_t m, CandRecord* out, hipStream_t s);
+
 // --- string tables and the Hamiltonian -----------------------------------------------
 struct StringTables {
   int64_t nA = 0, nB = 0, n = 0;

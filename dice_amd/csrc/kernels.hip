@@ -394,6 +394,24 @@ __global__ void __launch_bounds__(kBlock) kCandExact(const CandRecord* rec, int6
   }
 }
 
+__global__ void kGatherRecords(const CandRecord* rec, const int64_t* idx, int64_t m, CandRecord* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = rec[idx[i]];
+}
+
+// sampled records: flags[i] = (pass ? 1 : 0) | (score >= tau ? 2 : 0) for
+// record idx[i]; vals[i] = its score
+__global__ void kCandSample(const CandRecord* rec, const float* score, const int64_t* idx, int m, const double* cmax,
+                            double eps1, float tau, int* flags, float* vals) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const CandRecord c = rec[idx[i]];
+    const float sc = score[idx[i]];
+    const double eps = eps1 / fabs(cmax[c.src]);
+    flags[i] = ((double)fabsf(c.v) >= eps ? 1 : 0) | (sc >= tau ? 2 : 0);
+    vals[i] = sc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // string tables
 
@@ -1422,6 +1440,19 @@ void launchCandExact(const DevSystem& S, const CandRecord* rec, int64_t n, const
   if (n <= 0) return;
   DISPATCH_W(S.W, kCandExact<WW><<<gridFor(n), kBlock, 0, s>>>(rec, n, score, tau, dets, cmax, eps1, table,
                                                                   slots - 1, out, counter, cap, counts));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchCandSample(const CandRecord* rec, const float* score, const int64_t* idx, int m, const double* cmax,
+                      double eps1, float tau, int* flags, float* vals, hipStream_t s) {
+  if (m <= 0) return;
+  kCandSample<<<gridFor(m), kBlock, 0, s>>>(rec, score, idx, m, cmax, eps1, tau, flags, vals);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchGatherRecords(const CandRecord* rec, const int64_t* idx, int64_t m, CandRecord* out, hipStream_t s) {
+  if (m <= 0) return;
+  kGatherRecords<<<gridFor(m), kBlock, 0, s>>>(rec, idx, m, out);
   HIP_CHECK(hipGetLastError());
 }
 

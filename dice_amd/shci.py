@@ -44,7 +44,8 @@ _lib = None
 class RunOptions(ctypes.Structure):
     """shci_run_options of include/shci_gpu.h."""
     _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("davidson_block", ctypes.c_int),
-                ("autotune", ctypes.c_int), ("hip_graphs", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("autotune", ctypes.c_int), ("hip_graphs", ctypes.c_int), ("learned", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 class ShciError(RuntimeError):
@@ -98,6 +99,10 @@ def library():
         "shci_run_iteration": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int), pD], I32),
         "shci_run_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_run_ci": ([P, pD, I64], I32),
+        "shci_run_learned_count": ([P], I32),
+        "shci_run_learned": ([P, I32, ctypes.POINTER(ctypes.c_int), pI64, pI64, pI64, pD, pD,
+                              ctypes.POINTER(ctypes.c_int), pD], I32),
+        "shci_run_learned_training": ([P, pI64, pF, pF, pD], I32),
         "shci_mlp_param_count": ([I32, I32], I64),
         "shci_mlp_init": ([pF, I32, I32, ctypes.c_uint64], I32),
         "shci_mlp_forward": ([pF, I32, I32, pF, I64, pF], I32),
@@ -293,6 +298,24 @@ class VariationalResult:
             out.append((a.value, b.value, c.value, z.value))
         return out
 
+    def learned(self):
+        """Learned screening form: per ranked iteration {iteration, candidates, evaluated, accepted,
+        sampled_recall, fallback, seconds}, and the training record."""
+        L = self._lib
+        out = []
+        for k in range(L.shci_run_learned_count(self._h)):
+            it, fb = ctypes.c_int(), ctypes.c_int()
+            c, e, a = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+            rc, sf, sec = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            L.shci_run_learned(self._h, k, ctypes.byref(it), ctypes.byref(c), ctypes.byref(e), ctypes.byref(a),
+                               ctypes.byref(rc), ctypes.byref(sf), ctypes.byref(fb), ctypes.byref(sec))
+            out.append({"iteration": it.value, "candidates": c.value, "evaluated": e.value, "accepted": a.value,
+                        "sampled_recall": rc.value, "sampled_fraction": sf.value, "fallback": bool(fb.value),
+                        "seconds": sec.value})
+        n, f, l, sec = ctypes.c_int64(), ctypes.c_float(), ctypes.c_float(), ctypes.c_double()
+        L.shci_run_learned_training(self._h, ctypes.byref(n), ctypes.byref(f), ctypes.byref(l), ctypes.byref(sec))
+        return out, {"samples": n.value, "first_loss": f.value, "last_loss": l.value, "seconds": sec.value}
+
     def final_davidson_seconds(self):
         return self._lib.shci_run_final_davidson_seconds(self._h)
 
@@ -312,12 +335,13 @@ class VariationalResult:
 
 
 def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False, davidson_block=False,
-                   autotune=False, hip_graphs=False):
+                   autotune=False, hip_graphs=False, learned=False):
     """Run the variational step of ``Dice input.dat``; returns a VariationalResult.
 
     precision: PREC_FP64 (the drop-in) or PREC_FP32 (FP32 values, FP64
     accumulation and refinement); davidson_block / autotune / hip_graphs
-    select the batching form (see shci_run_options)."""
+    select the batching form, learned the learned screening form (see
+    shci_run_options)."""
     L = library()
     h = ctypes.c_void_p()
     opt = RunOptions()
@@ -326,6 +350,7 @@ def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma
     opt.davidson_block = 1 if davidson_block else 0
     opt.autotune = 1 if autotune else 0
     opt.hip_graphs = 1 if hip_graphs else 0
+    opt.learned = 2 if learned == "control" else (1 if learned else 0)
     st = L.shci_run_ex(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, ctypes.byref(opt),
                        ctypes.byref(h))
     if st != 0:

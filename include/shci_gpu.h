@@ -140,7 +140,14 @@ typedef struct {
                          workgroups) by timing on each Hamiltonian whose size has doubled */
   int hip_graphs;     /* nonzero (with davidson_block): replay block-Davidson iterations from captured
                          hipGraphs */
-  int reserved[3];
+  int learned;        /* 1: learned screening form (shci_run / shci_run_ex only): the first two
+                         iterations screen in full and log training samples, an MLP trained on them then
+                         predicts log10(|c| |v|) for the double-excitation candidates (heat-bath value
+                         >= eps / 10) and only those predicted above log10(eps1) - 0.8 (environment
+                         SHCI_LEARNED_MARGIN) are tested exactly; singles stay exact; when a sample shows
+                         that more than half would be tested, that iteration and the later ones screen in
+                         full.  2: the same with the untrained network (control) */
+  int reserved[2];
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,
@@ -168,6 +175,15 @@ shci_status shci_mlp_loss_grad(const float* params, int din, int hidden, const f
 /* Adam (lr, betas 0.9 / 0.999) over shuffled minibatches for `epochs` passes; params updated in place */
 shci_status shci_mlp_train(float* params, int din, int hidden, const float* X, const float* t, int64_t n, int epochs,
                            int batch, float lr, uint64_t seed, float* first_loss, float* last_loss);
+
+/* learned screening records of the last run: one per ranked iteration */
+int shci_run_learned_count(const shci_handle* h);
+shci_status shci_run_learned(const shci_handle* h, int k, int* iteration, int64_t* candidates, int64_t* evaluated,
+                             int64_t* accepted, double* sampled_recall, double* sampled_fraction, int* fallback,
+                             double* seconds);
+/* training of the learned form's network in the last run */
+shci_status shci_run_learned_training(const shci_handle* h, int64_t* samples, float* first_loss, float* last_loss,
+                                      double* seconds);
 
 /* per-iteration records of the last shci_run on this handle */
 int shci_run_iterations(const shci_handle* h);
