@@ -2,6 +2,10 @@
 // See cpu_ref.h for scope, citations and the documented deviations.
 #include "cpu_ref.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -12,6 +16,17 @@
 #include <sys/time.h>
 
 namespace oref {
+
+// Host threads for the loops the reference spreads over MPI ranks (rows of
+// H, screened determinants, the sigma rows): ORACLE_THREADS (default 1, the
+// serial reference order).  With more threads H and the determinant sets
+// are unchanged; sigma sums its per-thread partial vectors in thread order,
+// as the reference's MPI_Reduce sums per-rank partials.
+static int oracleThreads() {
+  const char* t = std::getenv("ORACLE_THREADS");
+  int n = t ? std::atoi(t) : 1;
+  return n < 1 ? 1 : n;
+}
 
 static double wallTime() {
   struct timeval tv;
@@ -706,6 +721,10 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
     H.connections[DetI - 1].push_back(DetJ - 1);
     H.Helements[DetI - 1].push_back(v);
   };
+  // each determinant belongs to one alpha string, so the rows written by
+  // different alpha strings are disjoint
+  const int nt = oracleThreads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (nt > 1)
   for (size_t i = 0; i < h.AlphaMajorToBeta.size(); i++) {
     for (size_t ii = 0; ii < h.AlphaMajorToBeta[i].size(); ii++) {
       int Astring = (int)i, Bstring = h.AlphaMajorToBeta[i][ii], DetI = h.AlphaMajorToDet[i][ii];
@@ -768,17 +787,32 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
 
 // Hmult2::operator(), Hmult.h:110-122 (serial branch)
 void hmult(const SparseHam& H, const double* x, double* y, size_t n) {
+  const int nt = oracleThreads();
+  std::vector<std::vector<double>> part(nt, std::vector<double>(nt > 1 ? n : 0, 0.0));
   std::vector<double> yt(n, 0.0);
-  for (size_t i = 0; i < H.connections.size(); i++) {
-    const std::vector<int>& c = H.connections[i];
-    const std::vector<double>& v = H.Helements[i];
-    for (size_t j = 0; j < c.size(); j++) {
-      double h = v[j];
-      size_t J = (size_t)c[j];
-      yt[i] += h * x[J];
-      if (J != i) yt[J] += h * x[i];
+#pragma omp parallel num_threads(nt) if (nt > 1)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    double* acc = nt > 1 ? part[t].data() : yt.data();
+    // rows dealt round-robin, as the reference deals rows over ranks
+    for (size_t i = (size_t)t; i < H.connections.size(); i += (size_t)nt) {
+      const std::vector<int>& c = H.connections[i];
+      const std::vector<double>& v = H.Helements[i];
+      for (size_t j = 0; j < c.size(); j++) {
+        double h = v[j];
+        size_t J = (size_t)c[j];
+        acc[i] += h * x[J];
+        if (J != i) acc[J] += h * x[i];
+      }
     }
   }
+  if (nt > 1)
+    for (int t = 0; t < nt; t++)
+      for (size_t j = 0; j < n; j++) yt[j] += part[t][j];
   for (size_t j = 0; j < n; j++) y[j] = yt[j];
 }
 
@@ -1106,8 +1140,20 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       cMax[j] = std::pow(cMax[j], 0.5);
     }
     std::vector<Det> newDets;
-    for (int i = 0; i < SortedDetsSize; i++)
-      screenDet(s, schd, Dets[i], epsilon1 / std::fabs(cMax[i]), SortedDets.data(), SortedDetsSize, newDets);
+    {
+      const int nt = oracleThreads();
+      std::vector<std::vector<Det>> found(nt);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (nt > 1)
+      for (int i = 0; i < SortedDetsSize; i++) {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        screenDet(s, schd, Dets[i], epsilon1 / std::fabs(cMax[i]), SortedDets.data(), SortedDetsSize, found[t]);
+      }
+      for (auto& f : found) newDets.insert(newDets.end(), f.begin(), f.end());
+    }
     std::sort(newDets.begin(), newDets.end(), less);
     newDets.erase(std::unique(newDets.begin(), newDets.end(), eq), newDets.end());
     std::vector<std::vector<double>> X0(nroots, std::vector<double>(DetsSize + newDets.size(), 0.0));

@@ -256,3 +256,24 @@ def test_parity_and_excitation_distance_bruteforce():
                     continue
                 bd = helpers.occ_to_det(list(occ), 2)
                 assert abs(o.hij(bd, kd)[0] - v) < 1e-12
+
+
+def test_threaded_oracle_matches_serial(tmp_path, golden_dir, monkeypatch):
+    """The oracle's rank-parallel loops on four threads (the CPU baseline's
+    configuration) give the serial run's determinant sets and Davidson
+    iteration counts, and its energies to 1e-10 (sigma sums per-thread
+    partial vectors, as the reference's MPI_Reduce sums per-rank ones)."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "h8_631g", "FCIDUMP")
+    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\n" % (fcidump, tmp_path)
+    _write(tmp_path / "dice.dat", inp)
+    monkeypatch.setenv("ORACLE_THREADS", "1")
+    serial = OracleRun(str(tmp_path / "dice.dat"), str(tmp_path / "log1"))
+    monkeypatch.setenv("ORACLE_THREADS", "4")
+    threaded = OracleRun(str(tmp_path / "dice.dat"), str(tmp_path / "log4"))
+    a, b = serial.iterations(), threaded.iterations()
+    assert [x["ndets"] for x in a] == [x["ndets"] for x in b]
+    assert [x["davidson"] for x in a] == [x["davidson"] for x in b]
+    for x, y in zip(a, b):
+        assert abs(x["energy"] - y["energy"]) < 1e-10
+    np.testing.assert_array_equal(serial.dets(1), threaded.dets(1))
