@@ -18,7 +18,7 @@ bandwidth uses algorithmic bytes per launch: stored values + column indices,
 row offsets, diagonal, x read once and y written once.  `traffic` is the HBM
 bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
-profiles/sigma_traffic_c1_<precision>.json when that summary exists.
+profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
 
 cpu_baseline: the CPU oracle (a restatement of the reference algorithm whose
 rank-parallel loops run on OpenMP threads, up to 16) on a bounded sample of
