@@ -90,6 +90,11 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   E.autotune = opt.autotune != 0;
   E.useGraphs = opt.hip_graphs != 0;
   E.learned = opt.learned != 0;
+  if (E.direct != (opt.direct_sigma != 0)) {  // switching between stored and direct H: rebuild
+    E.hamRows = -1;
+    E.permRows = -1;
+  }
+  E.direct = opt.direct_sigma != 0;
   E.learnedControl = opt.learned == 2;
   // SHCI_LEARNED_MARGIN: log10 margin below eps1 of the learned form's
   // evaluation threshold (default 0.8)
@@ -126,10 +131,10 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", err.c_str());
     return fail(h, SHCI_BAD_INPUT, err);
   }
-  if (schd.Trev != 0 || schd.davidsonType != 0 || schd.restart || schd.fullrestart || schd.doSOC ||
+  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.restart || schd.fullrestart || schd.doSOC ||
       schd.searchLowest || schd.doLCC) {
     std::string m =
-        "keyword outside the variational path implemented here (Treversal, direct/disk davidson, restart, "
+        "keyword outside the variational path implemented here (Treversal, diskdavidson, restart, "
         "fullrestart, dosoc, searchForLowestEnergyDet, dolcc)";
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
@@ -137,6 +142,12 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
   std::srand(schd.randomSeed + 0);
   E.load(schd.integralFile);
   E.precision = precision;
+  // directdavidson: no stored Hamiltonian (HmultDirect) and davidsonDirect
+  // (SHCI/SHCIbasics.cpp:1092,1167-1172,1206-1208)
+  E.direct = schd.davidsonType == 1 || opt.direct_sigma != 0;
+  E.davidsonDirectRules = schd.davidsonType == 1;
+  E.davidsonPrint = nullptr;
+  E.davidsonPrintStart = startofCalc;
   E.ncore = schd.ncore;
   E.nact = schd.nact;
   E.restrictions = schd.restrictions;
@@ -233,6 +244,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);
     E.appendNew();
     const int64_t N = E.n;
+    if (E.direct && log) fprintf(log, "New size of determinant space %8lld\n", (long long)N);
     // X0 = [ci; 0]
     X0.reserve((size_t)N * nroots * 8);
     HIP_CHECK(hipMemsetAsync(X0.p, 0, (size_t)N * nroots * 8, E.stream));
@@ -246,9 +258,13 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     const double t2 = wallTime();
     const double prevE0 = iter == 0 ? -10.0 : E0[0];
     int numIter = 0;
+    const int looseCopies = E.direct ? nroots + 2 : nroots + 4;
     E0 = E.davidsonMode == 1 ? E.davidsonBlock(X0.as<double>(), nroots, schd.davidsonTolLoose, numIter, false)
-                             : E.davidson(X0.as<double>(), nroots, nroots + 4, schd.davidsonTolLoose, numIter, false);
-    if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
+                             : E.davidson(X0.as<double>(), nroots, looseCopies, schd.davidsonTolLoose, numIter, false);
+    if ((int)E0.size() != nroots) {
+      if (log) fprintf(log, "%s\n", E.err.c_str());
+      return fail(h, SHCI_NO_CONVERGENCE, E.err);
+    }
     const double t3 = wallTime();
     const double t = t3 - startofCalc;
     if (log) {
@@ -277,8 +293,10 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
       if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
       const double tf0 = wallTime();
+      if (E.direct) E.davidsonPrint = log;  // davidsonDirect(..., print = true)
       E0 = E.davidsonMode == 1 ? E.davidsonBlock(E.ci.as<double>(), nroots, schd.davidsonTol, numIter, true)
                                : E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
+      E.davidsonPrint = nullptr;
       E.tFinalDavidson = wallTime() - tf0;
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <cstdio>
 #include <ctime>
+#include <sys/time.h>
 
 namespace shci {
 
@@ -172,6 +173,13 @@ void Engine::appendNew() {
   if (slots == oldSlots) hashInsert(S, dets.as<uint64_t>(), old, n, table.as<int32_t>(), slots, stream);
 }
 
+// gettimeofday seconds (the clock of the run's elapsed-time columns)
+static double wallClock() {
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
 static double wallSeconds() {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -193,6 +201,17 @@ void Engine::buildHamiltonian() {
   if (nDiag < n) launchEnergy(S, dets.as<uint64_t>() + nDiag * hs.W, n - nDiag, diag.as<double>() + nDiag, stream);
   nDiag = n;
   if (hamRows == n) return;
+  if (direct) {  // HmultDirect: sigma evaluates H from the string tables
+    buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
+    for (DevMem* m : {&rowPtr, &cols, &vals, &vals32, &valsLo, &rowPtrNew, &colsNew, &valsNew, &vals32New,
+                      &valsLoNew, &permCols, &permVals})
+      m->release();
+    nnzOff = 0;
+    permRows = -1;
+    hamRows = n;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return;
+  }
   const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
   // SHCI_TIMING: print the sub-phase wall times of this build to stderr
   const bool timing = std::getenv("SHCI_TIMING") != nullptr;
@@ -594,6 +613,11 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
                 2.0 * k * n * 8;
     HIP_CHECK(hipEventRecord(ev->start, stream));
   }
+  if (direct) {
+    launchSigmaDirect(S, dets.as<uint64_t>(), T, diag.as<double>(), dx, dy, k, ld, stream);
+    if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+    return;
+  }
   const bool perm = permRows == n && !(f32 && exact);
   const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
   if (perm) sigmaCounter.reserve(256);
@@ -708,6 +732,7 @@ std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double 
     for (int i = 0; i < cr0; i++) {
       if (std::sqrt(hres[i]) > tol) {
         convergedRoot = i;
+        if (davidsonPrint) std::fprintf(davidsonPrint, "going back to converged root %d\n", i);
         break;
       }
     }
@@ -720,21 +745,42 @@ std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double 
     } else {
       error = std::sqrt(hres[cr0]);
     }
+    if (davidsonPrint) {  // davidsonDirect's progress lines (SHCI/Davidson.cpp:526-530)
+      if (numIter == 0) std::fprintf(davidsonPrint, "nIter  Root               Energy                Error\n");
+      std::fprintf(davidsonPrint, "%5i  %4i   %18.10g   %18.10g  %10.2f\n", numIter, convergedRoot, w[cr0], error,
+                   wallClock() - davidsonPrintStart);
+    }
     numIter++;
     if (bsize == N) {
       for (int i = 0; i < nr; i++) {
         HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)i * N, col(B, i), N * 8, hipMemcpyDeviceToDevice, s));
         eroots.push_back(w[i]);
+        if (davidsonPrint)
+          std::fprintf(davidsonPrint, "#niter:%3d root:%3d -> Energy : %18.10g  \n", numIter, i, w[i]);
       }
       HIP_CHECK(hipStreamSynchronize(s));
       return eroots;
     }
-    if (error < tol || numIter > 800 * nr) {
+    // the reference's davidson caps at 800 iterations per root (and gives up
+    // at 2000); davidsonDirect at 400, with a message (Davidson.cpp:305-309,
+    // 548-555)
+    const int cap = davidsonDirectRules ? 400 * nr : 800 * nr;
+    if (error < tol || numIter > cap) {
+      if (davidsonDirectRules && numIter > cap) {
+        char m[128];
+        std::snprintf(m, sizeof m, "Davidson calculation did not converge for root %3d, #iter %5d",
+                      convergedRoot + 1, numIter);
+        err = m;
+        return {};
+      }
       if (numIter > 2000 * nr) {
         err = "Davidson calculation did not converge";
         return eroots;
       }
       convergedRoot++;
+      if (davidsonPrint)
+        std::fprintf(davidsonPrint, "#niter:%3d root:%3d -> Energy : %18.10g  \n", numIter, convergedRoot - 1,
+                     w[cr0]);
       if (convergedRoot == nr) {
         for (int i = 0; i < nr; i++) {
           HIP_CHECK(hipMemcpyAsync(dX0 + (int64_t)i * N, col(B, i), N * 8, hipMemcpyDeviceToDevice, s));
@@ -754,8 +800,9 @@ std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double 
       launchScaleInto(col(B, bsize), r, sc + 2, N, s);
       bsize++;
     } else {
-      bsize = nr + 3;
-      sigmaSize = nr + 2;
+      // restart (davidsonDirect keeps min(nroots + 3, maxCopies) vectors)
+      bsize = davidsonDirectRules ? std::min(nr + 3, maxCopies) : nr + 3;
+      sigmaSize = bsize - 1;
       launchScaleInto(col(B, bsize - 1), r, sc + 2, N, s);
     }
   }

@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <cstdio>
 #include <map>
 #include <string>
 #include <vector>
@@ -63,6 +64,13 @@ struct Engine {
 
   // Davidson scratch
   DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;
+  // the reference's direct mode (`directdavidson`): no stored H, sigma from
+  // the string tables (HmultDirect), and davidsonDirect's restart size,
+  // iteration cap and progress lines (printed to davidsonPrint when set)
+  bool direct = false;
+  bool davidsonDirectRules = false;
+  FILE* davidsonPrint = nullptr;
+  double davidsonPrintStart = 0;
 
   // batching form: block Davidson whose new block's sigma is one SpMM,
   // iterations replayed from captured hipGraphs, sigma launch shape chosen

@@ -94,6 +94,10 @@ void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables
                     int64_t* rowCount, hipStream_t s);
 void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
                    const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s);
+// direct sigma without a stored Hamiltonian (HmultDirect, SHCI/Hmult.h:153-382):
+// y[:, r] = diag .* x[:, r] + sum_J Hij(D_I, D_J) x[J, r]; y must not overlap x
+void launchSigmaDirect(const DevSystem& S, const uint64_t* dets, const StringTables& T, const double* diag,
+                       const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // fill with each value stored as FP32 hi = fp32(v) and lo = fp32(v - hi)
 void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
                         const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s);

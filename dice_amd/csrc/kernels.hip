@@ -576,6 +576,96 @@ __device__ __forceinline__ int64_t firstDetAtLeast(const uint32_t* a, int64_t lo
   return lo;
 }
 
+// Visits every connection of row I (all J != I one or two excitations
+// apart, restricted to J >= jmin): sink.hit(bool, J) is called by all 64
+// lanes together, once per candidate slot.  (a) the same-beta group: alpha
+// singles and alpha-alpha doubles; (b) the same-alpha group: beta singles
+// and beta-beta doubles; (c) alpha single x beta single: members of the
+// groups of alpha strings one excitation away whose beta string is one
+// excitation away.  Group members are in increasing determinant index, so
+// jmin skips a prefix of each group (binary search).
+template <class Sink>
+__device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint32_t jmin, int wave, int lane,
+                                             int (*sExcl)[65], int64_t (*sBeg)[64], Sink& sink) {
+  const uint64_t astr = T.alpha[I], bstr = T.beta[I];
+  const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
+  {
+    int64_t b0 = T.bStart[bi], b1 = T.bStart[bi + 1];
+    if (jmin) b0 = firstDetAtLeast(T.bMajorDet, b0, b1, jmin);
+    for (int64_t m0 = b0; m0 < b1; m0 += 64) {
+      int64_t m = m0 + lane;
+      bool h = false;
+      uint32_t J = 0;
+      if (m < b1) {
+        int dd = __popcll(T.bMajorA[m] ^ astr);
+        h = (dd == 2 || dd == 4);
+        J = T.bMajorDet[m];
+      }
+      sink.hit(h, J);
+    }
+  }
+  {
+    int64_t a0 = T.aStart[ai], a1 = T.aStart[ai + 1];
+    if (jmin) a0 = firstDetAtLeast(T.aMajorDet, a0, a1, jmin);
+    for (int64_t m0 = a0; m0 < a1; m0 += 64) {
+      int64_t m = m0 + lane;
+      bool h = false;
+      uint32_t J = 0;
+      if (m < a1) {
+        int dd = __popcll(T.aMajorB[m] ^ bstr);
+        h = (dd == 2 || dd == 4);
+        J = T.aMajorDet[m];
+      }
+      sink.hit(h, J);
+    }
+  }
+  {
+    int64_t s0 = T.aSinglesStart[ai], s1 = T.aSinglesStart[ai + 1];
+    for (int64_t c0 = s0; c0 < s1; c0 += 64) {
+      int64_t c = c0 + lane;
+      int len = 0;
+      int64_t beg = 0;
+      if (c < s1) {
+        uint32_t sa = T.aSingles[c];
+        beg = T.aStart[sa];
+        int64_t end = T.aStart[sa + 1];
+        if (jmin) beg = firstDetAtLeast(T.aMajorDet, beg, end, jmin);
+        len = (int)(end - beg);
+      }
+      int inc = len;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      sExcl[wave][lane] = inc - len;
+      sBeg[wave][lane] = beg;
+      int total = __shfl(inc, 63, 64);
+      __syncwarp();
+      for (int k0 = 0; k0 < total; k0 += 64) {
+        int k = k0 + lane;
+        bool h = false;
+        uint32_t J = 0;
+        if (k < total) {
+          int lo = 0, hi = 63;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sExcl[wave][mid] <= k)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          int64_t m = sBeg[wave][lo] + (k - sExcl[wave][lo]);
+          h = __popcll(T.aMajorB[m] ^ bstr) == 2;
+          J = T.aMajorDet[m];
+        }
+        sink.hit(h, J);
+      }
+      __syncwarp();
+    }
+  }
+}
+
 // Rows I < nOld enumerate only connections to J >= nOld (the columns the
 // previous Hamiltonian lacks); rows I >= nOld enumerate all connections.
 template <int W, bool FILL, bool SPLIT>
@@ -589,8 +679,6 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     uint64_t dI[W];
 #pragma unroll
     for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
-    const uint64_t astr = T.alpha[I], bstr = T.beta[I];
-    const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
     const uint32_t jmin = I >= nOld ? 0u : (uint32_t)nOld;
     RowSink<W, FILL, SPLIT> sink;
     sink.S = &S;
@@ -603,90 +691,69 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     sink.lo = lo;
     sink.pos = FILL ? fillStart[I] : 0;
     sink.cnt = 0;
-    // (a) same beta string: alpha singles and alpha-alpha doubles
-    {
-      int64_t b0 = T.bStart[bi], b1 = T.bStart[bi + 1];
-      if (jmin) b0 = firstDetAtLeast(T.bMajorDet, b0, b1, jmin);
-      for (int64_t m0 = b0; m0 < b1; m0 += 64) {
-        int64_t m = m0 + lane;
-        bool h = false;
-        uint32_t J = 0;
-        if (m < b1) {
-          int dd = __popcll(T.bMajorA[m] ^ astr);
-          h = (dd == 2 || dd == 4);
-          J = T.bMajorDet[m];
-        }
-        sink.hit(h, J);
-      }
-    }
-    // (b) same alpha string: beta singles and beta-beta doubles
-    {
-      int64_t a0 = T.aStart[ai], a1 = T.aStart[ai + 1];
-      if (jmin) a0 = firstDetAtLeast(T.aMajorDet, a0, a1, jmin);
-      for (int64_t m0 = a0; m0 < a1; m0 += 64) {
-        int64_t m = m0 + lane;
-        bool h = false;
-        uint32_t J = 0;
-        if (m < a1) {
-          int dd = __popcll(T.aMajorB[m] ^ bstr);
-          h = (dd == 2 || dd == 4);
-          J = T.aMajorDet[m];
-        }
-        sink.hit(h, J);
-      }
-    }
-    // (c) alpha single x beta single: members of the groups of alpha strings
-    // one excitation away whose beta string is one excitation away
-    {
-      int64_t s0 = T.aSinglesStart[ai], s1 = T.aSinglesStart[ai + 1];
-      for (int64_t c0 = s0; c0 < s1; c0 += 64) {
-        int64_t c = c0 + lane;
-        int len = 0;
-        int64_t beg = 0;
-        if (c < s1) {
-          uint32_t sa = T.aSingles[c];
-          beg = T.aStart[sa];
-          int64_t end = T.aStart[sa + 1];
-          if (jmin) beg = firstDetAtLeast(T.aMajorDet, beg, end, jmin);
-          len = (int)(end - beg);
-        }
-        int inc = len;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          int t = __shfl_up(inc, o, 64);
-          if (lane >= o) inc += t;
-        }
-        sExcl[wave][lane] = inc - len;
-        sBeg[wave][lane] = beg;
-        int total = __shfl(inc, 63, 64);
-        __syncwarp();
-        for (int k0 = 0; k0 < total; k0 += 64) {
-          int k = k0 + lane;
-          bool h = false;
-          uint32_t J = 0;
-          if (k < total) {
-            int lo = 0, hi = 63;
-            while (lo < hi) {
-              int mid = (lo + hi + 1) >> 1;
-              if (sExcl[wave][mid] <= k)
-                lo = mid;
-              else
-                hi = mid - 1;
-            }
-            int64_t m = sBeg[wave][lo] + (k - sExcl[wave][lo]);
-            h = __popcll(T.aMajorB[m] ^ bstr) == 2;
-            J = T.aMajorDet[m];
-          }
-          sink.hit(h, J);
-        }
-        __syncwarp();
-      }
-    }
+    enumerateRow(T, I, jmin, wave, lane, sExcl, sBeg, sink);
     if (!FILL) {
       int c = sink.cnt;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
       if (lane == 0) rowCount[I] = c;
+    }
+  }
+}
+
+// Direct sigma (no stored H): y[:, r] = diag .* x[:, r] + sum over the
+// connections J of row I of Hij(bra = D_I, ket = D_J) x[J, r], the
+// orientation of the reference's direct product; each element is evaluated
+// once for all k <= 8 vectors.
+template <int W>
+struct DirectSink {
+  const DevSystem* S;
+  const uint64_t* dets;
+  const uint64_t* dI;
+  const double* x;
+  int64_t ld;
+  int k;
+  double acc[8];
+  __device__ __forceinline__ void hit(bool h, uint32_t J) {
+    if (!h) return;
+    uint64_t dJ[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) dJ[w] = dets[(int64_t)J * W + w];
+    const double v = hijElement<W>(*S, dI, dJ);
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+      if (r < k) acc[r] = fma(v, x[r * ld + J], acc[r]);
+  }
+};
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) kSigmaDirect(DevSystem S, const uint64_t* dets, RowTables T,
+                                                       const double* diag, const double* x, double* y, int k,
+                                                       int64_t ld) {
+  __shared__ int sExcl[kWaves][65];
+  __shared__ int64_t sBeg[kWaves][64];
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < T.n; I += (int64_t)gridDim.x * kWaves) {
+    uint64_t dI[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
+    DirectSink<W> sink;
+    sink.S = &S;
+    sink.dets = dets;
+    sink.dI = dI;
+    sink.x = x;
+    sink.ld = ld;
+    sink.k = k;
+#pragma unroll
+    for (int r = 0; r < 8; r++) sink.acc[r] = 0.0;
+    enumerateRow(T, I, 0u, wave, lane, sExcl, sBeg, sink);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      if (r >= k) break;
+      double v = sink.acc[r];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) y[r * ld + I] = fma(diag[I], x[r * ld + I], v);
     }
   }
 }
@@ -1567,6 +1634,19 @@ void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTa
   RowTables r = rowTables(T);
   DISPATCH_W(S.W, kCsrRows<WW, true, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, nullptr,
                                                                   hi, lo));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaDirect(const DevSystem& S, const uint64_t* dets, const StringTables& T, const double* diag,
+                       const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  if (T.n <= 0 || k <= 0) return;
+  unsigned g = gridFor(T.n, kWaves, 1 << 16);
+  RowTables r = rowTables(T);
+  for (int c0 = 0; c0 < k; c0 += 8) {
+    const int kk = std::min(8, k - c0);
+    DISPATCH_W(S.W, kSigmaDirect<WW><<<g, kBlock, 0, s>>>(S, dets, r, diag, x + (int64_t)c0 * ld,
+                                                             y + (int64_t)c0 * ld, kk, ld));
+  }
   HIP_CHECK(hipGetLastError());
 }
 

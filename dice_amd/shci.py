@@ -45,7 +45,7 @@ class RunOptions(ctypes.Structure):
     """shci_run_options of include/shci_gpu.h."""
     _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("davidson_block", ctypes.c_int),
                 ("autotune", ctypes.c_int), ("hip_graphs", ctypes.c_int), ("learned", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("direct_sigma", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
 
 
 class ShciError(RuntimeError):
@@ -169,9 +169,10 @@ class System:
         self._check(self._lib.shci_set_precision(self._h, prec))
 
     def set_options(self, precision=PREC_FP64, davidson_block=False, autotune=False, hip_graphs=False,
-                    profile_sigma=False):
+                    profile_sigma=False, direct_sigma=False):
         """Method options for the piecewise calls (see shci_run_options)."""
         o = RunOptions()
+        o.direct_sigma = int(bool(direct_sigma))
         o.precision = precision
         o.profile_sigma = int(bool(profile_sigma))
         o.davidson_block = int(bool(davidson_block))

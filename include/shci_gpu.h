@@ -147,7 +147,11 @@ typedef struct {
                          SHCI_LEARNED_MARGIN) are tested exactly; singles stay exact; when a sample shows
                          that more than half would be tested, that iteration and the later ones screen in
                          full.  2: the same with the untrained network (control) */
-  int reserved[2];
+  int direct_sigma;   /* nonzero: no stored Hamiltonian; every sigma evaluates the elements from the string
+                         tables (HmultDirect, SHCI/Hmult.h:153-382).  shci_run_ex also turns it on, with
+                         davidsonDirect's rules (SHCI/Davidson.cpp:353-590), for the `directdavidson`
+                         keyword */
+  int reserved[1];
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,

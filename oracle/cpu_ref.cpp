@@ -816,6 +816,24 @@ void hmult(const SparseHam& H, const double* x, double* y, size_t n) {
   for (size_t j = 0; j < n; j++) y[j] = yt[j];
 }
 
+void hmultDirect(const System& s, const std::vector<Det>& dets, const SparseHam& H, const double* x, double* y,
+                 size_t n) {
+  std::vector<double> yt(n, 0.0);
+  for (size_t i = 0; i < H.connections.size(); i++) {
+    const std::vector<int>& c = H.connections[i];
+    for (size_t j = 0; j < c.size(); j++) {
+      const size_t J = (size_t)c[j];
+      if (J == i) {
+        yt[i] += energy(s, dets[i]) * x[i];
+      } else {
+        yt[i] += hij(s, dets[i], dets[J]) * x[J];
+        yt[J] += hij(s, dets[J], dets[i]) * x[i];
+      }
+    }
+  }
+  for (size_t j = 0; j < n; j++) y[j] = yt[j];
+}
+
 // ---------------------------------------------------------------------------
 // cyclic Jacobi eigensolver for the small symmetric subspace matrix.
 // A row-major m x m; w ascending; V row-major with eigenvectors in columns.
@@ -885,8 +903,16 @@ static void setRandom(double* a, size_t n) {  // Eigen random(): -1 + 2*rand()/R
 std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>>& x0,
                              const std::vector<double>& diag, int maxCopies, double tol, int& numIter,
                              bool print, FILE* out) {
+  const size_t n = x0[0].size();
+  return davidson([&](const double* x, double* y) { hmult(H, x, y, n); }, n, x0, diag, maxCopies, tol, numIter,
+                  print, out);
+}
+
+std::vector<double> davidson(const std::function<void(const double*, double*)>& op, size_t N,
+                             std::vector<std::vector<double>>& x0, const std::vector<double>& diag, int maxCopies,
+                             double tol, int& numIter, bool print, FILE* out, bool directRules,
+                             double startofCalc) {
   std::vector<double> eroots;
-  size_t N = x0[0].size();
   int nroots = (int)x0.size();
   std::vector<double> b(N * maxCopies, 0.0), sigma(N * maxCopies, 0.0), r(N, 0.0);
   auto col = [&](std::vector<double>& M, int c) { return M.data() + (size_t)c * N; };
@@ -917,7 +943,7 @@ std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>
   std::vector<double> hsub, w, V, tmp;
   while (true) {
     for (int i = sigmaSize; i < bsize; i++) {
-      hmult(H, col(b, i), col(sigma, i), N);
+      op(col(b, i), col(sigma, i));
       sigmaSize++;
     }
     hsub.assign((size_t)bsize * bsize, 0.0);
@@ -957,15 +983,35 @@ std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>
     }
     for (size_t k = 0; k < N; k++) r[k] = col(sigma, convergedRoot)[k] - ei * col(b, convergedRoot)[k];
     double error = norm(r.data(), N);
+    if (directRules && print && out) {  // SHCI/Davidson.cpp:526-530
+      if (numIter == 0) fprintf(out, "nIter  Root               Energy                Error\n");
+      fprintf(out, "%5i  %4i   %18.10g   %18.10g  %10.2f\n", numIter, convergedRoot, ei, error,
+              wallTime() - startofCalc);
+    }
     numIter++;
     if ((size_t)bsize == N) {
       for (int i = 0; i < nroots; i++) {
         std::copy(col(b, i), col(b, i) + N, x0[i].begin());
         eroots.push_back(w[i]);
+        if (directRules && print && out) fprintf(out, "#niter:%3d root:%3d -> Energy : %18.10g  \n", numIter, i, w[i]);
       }
       return eroots;
     }
-    if (error < tol || numIter > 800 * nroots) {
+    if (directRules && (error < tol || numIter > 400 * nroots)) {  // SHCI/Davidson.cpp:548-565
+      if (numIter > 400 * nroots) {
+        if (out) fprintf(out, "Davidson calculation did not converge for root %3d, #iter %5d\n", convergedRoot + 1, numIter);
+        return {};
+      }
+      convergedRoot++;
+      if (print && out) fprintf(out, "#niter:%3d root:%3d -> Energy : %18.10g  \n", numIter, convergedRoot - 1, ei);
+      if (convergedRoot == nroots) {
+        for (int i = 0; i < convergedRoot; i++) {
+          std::copy(col(b, i), col(b, i) + N, x0[i].begin());
+          eroots.push_back(w[i]);
+        }
+        return eroots;
+      }
+    } else if (!directRules && (error < tol || numIter > 800 * nroots)) {
       if (numIter > 2000 * nroots) {
         if (out) fprintf(out, "Davidson calculation did not converge for root %3d, #iter %5d\n", convergedRoot + 1, numIter);
         return eroots;
@@ -996,8 +1042,8 @@ std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>
       for (size_t k = 0; k < N; k++) col(b, bsize)[k] = r[k] / nr;
       bsize++;
     } else {
-      bsize = nroots + 3;
-      sigmaSize = nroots + 2;
+      bsize = directRules ? std::min(nroots + 3, maxCopies) : nroots + 3;
+      sigmaSize = bsize - 1;
       for (size_t k = 0; k < N; k++) col(b, bsize - 1)[k] = r[k] / nr;
     }
   }
@@ -1048,10 +1094,10 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     if (log) fprintf(log, "%s\n", err.c_str());
     return;
   }
-  if (schd.Trev != 0 || schd.davidsonType != 0 || schd.restart || schd.fullrestart || schd.doSOC ||
+  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.restart || schd.fullrestart || schd.doSOC ||
       schd.searchForLowestEnergyDet || schd.doLCC) {
     res.status = 2;
-    res.error = "keyword outside the restated variational path (Treversal/direct/disk/restart/soc/search/lcc)";
+    res.error = "keyword outside the restated variational path (Treversal/disk/restart/soc/search/lcc)";
     if (log) fprintf(log, "%s\n", res.error.c_str());
     return;
   }
@@ -1132,6 +1178,9 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
   populateHelpers(s, helpers, Dets, 0);
   makeHRows(s, helpers, Dets, 0, H);
   if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
+  // directdavidson: the same product with every element evaluated on the fly
+  const bool direct = schd.davidsonType == 1;
+  auto directOp = [&](const double* x, double* y) { hmultDirect(s, Dets, H, x, y, (size_t)DetsSize); };
   for (int iter = 0; iter < (int)schd.epsilon1.size(); iter++) {
     double epsilon1 = schd.epsilon1[iter];
     std::vector<double> cMax(DetsSize, 0.0);
@@ -1160,6 +1209,7 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     for (int i = 0; i < nroots; i++) std::copy(ci[i].begin(), ci[i].end(), X0[i].begin());
     Dets.insert(Dets.end(), newDets.begin(), newDets.end());
     DetsSize = (int)Dets.size();
+    if (direct && log) fprintf(log, "New size of determinant space %8i\n", DetsSize);  // SHCIbasics.cpp:1083-1084
     populateHelpers(s, helpers, Dets, SortedDetsSize);
     makeHRows(s, helpers, Dets, SortedDetsSize, H);
     SortedDets = Dets;
@@ -1171,7 +1221,16 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     double prevE0 = E0[0];
     if (iter == 0) prevE0 = -10.0;
     int numIter = 0;
-    E0 = davidson(H, X0, diag, nroots + 4, schd.davidsonTolLoose, numIter, schd.outputlevel > 0, log);
+    if (direct)  // davidsonDirect(Hdirect, X0, diag, nroots + 2, ...), SHCIbasics.cpp:1167-1169
+      E0 = davidson(directOp, (size_t)DetsSize, X0, diag, nroots + 2, schd.davidsonTolLoose, numIter,
+                    schd.outputlevel > 0, log, true, startofCalc);
+    else
+      E0 = davidson(H, X0, diag, nroots + 4, schd.davidsonTolLoose, numIter, schd.outputlevel > 0, log);
+    if ((int)E0.size() != nroots) {
+      res.status = 3;
+      res.error = "Davidson calculation did not converge";
+      return;
+    }
     double t = wallTime() - startofCalc;
     if (log) {
       fprintf(log, "%4d %4d  %10.2e  %10d", iter, 0, schd.epsilon1[iter], DetsSize);
@@ -1194,7 +1253,16 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     for (int i = 0; i < nroots; i++) ci[i] = X0[i];
     if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
       if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
-      E0 = davidson(H, ci, diag, nroots + 4, schd.davidsonTol, numIter, false, log);
+      if (direct)  // SHCIbasics.cpp:1206-1208
+        E0 = davidson(directOp, (size_t)DetsSize, ci, diag, nroots + 4, schd.davidsonTol, numIter, true, log, true,
+                      startofCalc);
+      else
+        E0 = davidson(H, ci, diag, nroots + 4, schd.davidsonTol, numIter, false, log);
+      if ((int)E0.size() != nroots) {
+        res.status = 3;
+        res.error = "Davidson calculation did not converge";
+        return;
+      }
       if (log) fprintf(log, "Exiting variational iterations\n");
       break;
     }

@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -141,6 +142,18 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets,
                int start, SparseHam& H);
 void hmult(const SparseHam& H, const double* x, double* y, size_t n);
 void symEig(int m, const double* A, double* w, double* V);
+// HmultDirect (SHCI/Hmult.h:153-382): y = H x with every element evaluated
+// on the fly as Hij(bra = D_I, ket = D_J) for row I; the connectivity is
+// taken from the stored rows of H
+void hmultDirect(const System& s, const std::vector<Det>& dets, const SparseHam& H, const double* x, double* y,
+                 size_t n);
+// davidson (SHCI/Davidson.cpp:111-348) on the product `op`; directRules
+// selects davidsonDirect's restart size, iteration cap and progress lines
+// (SHCI/Davidson.cpp:353-590)
+std::vector<double> davidson(const std::function<void(const double*, double*)>& op, size_t N,
+                             std::vector<std::vector<double>>& x0, const std::vector<double>& diag, int maxCopies,
+                             double tol, int& numIter, bool print, FILE* out, bool directRules = false,
+                             double startofCalc = 0.0);
 std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>>& x0,
                              const std::vector<double>& diag, int maxCopies,
                              double tol, int& numIter, bool print, FILE* out);

@@ -137,6 +137,11 @@ void oref_sigma(void* p, const double* x, double* y) {
   Handle* h = (Handle*)p;
   hmult(h->H, x, y, h->H.connections.size());
 }
+// HmultDirect over the determinants appended with oref_csr_append
+void oref_sigma_direct(void* p, const double* x, double* y) {
+  Handle* h = (Handle*)p;
+  hmultDirect(h->s, h->dets, h->H, x, y, h->H.connections.size());
+}
 // Davidson on the stored H (x0: nroots columns of length n, in/out).
 int oref_davidson(void* p, double* x0, int nroots, const double* diag, int maxCopies, double tol, double* eroots) {
   Handle* h = (Handle*)p;

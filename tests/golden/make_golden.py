@@ -30,6 +30,7 @@ ENERGIES = {
     "restart": "tests/SHCI/restart/trusted_hci.e",
     "n2_RAS": "tests/SHCI/n2_RAS/trusted_hci.e",
     "o2_det_var_plus_pt": "tests/SHCI/o2_det/trusted_hci.e",
+    "o2_det_direct": "tests/SHCI/o2_det_direct/trusted_hci.e",
 }
 
 

@@ -51,6 +51,7 @@ def load():
         "oref_csr_append": ([P, pI64, I64], I64),
         "oref_csr_get": ([P, pI64, ctypes.POINTER(ctypes.c_int32), pD], None),
         "oref_sigma": ([P, pD, pD], None),
+        "oref_sigma_direct": ([P, pD, pD], None),
         "oref_davidson": ([P, pD, I32, pD, I32, D, pD], I32),
         "oref_sym_eig": ([I32, pD, pD, pD], None),
         "oref_run": ([ctypes.c_char_p, ctypes.c_char_p, I32], P),
@@ -145,6 +146,13 @@ class OracleSystem:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         self.L.oref_sigma(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+        return y
+
+    def sigma_direct(self, x):
+        """HmultDirect: every element evaluated on the fly, Hij(D_I, D_J) for row I."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.L.oref_sigma_direct(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
         return y
 
     def davidson(self, x0, diag, max_copies, tol):

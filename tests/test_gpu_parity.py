@@ -497,3 +497,72 @@ def test_learned_form_gain_over_untrained_network(tmp_path, monkeypatch):
     assert abs(ctl.energies()[0] - plain) < 1e-10
     # fewer determinants can only raise the variational energy
     assert r.energies()[0] >= plain - 1e-9
+
+
+# ---- direct mode (no stored Hamiltonian) ----
+
+@pytest.mark.parametrize("name", ["h8", "c0", "w2", "hub"])
+def test_direct_sigma_matches_oracle(name, systems):
+    """Sigma with every element evaluated from the string tables (HmultDirect
+    orientation Hij(D_I, D_J)) equals the oracle's direct product and the
+    stored-matrix product to 1e-12 relative, for one and for three vectors."""
+    path = systems[name][0]
+    dets = _space(name, systems, 2500, 1)
+    g, o = _gpu(path), OracleSystem(path)
+    g.set_options(direct_sigma=True)
+    g.set_dets(dets)
+    g.make_hamiltonian()
+    o.csr_append(dets)
+    x = np.random.default_rng(9).normal(size=(len(dets), 3))
+    y = g.hmult(x)
+    for r in range(3):
+        yo = o.sigma_direct(np.ascontiguousarray(x[:, r]))
+        assert np.max(np.abs(y[:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
+        assert np.max(np.abs(y[:, r] - o.sigma(np.ascontiguousarray(x[:, r])))) <= 1e-12 * np.max(np.abs(yo))
+
+
+O2_INPUT = ("nocc 12\n0 2 4 6 8 10 1 3 5 7 9 11\nend\nepsilon2 1e-07\nsampleN 200\ndavidsonTol 5e-05\n"
+            "dE 1e-08\nschedule\n0 0.001\n3 0.0001\nend\ndeterministic\nnoio\nmaxiter 9\nnPTiter 0\n")
+
+
+def test_run_o2_det_trusted(tmp_path):
+    """tests/SHCI/o2_det: the trusted E_var to 1e-9 Ha, and the oracle's run
+    (determinant sets, energies) on the GPU."""
+    import json
+    with open(os.path.join(GOLD, "golden.json")) as f:
+        g = json.load(f)
+    inp = _write_input(tmp_path, O2_INPUT + "orbitals %s\nprefix %s\n" % (os.path.join(GOLD, "o2", "FCIDUMP"),
+                                                                       tmp_path))
+    o, r = _compare_runs(inp, tmp_path, 1)
+    assert abs(r.energies()[0] - g["trusted_energy"]["o2_det_var_plus_pt"]) < 1e-9
+
+
+def test_run_direct_davidson_o2(tmp_path):
+    """`directdavidson` (HmultDirect + davidsonDirect) on the o2_det_direct
+    input: the oracle's determinant sets, per-iteration energies and
+    Davidson iteration counts, and the direct mode's log lines."""
+    from dice_amd import shci
+    inp = _write_input(tmp_path, O2_INPUT + "orbitals %s\nprefix %s\ndirectdavidson\n" % (
+        os.path.join(GOLD, "o2", "FCIDUMP"), tmp_path))
+    o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"))
+    oi, gi = o.iterations(), r.iterations()
+    assert [x["ndets"] for x in oi] == [x["ndets"] for x in gi]
+    assert [x["davidson"] for x in oi] == [x["davidson"] for x in gi]
+    for a, b in zip(oi, gi):
+        assert abs(a["energy"] - b["energy"]) < 1e-10
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(r.dets(), o.dets(1))
+    log = open(tmp_path / "gpu.log").read()
+    assert "New size of determinant space" in log and "nIter  Root               Energy" in log
+    # progress lines of the final davidsonDirect match the oracle's: iteration
+    # and root exactly, energy to 1e-10 relative, residual norm to 1e-6
+    # relative (summation order), time column aside
+    def prog(path):
+        return [l.split()[:4] for l in open(path) if l[:5].strip().isdigit() and len(l.split()) == 5]
+    a, b = prog(tmp_path / "gpu.log"), prog(tmp_path / "oracle.log")
+    assert len(a) == len(b) > 0
+    for u, v in zip(a, b):
+        assert u[:2] == v[:2]
+        assert abs(float(u[2]) - float(v[2])) <= 1e-10 * abs(float(v[2]))
+        assert abs(float(u[3]) - float(v[3])) <= 1e-6 * abs(float(v[3]))

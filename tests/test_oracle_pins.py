@@ -277,3 +277,61 @@ def test_threaded_oracle_matches_serial(tmp_path, golden_dir, monkeypatch):
     for x, y in zip(a, b):
         assert abs(x["energy"] - y["energy"]) < 1e-10
     np.testing.assert_array_equal(serial.dets(1), threaded.dets(1))
+
+
+def test_direct_product_equals_stored_product(golden_dir):
+    """HmultDirect's product (every element evaluated on the fly as
+    Hij(D_I, D_J) for row I) equals the stored-matrix product to rounding:
+    the two orientations of a Slater-Condon single sum the same terms in a
+    different order."""
+    fcidump = os.path.join(golden_dir, "h8_631g", "FCIDUMP")
+    o = OracleSystem(fcidump)
+    dets = helpers.occ_to_det([0, 2, 4, 6, 1, 3, 5, 7], o.words)[None, :]
+    eps = 1e-2
+    while len(dets) < 1500:
+        new = o.screen(dets, np.ones(len(dets)), eps)
+        if len(new):
+            dets = np.concatenate([dets, new])
+        eps *= 0.5
+    o.csr_append(dets)
+    x = np.random.default_rng(3).normal(size=len(dets))
+    y, yd = o.sigma(x), o.sigma_direct(x)
+    assert np.max(np.abs(y - yd)) <= 1e-13 * np.max(np.abs(y))
+
+
+O2_INPUT = ("nocc 12\n0 2 4 6 8 10 1 3 5 7 9 11\nend\nepsilon2 1e-07\nsampleN 200\ndavidsonTol 5e-05\n"
+            "dE 1e-08\nschedule\n0 0.001\n3 0.0001\nend\ndeterministic\nnoio\nmaxiter 9\nnPTiter 0\n")
+
+
+def test_o2_det_trusted_energy(tmp_path, golden_dir):
+    """tests/SHCI/o2_det (8 orbitals, 12 electrons, eps1 1e-3 then 1e-4):
+    the stored trusted_hci.e is reproduced to 1e-9 Ha by the variational
+    step (the file holds E_var: shci.e is written after the variational
+    step)."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "o2", "FCIDUMP")
+    _write(tmp_path / "in.dat", O2_INPUT + "orbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+    r = OracleRun(str(tmp_path / "in.dat"), str(tmp_path / "log"))
+    assert abs(r.energies()[0] - g["trusted_energy"]["o2_det_var_plus_pt"]) < 1e-9
+
+
+def test_direct_davidson_run_o2(tmp_path, golden_dir):
+    """`directdavidson` on the same input: the run prints the direct mode's
+    lines and uses davidsonDirect with the loose tolerance the reference's
+    input reader forces for it (3e-2, SHCI/input.cpp:388).  With that
+    tolerance the loose Davidson of iteration 2 returns the previous energy
+    and the run stops early (dE test) at -149.6775786 Ha, 2.3e-5 above the
+    stored-matrix run; tests/SHCI/o2_det_direct/trusted_hci.e
+    (-149.6776014366) is 1.3e-7 from the stored-matrix run instead, which the
+    restated source reaches only without the 3e-2 override.  Parity of this
+    mode is therefore pinned on the oracle-GPU agreement, not on that file."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "o2", "FCIDUMP")
+    _write(tmp_path / "dir.dat", O2_INPUT + "orbitals %s\nprefix %s\ndirectdavidson\n" % (fcidump, tmp_path))
+    d = OracleRun(str(tmp_path / "dir.dat"), str(tmp_path / "dir.log"))
+    log = open(tmp_path / "dir.log").read()
+    assert "New size of determinant space" in log
+    assert "nIter  Root               Energy                Error" in log
+    e = d.energies()[0]
+    assert abs(e - (-149.67757858715754)) < 1e-9
+    assert e > g["trusted_energy"]["o2_det_var_plus_pt"]
