@@ -99,9 +99,9 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   // SHCI_LEARNED_MARGIN: log10 margin below eps1 of the learned form's
   // evaluation threshold (default 0.8)
   if (const char* m = std::getenv("SHCI_LEARNED_MARGIN")) E.learnedMargin = std::atof(m);
-  if (!E.autotune) {
-    E.sigmaCfg[0] = SigmaConfig{};
-    E.sigmaCfg[1] = SigmaConfig{};
+  if (!E.autotune) {  // the shapes measured fastest on the benchmark matrices
+    E.sigmaCfg[0] = SigmaConfig{8, true};
+    E.sigmaCfg[1] = SigmaConfig{8, false};
     E.tunedRows = 0;
   }
 }

@@ -340,8 +340,8 @@ void Engine::buildPermuted() {
   if (autotune && n >= 20000 && n >= 2 * tunedRows) tuneSigma();
 }
 
-// Times every relabelled-sigma launch shape (rows per work item 8 / 16 / 32,
-// static or persistent) on the current H, for one vector and, in the block
+// Times every relabelled-sigma launch shape (rows per work item 4 / 8 / 16 /
+// 32, static or persistent) on the current H, for one vector and, in the block
 // Davidson, for an SpMM of four, and keeps the fastest of each.
 void Engine::tuneSigma() {
   const int ks[2] = {1, 4};
@@ -359,7 +359,7 @@ void Engine::tuneSigma() {
     const int k = ks[which];
     SigmaConfig best = sigmaCfg[which];
     float bestMs = 1e30f;
-    for (int chunk : {8, 16, 32})
+    for (int chunk : {4, 8, 16, 32})
       for (bool pers : {false, true}) {
         sigmaCfg[which].chunk = chunk;
         sigmaCfg[which].persistent = pers;

@@ -78,7 +78,7 @@ struct Engine {
   int davidsonMode = 0;  // 0: the reference's single-vector Davidson, 1: block Davidson
   bool autotune = false;
   bool useGraphs = false;
-  SigmaConfig sigmaCfg[2];  // [0]: one vector, [1]: SpMM
+  SigmaConfig sigmaCfg[2] = {SigmaConfig{8, true}, SigmaConfig{8, false}};  // [0]: one vector, [1]: SpMM
   int64_t tunedRows = 0;
   std::string tuneLog;
   DevMem sigmaCounter, tuneX, tuneY;

@@ -1813,6 +1813,8 @@ static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* v
   if (n <= 0 || k <= 0) return;
   if (cfg.persistent && !counter) throw HipError("persistent sigma needs a work counter");
   switch (cfg.chunk * 2 + (cfg.persistent ? 1 : 0)) {
+    case 8: sigmaPermChunk<VT, XT, 4, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 9: sigmaPermChunk<VT, XT, 4, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
     case 16: sigmaPermChunk<VT, XT, 8, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
     case 17: sigmaPermChunk<VT, XT, 8, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
     case 32: sigmaPermChunk<VT, XT, 16, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
