@@ -186,10 +186,7 @@ static double wallSeconds() {
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-static void swapMem(DevMem& a, DevMem& b) {
-  std::swap(a.p, b.p);
-  std::swap(a.bytes, b.bytes);
-}
+static void swapMem(DevMem& a, DevMem& b) { a.swap(b); }
 
 // Builds H over the held determinants.  When a Hamiltonian over the first
 // hamRows determinants exists, only the connections that involve the new
@@ -213,16 +210,24 @@ void Engine::buildHamiltonian() {
     return;
   }
   const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
-  // SHCI_TIMING: print the sub-phase wall times of this build to stderr
+  // SHCI_TIMING: after the build, print per sub-phase the host time spent
+  // issuing it and the GPU time between its stream events (no added
+  // synchronisation, so the timed build runs as an untimed one does)
   const bool timing = std::getenv("SHCI_TIMING") != nullptr;
-  double t0 = timing ? wallSeconds() : 0.0;
+  struct Mark {
+    const char* what;
+    double host;
+    hipEvent_t ev;
+  };
+  std::vector<Mark> marks;
   auto mark = [&](const char* what) {
     if (!timing) return;
-    HIP_CHECK(hipStreamSynchronize(stream));
-    const double t = wallSeconds();
-    std::fprintf(stderr, "  build %-12s %8.2f ms\n", what, 1e3 * (t - t0));
-    t0 = t;
+    Mark m{what, wallSeconds(), nullptr};
+    HIP_CHECK(hipEventCreate(&m.ev));
+    HIP_CHECK(hipEventRecord(m.ev, stream));
+    marks.push_back(m);
   };
+  mark("start");
   buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
   mark("strings");
   rowPtrNew.reserve((size_t)(n + 1) * 8);
@@ -242,7 +247,7 @@ void Engine::buildHamiltonian() {
   const size_t perEntry = 12;
   {
     size_t freeB = 0, totalB = 0;
-    HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    freeB = deviceFreeBytes(&totalB);
     const size_t need = (size_t)nnzNew * perEntry;
     const size_t reusable = colsNew.bytes + valsNew.bytes + vals32New.bytes + valsLoNew.bytes;
     if (need > freeB + reusable)
@@ -289,7 +294,7 @@ void Engine::buildHamiltonian() {
   // the previous H's buffers are kept for the next build unless memory is short
   {
     size_t freeB = 0, totalB = 0;
-    HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    freeB = deviceFreeBytes(&totalB);
     if (freeB < totalB / 4) {
       colsNew.release();
       valsNew.release();
@@ -300,6 +305,21 @@ void Engine::buildHamiltonian() {
   buildPermuted();
   mark("relabel");
   HIP_CHECK(hipStreamSynchronize(stream));
+  if (timing) {
+    const double end = wallSeconds();
+    std::string line = "  build n=" + std::to_string(n) + " (host ms / gpu ms):";
+    char buf[96];
+    for (size_t k = 1; k < marks.size(); k++) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, marks[k - 1].ev, marks[k].ev));
+      std::snprintf(buf, sizeof buf, " %s %.1f/%.1f", marks[k].what, 1e3 * (marks[k].host - marks[k - 1].host), ms);
+      line += buf;
+    }
+    std::snprintf(buf, sizeof buf, " | sync wait %.1f ms, total %.1f ms", 1e3 * (end - marks.back().host),
+                  1e3 * (end - marks.front().host));
+    std::fprintf(stderr, "%s%s\n", line.c_str(), buf);
+    for (Mark& m : marks) (void)hipEventDestroy(m.ev);
+  }
 }
 
 // Alpha-major relabelled copy of H (FP64 values, or the FP32 hi part) for
@@ -310,7 +330,7 @@ void Engine::buildPermuted() {
   const bool f32 = precision == 1;
   const size_t need = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4;
   size_t freeB = 0, totalB = 0;
-  HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  freeB = deviceFreeBytes(&totalB);
   if (need + (size_t)n * 64 > freeB + permCols.bytes + permVals.bytes) {
     permCols.release();
     permVals.release();
@@ -410,7 +430,10 @@ void Engine::clearGraphs() {
 // once per (subspace size, block size) as a hipGraph and replayed.
 std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& numIter, bool exactSigma) {
   const int64_t N = n;
-  const int maxM = (int)std::min<int64_t>(N, std::min(32, std::max(6 * nr, nr + 16)));
+  // subspace size: min(32, max(6 nr, nr + 16)) unless SHCI_BLOCK_MAXM (nr + 2 .. 32) sets it
+  int maxDefault = std::min(32, std::max(6 * nr, nr + 16));
+  if (const char* e = std::getenv("SHCI_BLOCK_MAXM")) maxDefault = std::max(nr + 2, std::min(32, std::atoi(e)));
+  const int maxM = (int)std::min<int64_t>(N, maxDefault);
   if (nr > 8 || nr > maxM) throw std::invalid_argument("block Davidson supports up to 8 roots");
   hipStream_t s = stream;
   if (graphN != N) {

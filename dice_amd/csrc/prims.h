@@ -7,6 +7,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace shci {
 
@@ -22,39 +23,49 @@ struct HipError : std::runtime_error {
                              __FILE__ + ":" + std::to_string(__LINE__) + ")");            \
   } while (0)
 
-// Growable device buffer (bytes).  Grows by reallocating; contents are not
-// preserved unless `keep` is set.
+// Growable device buffer (bytes).  Buffers below kBigBuffer come from
+// hipMalloc and grow by reallocating (contents kept only with `keep`).
+// Larger buffers are a reserved virtual address range onto which physical
+// chunks of kBigChunk bytes are mapped as the buffer grows: growth maps
+// more chunks at the end and keeps the contents in place, and the chunks of
+// a released buffer go to a per-process pool that later buffers map again,
+// so the multi-gigabyte Hamiltonian buffers are neither copied nor freed
+// between iterations (freeing and reallocating them costs 30 ms per GB or
+// more on this part).  Idle chunks are returned to the driver when an
+// allocation fails, then the allocation is retried.  With SHCI_ALLOC_TIMING
+// set, every driver allocation or free that takes longer than 5 ms is
+// reported on stderr.
+constexpr size_t kBigChunk = size_t(256) << 20;
+constexpr size_t kBigBuffer = kBigChunk;
+
 struct DevMem {
   void* p = nullptr;
   size_t bytes = 0;
+  // chunk-mapped buffers: reserved address range and mapped chunks
+  size_t vaBytes = 0;
+  int device = -1;
+  std::vector<hipMemGenericAllocationHandle_t> chunks;
   DevMem() = default;
   DevMem(const DevMem&) = delete;
   DevMem& operator=(const DevMem&) = delete;
   ~DevMem() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void reserve(size_t nb, hipStream_t s = 0, bool keep = false) {
-    if (nb <= bytes) return;
-    size_t cap = bytes ? bytes : 256;
-    while (cap < nb) cap += cap / 2 + 256;
-    void* q = nullptr;
-    HIP_CHECK(hipMalloc(&q, cap));
-    if (keep && p && bytes) HIP_CHECK(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s));
-    if (p) {
-      if (keep) HIP_CHECK(hipStreamSynchronize(s));
-      HIP_CHECK(hipFree(p));
-    }
-    p = q;
-    bytes = cap;
-  }
+  // waits for the device before unmapping a chunk-mapped buffer
+  void release();
+  // after reserve(nb), at least nb bytes are addressable from p; the address
+  // may change (callers that captured it must refresh)
+  void reserve(size_t nb, hipStream_t s = 0, bool keep = false);
+  void swap(DevMem& o);
   template <typename T>
   T* as() const {
     return static_cast<T*>(p);
   }
 };
+
+// free device memory as hipMemGetInfo reports it plus the idle pooled chunks
+// of the current device (which any buffer can map without a driver call)
+size_t deviceFreeBytes(size_t* total = nullptr);
+// returns the idle pooled chunks of every device to the driver
+void trimChunkPool();
 
 // scratch for the primitives
 struct Workspace {

@@ -1,7 +1,232 @@
 // Device-wide scan / radix sort / unique for gfx950 (wave64).
 #include "prims.h"
 
+#include <time.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <utility>
+#include <algorithm>
+
 namespace shci {
+
+namespace {
+
+bool allocTiming() {
+  static const bool on = std::getenv("SHCI_ALLOC_TIMING") != nullptr;
+  return on;
+}
+
+double monoSeconds() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+void reportSlow(const char* what, size_t bytes, double t0) {
+  const double dt = monoSeconds() - t0;
+  if (dt > 5e-3) std::fprintf(stderr, "  %s %.2f GiB took %.1f ms\n", what, bytes / 1073741824.0, 1e3 * dt);
+}
+
+}  // namespace
+
+namespace {
+
+// Idle physical chunks per device (process lifetime; the driver reclaims
+// them at exit).
+struct ChunkPool {
+  std::mutex mu;
+  std::vector<std::pair<int, hipMemGenericAllocationHandle_t>> idle;
+};
+
+ChunkPool& chunkPool() {
+  static ChunkPool* pool = new ChunkPool;
+  return *pool;
+}
+
+hipMemAllocationProp chunkProp(int device) {
+  hipMemAllocationProp prop = {};
+  prop.type = hipMemAllocationTypePinned;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = device;
+  return prop;
+}
+
+size_t roundUp(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+hipMemGenericAllocationHandle_t takeChunk(int device) {
+  ChunkPool& pool = chunkPool();
+  {
+    std::lock_guard<std::mutex> g(pool.mu);
+    for (size_t k = pool.idle.size(); k-- > 0;)
+      if (pool.idle[k].first == device) {
+        hipMemGenericAllocationHandle_t h = pool.idle[k].second;
+        pool.idle.erase(pool.idle.begin() + (ptrdiff_t)k);
+        return h;
+      }
+  }
+  hipMemAllocationProp prop = chunkProp(device);
+  hipMemGenericAllocationHandle_t h{};
+  const double t0 = allocTiming() ? monoSeconds() : 0.0;
+  hipError_t e = hipMemCreate(&h, kBigChunk, &prop, 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trimChunkPool();
+    HIP_CHECK(hipMemCreate(&h, kBigChunk, &prop, 0));
+  }
+  if (allocTiming()) reportSlow("hipMemCreate", kBigChunk, t0);
+  return h;
+}
+
+void giveChunk(int device, hipMemGenericAllocationHandle_t h) {
+  ChunkPool& pool = chunkPool();
+  std::lock_guard<std::mutex> g(pool.mu);
+  pool.idle.emplace_back(device, h);
+}
+
+// maps chunks [first, last) of `handles` at base + k * kBigChunk
+void mapChunks(void* base, const std::vector<hipMemGenericAllocationHandle_t>& handles, size_t first, size_t last,
+               int device) {
+  hipMemAccessDesc acc = {};
+  acc.location.type = hipMemLocationTypeDevice;
+  acc.location.id = device;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  for (size_t k = first; k < last; k++) {
+    void* at = static_cast<char*>(base) + k * kBigChunk;
+    HIP_CHECK(hipMemMap(at, kBigChunk, 0, handles[k], 0));
+    HIP_CHECK(hipMemSetAccess(at, kBigChunk, &acc, 1));
+  }
+}
+
+void unmapChunks(void* base, size_t count) {
+  for (size_t k = 0; k < count; k++) HIP_CHECK(hipMemUnmap(static_cast<char*>(base) + k * kBigChunk, kBigChunk));
+}
+
+hipError_t mallocRetry(void** q, size_t bytes) {
+  hipError_t e = hipMalloc(q, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    trimChunkPool();
+    e = hipMalloc(q, bytes);
+  }
+  return e;
+}
+
+}  // namespace
+
+void trimChunkPool() {
+  ChunkPool& pool = chunkPool();
+  std::lock_guard<std::mutex> g(pool.mu);
+  for (auto& c : pool.idle) (void)hipMemRelease(c.second);
+  pool.idle.clear();
+}
+
+size_t deviceFreeBytes(size_t* total) {
+  size_t freeB = 0, totalB = 0;
+  HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  int device = 0;
+  HIP_CHECK(hipGetDevice(&device));
+  ChunkPool& pool = chunkPool();
+  std::lock_guard<std::mutex> g(pool.mu);
+  for (auto& c : pool.idle)
+    if (c.first == device) freeB += kBigChunk;
+  if (total) *total = totalB;
+  return freeB;
+}
+
+void DevMem::release() {
+  // (runs in destructors: errors are not thrown from here)
+  if (p && !chunks.empty()) {
+    // kernels still queued may read the range: wait before unmapping it
+    (void)hipDeviceSynchronize();
+    bool unmapped = true;
+    for (size_t k = 0; k < chunks.size(); k++)
+      unmapped = hipMemUnmap(static_cast<char*>(p) + k * kBigChunk, kBigChunk) == hipSuccess && unmapped;
+    (void)hipMemAddressFree(p, vaBytes);
+    // a chunk that could not be unmapped is released rather than reused
+    for (auto h : chunks) {
+      if (unmapped)
+        giveChunk(device, h);
+      else
+        (void)hipMemRelease(h);
+    }
+    chunks.clear();
+  } else if (p) {
+    const double t0 = allocTiming() ? monoSeconds() : 0.0;
+    (void)hipFree(p);
+    if (allocTiming()) reportSlow("hipFree", bytes, t0);
+  }
+  p = nullptr;
+  bytes = 0;
+  vaBytes = 0;
+  device = -1;
+}
+
+void DevMem::swap(DevMem& o) {
+  std::swap(p, o.p);
+  std::swap(bytes, o.bytes);
+  std::swap(vaBytes, o.vaBytes);
+  std::swap(device, o.device);
+  chunks.swap(o.chunks);
+}
+
+void DevMem::reserve(size_t nb, hipStream_t s, bool keep) {
+  if (nb <= bytes) return;
+  if (nb < kBigBuffer && chunks.empty()) {
+    size_t cap = bytes ? bytes : 256;
+    while (cap < nb) cap += cap / 2 + 256;
+    void* q = nullptr;
+    const double t0 = allocTiming() ? monoSeconds() : 0.0;
+    HIP_CHECK(mallocRetry(&q, cap));
+    if (allocTiming()) reportSlow("hipMalloc", cap, t0);
+    if (keep && p && bytes) HIP_CHECK(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s));
+    if (p) {
+      if (keep) HIP_CHECK(hipStreamSynchronize(s));
+      release();
+    }
+    p = q;
+    bytes = cap;
+    return;
+  }
+  const size_t need = roundUp(nb, kBigChunk);
+  const size_t have = chunks.size();
+  if (have == 0) {  // a hipMalloc'ed (or empty) buffer becomes chunk-mapped
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    const size_t va = roundUp(std::max(2 * need, size_t(4) << 30), kBigChunk);
+    void* base = nullptr;
+    HIP_CHECK(hipMemAddressReserve(&base, va, kBigChunk, nullptr, 0));
+    std::vector<hipMemGenericAllocationHandle_t> hs;
+    for (size_t k = 0; k < need / kBigChunk; k++) hs.push_back(takeChunk(dev));
+    mapChunks(base, hs, 0, hs.size(), dev);
+    if (keep && p && bytes) {
+      HIP_CHECK(hipMemcpyAsync(base, p, bytes, hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    release();
+    p = base;
+    bytes = need;
+    vaBytes = va;
+    device = dev;
+    chunks.swap(hs);
+    return;
+  }
+  if (need > vaBytes) {  // a larger address range: remap the same chunks
+    const size_t va = roundUp(2 * need, kBigChunk);
+    void* base = nullptr;
+    HIP_CHECK(hipMemAddressReserve(&base, va, kBigChunk, nullptr, 0));
+    HIP_CHECK(hipDeviceSynchronize());  // queued work may still use the old range
+    mapChunks(base, chunks, 0, have, device);
+    unmapChunks(p, have);
+    HIP_CHECK(hipMemAddressFree(p, vaBytes));
+    p = base;
+    vaBytes = va;
+  }
+  for (size_t k = have; k < need / kBigChunk; k++) chunks.push_back(takeChunk(device));
+  mapChunks(p, chunks, have, chunks.size(), device);
+  bytes = need;
+}
 
 namespace {
 
