@@ -323,8 +323,8 @@ bool readInput(const std::string& path, Schedule& schd, std::string& err, FILE* 
       }
     } else if (iequals(ArgName, "dospinrdm")) schd.DoRDM = true;
     else if (iequals(ArgName, "quasiq")) {}
-    else if (iequals(ArgName, "nblocks")) {}
-    else if (iequals(ArgName, "restrict") || iequals(ArgName, "restrictv")) {
+    else if (iequals(ArgName, "nblocks")) schd.nblocks = (int)atof(arg(1).c_str());
+    else if (iequals(ArgName, "restrict") || iequals(ArgName, "restrictv")) {  // input.cpp:267-306
       Restriction r;
       r.minElec = atoi(arg(1).c_str());
       r.maxElec = atoi(arg(2).c_str());
@@ -333,6 +333,7 @@ bool readInput(const std::string& path, Schedule& schd, std::string& err, FILE* 
         r.orbs.push_back(2 * atoi(tok[i].c_str()) + 1);
       }
       schd.restrictionsV.push_back(r);
+      if (iequals(ArgName, "restrict")) schd.restrictionsPT.push_back(r);
     } else if (iequals(ArgName, "applyB")) {
       err = "applyB can only be used with ZDice";
       return false;
@@ -624,6 +625,138 @@ void screenDet(const System& s, const Schedule& schd, const Det& d, double epsil
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// deterministic EN-PT2
+
+// Determinant::parity(i, j, a, b, sgn), Determinants.cpp:332-357: sign of
+// i -> a then j -> b applied to d
+static void parityDouble(Det d, int i, int j, int a, int b, double& sgn) {
+  parity(d, std::min(i, a), std::max(i, a), sgn);
+  setocc(d, i, false);
+  setocc(d, a, true);
+  parity(d, std::min(j, b), std::max(j, b), sgn);
+}
+
+// getDeterminantsDeterministicPT, SHCIgetdeterminants.cpp:40-171: candidates
+// of one source with numerator H_ai c_i (singles: Hij_1Excite, doubles: the
+// float heat-bath value times the parity of the excitation)
+static void ptCandidates(const System& s, const Schedule& schd, const Det& d, double epsilon, double ci1,
+                         std::vector<Det>& dets, std::vector<double>& num) {
+  int closed[128], open[128], nclosed = 0, nopen = 0;
+  for (int i = 0; i < s.nso; i++) {
+    if (getocc(d, i))
+      closed[nclosed++] = i;
+    else
+      open[nopen++] = i;
+  }
+  std::vector<int> cur(schd.restrictionsPT.size());
+  for (size_t r = 0; r < schd.restrictionsPT.size(); r++) {
+    int c = 0;
+    for (int k = 0; k < nclosed; k++)
+      for (int x : schd.restrictionsPT[r].orbs)
+        if (closed[k] == x) c++;
+    cur[r] = c;
+  }
+  for (int ia = 0; ia < nopen * nclosed; ia++) {
+    int i = ia / nopen, a = ia % nopen;
+    bool ok = true;
+    for (size_t r = 0; r < schd.restrictionsPT.size(); r++)
+      if (!oneElecAllowed(schd.restrictionsPT[r], cur[r], closed[i], open[a])) ok = false;
+    if (!ok) continue;
+    // (across spins only the one-electron term survives: I1 times the parity)
+    double integral = hij1Free(s, open[a], closed[i], closed, nclosed);
+    if (std::fabs(integral) > epsilon) {
+      Det di = d;
+      setocc(di, open[a], true);
+      setocc(di, closed[i], false);
+      dets.push_back(di);
+      num.push_back(integral * ci1);
+    }
+  }
+  if (std::fabs(s.maxEntry) < epsilon) return;
+  for (int ij = 0; ij < nclosed * nclosed; ij++) {
+    int i = ij / nclosed, j = ij % nclosed;
+    if (i <= j) continue;
+    int I = closed[i] / 2, J = closed[j] / 2;
+    int X = std::max(I, J), Y = std::min(I, J);
+    int pairIndex = X * (X + 1) / 2 + Y;
+    bool sameSpin = closed[i] % 2 == closed[j] % 2;
+    size_t start = sameSpin ? s.hbSameStart[pairIndex] : s.hbOppStart[pairIndex];
+    size_t end = sameSpin ? s.hbSameStart[pairIndex + 1] : s.hbOppStart[pairIndex + 1];
+    const float* integrals = sameSpin ? s.hbSameVal.data() : s.hbOppVal.data();
+    const int16_t* orbIndices = sameSpin ? s.hbSamePair.data() : s.hbOppPair.data();
+    for (size_t index = start; index < end; index++) {
+      if (std::fabs(integrals[index]) < epsilon) break;
+      int a = 2 * orbIndices[2 * index] + closed[i] % 2, b = 2 * orbIndices[2 * index + 1] + closed[j] % 2;
+      bool ok = true;
+      for (size_t r = 0; r < schd.restrictionsPT.size(); r++)
+        if (!twoElecAllowed(schd.restrictionsPT[r], cur[r], closed[i], closed[j], a, b)) ok = false;
+      if (!ok) continue;
+      if (!(getocc(d, a) || getocc(d, b))) {
+        Det di = d;
+        setocc(di, a, true);
+        setocc(di, b, true);
+        setocc(di, closed[i], false);
+        setocc(di, closed[j], false);
+        double sgn = 1.0;
+        parityDouble(di, a, b, closed[i], closed[j], sgn);
+        dets.push_back(di);
+        num.push_back((double)integrals[index] * sgn * ci1);
+      }
+    }
+  }
+}
+
+double ptDeterministic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
+                       double E0, double eps2, double* psi1Norm, size_t* nExternal) {
+  const int W = s.W;
+  const size_t N = dets.size();
+  // sources in contiguous blocks per thread, concatenated in source order
+  const int nt = std::max(1, std::min<int>(oracleThreads(), (int)std::max<size_t>(N, 1)));
+  std::vector<std::vector<Det>> cd(nt);
+  std::vector<std::vector<double>> cn(nt);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int t = 0; t < nt; t++) {
+    const size_t b0 = N * t / nt, b1 = N * (t + 1) / nt;
+    for (size_t i = b0; i < b1; i++)
+      ptCandidates(s, schd, dets[i], std::fabs(eps2 / ci[i]), ci[i], cd[t], cn[t]);
+  }
+  std::vector<Det> cand;
+  std::vector<double> num;
+  for (int t = 0; t < nt; t++) {
+    cand.insert(cand.end(), cd[t].begin(), cd[t].end());
+    num.insert(num.end(), cn[t].begin(), cn[t].end());
+  }
+  // MergeSortAndRemoveDuplicates (SHCISortMpiUtils.cpp:541-686): stable
+  // sort, numerators of equal determinants summed in generation order
+  std::vector<size_t> order(cand.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return detLess(cand[x], cand[y], W); });
+  std::vector<Det> sortedVar = dets;
+  std::sort(sortedVar.begin(), sortedVar.end(), [W](const Det& x, const Det& y) { return detLess(x, y, W); });
+  double pt = 0.0, norm = 0.0;
+  size_t next = 0;
+  for (size_t k = 0; k < order.size();) {
+    const Det& d = cand[order[k]];
+    double sum = num[order[k]];
+    size_t k1 = k + 1;
+    while (k1 < order.size() && detEq(cand[order[k1]], d, W)) sum += num[order[k1++]];
+    k = k1;
+    // RemoveDetsPresentIn (SHCISortMpiUtils.cpp:750-860)
+    if (std::binary_search(sortedVar.begin(), sortedVar.end(), d,
+                           [W](const Det& x, const Det& y) { return detLess(x, y, W); }))
+      continue;
+    const double Ea = energy(s, d);
+    pt += std::pow(std::fabs(sum), 2) / (E0 - Ea);
+    norm += std::pow(std::fabs(sum / (E0 - Ea)), 2);
+    next++;
+  }
+  if (psi1Norm) *psi1Norm = norm;
+  if (nExternal) *nExternal = next;
+  return pt;
 }
 
 // ---------------------------------------------------------------------------
@@ -1305,6 +1438,35 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       }
     }
     fflush(log);
+  }
+  // deterministic PT (SHCI/SHCI.cpp:706-721): shci.e becomes E_var + E_PT
+  if (!schd.stochastic && schd.nblocks == 1) {
+    if (log) {  // log_pt, SHCI.cpp:140-156
+      fprintf(log, "\n\n**************************************************************\n");
+      fprintf(log, "PERTURBATION THEORY STEP  \n");
+      fprintf(log, "**************************************************************\n");
+    }
+    std::vector<double> ept(nroots);
+    for (int root = 0; root < nroots; root++) {
+      const double e0 = E0[root] - coreEbkp;
+      const double pt = ptDeterministic(s, schd, Dets, ci[root].data(), e0, schd.epsilon2);
+      ept[root] = E0[root] + pt;
+      if (log) {  // SHCIbasics.cpp:664-668
+        fprintf(log, "Deterministic PT calculation converged\n");
+        fprintf(log, "PTEnergy: %18.10f\n", ept[root]);
+        fprintf(log, "Time(s):  %10.2f\n", wallTime() - startofCalc);
+        fflush(log);
+      }
+    }
+    res.ptEnergies = ept;
+    if (writeFiles) {
+      std::string efile = schd.prefix[0] + "/shci.e";
+      FILE* f = fopen(efile.c_str(), "wb");
+      if (f) {
+        for (double e : ept) fwrite(&e, 1, sizeof(double), f);
+        fclose(f);
+      }
+    }
   }
 }
 

@@ -15,11 +15,17 @@
 //   * sigma = H x                               SHCI/Hmult.h:52-150
 //   * Davidson                                  SHCI/Davidson.cpp:85-348
 //   * DoVariational driver                      SHCI/SHCIbasics.cpp:783-1378
+//   * deterministic EN-PT2                      SHCI/SHCIbasics.cpp:453-728,
+//                                               SHCI/SHCIgetdeterminants.cpp:40-171,
+//                                               SHCI/SHCISortMpiUtils.cpp:541-690
 //   * input.dat grammar                         SHCI/input.cpp:35-404
 //   * RAS restrictions                          SHCI/OccRestrictions.cpp
 // Deviations (documented, none changes a result beyond rounding):
 //   * the m x m Rayleigh-Ritz eigenproblem uses cyclic Jacobi instead of
 //     Eigen's SelfAdjointEigenSolver (same eigenpairs to ~1e-15);
+//   * the PT energies of the external determinants are full Energy()
+//     evaluations instead of EnergyAfterExcitation's update of the source's
+//     energy (the same value up to rounding);
 //   * half-determinant strings are held in one 64-bit word, so n <= 63
 //     spatial orbitals (the reference allows 95); single-excitation string
 //     probes stop at orbital n-1 (strings with higher bits never exist).
@@ -65,6 +71,8 @@ struct Schedule {  // subset of SHCI/input.h:35-194 that the variational path re
   bool printAllDeterminants = false;
   int writeBestDeterminants = 0;
   std::vector<Restriction> restrictionsV;
+  std::vector<Restriction> restrictionsPT;  // "restrict" (not "restrictv") also restricts the PT space
+  int nblocks = 1;
   int Trev = 0;
   int davidsonType = 0;  // 0 MEMORY, 1 DIRECT, 2 DISK
   bool restart = false, fullrestart = false, doSOC = false;
@@ -158,6 +166,15 @@ std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>
                              const std::vector<double>& diag, int maxCopies,
                              double tol, int& numIter, bool print, FILE* out);
 
+// Deterministic Epstein-Nesbet PT2 of one root (SHCI/SHCIbasics.cpp:453-728):
+// every determinant outside `dets` reached from D_i by a single with
+// |Hij| > eps2/|c_i| or a double whose heat-bath value passes eps2/|c_i|
+// contributes (sum_i H_ai c_i)^2 / (E0 - E_a).  dets need not be sorted;
+// E0 is the root's variational energy without the core energy.
+// psi1Norm (optional) receives sum (num / (E0 - E_a))^2.
+double ptDeterministic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
+                       double E0, double eps2, double* psi1Norm = nullptr, size_t* nExternal = nullptr);
+
 struct IterRecord {
   int iter;
   double eps1;
@@ -176,9 +193,12 @@ struct RunResult {
   std::vector<double> energies;
   std::vector<std::vector<Det>> detsPerIter;  // recorded when requested
   double coreE = 0.0;
+  std::vector<double> ptEnergies;  // E_var + E_PT per root (written to shci.e) when PT ran
 };
 
-// Full variational step of the Dice executable (SHCI/SHCI.cpp:157-446 up to shci.e).
+// Full variational step of the Dice executable (SHCI/SHCI.cpp:157-446 up to
+// shci.e), then the deterministic PT step when the input asks for it
+// (SHCI/SHCI.cpp:706-721).
 void runVariational(const std::string& inputPath, RunResult& res, FILE* log,
                     bool recordIters, bool writeFiles);
 

@@ -77,6 +77,31 @@ void oref_set_active(void* p, int ncore, int nact) {
   ((Handle*)p)->schd.nact = nact;
 }
 
+// restriction applied to the PT space (the "restrict" keyword's PT half)
+void oref_add_pt_restriction(void* p, int minElec, int maxElec, const int* spatial, int n) {
+  Restriction r;
+  r.minElec = minElec;
+  r.maxElec = maxElec;
+  for (int k = 0; k < n; k++) {
+    r.orbs.push_back(2 * spatial[k]);
+    r.orbs.push_back(2 * spatial[k] + 1);
+  }
+  ((Handle*)p)->schd.restrictionsPT.push_back(r);
+}
+
+// deterministic EN-PT2 of the n determinants with coefficients ci (one root)
+// and variational energy e0 (no core energy)
+double oref_pt_deterministic(void* p, const int64_t* dets, int64_t n, const double* ci, double e0, double eps2,
+                             double* psi1Norm, int64_t* nExternal) {
+  Handle* h = (Handle*)p;
+  std::vector<Det> d(n);
+  for (int64_t i = 0; i < n; i++) d[i] = loadDet(dets + i * h->s.W, h->s.W);
+  size_t next = 0;
+  const double pt = ptDeterministic(h->s, h->schd, d, ci, e0, eps2, psi1Norm, &next);
+  if (nExternal) *nExternal = (int64_t)next;
+  return pt;
+}
+
 double oref_energy(void* p, const int64_t* det) {
   Handle* h = (Handle*)p;
   return energy(h->s, loadDet(det, h->s.W));
@@ -183,6 +208,12 @@ void oref_run_iter(void* p, int it, int64_t* ndets, double* e0, int* ndav) {
 void oref_run_energies(void* p, double* e) {
   RunResult* r = (RunResult*)p;
   for (size_t i = 0; i < r->energies.size(); i++) e[i] = r->energies[i];
+}
+// E_var + E_PT per root when the run did the deterministic PT step; returns the count (0 otherwise)
+int oref_run_pt(void* p, double* e) {
+  RunResult* r = (RunResult*)p;
+  for (size_t i = 0; i < r->ptEnergies.size(); i++) e[i] = r->ptEnergies[i];
+  return (int)r->ptEnergies.size();
 }
 int64_t oref_run_ndets(void* p) { return (int64_t)((RunResult*)p)->dets.size(); }
 int64_t oref_run_iter_ndets(void* p, int it) { return (int64_t)((RunResult*)p)->detsPerIter[it].size(); }

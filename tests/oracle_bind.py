@@ -66,6 +66,9 @@ def load():
         "oref_run_dets": ([P, I32, pI64, I32], None),
         "oref_run_ci": ([P, pD], None),
         "oref_run_free": ([P], None),
+        "oref_run_pt": ([P, pD], I32),
+        "oref_add_pt_restriction": ([P, I32, I32, ctypes.POINTER(ctypes.c_int), I32], None),
+        "oref_pt_deterministic": ([P, pI64, I64, pD, D, D, pD, pI64], D),
         "oref_mlp_forward": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float), I64, I32,
                               pD], None),
         "oref_mlp_loss_grad": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float),
@@ -155,6 +158,20 @@ class OracleSystem:
         self.L.oref_sigma_direct(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
         return y
 
+    def add_pt_restriction(self, min_elec, max_elec, spatial_orbs):
+        o = np.ascontiguousarray(spatial_orbs, dtype=np.int32)
+        self.L.oref_add_pt_restriction(self.h, min_elec, max_elec, o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       len(o))
+
+    def pt_deterministic(self, dets, ci, e0, eps2):
+        """Deterministic EN-PT2 of one root: (E_PT, psi1 norm, number of external determinants)."""
+        d = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.words)
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        psi, nx = ctypes.c_double(), ctypes.c_int64()
+        e = self.L.oref_pt_deterministic(self.h, _p(d, ctypes.c_int64), len(d), _p(c, ctypes.c_double), e0, eps2,
+                                         ctypes.byref(psi), ctypes.byref(nx))
+        return e, psi.value, nx.value
+
     def davidson(self, x0, diag, max_copies, tol):
         x = np.array(np.atleast_2d(np.asarray(x0, dtype=np.float64).T).T, order="F")
         if x.ndim == 1:
@@ -192,6 +209,12 @@ class OracleRun:
             self.L.oref_run_iter(self.h, i, ctypes.byref(nd), _p(e, ctypes.c_double), ctypes.byref(it))
             out.append({"ndets": nd.value, "energy": float(e[0]), "energies": e.copy(), "davidson": it.value})
         return out
+
+    def pt_energies(self):
+        """E_var + E_PT per root when the run did the deterministic PT step (empty otherwise)."""
+        e = np.zeros(64)
+        n = self.L.oref_run_pt(self.h, _p(e, ctypes.c_double))
+        return e[:n].copy()
 
     def energies(self):
         e = np.zeros(self.L.oref_run_nroots(self.h))

@@ -5,7 +5,10 @@ term, a wrong sign or index, a transposed operand, a > vs >= slip) fails:
   * the reference's golden variational log and final determinant set
     (examples/h8_631g/dice.out, examples/h8_631g/dets);
   * the reference's trusted variational energies (tests/SHCI/restart,
-    tests/SHCI/n2_RAS);
+    tests/SHCI/n2_RAS) and variational + PT energies (tests/SHCI/o2_det,
+    tests/SHCI/o2_det_direct, whose PT energy is nonzero);
+  * deterministic EN-PT2 at eps2 = 0 against the brute-force
+    Epstein-Nesbet sum over the whole FCI space;
   * Slater-Condon elements against brute-force second quantisation;
   * the FCI limit against dense diagonalisation;
   * iteration-0 screening against a brute-force enumeration of the
@@ -304,15 +307,18 @@ O2_INPUT = ("nocc 12\n0 2 4 6 8 10 1 3 5 7 9 11\nend\nepsilon2 1e-07\nsampleN 20
 
 
 def test_o2_det_trusted_energy(tmp_path, golden_dir):
-    """tests/SHCI/o2_det (8 orbitals, 12 electrons, eps1 1e-3 then 1e-4):
-    the stored trusted_hci.e is reproduced to 1e-9 Ha by the variational
-    step (the file holds E_var: shci.e is written after the variational
-    step)."""
+    """tests/SHCI/o2_det (8 orbitals, 12 electrons, eps1 1e-3 then 1e-4,
+    `deterministic`): shci.e holds E_var + E_PT (SHCI/SHCI.cpp:706-721) and
+    reproduces the stored trusted_hci.e to 1e-9 Ha.  Here the variational
+    space holds every determinant the heat-bath lists reach, so E_PT is 0."""
     g = _golden(golden_dir)
     fcidump = os.path.join(golden_dir, "o2", "FCIDUMP")
     _write(tmp_path / "in.dat", O2_INPUT + "orbitals %s\nprefix %s\n" % (fcidump, tmp_path))
     r = OracleRun(str(tmp_path / "in.dat"), str(tmp_path / "log"))
     assert abs(r.energies()[0] - g["trusted_energy"]["o2_det_var_plus_pt"]) < 1e-9
+    assert r.pt_energies()[0] == r.energies()[0]
+    with open(tmp_path / "shci.e", "rb") as f:
+        assert abs(struct.unpack("<d", f.read(8))[0] - g["trusted_energy"]["o2_det_var_plus_pt"]) < 1e-9
 
 
 def test_direct_davidson_run_o2(tmp_path, golden_dir):
@@ -320,11 +326,10 @@ def test_direct_davidson_run_o2(tmp_path, golden_dir):
     lines and uses davidsonDirect with the loose tolerance the reference's
     input reader forces for it (3e-2, SHCI/input.cpp:388).  With that
     tolerance the loose Davidson of iteration 2 returns the previous energy
-    and the run stops early (dE test) at -149.6775786 Ha, 2.3e-5 above the
-    stored-matrix run; tests/SHCI/o2_det_direct/trusted_hci.e
-    (-149.6776014366) is 1.3e-7 from the stored-matrix run instead, which the
-    restated source reaches only without the 3e-2 override.  Parity of this
-    mode is therefore pinned on the oracle-GPU agreement, not on that file."""
+    and the run stops early (dE test) at E_var = -149.6775786 Ha with a
+    smaller space, so the PT step adds -2.28e-5 Ha; E_var + E_PT reproduces
+    tests/SHCI/o2_det_direct/trusted_hci.e (-149.6776014366) to 2e-9 Ha, a
+    pin of a nonzero PT energy against the reference's own output."""
     g = _golden(golden_dir)
     fcidump = os.path.join(golden_dir, "o2", "FCIDUMP")
     _write(tmp_path / "dir.dat", O2_INPUT + "orbitals %s\nprefix %s\ndirectdavidson\n" % (fcidump, tmp_path))
@@ -334,4 +339,109 @@ def test_direct_davidson_run_o2(tmp_path, golden_dir):
     assert "nIter  Root               Energy                Error" in log
     e = d.energies()[0]
     assert abs(e - (-149.67757858715754)) < 1e-9
-    assert e > g["trusted_energy"]["o2_det_var_plus_pt"]
+    ept = d.pt_energies()[0]
+    assert ept - e < -2e-5
+    assert abs(ept - g["trusted_energy"]["o2_det_direct"]) < 2e-9
+    with open(tmp_path / "shci.e", "rb") as f:
+        assert struct.unpack("<d", f.read(8))[0] == ept
+
+
+# ---- deterministic EN-PT2 (SHCI/SHCIbasics.cpp:453-728) ----
+
+def _pt_system(tmp_path, eps1):
+    """An 8-orbital synthetic molecule (4+4 electrons, 4,900-determinant FCI
+    space) selected at a loose eps1 so most of the FCI space is external."""
+    from dice_amd import synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 5, 8, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, eps1)], 3)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    run = OracleRun(inp, None)
+    o = OracleSystem(path)
+    dets = run.dets(o.words)
+    ci = run.ci()[:, 0]
+    core = o.L.oref_core_energy(o.h)
+    return o, dets, ci, run.energies()[0] - core
+
+
+def _pt_brute(o, dets, ci, e0, eps2):
+    """sum_a (sum_i H_ai c_i)^2 / (E0 - E_a) over every FCI determinant a
+    outside the variational space, each term H_ai c_i kept when
+    |H_ai| > eps2 / |c_i| (the per-term screening of the PT step)."""
+    space = helpers.fci_space(8, 4, 4)
+    var = {int(d[0]) for d in dets}
+    ext = np.array([helpers.occ_to_det(s, o.words) for s in space])
+    ext = ext[[int(d[0]) not in var for d in ext]]
+    ev = np.array([int(d[0]) for d in ext], dtype=np.int64)
+    total, count = 0.0, 0
+    eps = np.abs(eps2 / ci)
+    for a, da in zip(ev, ext):
+        x = np.array([bin((int(a) ^ int(d[0])) & ((1 << 64) - 1)).count("1") for d in dets])
+        near = np.nonzero(x <= 4)[0]
+        if len(near) == 0:
+            continue
+        h = o.hij(np.repeat(da[None, :], len(near), axis=0), dets[near])
+        keep = np.abs(h) > eps[near]
+        if not keep.any():
+            continue
+        num = float(np.sum(h[keep] * ci[near][keep]))
+        total += num * num / (e0 - o.energy(da)[0])
+        count += 1
+    return total, count
+
+
+def test_pt_deterministic_eps2_zero_equals_brute_force(tmp_path):
+    """With eps2 = 0 the deterministic PT step is the full second-order
+    Epstein-Nesbet energy: every external determinant's numerator
+    sum_i H_ai c_i (singles exact, doubles from the float heat-bath values, so
+    agreement to float precision) over E0 - E_a."""
+    o, dets, ci, e0 = _pt_system(tmp_path, 2e-2)
+    assert 50 < len(dets) < 2000
+    pt, psi1, next_ = o.pt_deterministic(dets, ci, e0, 0.0)
+    brute, count = _pt_brute(o, dets, ci, e0, 0.0)
+    assert pt < 0 and brute < 0
+    assert abs(pt - brute) <= 2e-6 * abs(brute), (pt, brute)
+    assert next_ == count
+
+
+def test_pt_deterministic_screening_per_term(tmp_path):
+    """eps2 > 0 drops each term H_ai c_i with |H_ai| <= eps2/|c_i| separately
+    (a determinant keeps the terms of the sources that pass), and the energy
+    moves toward the eps2 = 0 value as eps2 shrinks."""
+    o, dets, ci, e0 = _pt_system(tmp_path, 2e-2)
+    full = o.pt_deterministic(dets, ci, e0, 0.0)[0]
+    last = 0.0
+    for eps2 in (1e-2, 1e-3, 1e-4):
+        pt, _, nx = o.pt_deterministic(dets, ci, e0, eps2)
+        brute, count = _pt_brute(o, dets, ci, e0, eps2)
+        assert abs(pt - brute) <= 2e-6 * abs(brute) + 1e-12, (eps2, pt, brute)
+        assert nx == count
+        assert abs(full - pt) <= abs(full - last) + 1e-12
+        last = pt
+
+
+def test_pt_run_writes_var_plus_pt(tmp_path):
+    """`deterministic` in the input: the run's shci.e holds E_var + E_PT per
+    root (SHCI/SHCI.cpp:706-721), E_PT being the PT step on the final
+    variational space and coefficients."""
+    from dice_amd import synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 5, 8, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 2e-2)], 3,
+                      extra=["deterministic", "epsilon2 1e-5", "prefix %s" % tmp_path])
+    run = OracleRun(inp, str(tmp_path / "log"))
+    o = OracleSystem(path)
+    core = o.L.oref_core_energy(o.h)
+    ev = run.energies()
+    ept = run.pt_energies()
+    assert len(ept) == 1
+    pt = o.pt_deterministic(run.dets(o.words), run.ci()[:, 0], ev[0] - core, 1e-5)[0]
+    assert ept[0] == ev[0] + pt
+    with open(tmp_path / "shci.e", "rb") as f:
+        assert struct.unpack("<d", f.read(8))[0] == ept[0]
+    log = open(tmp_path / "log").read()
+    assert "PERTURBATION THEORY STEP" in log
+    assert "PTEnergy: %18.10f" % ept[0] in log
