@@ -151,7 +151,9 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
   E.ncore = schd.ncore;
   E.nact = schd.nact;
   E.restrictions = schd.restrictions;
+  E.restrictionsPT = schd.restrictionsPT;
   E.uploadRestrictions();
+  E.ptEnergies.clear();
   const HostSystem& hs = E.hs;
   const int W = hs.W, nso = hs.nso;
   if ((int)schd.occupied[0].size() != hs.nelec) {
@@ -344,6 +346,33 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     }
     fflush(log);
   }
+  // deterministic PT step (SHCI/SHCI.cpp:706-721): shci.e becomes E_var + E_PT
+  if (!schd.stochastic && schd.nblocks == 1) {
+    if (log) {  // log_pt, SHCI/SHCI.cpp:140-156
+      fprintf(log, "\n\n**************************************************************\n");
+      fprintf(log, "PERTURBATION THEORY STEP  \n");
+      fprintf(log, "**************************************************************\n");
+    }
+    std::vector<double> ept(E0.size());
+    for (size_t root = 0; root < E0.size(); root++) {
+      const double pt = E.ptDeterministic(E.ci.as<double>() + (int64_t)root * E.n, E0[root] - coreEbkp,
+                                          schd.epsilon2, nullptr, nullptr);
+      ept[root] = E0[root] + pt;
+      if (log) {  // SHCI/SHCIbasics.cpp:664-668
+        fprintf(log, "Deterministic PT calculation converged\n");
+        fprintf(log, "PTEnergy: %18.10f\n", ept[root]);
+        fprintf(log, "Time(s):  %10.2f\n", wallTime() - startofCalc);
+        fflush(log);
+      }
+    }
+    E.ptEnergies = ept;
+    std::string efile = schd.prefix[0] + "/shci.e";
+    FILE* f = fopen(efile.c_str(), "wb");
+    if (f) {
+      for (double e : ept) fwrite(&e, 1, sizeof(double), f);
+      fclose(f);
+    }
+  }
   return SHCI_OK;
 }
 
@@ -417,7 +446,9 @@ shci_status shci_add_restriction(shci_handle* h, int min_elec, int max_elec, con
       r.orbs.push_back(2 * spatial_orbs[i]);
       r.orbs.push_back(2 * spatial_orbs[i] + 1);
     }
+    // as the input's "restrict": the variational and the PT space
     h->eng.restrictions.push_back(r);
+    h->eng.restrictionsPT.push_back(r);
     h->eng.uploadRestrictions();
     return SHCI_OK;
   });
@@ -615,6 +646,20 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
   });
 }
 
+shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, double eps2, double* e_pt,
+                                  double* psi1_norm, int64_t* n_external) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (!ci || !e_pt) throw std::invalid_argument("null argument");
+    const int64_t n = E.n;
+    DevMem dc;
+    dc.reserve((size_t)std::max<int64_t>(n, 1) * 8);
+    if (n) HIP_CHECK(hipMemcpyAsync(dc.p, ci, n * 8, hipMemcpyHostToDevice, E.stream));
+    *e_pt = E.ptDeterministic(dc.as<double>(), e0, eps2, psi1_norm, n_external);
+    return SHCI_OK;
+  });
+}
+
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out) {
   shci_run_options opt;
   std::memset(&opt, 0, sizeof(opt));
@@ -687,6 +732,12 @@ shci_status shci_run_iteration(const shci_handle* h, int it, int64_t* ndets, dou
 shci_status shci_run_energies(const shci_handle* h, double* energies, int* nroots) {
   for (size_t i = 0; i < h->eng.energies.size(); i++) energies[i] = h->eng.energies[i];
   if (nroots) *nroots = (int)h->eng.energies.size();
+  return SHCI_OK;
+}
+
+shci_status shci_run_pt_energies(const shci_handle* h, double* energies, int* nroots) {
+  for (size_t i = 0; i < h->eng.ptEnergies.size(); i++) energies[i] = h->eng.ptEnergies[i];
+  if (nroots) *nroots = (int)h->eng.ptEnergies.size();
   return SHCI_OK;
 }
 

@@ -85,7 +85,82 @@ void Engine::uploadRestrictions() {
   }
   upload(dRmult, mult, stream);
   S.rmult = dRmult.as<int8_t>();
+  if (restrictionsPT.size() > SHCI_MAX_RESTRICT) throw std::invalid_argument("too many restrict keywords (max 4)");
+  std::vector<int8_t> multPt(std::max<size_t>(restrictionsPT.size(), 1) * hs.nso, 0);
+  for (size_t r = 0; r < restrictionsPT.size(); r++)
+    for (int o : restrictionsPT[r].orbs)
+      if (o >= 0 && o < hs.nso) multPt[r * hs.nso + o]++;
+  upload(dRmultPt, multPt, stream);
   HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+DevSystem Engine::ptSystem() const {
+  DevSystem P = S;
+  P.nrestrict = (int)restrictionsPT.size();
+  for (size_t r = 0; r < restrictionsPT.size(); r++) {
+    P.rmin[r] = restrictionsPT[r].minElec;
+    P.rmax[r] = restrictionsPT[r].maxElec;
+  }
+  P.rmult = dRmultPt.as<int8_t>();
+  return P;
+}
+
+double Engine::ptDeterministic(const double* dCi, double e0, double eps2, double* psi1Norm, int64_t* nExternal,
+                               int* nPartitions) {
+  const int W = hs.W;
+  double pt = 0.0, psi = 0.0;
+  int64_t next = 0;
+  uint32_t nparts = 1;
+  if (n > 0) {
+    ensureTable(n);
+    const DevSystem P = ptSystem();
+    ptCounts.reserve((size_t)(n + 1) * 8);
+    int64_t* cnt = ptCounts.as<int64_t>();
+    // per-source candidate counts of one partition, scanned into offsets
+    auto countPass = [&](uint32_t parts, uint32_t part) {
+      HIP_CHECK(hipMemsetAsync(cnt + n, 0, 8, stream));
+      launchPtGen(P, dets.as<uint64_t>(), n, dCi, eps2, table.as<int32_t>(), slots, parts, part, cnt, nullptr,
+                  nullptr, nullptr, stream);
+      exclusiveScanI64(cnt, n + 1, ws, stream);
+      int64_t total = 0;
+      HIP_CHECK(hipMemcpyAsync(&total, cnt + n, 8, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      return total;
+    };
+    const int64_t total = countPass(1, 0);
+    // keys and their sort scratch, numerator, 32-bit payload and its scratch
+    const size_t perCand = (size_t)16 * W + 16;
+    const size_t budget = std::min<size_t>(deviceFreeBytes() / 3, size_t(32) << 30);
+    const int64_t maxPer = std::max<int64_t>(1 << 20, std::min<int64_t>((int64_t)(budget / perCand), INT32_MAX));
+    nparts = (uint32_t)std::max<int64_t>(1, (total + maxPer - 1) / maxPer);
+    // SHCI_PT_PARTITIONS: at least this many partitions
+    if (const char* e = std::getenv("SHCI_PT_PARTITIONS")) nparts = std::max<uint32_t>(nparts, (uint32_t)std::atoi(e));
+    ptOut.reserve(64);
+    double host[3];
+    for (uint32_t part = 0; part < nparts; part++) {
+      const int64_t m = nparts == 1 ? total : countPass(nparts, part);
+      if (m == 0) continue;
+      if (m > INT32_MAX) throw std::invalid_argument("PT partition exceeds 2^31 candidates");
+      ptKeys.reserve((size_t)m * W * 8);
+      ptNum.reserve((size_t)m * 8);
+      ptIdx.reserve((size_t)m * 4);
+      launchPtGen(P, dets.as<uint64_t>(), n, dCi, eps2, table.as<int32_t>(), slots, nparts, part, nullptr, cnt,
+                  ptKeys.as<uint64_t>(), ptNum.as<double>(), stream);
+      launchIota(ptIdx.as<uint32_t>(), m, stream);
+      radixSortKeys(ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), m, W, hs.nso, true, ws, stream);
+      launchPtReduce(P, ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), ptNum.as<double>(), m, e0, ptPartial,
+                     ptOut.as<double>(), stream);
+      HIP_CHECK(hipMemcpyAsync(host, ptOut.p, 3 * 8, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      pt += host[0];
+      psi += host[1];
+      next += (int64_t)host[2];
+    }
+  }
+  if (psi1Norm) *psi1Norm = psi;
+  if (nExternal) *nExternal = next;
+  if (nPartitions) *nPartitions = (int)nparts;
+  return pt;
 }
 
 void Engine::ensureTable(int64_t need) {

@@ -32,9 +32,10 @@ struct Engine {
   int precision = 0;
   int ncore = 0, nact = 1000000;
   std::vector<Restriction> restrictions;
+  std::vector<Restriction> restrictionsPT;  // the PT space's ("restrict", not "restrictv")
 
   // system tables
-  DevMem dH1, dI2, dDirect, dExch, dTsame, dTopp, dSingles, dHbVal[2], dHbPair[2], dHbStart[2], dRmult;
+  DevMem dH1, dI2, dDirect, dExch, dTsame, dTopp, dSingles, dHbVal[2], dHbPair[2], dHbStart[2], dRmult, dRmultPt;
 
   // variational space
   int64_t n = 0;
@@ -130,9 +131,18 @@ struct Engine {
   void learnedTrain();
   int64_t screenLearned(const double* dCmax, double eps1, int iter);
 
+  // deterministic EN-PT2 (SHCI/SHCIbasics.cpp:453-728) of root vector dCi
+  // (device, n values) with variational energy e0 (no core energy): E_PT;
+  // candidates are processed in hash partitions sized to the free memory
+  DevMem ptCounts, ptKeys, ptNum, ptIdx, ptOut, ptPartial;
+  double ptDeterministic(const double* dCi, double e0, double eps2, double* psi1Norm, int64_t* nExternal,
+                         int* nPartitions = nullptr);
+  DevSystem ptSystem() const;
+
   // last run
   std::vector<IterRecord> iters;
   std::vector<double> energies;
+  std::vector<double> ptEnergies;  // E_var + E_PT per root when the run did the PT step
   DevMem ci;  // n x nroots, column-major, ld = n
   int nroots = 1;
   std::string err;

@@ -308,6 +308,9 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
         r.orbs.push_back(2 * std::atoi(t[k].c_str()) + 1);
       }
       schd.restrictions.push_back(r);
+      if (same(key, "restrict")) schd.restrictionsPT.push_back(r);
+    } else if (same(key, "nblocks")) {
+      schd.nblocks = (int)std::atof(arg(1).c_str());
     } else if (same(key, "applyB")) {
       err = "applyB can only be used with ZDice";
       return false;
@@ -344,7 +347,7 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
              same(key, "maxexcitation") || same(key, "SenioAndExc") || same(key, "dogtensor") ||
              same(key, "targetError") || same(key, "dosocqdpt") || same(key, "socmultiplier") ||
              same(key, "epsilon2Large") || same(key, "onlyperturbative") || same(key, "DoSpinOneRDM") ||
-             same(key, "TransitionRDM") || same(key, "quasiq") || same(key, "nblocks") ||
+             same(key, "TransitionRDM") || same(key, "quasiq") ||
              same(key, "cdfciTol") || same(key, "excitation") || same(key, "nvirt") || same(key, "eps") ||
              same(key, "pointGroup") || same(key, "spin") || same(key, "irrep") || same(key, "DoThreeRDM") ||
              same(key, "DoFourRDM") || same(key, "cdfciIter") || same(key, "reportInterval") ||

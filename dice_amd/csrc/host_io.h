@@ -47,7 +47,9 @@ struct HostSchedule {
   bool stochastic = true;
   int outputlevel = 0, printBestDeterminants = 0, writeBestDeterminants = 0;
   bool printAllDeterminants = false;
-  std::vector<Restriction> restrictions;
+  std::vector<Restriction> restrictions;    // variational space ("restrict", "restrictv")
+  std::vector<Restriction> restrictionsPT;  // PT space ("restrict")
+  int nblocks = 1;
   int Trev = 0, davidsonType = 0;
   bool restart = false, fullrestart = false, doSOC = false, searchLowest = false, doLCC = false;
   bool doRDM = false, doOneRDM = false;

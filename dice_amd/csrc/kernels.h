@@ -27,6 +27,20 @@ void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cma
 // (which may exceed cap: the caller then grows `out` and runs again).
 // Screens the source determinants [begin, end) of `dets` (the whole
 // variational space is probed for membership).
+// Deterministic EN-PT2 candidates of sources [0, n) (SHCI/SHCIgetdeterminants.cpp:40-171):
+// with fillStart == nullptr the count pass writes counts[src]; otherwise the
+// fill pass writes each source's candidates (determinants outD, numerators
+// H_ai c_i outN) from fillStart[src] in the reference's generation order.
+// Only candidates outside the membership table and in partition `part` of
+// `nparts` are kept.
+void launchPtGen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* ci, double eps2,
+                 const int32_t* table, uint64_t slots, uint32_t nparts, uint32_t part, int64_t* counts,
+                 const int64_t* fillStart, uint64_t* outD, double* outN, hipStream_t s);
+void launchIota(uint32_t* v, int64_t n, hipStream_t s);
+// m candidates sorted by determinant (payload idx into num): out[0..2]
+// (device) = E_PT, psi1 norm, number of external determinants
+void launchPtReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num, int64_t m,
+                    double e0, DevMem& partial, double* out, hipStream_t s);
 void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
                   double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
                   unsigned long long* counter, int64_t cap, hipStream_t s);

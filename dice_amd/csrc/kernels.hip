@@ -323,6 +323,220 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
 }
 
 // ---------------------------------------------------------------------------
+// deterministic EN-PT2 (SHCI/SHCIbasics.cpp:453-728 with the candidate rules of
+// SHCI/SHCIgetdeterminants.cpp:40-171)
+
+// partition of a candidate among nparts passes (hash bits above the ones the
+// membership table uses)
+template <int W>
+__device__ __forceinline__ uint32_t ptPart(const uint64_t* d, uint32_t nparts) {
+  return nparts == 1 ? 0u : (uint32_t)((detHash<W>(d) >> 40) % nparts);
+}
+
+// Count pass: per-lane counters.  Fill pass: wave-uniform cursor with ballot
+// positions, so a source's candidates land in the reference's generation
+// order (singles by (closed i, open a), then doubles by (i > j) pair and
+// heat-bath order).
+template <int W, bool FILL>
+struct PtSink {
+  int64_t pos;
+  int cnt;
+  uint64_t* outD;
+  double* outN;
+  __device__ __forceinline__ void put(bool keep, const uint64_t* cand, double num) {
+    if (!FILL) {
+      cnt += keep ? 1 : 0;
+      return;
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+      const int64_t p = pos + __popcll(m & lanemaskLt());
+#pragma unroll
+      for (int w = 0; w < W; w++) outD[p * W + w] = cand[w];
+      outN[p] = num;
+    }
+    pos += __popcll(m);
+  }
+};
+
+// Candidates D_a of sources [0, ndets) with numerator H_ai c_i: singles with
+// |Hij_1Excite| > eps, doubles while the float heat-bath value passes eps
+// (numerator = value x parity of the excitation x c_i), eps = |eps2 / c_i|,
+// no active-space limits, the PT restrictions in S; kept when D_a is not a
+// variational determinant and falls in partition `part`.
+template <int W, bool FILL>
+__global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* dets, int64_t ndets, const double* ci,
+                                                 double eps2, const int32_t* table, uint64_t mask, uint32_t nparts,
+                                                 uint32_t part, int64_t* counts, const int64_t* fillStart,
+                                                 uint64_t* outD, double* outN) {
+  __shared__ uint8_t sClosed[kWaves][128];
+  __shared__ uint8_t sOpen[kWaves][128];
+  __shared__ int sExcl[kWaves][65];
+  __shared__ int64_t sStart[kWaves][64];
+  __shared__ int sPair[kWaves][64];
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int nso = S.nso;
+  for (int64_t di = (int64_t)blockIdx.x * kWaves + wave; di < ndets; di += (int64_t)gridDim.x * kWaves) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[di * W + w];
+    const double c = ci[di];
+    const double eps = fabs(eps2 / c);
+    PtSink<W, FILL> sink;
+    sink.pos = FILL ? fillStart[di] : 0;
+    sink.cnt = 0;
+    sink.outD = outD;
+    sink.outN = outN;
+    int nclosed = 0, nopen = 0;
+    for (int base = 0; base < nso; base += 64) {
+      int k = base + lane;
+      bool valid = k < nso;
+      bool o = valid && occ<W>(d, k);
+      uint64_t mc = __ballot(o), mo = __ballot(valid && !o);
+      if (o) sClosed[wave][nclosed + __popcll(mc & lanemaskLt())] = (uint8_t)k;
+      if (valid && !o) sOpen[wave][nopen + __popcll(mo & lanemaskLt())] = (uint8_t)k;
+      nclosed += __popcll(mc);
+      nopen += __popcll(mo);
+    }
+    __syncwarp();
+    int cur[SHCI_MAX_RESTRICT];
+    for (int r = 0; r < S.nrestrict; r++) {
+      int cc = 0;
+      for (int k = 0; k < nclosed; k++) cc += S.rmult[r * nso + sClosed[wave][k]];
+      cur[r] = cc;
+    }
+    // ---- single excitations
+    const int nsingle = nclosed * nopen;
+    for (int t0 = 0; t0 < nsingle; t0 += 64) {
+      int t = t0 + lane;
+      bool keep = false;
+      double num = 0.0;
+      uint64_t cand[W];
+#pragma unroll
+      for (int w = 0; w < W; w++) cand[w] = d[w];
+      if (t < nsingle) {
+        int i = t / nopen, a = t - (t / nopen) * nopen;
+        int cI = sClosed[wave][i], oa = sOpen[wave][a];
+        bool ok = true;
+        for (int r = 0; r < S.nrestrict && ok; r++) {
+          int e = cur[r] - S.rmult[r * nso + cI] + S.rmult[r * nso + oa];
+          if (e < S.rmin[r] || e > S.rmax[r]) ok = false;
+        }
+        // (a spin-changing single has only the one-electron term, which is 0)
+        const double integral = (ok && ((cI ^ oa) & 1) == 0) ? hijSingle<W>(S, d, oa, cI) : 0.0;
+        if (ok && fabs(integral) > eps) {
+          cand[oa >> 6] |= (1ull << (oa & 63));
+          cand[cI >> 6] &= ~(1ull << (cI & 63));
+          num = __dmul_rn(integral, c);
+          keep = ptPart<W>(cand, nparts) == part && !hashContains<W>(table, mask, dets, cand);
+        }
+      }
+      sink.put(keep, cand, num);
+    }
+    // ---- double excitations
+    if (!(S.maxEntryAbs < eps)) {
+      const int npairs = nclosed * (nclosed - 1) / 2;
+      for (int p0 = 0; p0 < npairs; p0 += 64) {
+        int p = p0 + lane;
+        int len = 0;
+        int64_t st = 0;
+        if (p < npairs) {
+          int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+          while (i * (i - 1) / 2 > p) i--;
+          while ((i + 1) * i / 2 <= p) i++;
+          int j = p - i * (i - 1) / 2;
+          int cI = sClosed[wave][i], cJ = sClosed[wave][j];
+          int I = cI >> 1, J = cJ >> 1;
+          int X = I > J ? I : J, Y = I > J ? J : I;
+          int pairIndex = X * (X + 1) / 2 + Y;
+          int sameSpin = ((cI ^ cJ) & 1) == 0 ? 1 : 0;
+          int64_t s0 = S.hbStart[sameSpin][pairIndex], s1 = S.hbStart[sameSpin][pairIndex + 1];
+          const float* v = S.hbVal[sameSpin];
+          int64_t lo = s0, hi = s1;
+          while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if ((double)fabsf(v[mid]) < eps)
+              hi = mid;
+            else
+              lo = mid + 1;
+          }
+          len = (int)(lo - s0);
+          st = s0;
+          sPair[wave][lane] = i | (j << 8) | (sameSpin << 16);
+        }
+        int inc = len;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          int tt = __shfl_up(inc, o, 64);
+          if (lane >= o) inc += tt;
+        }
+        sExcl[wave][lane] = inc - len;
+        sStart[wave][lane] = st;
+        int total = __shfl(inc, 63, 64);
+        __syncwarp();
+        for (int k0 = 0; k0 < total; k0 += 64) {
+          int k = k0 + lane;
+          bool keep = false;
+          double num = 0.0;
+          uint64_t cand[W];
+#pragma unroll
+          for (int w = 0; w < W; w++) cand[w] = d[w];
+          if (k < total) {
+            int lo = 0, hi = 63;
+            while (lo < hi) {
+              int mid = (lo + hi + 1) >> 1;
+              if (sExcl[wave][mid] <= k)
+                lo = mid;
+              else
+                hi = mid - 1;
+            }
+            int q = lo;
+            int pk = sPair[wave][q];
+            int i = pk & 0xFF, j = (pk >> 8) & 0xFF, same = (pk >> 16) & 1;
+            int cI = sClosed[wave][i], cJ = sClosed[wave][j];
+            int64_t idx = sStart[wave][q] + (k - sExcl[wave][q]);
+            uint32_t ab = S.hbPair[same][idx];
+            int a = 2 * (int)(ab & 0xFFFF) + (cI & 1), b = 2 * (int)(ab >> 16) + (cJ & 1);
+            bool ok = true;
+            for (int r = 0; r < S.nrestrict && ok; r++) {
+              const int8_t* m = S.rmult + r * nso;
+              int e = cur[r] - m[cI] - m[cJ] + m[a] + m[b];
+              if (e < S.rmin[r] || e > S.rmax[r]) ok = false;
+            }
+            if (ok && !(occ<W>(d, a) || occ<W>(d, b))) {
+              cand[a >> 6] |= (1ull << (a & 63));
+              cand[b >> 6] |= (1ull << (b & 63));
+              cand[cI >> 6] &= ~(1ull << (cI & 63));
+              cand[cJ >> 6] &= ~(1ull << (cJ & 63));
+              // Determinant::parity(a, b, cI, cJ) on the new determinant
+              // (SHCI/Determinants.cpp:332-357): a -> cI, then b -> cJ
+              int nb = countBetween<W>(cand, a < cI ? a : cI, a < cI ? cI : a);
+              uint64_t mid2[W];
+#pragma unroll
+              for (int w = 0; w < W; w++) mid2[w] = cand[w];
+              mid2[a >> 6] &= ~(1ull << (a & 63));
+              mid2[cI >> 6] |= (1ull << (cI & 63));
+              nb += countBetween<W>(mid2, b < cJ ? b : cJ, b < cJ ? cJ : b);
+              const double sgn = (nb & 1) ? -1.0 : 1.0;
+              num = __dmul_rn(__dmul_rn((double)S.hbVal[same][idx], sgn), c);
+              keep = ptPart<W>(cand, nparts) == part && !hashContains<W>(table, mask, dets, cand);
+            }
+          }
+          sink.put(keep, cand, num);
+        }
+        __syncwarp();
+      }
+    }
+    if (!FILL) {
+      int cc = sink.cnt;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) cc += __shfl_xor(cc, o, 64);
+      if (lane == 0) counts[di] = cc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // learned screening: features, targets and the exact test of ranked records
 
 // X[r * din + f]: f < nso occupation bit f of the source determinant; then
@@ -1262,6 +1476,42 @@ __global__ void __launch_bounds__(kBlock) kFinalSum(const double* partial, int n
   }
 }
 
+__global__ void kIota(uint32_t* v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (uint32_t)i;
+}
+
+// Over candidates sorted by determinant (stable, so equal determinants keep
+// their generation order): the first entry of each run sums the run's
+// numerators in that order (the reference's merge) and contributes
+// num^2 / (E0 - E_a) and (num / (E0 - E_a))^2; per-block partial sums of
+// (E_PT, psi1 norm, number of external determinants) in fixed order.
+template <int W>
+__global__ void __launch_bounds__(kBlock) kPtReduce(DevSystem S, const uint64_t* keys, const uint32_t* idx,
+                                                    const double* num, int64_t m, double e0, double* partial) {
+  double sPt = 0.0, sPsi = 0.0, sCnt = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i > 0 && detEqual<W>(keys + i * W, keys + (i - 1) * W)) continue;
+    double sum = num[idx[i]];
+    for (int64_t j = i + 1; j < m && detEqual<W>(keys + j * W, keys + i * W); j++) sum = __dadd_rn(sum, num[idx[j]]);
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = keys[i * W + w];
+    const double den = __dsub_rn(e0, detEnergy<W>(S, d));
+    sPt = __dadd_rn(sPt, __ddiv_rn(__dmul_rn(fabs(sum), fabs(sum)), den));
+    const double r = __ddiv_rn(sum, den);
+    sPsi = __dadd_rn(sPsi, __dmul_rn(fabs(r), fabs(r)));
+    sCnt += 1.0;
+  }
+  const double t0 = blockSum(sPt), t1 = blockSum(sPsi), t2 = blockSum(sCnt);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = t0;
+    partial[3 * blockIdx.x + 1] = t1;
+    partial[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+
 __global__ void kPrecondition(double* r, const double* diag, const double* e, int64_t n) {
   const double ev = e[0];
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
@@ -1464,6 +1714,36 @@ void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64
   unsigned g = gridFor(end - begin, kWaves, 1 << 16);
   DISPATCH_W(S.W, kScreen<WW, 0><<<g, kBlock, 0, s>>>(S, dets, begin, end, cmax, eps1, ncore, nact, table,
                                                          slots - 1, out, counter, cap, 1.0, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPtGen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* ci, double eps2,
+                 const int32_t* table, uint64_t slots, uint32_t nparts, uint32_t part, int64_t* counts,
+                 const int64_t* fillStart, uint64_t* outD, double* outN, hipStream_t s) {
+  if (n <= 0) return;
+  if (nparts == 0 || part >= nparts) throw HipError("launchPtGen: bad partition");
+  unsigned g = gridFor(n, kWaves, 1 << 16);
+  if (fillStart)
+    DISPATCH_W(S.W, kPtGen<WW, true><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts, part,
+                                                            counts, fillStart, outD, outN));
+  else
+    DISPATCH_W(S.W, kPtGen<WW, false><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts, part,
+                                                             counts, nullptr, nullptr, nullptr));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchIota(uint32_t* v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  kIota<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(v, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPtReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num, int64_t m,
+                    double e0, DevMem& partial, double* out, hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + kBlock - 1) / kBlock));
+  partial.reserve((size_t)nb * 3 * sizeof(double));
+  DISPATCH_W(S.W, kPtReduce<WW><<<nb, kBlock, 0, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
+  kFinalSum<<<3, kBlock, 0, s>>>(partial.as<double>(), nb, 3, 3, out);
   HIP_CHECK(hipGetLastError());
 }
 

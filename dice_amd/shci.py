@@ -90,6 +90,7 @@ def library():
         "shci_sigma": ([P, pD, pD, I32], I32),
         "shci_sigma_device": ([P, P, P, I32, I64], I32),
         "shci_davidson": ([P, pD, I32, I32, D, pD, ctypes.POINTER(ctypes.c_int)], I32),
+        "shci_pt_deterministic": ([P, pD, D, D, pD, pD, pI64], I32),
         "shci_run": ([ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(P)], I32),
         "shci_run_ex": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RunOptions), ctypes.POINTER(P)], I32),
         "shci_run_iteration_phases": ([P, I32, pD, pD, pD, pI64], I32),
@@ -99,6 +100,7 @@ def library():
         "shci_run_iteration": ([P, I32, pI64, pD, ctypes.POINTER(ctypes.c_int), pD], I32),
         "shci_run_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_run_ci": ([P, pD, I64], I32),
+        "shci_run_pt_energies": ([P, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_run_learned_count": ([P], I32),
         "shci_run_learned": ([P, I32, ctypes.POINTER(ctypes.c_int), pI64, pI64, pI64, pD, pD,
                               ctypes.POINTER(ctypes.c_int), pD], I32),
@@ -263,6 +265,20 @@ class System:
         return e, x, it.value
 
 
+    def pt_deterministic(self, ci, e0, eps2):
+        """Deterministic EN-PT2 of one root over the held determinants
+        (SHCIbasics::DoPerturbativeDeterministic, SHCI/SHCIbasics.cpp:453-728):
+        ci = the root's coefficients, e0 = its variational energy without the
+        core energy.  Returns (E_PT, psi1 norm, number of external determinants)."""
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        if len(c) != self.num_dets():
+            raise ValueError("need one coefficient per held determinant")
+        e, psi, nx = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._check(self._lib.shci_pt_deterministic(self._h, _p(c, ctypes.c_double), e0, eps2, ctypes.byref(e),
+                                                    ctypes.byref(psi), ctypes.byref(nx)))
+        return e.value, psi.value, nx.value
+
+
 class VariationalResult:
     def __init__(self, handle):
         self._h = handle
@@ -288,6 +304,22 @@ class VariationalResult:
         n = ctypes.c_int()
         self._lib.shci_run_energies(self._h, _p(e, ctypes.c_double), ctypes.byref(n))
         return e[:n.value].copy()
+
+    def pt_energies(self):
+        """E_var + E_PT per root when the run did the deterministic PT step (empty otherwise)."""
+        e = np.zeros(64)
+        n = ctypes.c_int()
+        self._lib.shci_run_pt_energies(self._h, _p(e, ctypes.c_double), ctypes.byref(n))
+        return e[:n.value].copy()
+
+    def ci(self):
+        """Final coefficients, n x nroots."""
+        n = ctypes.c_int64()
+        self._lib.shci_num_dets(self._h, ctypes.byref(n))
+        nr = len(self.energies())
+        out = np.zeros(n.value * nr)
+        self._lib.shci_run_ci(self._h, _p(out, ctypes.c_double), len(out))
+        return out.reshape(nr, n.value).T
 
     def phases(self):
         """Per iteration: (screen s, Hamiltonian s, Davidson s, lower-triangle element count)."""

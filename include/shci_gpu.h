@@ -62,7 +62,8 @@ double shci_core_energy(const shci_handle* h);
 /* Hamiltonian value precision for subsequent shci_build_hamiltonian calls. */
 shci_status shci_set_precision(shci_handle* h, int precision);
 /* Restrict excitations as the ncore / nact keywords do (SHCI/SHCIgetdeterminants.cpp:649,694,711)
- * and add RAS restrictions (`restrict min max orbs...`, SHCI/OccRestrictions.cpp). */
+ * and add RAS restrictions (`restrict min max orbs...`, SHCI/OccRestrictions.cpp,
+ * SHCI/input.cpp:267-276: to the variational and the PT space). */
 shci_status shci_set_active_space(shci_handle* h, int ncore, int nact);
 shci_status shci_add_restriction(shci_handle* h, int min_elec, int max_elec, const int* spatial_orbs, int n);
 
@@ -119,13 +120,27 @@ shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, in
  * and max_copies is ignored. */
 shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies, double tol, double* energies,
                           int* iterations);
+/* Deterministic Epstein-Nesbet PT2 of one root over the held determinants
+ * (SHCIbasics::DoPerturbativeDeterministic, SHCI/SHCIbasics.cpp:453-728, with
+ * the candidates of SHCIgetdeterminants::getDeterminantsDeterministicPT,
+ * SHCI/SHCIgetdeterminants.cpp:40-171): ci = the root's n coefficients
+ * (host), e0 = its variational energy without the core energy; *e_pt = the
+ * PT energy (add e0 and the core energy for the reference's PTEnergy);
+ * psi1_norm / n_external (optional) = sum (num / (e0 - E_a))^2 and the
+ * number of external determinants.  The PT space honours the restrictions
+ * of shci_add_restriction (the "restrict" keyword), not ncore / nact. */
+shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, double eps2, double* e_pt,
+                                  double* psi1_norm, int64_t* n_external);
 
 /* ---- whole variational step ----------------------------------------------------------- */
 /* Run the variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446,
  * SHCIbasics::DoVariational SHCI/SHCIbasics.cpp:783-1378): reads the input
  * file and the FCIDUMP it names, writes <prefix>/shci.e, and prints the
- * reference's tables to log_path (stdout when NULL or empty).  precision as
- * shci_precision.  The result handle holds the final space for inspection. */
+ * reference's tables to log_path (stdout when NULL or empty).  With the
+ * `deterministic` keyword (and nblocks 1) the deterministic PT step follows
+ * (SHCI/SHCI.cpp:706-721) and shci.e holds E_var + E_PT per root.
+ * precision as shci_precision.  The result handle holds the final space for
+ * inspection. */
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);
 
 /* Options of shci_run_ex and shci_set_options (zero-initialise, then set
@@ -203,6 +218,8 @@ double shci_run_final_davidson_seconds(const shci_handle* h);
  * events) and summed algorithmic bytes (stored values + indices, row offsets, diagonal, x and y once) */
 shci_status shci_profile_sigma_read(shci_handle* h, int64_t* calls, double* milliseconds, double* bytes);
 shci_status shci_run_energies(const shci_handle* h, double* energies, int* nroots);
+/* E_var + E_PT per root when the run did the deterministic PT step (*nroots = 0 otherwise) */
+shci_status shci_run_pt_energies(const shci_handle* h, double* energies, int* nroots);
 /* final coefficients, n x nroots column-major */
 shci_status shci_run_ci(shci_handle* h, double* out, int64_t cap);
 
