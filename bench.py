@@ -20,6 +20,10 @@ bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
 
+pt_deterministic: the deterministic EN-PT2 step (after the variational one)
+on the final space at eps2 = 1e-4, and GPU vs CPU oracle on the first
+iteration's space at eps2 = 1e-3.
+
 cpu_baseline: the CPU oracle (a restatement of the reference algorithm whose
 rank-parallel loops run on OpenMP threads, up to 16) on a bounded sample of
 the same workload: the first variational iteration from HF (27,685
@@ -40,6 +44,8 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 EPS1 = 5e-3
 NITER = 6
+PT_EPS2 = 1e-4
+PT_SAMPLE_EPS2 = 1e-3  # keeps the CPU oracle's PT sample to tens of seconds
 
 
 def traffic_bytes(precision):
@@ -135,6 +141,52 @@ def batching_form(shci, args):
     return res
 
 
+def pt_measure(shci, full, sample, log, args):
+    """Deterministic EN-PT2 (the step after the variational one) on the final
+    FP64 space of the workload at eps2 = PT_EPS2, and on the first-iteration
+    sample at eps2 = PT_SAMPLE_EPS2 on the GPU and in the CPU oracle (16
+    threads at most)."""
+    r = shci.do_variational(full, log)
+    dets, ci, e = r.dets(), r.ci()[:, 0], float(r.energies()[0])
+    del r
+    S = shci.System.from_fcidump(os.path.join(os.path.dirname(full), "FCIDUMP"))
+    core = S.core_energy
+    S.set_dets(dets)
+    S.pt_deterministic(ci, e - core, PT_EPS2)  # warm-up
+    t0 = time.time()
+    ept, psi1, next_ = S.pt_deterministic(ci, e - core, PT_EPS2)
+    gpu_s = time.time() - t0
+    S.close()
+    res = {"eps2": PT_EPS2, "ndets": len(dets), "seconds": round(gpu_s, 4), "E_PT": ept, "E_var_plus_PT": e + ept,
+           "external_dets": next_, "external_per_s": round(next_ / gpu_s, 1)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import OracleRun, OracleSystem
+        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), os.cpu_count() or 1))
+        os.environ["ORACLE_THREADS"] = str(threads)
+        fcidump = os.path.join(os.path.dirname(full), "FCIDUMP")
+        run = OracleRun(sample, None)
+        o = OracleSystem(fcidump)
+        sd, sc, se = run.dets(o.words), run.ci()[:, 0], float(run.energies()[0]) - core
+        t0 = time.time()
+        cpu_pt = o.pt_deterministic(sd, sc, se, PT_SAMPLE_EPS2)
+        cpu_s = time.time() - t0
+        G = shci.System.from_fcidump(fcidump)
+        G.set_dets(sd)
+        G.pt_deterministic(sc, se, PT_SAMPLE_EPS2)
+        t0 = time.time()
+        gpu_pt = G.pt_deterministic(sc, se, PT_SAMPLE_EPS2)
+        gpu_sample_s = time.time() - t0
+        G.close()
+        res["cpu_sample"] = {"sample": "first variational iteration's space (%d determinants), eps2 %.0e"
+                                       % (len(sd), PT_SAMPLE_EPS2),
+                             "cpu_s": round(cpu_s, 3), "cores": threads, "kind": "port", "gpu_s": round(gpu_sample_s, 4),
+                             "E_PT_cpu": cpu_pt[0], "E_PT_gpu": gpu_pt[0], "external_dets": cpu_pt[2]}
+    print("# PT eps2 %.0e: %.3f s, %d external determinants, E_PT %.10f" % (PT_EPS2, gpu_s, next_, ept),
+          file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +195,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-forms", action="store_true", help="time only the --precision form")
+    ap.add_argument("--davidson-block", action="store_true",
+                    help="the --precision form with the block Davidson and hipGraph replay (batching form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the variational step on a single MI355X "
@@ -156,8 +210,10 @@ def main():
     primary = shci.PREC_FP32 if args.precision == "fp32" else shci.PREC_FP64
     results = {}
     forms = [primary] if args.no_forms else [primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]
+    block = dict(davidson_block=True, hip_graphs=True) if args.davidson_block else {}
     for prec in forms:
-        results[prec] = run_form(shci, full, log, prec, args.steps, args.warmup)
+        results[prec] = run_form(shci, full, log, prec, args.steps, args.warmup, profile=not block,
+                                 **(block if prec == primary else {}))
     prec = primary
     R = results[prec]
     iter_times, step_times, calls, ms, nbytes, energies, per_iter = (
@@ -255,6 +311,9 @@ def main():
             "iterations": B["per_iter"],
         }
         out["forms"]["batched_block"] = batching_form(shci, args)
+
+    if not args.no_forms:
+        out["pt_deterministic"] = pt_measure(shci, full, sample, log, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
