@@ -323,12 +323,54 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       fclose(f);
     }
   }
-  if (log) {  // SHCI/SHCI.cpp:526-551
-    const int64_t N = E.n;
-    std::vector<int64_t> dd(N * W);
-    std::vector<double> cc(N * nroots);
+  std::vector<int64_t> dd;
+  std::vector<double> cc;
+  const int64_t N = E.n;
+  if (log || schd.writeBestDeterminants > 0) {
+    dd.resize(N * W);
+    cc.resize(N * nroots);
     HIP_CHECK(hipMemcpy(dd.data(), E.dets.p, N * W * 8, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(cc.data(), E.ci.p, N * nroots * 8, hipMemcpyDeviceToHost));
+  }
+  if (log && schd.printAllDeterminants) {  // SHCI/SHCI.cpp:449-462
+    fprintf(log, "Printing all determinants\n");
+    fprintf(log, "%4s %10s  Determinant string\n", "Det", "weight");
+    for (int root = 0; root < nroots; root++) {
+      fprintf(log, "State :%d\n", root);
+      for (int64_t i = 0; i < N; i++)
+        fprintf(log, "%18.8e  %s\n", cc[root * N + i] * parityDiceToAlphaBeta(&dd[i * W], nso),
+                detString(&dd[i * W], nso).c_str());
+    }
+  }
+  if (schd.writeBestDeterminants > 0) {  // SHCI/SHCI.cpp:464-505: dets.bin in the working directory
+    const int num = (int)std::min<int64_t>(schd.writeBestDeterminants, N);
+    const int nspatorbs = nso / 2;
+    for (int root = 0; root < nroots; root++) {
+      const std::string fname = root == 0 ? "dets.bin" : "dets_" + std::to_string(root) + ".bin";
+      FILE* f = fopen(fname.c_str(), "wb");
+      if (!f) continue;
+      fwrite(&num, sizeof(int), 1, f);
+      fwrite(&nspatorbs, sizeof(int), 1, f);
+      // |c| descending; ties (unordered in the reference) in determinant order
+      std::vector<int64_t> idx(N);
+      for (int64_t k = 0; k < N; k++) idx[k] = k;
+      const double* c = cc.data() + root * N;
+      std::stable_sort(idx.begin(), idx.end(), [c](int64_t a, int64_t b) { return std::fabs(c[a]) > std::fabs(c[b]); });
+      for (int i = 0; i < num; i++) {
+        const int64_t m = idx[i];
+        const double w = parityDiceToAlphaBeta(&dd[m * W], nso) * c[m];
+        fwrite(&w, sizeof(double), 1, f);
+        for (int k = 0; k < nspatorbs; k++) {
+          const bool a = (dd[m * W + (2 * k) / 64] >> ((2 * k) % 64)) & 1;
+          const bool b = (dd[m * W + (2 * k + 1) / 64] >> ((2 * k + 1) % 64)) & 1;
+          const char o = a ? (b ? '2' : 'a') : (b ? 'b' : '0');
+          fwrite(&o, 1, 1, f);
+        }
+      }
+      fclose(f);
+    }
+  }
+  if (log) {  // SHCI/SHCI.cpp:526-551
     fprintf(log, "Printing most important determinants\n");
     fprintf(log, "%4s %10s  Determinant string\n", "Det", "weight");
     for (int root = 0; root < nroots; root++) {

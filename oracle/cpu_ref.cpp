@@ -1421,6 +1421,44 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       fclose(f);
     }
   }
+  if (log && schd.printAllDeterminants) {  // SHCI.cpp:449-462
+    fprintf(log, "Printing all determinants\n");
+    fprintf(log, "%4s %10s  Determinant string\n", "Det", "weight");
+    for (int root = 0; root < nroots; root++) {
+      fprintf(log, "State :%d\n", root);
+      for (int i = 0; i < DetsSize; i++)
+        fprintf(log, "%18.8e  %s\n", ci[root][i] * parityDiceToAlphaBeta(Dets[i], s.nso),
+                detString(Dets[i], s.nso).c_str());
+    }
+  }
+  if (writeFiles && schd.writeBestDeterminants > 0) {  // SHCI.cpp:464-505
+    const int num = std::min(schd.writeBestDeterminants, DetsSize);
+    const int nspatorbs = s.nso / 2;
+    for (int root = 0; root < nroots; root++) {
+      std::string fname = root == 0 ? "dets.bin" : "dets_" + std::to_string(root) + ".bin";
+      FILE* f = fopen(fname.c_str(), "wb");
+      if (!f) continue;
+      fwrite(&num, sizeof(int), 1, f);
+      fwrite(&nspatorbs, sizeof(int), 1, f);
+      // |c| descending (ties, which the reference's sort leaves unordered,
+      // in determinant order)
+      std::vector<size_t> idx(DetsSize);
+      for (size_t k = 0; k < idx.size(); k++) idx[k] = k;
+      const std::vector<double>& c = ci[root];
+      std::stable_sort(idx.begin(), idx.end(), [&c](size_t a, size_t b) { return std::fabs(c[a]) > std::fabs(c[b]); });
+      for (int i = 0; i < num; i++) {
+        const size_t m = idx[i];
+        const double w = parityDiceToAlphaBeta(Dets[m], s.nso) * c[m];
+        fwrite(&w, sizeof(double), 1, f);
+        for (int k = 0; k < nspatorbs; k++) {
+          const bool a = getocc(Dets[m], 2 * k), b = getocc(Dets[m], 2 * k + 1);
+          const char o = a ? (b ? '2' : 'a') : (b ? 'b' : '0');
+          fwrite(&o, 1, 1, f);
+        }
+      }
+      fclose(f);
+    }
+  }
   if (log) {  // SHCI.cpp:526-551
     fprintf(log, "Printing most important determinants\n");
     fprintf(log, "%4s %10s  Determinant string\n", "Det", "weight");

@@ -181,3 +181,32 @@ def fci_space(norb, na, nb):
         for b in itertools.combinations(range(norb), nb):
             res.append(sorted([2 * x for x in a] + [2 * x + 1 for x in b]))
     return res
+
+
+def read_dets_bin(path):
+    """dets.bin of writeBestDeterminants (SHCI/SHCI.cpp:464-505): int count,
+    int spatial orbitals, then per determinant a double coefficient (times the
+    Dice -> alpha-beta parity) and one char per spatial orbital (0, a, b, 2).
+    Returns (norb, [(coefficient, string)])."""
+    import struct
+    with open(path, "rb") as f:
+        b = f.read()
+    num, norb = struct.unpack("<ii", b[:8])
+    if len(b) != 8 + num * (8 + norb):
+        raise ValueError("dets.bin: %d bytes, expected %d" % (len(b), 8 + num * (8 + norb)))
+    out, off = [], 8
+    for _ in range(num):
+        c = struct.unpack("<d", b[off:off + 8])[0]
+        s = b[off + 8:off + 8 + norb].decode("ascii")
+        if set(s) - set("0ab2"):
+            raise ValueError("dets.bin: bad occupation string %r" % s)
+        out.append((c, s))
+        off += 8 + norb
+    return norb, out
+
+
+def occ_string(occ, norb):
+    """Spin-orbital list -> the 0/a/b/2 string of dets.bin."""
+    o = set(occ)
+    return "".join("2" if (2 * i in o and 2 * i + 1 in o) else "a" if 2 * i in o else "b" if 2 * i + 1 in o
+                   else "0" for i in range(norb))

@@ -566,3 +566,42 @@ def test_run_direct_davidson_o2(tmp_path):
         assert u[:2] == v[:2]
         assert abs(float(u[2]) - float(v[2])) <= 1e-10 * abs(float(v[2]))
         assert abs(float(u[3]) - float(v[3])) <= 1e-6 * abs(float(v[3]))
+
+
+def test_run_output_files_match_oracle(tmp_path, monkeypatch):
+    """writeBestDeterminants / printAllDeterminants (SHCI/SHCI.cpp:448-505) on
+    the h8 golden input: the drop-in's dets.bin holds the oracle's 101
+    determinants with the same coefficients (1e-9), in non-increasing |c|, and
+    its log lists every determinant."""
+    import json
+    from dice_amd import shci
+    g = json.load(open(os.path.join(GOLD, "golden.json")))
+    fcidump = os.path.join(GOLD, "h8_631g", "FCIDUMP")
+    res = {}
+    for who in ("gpu", "oracle"):
+        d = tmp_path / who
+        d.mkdir()
+        inp = d / "dice.dat"
+        inp.write_text(g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nwriteBestDeterminants 101\n"
+                       "printAllDeterminants\n" % (fcidump, d))
+        monkeypatch.chdir(d)
+        if who == "gpu":
+            r = shci.do_variational(str(inp), str(d / "log"))
+            n = len(r.dets())
+        else:
+            OracleRun(str(inp), str(d / "log"))
+        res[who] = helpers.read_dets_bin(str(d / "dets.bin"))
+        log = open(d / "log").read()
+        block = log.split("Printing all determinants")[1].split("Printing most important")[0]
+        res[who + "_lines"] = len([l for l in block.splitlines()
+                                   if l.strip() and l.strip()[0] in "-0123456789" and "e" in l.split()[0]])
+    (ng, rg), (no, ro) = res["gpu"], res["oracle"]
+    assert ng == no == 16 and len(rg) == len(ro) == 101
+    mine = {s: c for c, s in rg}
+    ref = {s: c for c, s in ro}
+    assert set(mine) == set(ref)
+    for s, c in ref.items():
+        assert abs(mine[s] - c) < 1e-9
+    for (c0, _), (c1, _) in zip(rg, rg[1:]):
+        assert abs(c1) <= abs(c0)
+    assert res["gpu_lines"] == res["oracle_lines"] == n

@@ -445,3 +445,52 @@ def test_pt_run_writes_var_plus_pt(tmp_path):
     log = open(tmp_path / "log").read()
     assert "PERTURBATION THEORY STEP" in log
     assert "PTEnergy: %18.10f" % ept[0] in log
+
+
+# ---- output files of the run (SHCI/SHCI.cpp:448-505) ----
+
+def test_dets_bin_fixture_format(golden_dir):
+    """The reference's dets.bin fixture (tests/DQMC/multislater_rhf/dets.bin:
+    396 determinants, 8 orbitals) read with the same parser the output tests
+    use: header, record size, occupation alphabet, constant electron counts,
+    |c| non-increasing (up to 1e-12 relative near-ties)."""
+    norb, recs = helpers.read_dets_bin(os.path.join(golden_dir, "multislater_rhf", "dets.bin"))
+    assert norb == 8 and len(recs) == 396
+    na = {s.count("a") + s.count("2") for _, s in recs}
+    nb = {s.count("b") + s.count("2") for _, s in recs}
+    assert na == {5} and nb == {5}
+    assert abs(recs[0][0] - 0.9643137363372913) < 1e-15 and recs[0][1] == "22222000"
+    for (c0, _), (c1, _) in zip(recs, recs[1:]):
+        assert abs(c1) <= abs(c0) * (1 + 1e-12)
+
+
+def test_run_writes_dets_bin_and_all_determinants(tmp_path, golden_dir, monkeypatch):
+    """writeBestDeterminants 101 and printAllDeterminants on the h8 golden
+    input: dets.bin (written to the working directory, as the reference does)
+    holds the 101 largest |c| (the 101st is followed by a 0.9% gap; ranks
+    100 and 101 are a tie) with the parity-adjusted sign of the log's
+    "most important determinants" table, and the log lists every
+    determinant."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "h8_631g", "FCIDUMP")
+    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nwriteBestDeterminants 101\nprintAllDeterminants\n" % (
+        fcidump, tmp_path)
+    _write(tmp_path / "dice.dat", inp)
+    monkeypatch.chdir(tmp_path)
+    run = OracleRun(str(tmp_path / "dice.dat"), str(tmp_path / "log"))
+    norb, recs = helpers.read_dets_bin(str(tmp_path / "dets.bin"))
+    assert norb == 16 and len(recs) == 101
+    dets = run.dets(1)
+    ci = run.ci()[:, 0]
+    index = {helpers.occ_string(helpers.det_to_occ(d), 16): k for k, d in enumerate(dets)}
+    order = np.argsort(-np.abs(ci), kind="stable")
+    assert [index[s] for _, s in recs] == list(order[:101])
+    for c, s in recs:
+        assert abs(abs(c) - abs(ci[index[s]])) < 1e-15
+    log = open(tmp_path / "log").read()
+    top = log.split("Printing most important determinants")[1].splitlines()[3:9]
+    for (c, s), line in zip(recs[:6], top):
+        assert abs(float(line.split()[1]) - c) < 1e-10
+    block = log.split("Printing all determinants")[1].split("Printing most important")[0].splitlines()
+    assert block[2] == "State :0"
+    assert len([l for l in block if l.strip() and l.strip()[0] in "-0123456789" and "e" in l.split()[0]]) == len(dets)
