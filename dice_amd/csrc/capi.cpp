@@ -108,6 +108,64 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
 
 // The variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446 and
 // SHCIbasics::DoVariational, SHCI/SHCIbasics.cpp:783-1378) on the device.
+// Checkpoint of the variational step in this build's own portable format
+// (the role of SHCIbasics::writeVariationalResult / readVariationalResult,
+// SHCI/SHCIbasics.cpp:1481-1575, whose files are Boost archives):
+// <prefix>/shci_var.bkp, little-endian: char[8] "SHCIVAR1", int32 W,
+// nroots, iter, converged, int64 n, double E0[nroots] (no core energy),
+// int64 dets[n][W] (run order), double ci[nroots][n].
+struct Checkpoint {
+  int W = 0, nroots = 0, iter = 0, converged = 0;
+  std::vector<double> E0;
+  std::vector<int64_t> dets;
+  std::vector<double> ci;
+  int64_t n = 0;
+};
+
+bool writeCheckpoint(const std::string& path, const Checkpoint& c) {
+  const std::string tmp = path + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return false;
+  const char magic[8] = {'S', 'H', 'C', 'I', 'V', 'A', 'R', '1'};
+  const int32_t hdr[4] = {c.W, c.nroots, c.iter, c.converged};
+  bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(hdr, 4, 4, f) == 4 && fwrite(&c.n, 8, 1, f) == 1 &&
+            fwrite(c.E0.data(), 8, c.nroots, f) == (size_t)c.nroots &&
+            fwrite(c.dets.data(), 8, c.n * c.W, f) == (size_t)(c.n * c.W) &&
+            fwrite(c.ci.data(), 8, c.n * c.nroots, f) == (size_t)(c.n * c.nroots);
+  ok = fclose(f) == 0 && ok;
+  return ok && std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+bool readCheckpoint(const std::string& path, Checkpoint& c, std::string& err) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) {
+    err = "cannot open the checkpoint " + path;
+    return false;
+  }
+  char magic[8];
+  int32_t hdr[4];
+  bool ok = fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SHCIVAR1", 8) == 0 && fread(hdr, 4, 4, f) == 4 &&
+            fread(&c.n, 8, 1, f) == 1;
+  if (ok) {
+    c.W = hdr[0];
+    c.nroots = hdr[1];
+    c.iter = hdr[2];
+    c.converged = hdr[3];
+    ok = c.W >= 1 && c.W <= 2 && c.nroots >= 1 && c.nroots <= 64 && c.n >= 1 && c.n < ((int64_t)1 << 32);
+  }
+  if (ok) {
+    c.E0.resize(c.nroots);
+    c.dets.resize(c.n * c.W);
+    c.ci.resize(c.n * c.nroots);
+    ok = fread(c.E0.data(), 8, c.nroots, f) == (size_t)c.nroots &&
+         fread(c.dets.data(), 8, c.n * c.W, f) == (size_t)(c.n * c.W) &&
+         fread(c.ci.data(), 8, c.n * c.nroots, f) == (size_t)(c.n * c.nroots);
+  }
+  fclose(f);
+  if (!ok) err = "the checkpoint " + path + " is not a valid variational checkpoint";
+  return ok;
+}
+
 shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, const shci_run_options& opt) {
   Engine& E = h->eng;
   const int precision = opt.precision;
@@ -131,11 +189,10 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", err.c_str());
     return fail(h, SHCI_BAD_INPUT, err);
   }
-  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.restart || schd.fullrestart || schd.doSOC ||
-      schd.searchLowest || schd.doLCC) {
+  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchLowest || schd.doLCC) {
     std::string m =
-        "keyword outside the variational path implemented here (Treversal, diskdavidson, restart, "
-        "fullrestart, dosoc, searchForLowestEnergyDet, dolcc)";
+        "keyword outside the variational path implemented here (Treversal, diskdavidson, dosoc, "
+        "searchForLowestEnergyDet, dolcc)";
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
   }
@@ -224,7 +281,53 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
   E.iters.clear();
   if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
   DevMem X0;
-  for (int iter = 0; iter < (int)schd.epsilon1.size(); iter++) {
+  const std::string ckptPath = schd.prefix[0] + "/shci_var.bkp";
+  auto saveCheckpoint = [&](int iter) {  // SHCI/SHCIbasics.cpp:1224-1227,1345-1352 (always "converged")
+    Checkpoint c;
+    c.W = W;
+    c.nroots = nroots;
+    c.iter = iter;
+    c.converged = 1;
+    c.n = E.n;
+    c.E0 = E0;
+    c.dets.resize(c.n * W);
+    c.ci.resize(c.n * nroots);
+    HIP_CHECK(hipMemcpy(c.dets.data(), E.dets.p, c.n * W * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(c.ci.data(), E.ci.p, c.n * nroots * 8, hipMemcpyDeviceToHost));
+    if (!writeCheckpoint(ckptPath, c) && log) fprintf(log, "could not write the checkpoint %s\n", ckptPath.c_str());
+  };
+  // restart from the checkpoint (SHCI/SHCIbasics.cpp:871-941)
+  int iterstart = 0;
+  bool fromConverged = false;
+  if (schd.restart || schd.fullrestart) {
+    Checkpoint c;
+    std::string e;
+    if (!readCheckpoint(ckptPath, c, e) || c.W != W || c.nroots != nroots) {
+      if (e.empty()) e = "the checkpoint " + ckptPath + " does not match this system (words or roots)";
+      if (log) fprintf(log, "%s\n", e.c_str());
+      return fail(h, SHCI_BAD_INPUT, e);
+    }
+    E.setDets(c.dets.data(), c.n);
+    E.buildHamiltonian();
+    E.ci.reserve((size_t)c.n * nroots * 8);
+    HIP_CHECK(hipMemcpy(E.ci.p, c.ci.data(), c.n * nroots * 8, hipMemcpyHostToDevice));
+    E0 = c.E0;
+    const int last = (int)schd.epsilon1.size() - 1;
+    iterstart = c.iter;
+    for (int i = 0; i < nroots; i++)
+      if (log)
+        fprintf(log, "%4d %4d  %10.2e  %10lld -   %18.10f  %10.2f\n", iterstart, i,
+                schd.epsilon1[std::min(iterstart, last)], (long long)c.n, E0[i] + coreEbkp, wallTime() - startofCalc);
+    iterstart = schd.fullrestart ? 0 : iterstart + 1;
+    // (a full restart re-runs the schedule: the reference's test would read
+    // epsilon1[-1] there)
+    if (!schd.fullrestart && c.converged &&
+        (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
+      if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
+      fromConverged = true;
+    }
+  }
+  for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
     const double eps1 = schd.epsilon1[iter];
     const double t0 = wallTime();
     const int64_t nold = E.n;
@@ -302,18 +405,20 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       E.tFinalDavidson = wallTime() - tf0;
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");
+      if (schd.io) saveCheckpoint(iter);
       break;
     }
+    if (schd.io) saveCheckpoint(iter);
   }
-  if (log) {
+  if (log && !fromConverged) {
     fprintf(log, "\nVariational calculation result\n");
     fprintf(log, "%4s %18s  %10s\n", "Root", "Energy", "Time(s)");
   }
   for (size_t i = 0; i < E0.size(); i++) {
     E0[i] += coreEbkp;
-    if (log) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
+    if (log && !fromConverged) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
   }
-  if (log) fprintf(log, "\n");
+  if (log && !fromConverged) fprintf(log, "\n");
   E.energies = E0;
   {  // SHCI/SHCI.cpp:437-446
     std::string efile = schd.prefix[0] + "/shci.e";

@@ -1227,10 +1227,9 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     if (log) fprintf(log, "%s\n", err.c_str());
     return;
   }
-  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.restart || schd.fullrestart || schd.doSOC ||
-      schd.searchForLowestEnergyDet || schd.doLCC) {
+  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchForLowestEnergyDet || schd.doLCC) {
     res.status = 2;
-    res.error = "keyword outside the restated variational path (Treversal/disk/restart/soc/search/lcc)";
+    res.error = "keyword outside the restated variational path (Treversal/disk/soc/search/lcc)";
     if (log) fprintf(log, "%s\n", res.error.c_str());
     return;
   }
@@ -1314,7 +1313,77 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
   // directdavidson: the same product with every element evaluated on the fly
   const bool direct = schd.davidsonType == 1;
   auto directOp = [&](const double* x, double* y) { hmultDirect(s, Dets, H, x, y, (size_t)DetsSize); };
-  for (int iter = 0; iter < (int)schd.epsilon1.size(); iter++) {
+  // checkpoint: the role of writeVariationalResult / readVariationalResult
+  // (SHCIbasics.cpp:1481-1575) in the drop-in's own format: <prefix>/shci_var.bkp,
+  // "SHCIVAR1", int32 W, nroots, iter, converged, int64 n, E0[nroots] (no
+  // core energy), int64 dets[n][W], double ci[nroots][n]
+  const std::string ckpt = schd.prefix[0] + "/shci_var.bkp";
+  auto saveCheckpoint = [&](int iter) {  // SHCIbasics.cpp:1224-1227,1345-1352 (always "converged")
+    if (!writeFiles || !schd.io) return;
+    FILE* f = fopen(ckpt.c_str(), "wb");
+    if (!f) return;
+    const int32_t hdr[4] = {s.W, nroots, iter, 1};
+    const int64_t n = DetsSize;
+    fwrite("SHCIVAR1", 1, 8, f);
+    fwrite(hdr, 4, 4, f);
+    fwrite(&n, 8, 1, f);
+    fwrite(E0.data(), 8, nroots, f);
+    for (const Det& d : Dets) fwrite(d.r, 8, s.W, f);
+    for (int r = 0; r < nroots; r++) fwrite(ci[r].data(), 8, n, f);
+    fclose(f);
+  };
+  // restart (SHCIbasics.cpp:871-941)
+  int iterstart = 0;
+  bool fromConverged = false;
+  if (schd.restart || schd.fullrestart) {
+    FILE* f = fopen(ckpt.c_str(), "rb");
+    char magic[8];
+    int32_t hdr[4];
+    int64_t n = 0;
+    bool ok = f && fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SHCIVAR1", 8) == 0 && fread(hdr, 4, 4, f) == 4 &&
+              fread(&n, 8, 1, f) == 1 && hdr[0] == s.W && hdr[1] == nroots && n > 0;
+    std::vector<double> e(nroots);
+    std::vector<Det> d(ok ? n : 0);
+    std::vector<std::vector<double>> c(nroots, std::vector<double>(ok ? n : 0));
+    if (ok) {
+      ok = fread(e.data(), 8, nroots, f) == (size_t)nroots;
+      for (int64_t k = 0; ok && k < n; k++) ok = fread(d[k].r, 8, s.W, f) == (size_t)s.W;
+      for (int r = 0; ok && r < nroots; r++) ok = fread(c[r].data(), 8, n, f) == (size_t)n;
+    }
+    if (f) fclose(f);
+    if (!ok) {
+      res.status = 1;
+      res.error = "cannot read the checkpoint " + ckpt;
+      if (log) fprintf(log, "%s\n", res.error.c_str());
+      return;
+    }
+    Dets = d;
+    ci = c;
+    E0 = e;
+    DetsSize = (int)n;
+    SortedDets = Dets;
+    std::sort(SortedDets.begin(), SortedDets.end(), less);
+    SortedDetsSize = DetsSize;
+    helpers = Helpers();
+    H = SparseHam();
+    populateHelpers(s, helpers, Dets, 0);
+    makeHRows(s, helpers, Dets, 0, H);
+    diag.resize(DetsSize);
+    for (int k = 0; k < DetsSize; k++) diag[k] = energy(s, Dets[k]);
+    const int last = (int)schd.epsilon1.size() - 1;
+    iterstart = hdr[2];
+    for (int i = 0; i < nroots; i++)
+      if (log)
+        fprintf(log, "%4d %4d  %10.2e  %10d -   %18.10f  %10.2f\n", iterstart, i,
+                schd.epsilon1[std::min(iterstart, last)], DetsSize, E0[i] + coreEbkp, wallTime() - startofCalc);
+    iterstart = schd.fullrestart ? 0 : iterstart + 1;
+    if (!schd.fullrestart && hdr[3] &&
+        (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
+      if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
+      fromConverged = true;
+    }
+  }
+  for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
     double epsilon1 = schd.epsilon1[iter];
     std::vector<double> cMax(DetsSize, 0.0);
     for (int j = 0; j < DetsSize; j++) {
@@ -1397,18 +1466,20 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
         return;
       }
       if (log) fprintf(log, "Exiting variational iterations\n");
+      saveCheckpoint(iter);
       break;
     }
+    saveCheckpoint(iter);
   }
-  if (log) {
+  if (log && !fromConverged) {
     fprintf(log, "\nVariational calculation result\n");
     fprintf(log, "%4s %18s  %10s\n", "Root", "Energy", "Time(s)");
   }
   for (size_t i = 0; i < E0.size(); i++) {
     E0[i] += coreEbkp;
-    if (log) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
+    if (log && !fromConverged) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
   }
-  if (log) fprintf(log, "\n");
+  if (log && !fromConverged) fprintf(log, "\n");
   res.energies = E0;
   res.dets = Dets;
   res.ci = ci;

@@ -25,12 +25,14 @@ COPIES = {
     "o2/FCIDUMP": "tests/SHCI/o2_det/FCIDUMP",
     "h2o/FCIDUMP": "tests/SHCI/integrals/h2o_FCIDUMP_symm",
     "h2h2/FCIDUMP": "tests/SHCI/integrals/H2H2_FCIDUMP_nosym",
+    "multislater_rhf/dets.bin": "tests/DQMC/multislater_rhf/dets.bin",
 }
 ENERGIES = {
     "restart": "tests/SHCI/restart/trusted_hci.e",
     "n2_RAS": "tests/SHCI/n2_RAS/trusted_hci.e",
     "o2_det_var_plus_pt": "tests/SHCI/o2_det/trusted_hci.e",
     "o2_det_direct": "tests/SHCI/o2_det_direct/trusted_hci.e",
+    "fullrestart": "tests/SHCI/fullrestart/trusted_hci.e",
 }
 
 
@@ -55,6 +57,9 @@ def main():
                        "input": open(os.path.join(REF, "examples/h8_631g/dice.dat")).read()}
     # inputs of the reference tests, kept as text data
     gold["restart_input3"] = open(os.path.join(REF, "tests/SHCI/restart/input3.dat")).read()
+    # (tests/SHCI/fullrestart uses the restart test's FCIDUMP, byte for byte)
+    for name in ("restart/input2", "fullrestart/input2", "fullrestart/input3"):
+        gold[name.replace("/", "_")] = open(os.path.join(REF, "tests/SHCI/%s.dat" % name)).read()
     gold["n2_RAS_input"] = open(os.path.join(REF, "tests/SHCI/n2_RAS/input.dat")).read()
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(gold, f, indent=1)

@@ -605,3 +605,44 @@ def test_run_output_files_match_oracle(tmp_path, monkeypatch):
     for (c0, _), (c1, _) in zip(rg, rg[1:]):
         assert abs(c1) <= abs(c0)
     assert res["gpu_lines"] == res["oracle_lines"] == n
+
+
+def test_run_restart_chain_trusted(tmp_path):
+    """tests/SHCI/restart as the reference runs it, on the GPU: input2 (io)
+    checkpoints every iteration, input3 (restart) continues the schedule to
+    eps1 5e-5; shci.e matches trusted_hci.e (runner tolerance 5e-5)."""
+    import json
+    import struct
+    from dice_amd import shci
+    g = json.load(open(os.path.join(GOLD, "golden.json")))
+    fcidump = os.path.join(GOLD, "restart", "FCIDUMP")
+    for k, name in enumerate(("restart_input2", "restart_input3")):
+        text = "\n".join(l for l in g[name].splitlines() if not l.strip().lower().startswith("prefix"))
+        p = tmp_path / ("in%d.dat" % k)
+        p.write_text(text + "\norbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+        shci.do_variational(str(p), str(tmp_path / ("log%d" % k)))
+    with open(tmp_path / "shci.e", "rb") as f:
+        e = struct.unpack("<d", f.read(8))[0]
+    assert abs(e - g["trusted_energy"]["restart"]) < 1e-6, e
+
+
+def test_run_restart_from_oracle_checkpoint_moves_to_pt(tmp_path):
+    """The checkpoint format is shared: the GPU restarts from the oracle's
+    checkpoint of an O2 run at eps1 1e-2, skips the iterations (converged at
+    or below its own eps1 2e-2) and gives the oracle's E_var and E_var + E_PT
+    (1e-10 Ha)."""
+    from dice_amd import shci
+    from test_oracle_pins import O2_INPUT
+    fcidump = os.path.join(GOLD, "o2", "FCIDUMP")
+    base = O2_INPUT.replace("deterministic\n", "").replace("noio\n", "").replace(
+        "schedule\n0 0.001\n3 0.0001\nend\n", "schedule\n0 0.01\nend\n")
+    (tmp_path / "a.dat").write_text(base + "io\norbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+    OracleRun(str(tmp_path / "a.dat"), str(tmp_path / "a.log"))
+    b = base.replace("schedule\n0 0.01\nend\n", "schedule\n0 0.02\nend\n")
+    (tmp_path / "b.dat").write_text(b + "restart\ndeterministic\nnoio\norbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+    rg = shci.do_variational(str(tmp_path / "b.dat"), str(tmp_path / "g.log"))
+    ro = OracleRun(str(tmp_path / "b.dat"), str(tmp_path / "o.log"))
+    assert "moving to perturbative part" in open(tmp_path / "g.log").read()
+    np.testing.assert_allclose(rg.energies(), ro.energies(), rtol=0, atol=1e-10)
+    np.testing.assert_allclose(rg.pt_energies(), ro.pt_energies(), rtol=0, atol=1e-10)
+    assert rg.pt_energies()[0] < rg.energies()[0] - 1e-6
