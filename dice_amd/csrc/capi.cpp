@@ -519,6 +519,25 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       for (double e : ept) fwrite(&e, 1, sizeof(double), f);
       fclose(f);
     }
+  } else if (schd.SampleN != -1 && schd.nPTiter != 0) {  // semistochastic PT, SHCI/SHCI.cpp:722-741
+    if (log) {
+      fprintf(log, "\n\n**************************************************************\n");
+      fprintf(log, "PERTURBATION THEORY STEP  \n");
+      fprintf(log, "**************************************************************\n");
+    }
+    std::vector<double> ept(E0.size());
+    for (size_t root = 0; root < E0.size(); root++)
+      ept[root] = E0[root] + E.ptSemistochastic(E.ci.as<double>() + (int64_t)root * E.n, E0[root] - coreEbkp,
+                                                schd.epsilon2, schd.epsilon2Large, schd.SampleN, schd.nPTiter,
+                                                schd.targetError, coreEbkp, (int)root, log, startofCalc, nullptr,
+                                                nullptr, nullptr);
+    E.ptEnergies = ept;
+    std::string efile = schd.prefix[0] + "/shci.e";
+    FILE* f = fopen(efile.c_str(), "wb");
+    if (f) {
+      for (double e : ept) fwrite(&e, 1, sizeof(double), f);
+      fclose(f);
+    }
   }
   return SHCI_OK;
 }
@@ -803,6 +822,27 @@ shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, d
     dc.reserve((size_t)std::max<int64_t>(n, 1) * 8);
     if (n) HIP_CHECK(hipMemcpyAsync(dc.p, ci, n * 8, hipMemcpyHostToDevice, E.stream));
     *e_pt = E.ptDeterministic(dc.as<double>(), e0, eps2, psi1_norm, n_external);
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_pt_semistochastic(shci_handle* h, const double* ci, double e0, double eps2, double eps2_large,
+                                   int sample_n, int n_iter, double target_error, unsigned seed, double* e_pt,
+                                   double* per_iter, int cap, int* iterations, double* error) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (!ci || !e_pt) throw std::invalid_argument("null argument");
+    if (sample_n < 2) throw std::invalid_argument("sample_n must be at least 2");
+    const int64_t n = E.n;
+    DevMem dc;
+    dc.reserve((size_t)std::max<int64_t>(n, 1) * 8);
+    if (n) HIP_CHECK(hipMemcpyAsync(dc.p, ci, n * 8, hipMemcpyHostToDevice, E.stream));
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    std::srand(seed);
+    std::vector<double> per;
+    *e_pt = E.ptSemistochastic(dc.as<double>(), e0, eps2, eps2_large, sample_n, n_iter, target_error, 0.0, 0,
+                               nullptr, 0.0, error, iterations, &per);
+    for (int k = 0; per_iter && k < cap && k < (int)per.size(); k++) per_iter[k] = per[k];
     return SHCI_OK;
   });
 }

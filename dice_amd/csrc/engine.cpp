@@ -163,6 +163,145 @@ double Engine::ptDeterministic(const double* dCi, double e0, double eps2, double
   return pt;
 }
 
+double Engine::ptSemistochastic(const double* dCi, double e0, double eps2, double eps2Large, int sampleN, int nPTiter,
+                                double targetError, double coreE, int root, FILE* log, double startofCalc,
+                                double* error, int* iters, std::vector<double>* perIter) {
+  if (nPTiter == 0) return 0.0;
+  auto now = [] {
+    timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+  };
+  const int W = hs.W;
+  const double E0c = e0 + coreE;
+  if (log) fprintf(log, "Performing (semi)stochastic PT for state: %3d\n", root);
+  // the deterministic part always runs; only its heading depends on the
+  // threshold (SHCI/SHCIbasics.cpp:69-75)
+  if (log && eps2Large < 999) fprintf(log, "\n1/ Deterministic calculation with epsilon2=%g\n", eps2Large);
+  const double eptLarge = ptDeterministic(dCi, e0, eps2Large, nullptr, nullptr);
+  if (log) {  // DoPerturbativeDeterministic's lines
+    fprintf(log, "Deterministic PT calculation converged\n");
+    fprintf(log, "PTEnergy: %18.10f\n", E0c + eptLarge);
+    fprintf(log, "Time(s):  %10.2f\n", now() - startofCalc);
+    fprintf(log, "\n2/ Stochastic calculation with epsilon2=%g\n", eps2);
+  }
+  const int N = (int)n;
+  std::vector<double> c(N);
+  HIP_CHECK(hipMemcpy(c.data(), dCi, (size_t)N * 8, hipMemcpyDeviceToHost));
+  const int Nmc = sampleN;
+  double cumulative = 0.0;
+  for (int i = 0; i < N; i++) cumulative += std::fabs(c[i]);
+  // alias tables (SHCI/SHCIsampledeterminants.cpp:58-78)
+  std::vector<int> alias(N, 0);
+  std::vector<double> prob(N, 0.0);
+  {
+    std::vector<double> larger, smaller;
+    for (int i = 0; i < N; i++) {
+      prob[i] = std::fabs(c[i]) * N / cumulative;
+      (prob[i] < 1.0 ? smaller : larger).push_back(i);
+    }
+    while (!larger.empty() && !smaller.empty()) {
+      const int l = (int)larger.back(), sm = (int)smaller.back();
+      larger.pop_back();
+      smaller.pop_back();
+      alias[sm] = l;
+      prob[l] = prob[l] - (1.0 - prob[sm]);
+      (prob[l] < 1.0 ? smaller : larger).push_back(l);
+    }
+  }
+  std::vector<int> sample(Nmc, -1);
+  std::vector<double> wts(Nmc, 0.0);
+  std::vector<uint32_t> src32;
+  ensureTable(n);
+  const DevSystem P = ptSystem();
+  ptCounts.reserve((size_t)(Nmc + 1) * 8);
+  ptOut.reserve(64);
+  int64_t* cnt = ptCounts.as<int64_t>();
+  if (log) fprintf(log, "%6s  %18s  %5s %18s %10s  %10s\n", "Iter", "EPTcurrent", "State", "EPTavg", "Error", "Time(s)");
+  double avg = 0.0, avg2 = 0.0, stddev = 0.0;
+  int currentIter = 0;
+  for (int iter = 0; iter < nPTiter; iter++) {
+    // sample_N2_alias (SHCI/SHCIsampledeterminants.cpp:148-176)
+    std::fill(sample.begin(), sample.end(), -1);
+    int distinct = 0;
+    for (int index = 0; index < Nmc; index++) {
+      int detIndex = (int)std::floor(1. * ((double)std::rand() / (RAND_MAX)) * N);
+      const double r = ((double)std::rand() / (RAND_MAX));
+      if (r >= prob[detIndex]) detIndex = alias[detIndex];
+      auto it = std::find(sample.begin(), sample.end(), detIndex);
+      if (it == sample.end()) {
+        sample[distinct] = detIndex;
+        wts[distinct] = cumulative * c[detIndex] / std::fabs(c[detIndex]);
+        distinct++;
+      } else {
+        wts[std::distance(sample.begin(), it)] += cumulative * c[detIndex] / std::fabs(c[detIndex]);
+      }
+    }
+    for (int i = 0; i < Nmc; i++) wts[i] /= Nmc;
+    src32.assign(sample.begin(), sample.begin() + distinct);
+    ptSrc.reserve((size_t)std::max(distinct, 1) * 4);
+    ptW.reserve((size_t)std::max(distinct, 1) * 8);
+    HIP_CHECK(hipMemcpyAsync(ptSrc.p, src32.data(), (size_t)distinct * 4, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(ptW.p, wts.data(), (size_t)distinct * 8, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemsetAsync(cnt + distinct, 0, 8, stream));
+    launchPtGenStoch(P, dets.as<uint64_t>(), ptSrc.as<uint32_t>(), ptW.as<double>(), distinct, dCi, eps2, eps2Large,
+                     (double)Nmc, table.as<int32_t>(), slots, cnt, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     stream);
+    exclusiveScanI64(cnt, distinct + 1, ws, stream);
+    int64_t m = 0;
+    HIP_CHECK(hipMemcpyAsync(&m, cnt + distinct, 8, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (m > INT32_MAX) throw std::invalid_argument("semistochastic PT: more than 2^31 candidates in one iteration");
+    double host[2] = {0.0, 0.0};
+    if (m > 0) {
+      ptKeys.reserve((size_t)m * W * 8);
+      ptNum.reserve((size_t)m * 8);
+      ptN2.reserve((size_t)m * 8);
+      ptP.reserve((size_t)m);
+      ptIdx.reserve((size_t)m * 4);
+      launchPtGenStoch(P, dets.as<uint64_t>(), ptSrc.as<uint32_t>(), ptW.as<double>(), distinct, dCi, eps2,
+                       eps2Large, (double)Nmc, table.as<int32_t>(), slots, nullptr, cnt, ptKeys.as<uint64_t>(),
+                       ptNum.as<double>(), ptN2.as<double>(), ptP.as<uint8_t>(), stream);
+      launchIota(ptIdx.as<uint32_t>(), m, stream);
+      radixSortKeys(ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), m, W, hs.nso, true, ws, stream);
+      launchPtStochReduce(P, ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), ptNum.as<double>(), ptN2.as<double>(),
+                          ptP.as<uint8_t>(), m, e0, (double)Nmc, ptPartial, ptOut.as<double>(), stream);
+      HIP_CHECK(hipMemcpyAsync(host, ptOut.p, 16, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    // the running mean and standard error (SHCI/SHCIbasics.cpp:404-440)
+    currentIter++;
+    const double est = -host[0] + host[1] + eptLarge;
+    if (perIter) perIter->push_back(est);
+    avg += est;
+    avg2 += std::pow(est, 2);
+    stddev = currentIter < 5
+                 ? 1e4
+                 : std::pow((currentIter * avg2 - std::pow(avg, 2)) / currentIter / (currentIter - 1) / currentIter, 0.5);
+    if (log) {
+      if (currentIter < 5)
+        fprintf(log, "%6d  %18.10f  %5d %18.10f %10s  %10.2f\n", currentIter, E0c + est, root, E0c + avg / currentIter,
+                "--", now() - startofCalc);
+      else
+        fprintf(log, "%6d  %18.10f  %5d %18.10f %10.2e  %10.2f\n", currentIter, E0c + est, root,
+                E0c + avg / currentIter, stddev, now() - startofCalc);
+      fflush(log);
+    }
+    if (stddev < targetError) {
+      avg /= currentIter;
+      if (log) {
+        fprintf(log, "Semistochastic PT calculation converged\n");
+        fprintf(log, "PTEnergy: %18.10f +/- %8.2e\n", E0c + avg, stddev);
+        fprintf(log, "Time(s):  %10.2f\n", now() - startofCalc);
+      }
+      break;
+    }
+  }
+  if (error) *error = stddev;
+  if (iters) *iters = currentIter;
+  return avg;
+}
+
 void Engine::ensureTable(int64_t need) {
   uint64_t want = 1024;
   while (want < (uint64_t)need * 2) want <<= 1;

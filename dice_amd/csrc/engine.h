@@ -138,6 +138,17 @@ struct Engine {
   double ptDeterministic(const double* dCi, double e0, double eps2, double* psi1Norm, int64_t* nExternal,
                          int* nPartitions = nullptr);
   DevSystem ptSystem() const;
+  // semistochastic EN-PT2 (SHCI/SHCIbasics.cpp:55-452) of root vector dCi
+  // (device, n values): the deterministic part at eps2Large, then sampled
+  // iterations (C library rand(), as the reference) until the standard error
+  // is below targetError or nPTiter iterations ran; log lines as the
+  // reference's when log is set (coreE only shifts printed energies).
+  // Returns the reference function's value (the average correction, or the
+  // unaveraged sum when nPTiter runs out).
+  DevMem ptN2, ptP, ptSrc, ptW;
+  double ptSemistochastic(const double* dCi, double e0, double eps2, double eps2Large, int sampleN, int nPTiter,
+                          double targetError, double coreE, int root, FILE* log, double startofCalc, double* error,
+                          int* iters, std::vector<double>* perIter);
 
   // last run
   std::vector<IterRecord> iters;

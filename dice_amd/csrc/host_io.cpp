@@ -286,6 +286,8 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
     else if (same(key, "orbitals")) schd.integralFile = arg(1);
     else if (same(key, "nptiter")) schd.nPTiter = iarg(1);
     else if (same(key, "epsilon2")) schd.epsilon2 = farg(1);
+    else if (same(key, "epsilon2Large")) schd.epsilon2Large = farg(1);
+    else if (same(key, "targetError")) schd.targetError = farg(1);
     else if (same(key, "seed")) schd.randomSeed = (unsigned int)farg(1);
     else if (same(key, "nroots")) schd.nroots = (int)farg(1);
     else if (same(key, "printbestdeterminants")) schd.printBestDeterminants = iarg(1);
@@ -345,8 +347,8 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
     else if (same(key, "relaxedRDM") || same(key, "num_thrds") || same(key, "extrapolate") ||
              same(key, "algorithm") || same(key, "doresponse") || same(key, "maxseniority") ||
              same(key, "maxexcitation") || same(key, "SenioAndExc") || same(key, "dogtensor") ||
-             same(key, "targetError") || same(key, "dosocqdpt") || same(key, "socmultiplier") ||
-             same(key, "epsilon2Large") || same(key, "onlyperturbative") || same(key, "DoSpinOneRDM") ||
+             same(key, "dosocqdpt") || same(key, "socmultiplier") ||
+             same(key, "onlyperturbative") || same(key, "DoSpinOneRDM") ||
              same(key, "TransitionRDM") || same(key, "quasiq") ||
              same(key, "cdfciTol") || same(key, "excitation") || same(key, "nvirt") || same(key, "eps") ||
              same(key, "pointGroup") || same(key, "spin") || same(key, "irrep") || same(key, "DoThreeRDM") ||

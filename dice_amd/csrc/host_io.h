@@ -37,6 +37,7 @@ struct HostSchedule {
   std::vector<std::vector<int>> occupied;
   std::vector<double> epsilon1;
   double davidsonTol = 5e-5, davidsonTolLoose = 5e-5, dE = 1e-8, epsilon2 = 1e-8;
+  double epsilon2Large = 1000.0, targetError = 1e-4;
   int nroots = 1;
   unsigned int randomSeed = 0;
   bool io = true;

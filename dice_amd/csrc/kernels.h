@@ -36,6 +36,19 @@ void launchCmax(const double* ci, int nroots, int64_t ld, int64_t n, double* cma
 void launchPtGen(const DevSystem& S, const uint64_t* dets, int64_t n, const double* ci, double eps2,
                  const int32_t* table, uint64_t slots, uint32_t nparts, uint32_t part, int64_t* counts,
                  const int64_t* fillStart, uint64_t* outD, double* outN, hipStream_t s);
+// Semistochastic PT candidates of the sampled sources src[0..nsrc) with
+// weights w (count pass when fillStart == nullptr): numerators H_ai w, second
+// moments H_ai^2 w (w nmc/(nmc-1) - c) and the epsilon2Large flag
+// (SHCI/SHCIgetdeterminants.cpp:889-1046), in generation order
+void launchPtGenStoch(const DevSystem& S, const uint64_t* dets, const uint32_t* src, const double* w, int64_t nsrc,
+                      const double* ci, double eps2, double eps2Large, double nmc, const int32_t* table, uint64_t slots,
+                      int64_t* counts, const int64_t* fillStart, uint64_t* outD, double* outN, double* outN2,
+                      uint8_t* outP, hipStream_t s);
+// out[0..1] (device) = the two energy sums of one semistochastic iteration
+// over m candidates sorted by determinant (payload idx)
+void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num,
+                         const double* num2, const uint8_t* pres, int64_t m, double e0, double nmc, DevMem& partial,
+                         double* out, hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // m candidates sorted by determinant (payload idx into num): out[0..2]
 // (device) = E_PT, psi1 norm, number of external determinants

@@ -343,7 +343,10 @@ struct PtSink {
   int cnt;
   uint64_t* outD;
   double* outN;
-  __device__ __forceinline__ void put(bool keep, const uint64_t* cand, double num) {
+  double* outN2;     // semistochastic: second-moment numerator
+  uint8_t* outP;     // semistochastic: term passes epsilon2Large
+  __device__ __forceinline__ void put(bool keep, const uint64_t* cand, double num, double num2 = 0.0,
+                                      bool present = false) {
     if (!FILL) {
       cnt += keep ? 1 : 0;
       return;
@@ -354,9 +357,26 @@ struct PtSink {
 #pragma unroll
       for (int w = 0; w < W; w++) outD[p * W + w] = cand[w];
       outN[p] = num;
+      if (outN2) {
+        outN2[p] = num2;
+        outP[p] = present ? 1 : 0;
+      }
     }
     pos += __popcll(m);
   }
+};
+
+// Semistochastic mode (SHCI/SHCIgetdeterminants.cpp:889-1046): source k is
+// determinant src[k] with sample weight w[k]; num = H_ai w, num2 = H_ai^2 w
+// (w Nmc / (Nmc - 1) - c) with a double's H_ai^2 formed in float as there,
+// present = |H_ai| > epsilon2Large / |c|.
+struct PtStoch {
+  const uint32_t* src;
+  const double* w;
+  double eps2Large;
+  double nmc;
+  double* outN2;
+  uint8_t* outP;
 };
 
 // Candidates D_a of sources [0, ndets) with numerator H_ai c_i: singles with
@@ -364,11 +384,11 @@ struct PtSink {
 // (numerator = value x parity of the excitation x c_i), eps = |eps2 / c_i|,
 // no active-space limits, the PT restrictions in S; kept when D_a is not a
 // variational determinant and falls in partition `part`.
-template <int W, bool FILL>
+template <int W, bool FILL, bool STOCH>
 __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* dets, int64_t ndets, const double* ci,
                                                  double eps2, const int32_t* table, uint64_t mask, uint32_t nparts,
                                                  uint32_t part, int64_t* counts, const int64_t* fillStart,
-                                                 uint64_t* outD, double* outN) {
+                                                 uint64_t* outD, double* outN, PtStoch st) {
   __shared__ uint8_t sClosed[kWaves][128];
   __shared__ uint8_t sOpen[kWaves][128];
   __shared__ int sExcl[kWaves][65];
@@ -377,16 +397,22 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   const int nso = S.nso;
   for (int64_t di = (int64_t)blockIdx.x * kWaves + wave; di < ndets; di += (int64_t)gridDim.x * kWaves) {
+    const int64_t src = STOCH ? (int64_t)st.src[di] : di;
     uint64_t d[W];
 #pragma unroll
-    for (int w = 0; w < W; w++) d[w] = dets[di * W + w];
-    const double c = ci[di];
-    const double eps = fabs(eps2 / c);
+    for (int w = 0; w < W; w++) d[w] = dets[src * W + w];
+    const double c = ci[src];
+    const double wt = STOCH ? st.w[di] : c;
+    const double eps = STOCH ? __ddiv_rn(eps2, fabs(c)) : fabs(eps2 / c);
+    const double epsL = STOCH ? __ddiv_rn(st.eps2Large, fabs(c)) : 0.0;
+    const double wfac = STOCH ? __dsub_rn(__ddiv_rn(__dmul_rn(wt, st.nmc), st.nmc - 1.0), c) : 0.0;
     PtSink<W, FILL> sink;
     sink.pos = FILL ? fillStart[di] : 0;
     sink.cnt = 0;
     sink.outD = outD;
     sink.outN = outN;
+    sink.outN2 = STOCH ? st.outN2 : nullptr;
+    sink.outP = STOCH ? st.outP : nullptr;
     int nclosed = 0, nopen = 0;
     for (int base = 0; base < nso; base += 64) {
       int k = base + lane;
@@ -409,8 +435,8 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
     const int nsingle = nclosed * nopen;
     for (int t0 = 0; t0 < nsingle; t0 += 64) {
       int t = t0 + lane;
-      bool keep = false;
-      double num = 0.0;
+      bool keep = false, pres = false;
+      double num = 0.0, num2 = 0.0;
       uint64_t cand[W];
 #pragma unroll
       for (int w = 0; w < W; w++) cand[w] = d[w];
@@ -427,11 +453,15 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
         if (ok && fabs(integral) > eps) {
           cand[oa >> 6] |= (1ull << (oa & 63));
           cand[cI >> 6] &= ~(1ull << (cI & 63));
-          num = __dmul_rn(integral, c);
+          num = __dmul_rn(integral, wt);
+          if (STOCH) {
+            num2 = __dmul_rn(__dmul_rn(__dmul_rn(integral, integral), wt), wfac);
+            pres = fabs(integral) > epsL;
+          }
           keep = ptPart<W>(cand, nparts) == part && !hashContains<W>(table, mask, dets, cand);
         }
       }
-      sink.put(keep, cand, num);
+      sink.put(keep, cand, num, num2, pres);
     }
     // ---- double excitations
     if (!(S.maxEntryAbs < eps)) {
@@ -476,8 +506,8 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
         __syncwarp();
         for (int k0 = 0; k0 < total; k0 += 64) {
           int k = k0 + lane;
-          bool keep = false;
-          double num = 0.0;
+          bool keep = false, pres = false;
+          double num = 0.0, num2 = 0.0;
           uint64_t cand[W];
 #pragma unroll
           for (int w = 0; w < W; w++) cand[w] = d[w];
@@ -518,11 +548,16 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
               mid2[cI >> 6] |= (1ull << (cI & 63));
               nb += countBetween<W>(mid2, b < cJ ? b : cJ, b < cJ ? cJ : b);
               const double sgn = (nb & 1) ? -1.0 : 1.0;
-              num = __dmul_rn(__dmul_rn((double)S.hbVal[same][idx], sgn), c);
+              const float hv = S.hbVal[same][idx];
+              num = __dmul_rn(__dmul_rn((double)hv, sgn), wt);
+              if (STOCH) {
+                num2 = __dmul_rn(__dmul_rn((double)__fmul_rn(hv, hv), wt), wfac);
+                pres = (double)fabsf(hv) > epsL;
+              }
               keep = ptPart<W>(cand, nparts) == part && !hashContains<W>(table, mask, dets, cand);
             }
           }
-          sink.put(keep, cand, num);
+          sink.put(keep, cand, num, num2, pres);
         }
         __syncwarp();
       }
@@ -1476,6 +1511,43 @@ __global__ void __launch_bounds__(kBlock) kFinalSum(const double* partial, int n
   }
 }
 
+// Semistochastic reduction (SHCI/SHCIbasics.cpp:264-385): per run of equal
+// determinants (sorted, generation order kept), a1/a2 = sums of num / num2,
+// b1/b2 = the same over the terms present at epsilon2Large; per-block partial
+// sums of (|a1|^2 Nmc/(Nmc-1) - a2) / (E_a - E0) and of the b version.
+template <int W>
+__global__ void __launch_bounds__(kBlock) kPtStochReduce(DevSystem S, const uint64_t* keys, const uint32_t* idx,
+                                                         const double* num, const double* num2, const uint8_t* pres,
+                                                         int64_t m, double e0, double nmc, double* partial) {
+  double sA = 0.0, sB = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i > 0 && detEqual<W>(keys + i * W, keys + (i - 1) * W)) continue;
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int64_t j = i; j < m && detEqual<W>(keys + j * W, keys + i * W); j++) {
+      const uint32_t q = idx[j];
+      a1 = __dadd_rn(a1, num[q]);
+      a2 = __dadd_rn(a2, num2[q]);
+      if (pres[q]) {
+        b1 = __dadd_rn(b1, num[q]);
+        b2 = __dadd_rn(b2, num2[q]);
+      }
+    }
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = keys[i * W + w];
+    const double den = __dsub_rn(detEnergy<W>(S, d), e0);
+    sA = __dadd_rn(sA, __ddiv_rn(__dsub_rn(__ddiv_rn(__dmul_rn(__dmul_rn(fabs(a1), fabs(a1)), nmc), nmc - 1.0), a2),
+                                 den));
+    sB = __dadd_rn(sB, __ddiv_rn(__dsub_rn(__ddiv_rn(__dmul_rn(__dmul_rn(fabs(b1), fabs(b1)), nmc), nmc - 1.0), b2),
+                                 den));
+  }
+  const double t0 = blockSum(sA), t1 = blockSum(sB);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = t0;
+    partial[2 * blockIdx.x + 1] = t1;
+  }
+}
+
 __global__ void kIota(uint32_t* v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     v[i] = (uint32_t)i;
@@ -1723,12 +1795,40 @@ void launchPtGen(const DevSystem& S, const uint64_t* dets, int64_t n, const doub
   if (n <= 0) return;
   if (nparts == 0 || part >= nparts) throw HipError("launchPtGen: bad partition");
   unsigned g = gridFor(n, kWaves, 1 << 16);
+  const PtStoch st{nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
   if (fillStart)
-    DISPATCH_W(S.W, kPtGen<WW, true><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts, part,
-                                                            counts, fillStart, outD, outN));
+    DISPATCH_W(S.W, kPtGen<WW, true, false><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts, part,
+                                                                   counts, fillStart, outD, outN, st));
   else
-    DISPATCH_W(S.W, kPtGen<WW, false><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts, part,
-                                                             counts, nullptr, nullptr, nullptr));
+    DISPATCH_W(S.W, kPtGen<WW, false, false><<<g, kBlock, 0, s>>>(S, dets, n, ci, eps2, table, slots - 1, nparts,
+                                                                    part, counts, nullptr, nullptr, nullptr, st));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPtGenStoch(const DevSystem& S, const uint64_t* dets, const uint32_t* src, const double* w, int64_t nsrc,
+                      const double* ci, double eps2, double eps2Large, double nmc, const int32_t* table, uint64_t slots,
+                      int64_t* counts, const int64_t* fillStart, uint64_t* outD, double* outN, double* outN2,
+                      uint8_t* outP, hipStream_t s) {
+  if (nsrc <= 0) return;
+  unsigned g = gridFor(nsrc, kWaves, 1 << 16);
+  const PtStoch st{src, w, eps2Large, nmc, outN2, outP};
+  if (fillStart)
+    DISPATCH_W(S.W, kPtGen<WW, true, true><<<g, kBlock, 0, s>>>(S, dets, nsrc, ci, eps2, table, slots - 1, 1u, 0u,
+                                                                  counts, fillStart, outD, outN, st));
+  else
+    DISPATCH_W(S.W, kPtGen<WW, false, true><<<g, kBlock, 0, s>>>(S, dets, nsrc, ci, eps2, table, slots - 1, 1u, 0u,
+                                                                   counts, nullptr, nullptr, nullptr, st));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num,
+                         const double* num2, const uint8_t* pres, int64_t m, double e0, double nmc, DevMem& partial,
+                         double* out, hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + kBlock - 1) / kBlock));
+  partial.reserve((size_t)nb * 2 * sizeof(double));
+  DISPATCH_W(S.W, kPtStochReduce<WW><<<nb, kBlock, 0, s>>>(S, keys, idx, num, num2, pres, m, e0, nmc,
+                                                           partial.as<double>()));
+  kFinalSum<<<2, kBlock, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
   HIP_CHECK(hipGetLastError());
 }
 

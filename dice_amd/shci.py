@@ -91,6 +91,8 @@ def library():
         "shci_sigma_device": ([P, P, P, I32, I64], I32),
         "shci_davidson": ([P, pD, I32, I32, D, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_pt_deterministic": ([P, pD, D, D, pD, pD, pI64], I32),
+        "shci_pt_semistochastic": ([P, pD, D, D, D, I32, I32, D, ctypes.c_uint, pD, pD, I32,
+                                    ctypes.POINTER(ctypes.c_int), pD], I32),
         "shci_run": ([ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(P)], I32),
         "shci_run_ex": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RunOptions), ctypes.POINTER(P)], I32),
         "shci_run_iteration_phases": ([P, I32, pD, pD, pD, pI64], I32),
@@ -277,6 +279,21 @@ class System:
         self._check(self._lib.shci_pt_deterministic(self._h, _p(c, ctypes.c_double), e0, eps2, ctypes.byref(e),
                                                     ctypes.byref(psi), ctypes.byref(nx)))
         return e.value, psi.value, nx.value
+
+
+    def pt_semistochastic(self, ci, e0, eps2, eps2_large, sample_n, n_iter, target_error, seed):
+        """Semistochastic EN-PT2 of one root (SHCI/SHCIbasics.cpp:55-452): returns
+        (the reference function's value, per-iteration estimates, iterations, standard error)."""
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        if len(c) != self.num_dets():
+            raise ValueError("need one coefficient per held determinant")
+        e, err, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        per = np.zeros(max(n_iter, 1))
+        self._check(self._lib.shci_pt_semistochastic(self._h, _p(c, ctypes.c_double), e0, eps2, eps2_large, sample_n,
+                                                     n_iter, target_error, seed, ctypes.byref(e),
+                                                     _p(per, ctypes.c_double), len(per), ctypes.byref(it),
+                                                     ctypes.byref(err)))
+        return e.value, per[:it.value].copy(), it.value, err.value
 
 
 class VariationalResult:

@@ -127,7 +127,9 @@ def write_input(path, fcidump, occ, schedule, maxiter, nroots=1, seed=1, davidso
             f.write("%d %g\n" % (it, eps))
         f.write("end\n")
         f.write("davidsonTol %g\ndE %g\nmaxiter %d\nseed %d\n" % (davidson_tol, dE, maxiter, seed))
-        f.write("noio\nsampleN 200\nnPTiter 0\n")
+        f.write("noio\nsampleN 200\n")
+        if not any(l.lower().startswith("nptiter") for l in extra):
+            f.write("nPTiter 0\n")
         for line in extra:
             f.write(line + "\n")
 

@@ -131,6 +131,19 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
  * of shci_add_restriction (the "restrict" keyword), not ncore / nact. */
 shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, double eps2, double* e_pt,
                                   double* psi1_norm, int64_t* n_external);
+/* Semistochastic EN-PT2 of one root over the held determinants
+ * (SHCIbasics::DoPerturbativeStochastic2SingleListDoubleEpsilon2AllTogether,
+ * SHCI/SHCIbasics.cpp:55-452, with the alias sampling of
+ * SHCI/SHCIsampledeterminants.cpp:37-181): the deterministic PT at
+ * eps2_large plus sampled iterations of sample_n draws (C library rand()
+ * after srand(seed)) until the standard error is below target_error or
+ * n_iter iterations ran.  *e_pt = the reference function's value (the mean
+ * correction; the unaveraged sum when n_iter runs out); per_iter (cap
+ * entries, optional) = each iteration's estimate; *iterations, *error
+ * (optional) = iterations run and the last standard error. */
+shci_status shci_pt_semistochastic(shci_handle* h, const double* ci, double e0, double eps2, double eps2_large,
+                                   int sample_n, int n_iter, double target_error, unsigned seed, double* e_pt,
+                                   double* per_iter, int cap, int* iterations, double* error);
 
 /* ---- whole variational step ----------------------------------------------------------- */
 /* Run the variational step of `Dice input.dat` (SHCI/SHCI.cpp:157-446,
@@ -138,7 +151,9 @@ shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, d
  * file and the FCIDUMP it names, writes <prefix>/shci.e, and prints the
  * reference's tables to log_path (stdout when NULL or empty).  With the
  * `deterministic` keyword (and nblocks 1) the deterministic PT step follows
- * (SHCI/SHCI.cpp:706-721) and shci.e holds E_var + E_PT per root.
+ * (SHCI/SHCI.cpp:706-721), otherwise with sampleN set and nPTiter != 0 the
+ * semistochastic one (SHCI/SHCI.cpp:722-741); shci.e then holds
+ * E_var + E_PT per root.
  * precision as shci_precision.  The result handle holds the final space for
  * inspection. */
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);

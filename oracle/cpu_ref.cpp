@@ -298,7 +298,7 @@ bool readInput(const std::string& path, Schedule& schd, std::string& err, FILE* 
     else if (iequals(ArgName, "maxexcitation")) {}
     else if (iequals(ArgName, "SenioAndExc")) {}
     else if (iequals(ArgName, "dogtensor")) {}
-    else if (iequals(ArgName, "targetError")) {}
+    else if (iequals(ArgName, "targetError")) schd.targetError = atof(arg(1).c_str());
     else if (iequals(ArgName, "orbitals")) schd.integralFile = arg(1);
     else if (iequals(ArgName, "dosocqdpt")) {}
     else if (iequals(ArgName, "nptiter")) schd.nPTiter = atoi(arg(1).c_str());
@@ -306,7 +306,7 @@ bool readInput(const std::string& path, Schedule& schd, std::string& err, FILE* 
     else if (iequals(ArgName, "socmultiplier")) {}
     else if (iequals(ArgName, "seed")) schd.randomSeed = (unsigned int)atof(arg(1).c_str());
     else if (iequals(ArgName, "nroots")) schd.nroots = (int)atof(arg(1).c_str());
-    else if (iequals(ArgName, "epsilon2Large")) {}
+    else if (iequals(ArgName, "epsilon2Large")) schd.epsilon2Large = atof(arg(1).c_str());
     else if (iequals(ArgName, "onlyperturbative")) {}
     else if (iequals(ArgName, "printbestdeterminants")) schd.printBestDeterminants = atoi(arg(1).c_str());
     else if (iequals(ArgName, "printalldeterminants")) schd.printAllDeterminants = true;
@@ -643,7 +643,7 @@ static void parityDouble(Det d, int i, int j, int a, int b, double& sgn) {
 // of one source with numerator H_ai c_i (singles: Hij_1Excite, doubles: the
 // float heat-bath value times the parity of the excitation)
 static void ptCandidates(const System& s, const Schedule& schd, const Det& d, double epsilon, double ci1,
-                         std::vector<Det>& dets, std::vector<double>& num) {
+                         std::vector<Det>& dets, std::vector<double>& num, std::vector<char>* isDouble = nullptr) {
   int closed[128], open[128], nclosed = 0, nopen = 0;
   for (int i = 0; i < s.nso; i++) {
     if (getocc(d, i))
@@ -673,6 +673,7 @@ static void ptCandidates(const System& s, const Schedule& schd, const Det& d, do
       setocc(di, closed[i], false);
       dets.push_back(di);
       num.push_back(integral * ci1);
+      if (isDouble) isDouble->push_back(0);
     }
   }
   if (std::fabs(s.maxEntry) < epsilon) return;
@@ -704,6 +705,7 @@ static void ptCandidates(const System& s, const Schedule& schd, const Det& d, do
         parityDouble(di, a, b, closed[i], closed[j], sgn);
         dets.push_back(di);
         num.push_back((double)integrals[index] * sgn * ci1);
+        if (isDouble) isDouble->push_back(1);
       }
     }
   }
@@ -757,6 +759,179 @@ double ptDeterministic(const System& s, const Schedule& schd, const std::vector<
   if (psi1Norm) *psi1Norm = norm;
   if (nExternal) *nExternal = next;
   return pt;
+}
+
+// alias tables for sampling i with probability |c_i| / cumulative
+// (SHCIsampledeterminants.cpp:58-78)
+static void aliasSetup(const double* ci, int n, double cumulative, std::vector<int>& alias, std::vector<double>& prob) {
+  alias.assign(n, 0);
+  prob.assign(n, 0.0);
+  std::vector<double> larger, smaller;  // (indices held as doubles, as there)
+  for (int i = 0; i < n; i++) {
+    prob[i] = std::fabs(ci[i]) * n / cumulative;
+    if (prob[i] < 1.0)
+      smaller.push_back(i);
+    else
+      larger.push_back(i);
+  }
+  while (larger.size() > 0 && smaller.size() > 0) {
+    int l = (int)larger.back();
+    larger.pop_back();
+    int sm = (int)smaller.back();
+    smaller.pop_back();
+    alias[sm] = l;
+    prob[l] = prob[l] - (1.0 - prob[sm]);
+    if (prob[l] < 1.0)
+      smaller.push_back(l);
+    else
+      larger.push_back(l);
+  }
+}
+
+// sample_N2_alias (SHCIsampledeterminants.cpp:148-176): Nmc draws with
+// replacement; distinct determinants in first-drawn order with weights
+// cumulative * sign(c) * (times drawn) / Nmc; returns the distinct count
+static int sampleAlias(const double* ci, int n, double cumulative, std::vector<int>& sample, std::vector<double>& wts,
+                       const std::vector<int>& alias, const std::vector<double>& prob) {
+  const int niter = (int)sample.size();
+  int sampleIndex = 0;
+  for (int index = 0; index < niter; index++) {
+    int detIndex = (int)std::floor(1. * ((double)rand() / (RAND_MAX)) * n);
+    double rand_no = ((double)rand() / (RAND_MAX));
+    if (rand_no >= prob[detIndex]) detIndex = alias[detIndex];
+    auto it = std::find(sample.begin(), sample.end(), detIndex);
+    if (it == sample.end()) {
+      sample[sampleIndex] = detIndex;
+      wts[sampleIndex] = cumulative * ci[detIndex] / std::fabs(ci[detIndex]);
+      sampleIndex++;
+    } else {
+      wts[std::distance(sample.begin(), it)] += cumulative * ci[detIndex] / std::fabs(ci[detIndex]);
+    }
+  }
+  for (int i = 0; i < niter; i++) wts[i] /= niter;
+  return sampleIndex;
+}
+
+// getDeterminantsStochastic2Epsilon (SHCIgetdeterminants.cpp:889-1046):
+// the deterministic candidates of D with w = the sample weight in place of c
+// in num1 = H_ai w, num2 = H_ai^2 w (w Nmc/(Nmc-1) - c), and present = the
+// term passes epsLarge
+static void ptStochCandidates(const System& s, const Schedule& schd, const Det& d, double eps, double epsLarge,
+                              double w, double c, int Nmc, std::vector<Det>& dets, std::vector<double>& num1,
+                              std::vector<double>& num2, std::vector<char>& present) {
+  std::vector<Det> cd;
+  std::vector<double> v;
+  std::vector<char> dbl;
+  ptCandidates(s, schd, d, eps, 1.0, cd, v, &dbl);  // v = H_ai (c = 1)
+  const double Nmcd = 1. * Nmc;
+  for (size_t k = 0; k < cd.size(); k++) {
+    dets.push_back(cd[k]);
+    num1.push_back(v[k] * w);
+    // (a double's value is a float, squared in float arithmetic there)
+    const double v2 = dbl[k] ? (double)((float)std::fabs(v[k]) * (float)std::fabs(v[k])) : v[k] * v[k];
+    num2.push_back(v2 * w * (w * Nmcd / (Nmcd - 1) - c));
+    present.push_back(std::fabs(v[k]) > epsLarge);
+  }
+}
+
+double ptSemistochastic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
+                        double E0, double coreE, int root, FILE* log, double startofCalc, double* error, int* iters,
+                        std::vector<double>* perIter) {
+  if (schd.nPTiter == 0) return 0.0;
+  const int W = s.W;
+  const int DetsSize = (int)dets.size();
+  const double E0c = E0 + coreE;  // the reference's E0 holds the core energy
+  if (log) fprintf(log, "Performing (semi)stochastic PT for state: %3d\n", root);
+  // the deterministic part always runs (only the message depends on the
+  // threshold, SHCIbasics.cpp:69-75)
+  if (log && schd.epsilon2Large < 999) fprintf(log, "\n1/ Deterministic calculation with epsilon2=%g\n", schd.epsilon2Large);
+  const double EptLarge = ptDeterministic(s, schd, dets, ci, E0, schd.epsilon2Large);
+  if (log) {  // DoPerturbativeDeterministic's own lines
+    fprintf(log, "Deterministic PT calculation converged\n");
+    fprintf(log, "PTEnergy: %18.10f\n", E0c + EptLarge);
+    fprintf(log, "Time(s):  %10.2f\n", wallTime() - startofCalc);
+    fprintf(log, "\n2/ Stochastic calculation with epsilon2=%g\n", schd.epsilon2);
+  }
+  std::vector<Det> sortedVar = dets;
+  std::sort(sortedVar.begin(), sortedVar.end(), [W](const Det& x, const Det& y) { return detLess(x, y, W); });
+  const int Nmc = schd.SampleN;
+  double cumulative = 0.0;
+  for (int i = 0; i < DetsSize; i++) cumulative += std::fabs(ci[i]);
+  std::vector<int> alias;
+  std::vector<double> prob;
+  aliasSetup(ci, DetsSize, cumulative, alias, prob);
+  std::vector<int> allSample(Nmc, -1);
+  std::vector<double> allwts(Nmc, 0.);
+  if (log) fprintf(log, "%6s  %18s  %5s %18s %10s  %10s\n", "Iter", "EPTcurrent", "State", "EPTavg", "Error", "Time(s)");
+  double Avg = 0.0, Avg2 = 0.0, stddev = 0.0;
+  int currentIter = 0;
+  for (int iter = 0; iter < schd.nPTiter; iter++) {
+    std::fill(allSample.begin(), allSample.end(), -1);
+    const int distinct = sampleAlias(ci, DetsSize, cumulative, allSample, allwts, alias, prob);
+    std::vector<Det> cand;
+    std::vector<double> n1, n2;
+    std::vector<char> pres;
+    for (int k = 0; k < distinct; k++) {
+      const int I = allSample[k];
+      ptStochCandidates(s, schd, dets[I], schd.epsilon2 / std::fabs(ci[I]), schd.epsilon2Large / std::fabs(ci[I]),
+                        allwts[k], ci[I], Nmc, cand, n1, n2, pres);
+    }
+    // MergeSort (stable), then one pass over the runs of equal determinants
+    // that are not variational (SHCIbasics.cpp:264-385)
+    std::vector<size_t> order(cand.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return detLess(cand[x], cand[y], W); });
+    double energyEN = 0.0, energyENLarge = 0.0;
+    for (size_t k = 0; k < order.size();) {
+      const Det& d = cand[order[k]];
+      double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+      size_t k1 = k;
+      for (; k1 < order.size() && detEq(cand[order[k1]], d, W); k1++) {
+        a1 += n1[order[k1]];
+        a2 += n2[order[k1]];
+        if (pres[order[k1]]) {
+          b1 += n1[order[k1]];
+          b2 += n2[order[k1]];
+        }
+      }
+      k = k1;
+      if (std::binary_search(sortedVar.begin(), sortedVar.end(), d,
+                             [W](const Det& x, const Det& y) { return detLess(x, y, W); }))
+        continue;
+      const double den = energy(s, d) - E0;
+      energyEN += (std::pow(std::fabs(a1), 2) * Nmc / (Nmc - 1) - a2) / den;
+      energyENLarge += (std::pow(std::fabs(b1), 2) * Nmc / (Nmc - 1) - b2) / den;
+    }
+    currentIter++;
+    const double est = -energyEN + energyENLarge + EptLarge;
+    if (perIter) perIter->push_back(est);
+    Avg += est;
+    Avg2 += std::pow(est, 2);
+    stddev = currentIter < 5 ? 1e4
+                             : std::pow((currentIter * Avg2 - std::pow(Avg, 2)) / currentIter / (currentIter - 1) /
+                                            currentIter,
+                                        0.5);
+    if (log) {
+      if (currentIter < 5)
+        fprintf(log, "%6d  %18.10f  %5d %18.10f %10s  %10.2f\n", currentIter, E0c + est, root,
+                E0c + Avg / currentIter, "--", wallTime() - startofCalc);
+      else
+        fprintf(log, "%6d  %18.10f  %5d %18.10f %10.2e  %10.2f\n", currentIter, E0c + est, root,
+                E0c + Avg / currentIter, stddev, wallTime() - startofCalc);
+    }
+    if (stddev < schd.targetError) {
+      Avg /= currentIter;
+      if (log) {
+        fprintf(log, "Semistochastic PT calculation converged\n");
+        fprintf(log, "PTEnergy: %18.10f +/- %8.2e\n", E0c + Avg, stddev);
+        fprintf(log, "Time(s):  %10.2f\n", wallTime() - startofCalc);
+      }
+      break;
+    }
+  }
+  if (error) *error = stddev;
+  if (iters) *iters = currentIter;
+  return Avg;
 }
 
 // ---------------------------------------------------------------------------
@@ -1567,6 +1742,25 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
         fflush(log);
       }
     }
+    res.ptEnergies = ept;
+    if (writeFiles) {
+      std::string efile = schd.prefix[0] + "/shci.e";
+      FILE* f = fopen(efile.c_str(), "wb");
+      if (f) {
+        for (double e : ept) fwrite(&e, 1, sizeof(double), f);
+        fclose(f);
+      }
+    }
+  } else if (schd.SampleN != -1 && schd.nPTiter != 0) {  // semistochastic PT, SHCI/SHCI.cpp:722-741
+    if (log) {
+      fprintf(log, "\n\n**************************************************************\n");
+      fprintf(log, "PERTURBATION THEORY STEP  \n");
+      fprintf(log, "**************************************************************\n");
+    }
+    std::vector<double> ept(nroots);
+    for (int root = 0; root < nroots; root++)
+      ept[root] = E0[root] + ptSemistochastic(s, schd, Dets, ci[root].data(), E0[root] - coreEbkp, coreEbkp, root,
+                                              log, startofCalc);
     res.ptEnergies = ept;
     if (writeFiles) {
       std::string efile = schd.prefix[0] + "/shci.e";

@@ -57,7 +57,7 @@ struct Schedule {  // subset of SHCI/input.h:35-194 that the variational path re
   std::vector<std::vector<int>> occupied;
   std::vector<double> epsilon1;
   double davidsonTol = 5e-5, davidsonTolLoose = 5e-5, dE = 1e-8;
-  double epsilon2 = 1e-8;
+  double epsilon2 = 1e-8, epsilon2Large = 1000.0, targetError = 1e-4;
   int nroots = 1;
   unsigned int randomSeed = 0;
   bool io = true;
@@ -174,6 +174,20 @@ std::vector<double> davidson(const SparseHam& H, std::vector<std::vector<double>
 // psi1Norm (optional) receives sum (num / (E0 - E_a))^2.
 double ptDeterministic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
                        double E0, double eps2, double* psi1Norm = nullptr, size_t* nExternal = nullptr);
+
+// Semistochastic EN-PT2 of one root (SHCIbasics.cpp:55-452): the
+// deterministic PT at epsilon2Large plus, per iteration, an unbiased estimate
+// of the remainder down to epsilon2 from SampleN determinants drawn with
+// probability |c_i| / sum |c| (alias method, SHCIsampledeterminants.cpp:37-181,
+// with the C library's rand()); stops when the standard error is below
+// targetError or after nPTiter iterations.  E0 without the core energy;
+// coreE only shifts the printed energies.  Returns what the reference's
+// function returns (the average energy correction, or the unaveraged sum
+// when nPTiter runs out); *error / *iters receive the last standard error and
+// the iteration count.
+double ptSemistochastic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
+                        double E0, double coreE, int root, FILE* log, double startofCalc, double* error = nullptr,
+                        int* iters = nullptr, std::vector<double>* perIter = nullptr);
 
 struct IterRecord {
   int iter;

@@ -1,6 +1,7 @@
 // C entry points of the CPU oracle, for the test suite and bench.py's
 // cpu_baseline leg (loaded with ctypes).  TEST INFRASTRUCTURE ONLY.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -100,6 +101,28 @@ double oref_pt_deterministic(void* p, const int64_t* dets, int64_t n, const doub
   const double pt = ptDeterministic(h->s, h->schd, d, ci, e0, eps2, psi1Norm, &next);
   if (nExternal) *nExternal = (int64_t)next;
   return pt;
+}
+
+// semistochastic EN-PT2 of one root (srand(seed) first, as the run's start
+// does); perIter (cap entries) receives each iteration's estimate; returns
+// the reference function's value
+double oref_pt_semistochastic(void* p, const int64_t* dets, int64_t n, const double* ci, double e0, double eps2,
+                              double eps2Large, int sampleN, int nPTiter, double targetError, unsigned seed,
+                              double* perIter, int cap, int* iters, double* error) {
+  Handle* h = (Handle*)p;
+  std::vector<Det> d(n);
+  for (int64_t i = 0; i < n; i++) d[i] = loadDet(dets + i * h->s.W, h->s.W);
+  Schedule schd = h->schd;
+  schd.epsilon2 = eps2;
+  schd.epsilon2Large = eps2Large;
+  schd.SampleN = sampleN;
+  schd.nPTiter = nPTiter;
+  schd.targetError = targetError;
+  std::srand(seed);
+  std::vector<double> est;
+  const double r = ptSemistochastic(h->s, schd, d, ci, e0, 0.0, 0, nullptr, 0.0, error, iters, &est);
+  for (int k = 0; k < (int)est.size() && k < cap; k++) perIter[k] = est[k];
+  return r;
 }
 
 double oref_energy(void* p, const int64_t* det) {

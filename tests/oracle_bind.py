@@ -69,6 +69,8 @@ def load():
         "oref_run_pt": ([P, pD], I32),
         "oref_add_pt_restriction": ([P, I32, I32, ctypes.POINTER(ctypes.c_int), I32], None),
         "oref_pt_deterministic": ([P, pI64, I64, pD, D, D, pD, pI64], D),
+        "oref_pt_semistochastic": ([P, pI64, I64, pD, D, D, D, I32, I32, D, ctypes.c_uint, pD, I32,
+                                    ctypes.POINTER(ctypes.c_int), pD], D),
         "oref_mlp_forward": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float), I64, I32,
                               pD], None),
         "oref_mlp_loss_grad": ([ctypes.POINTER(ctypes.c_float), I32, I32, ctypes.POINTER(ctypes.c_float),
@@ -171,6 +173,18 @@ class OracleSystem:
         e = self.L.oref_pt_deterministic(self.h, _p(d, ctypes.c_int64), len(d), _p(c, ctypes.c_double), e0, eps2,
                                          ctypes.byref(psi), ctypes.byref(nx))
         return e, psi.value, nx.value
+
+    def pt_semistochastic(self, dets, ci, e0, eps2, eps2_large, sample_n, n_iter, target_error, seed):
+        """Semistochastic EN-PT2 of one root: (returned value, per-iteration
+        estimates, iterations, standard error)."""
+        d = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.words)
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        per = np.zeros(max(n_iter, 1))
+        it, err = ctypes.c_int(), ctypes.c_double()
+        r = self.L.oref_pt_semistochastic(self.h, _p(d, ctypes.c_int64), len(d), _p(c, ctypes.c_double), e0, eps2,
+                                          eps2_large, sample_n, n_iter, target_error, seed, _p(per, ctypes.c_double),
+                                          len(per), ctypes.byref(it), ctypes.byref(err))
+        return r, per[:it.value].copy(), it.value, err.value
 
     def davidson(self, x0, diag, max_copies, tol):
         x = np.array(np.atleast_2d(np.asarray(x0, dtype=np.float64).T).T, order="F")

@@ -583,7 +583,7 @@ def test_run_output_files_match_oracle(tmp_path, monkeypatch):
         d.mkdir()
         inp = d / "dice.dat"
         inp.write_text(g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nwriteBestDeterminants 101\n"
-                       "printAllDeterminants\n" % (fcidump, d))
+                       "printAllDeterminants\nnPTiter 0\n" % (fcidump, d))
         monkeypatch.chdir(d)
         if who == "gpu":
             r = shci.do_variational(str(inp), str(d / "log"))

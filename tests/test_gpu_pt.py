@@ -117,3 +117,44 @@ def test_run_pt_two_roots_matches_oracle(tmp_path):
     assert len(r.pt_energies()) == 2
     np.testing.assert_allclose(r.pt_energies(), o.pt_energies(), rtol=0, atol=1e-10)
     assert np.all(r.pt_energies() < r.energies())
+
+
+# ---- semistochastic EN-PT2 (SHCI/SHCIbasics.cpp:55-452) ----
+
+@pytest.mark.parametrize("name,eps2,large", [("c0", 1e-4, 1e-3), ("w2", 1e-4, 1000.0), ("hub", 1e-4, 1e-3)])
+def test_pt_semistochastic_matches_oracle(name, eps2, large, systems):
+    """Same seed, same draws (the C library's rand() on the host in both):
+    every iteration's estimate agrees with the oracle's to 1e-12 relative
+    (the per-determinant sums are bit-identical; only the final sum's order
+    differs), and so do the iteration count and the returned value."""
+    from dice_amd import shci
+    dets, ci = _case(name, systems, 200, 31)
+    path = systems[name][0]
+    g, o = shci.System.from_fcidump(path), OracleSystem(path)
+    e0 = float(np.min(o.energy(dets))) - 0.5
+    g.set_dets(dets)
+    # (a negative target error runs all 12 iterations: when every term passes
+    # both thresholds the estimates are all equal and the error is 0 only up
+    # to rounding)
+    rg, pg, ig, sg = g.pt_semistochastic(ci, e0, eps2, large, 50, 12, -1.0, 11)
+    ro, po, io_, so = o.pt_semistochastic(dets, ci, e0, eps2, large, 50, 12, -1.0, 11)
+    assert ig == io_ == 12
+    np.testing.assert_allclose(pg, po, rtol=1e-12, atol=0)
+    assert abs(rg - ro) <= 1e-12 * abs(ro)
+
+
+def test_run_semistochastic_pt_matches_oracle(tmp_path):
+    """A whole run with the semistochastic PT step on the 8-orbital synthetic
+    molecule: E_var and E_var + E_PT equal the oracle's run."""
+    from dice_amd import shci, synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 5, 8, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 2e-2)], 3,
+                      extra=["nPTiter 200", "epsilon2 1e-4", "epsilon2Large 1e-3", "targetError 6e-4",
+                             "prefix %s" % tmp_path])
+    r = shci.do_variational(inp, str(tmp_path / "g.log"))
+    o = OracleRun(inp, str(tmp_path / "o.log"))
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=1e-10)
+    np.testing.assert_allclose(r.pt_energies(), o.pt_energies(), rtol=0, atol=1e-9)
+    assert "Semistochastic PT calculation converged" in open(tmp_path / "g.log").read()

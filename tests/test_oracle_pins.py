@@ -49,8 +49,13 @@ def test_h8_golden_log_and_det_set(tmp_path, golden_dir):
         assert mine["davidson"] == ref["davidson"]
     e = run.energies()
     assert abs(e[0] - g["h8_631g"]["final_energy"]) < 6e-11
+    # the input asks for the semistochastic PT (sampleN, no nPTiter 0): shci.e
+    # holds E_var + E_PT.  (dice.out's PT section, printed by an older Dice,
+    # shows E_PT = 0; the PT code of the reference tree applies no ncore /
+    # nact limit, so its PT reaches the eight orbitals outside nact 8.)
     with open(tmp_path / "shci.e", "rb") as f:
-        assert struct.unpack("<d", f.read(8))[0] == e[0]
+        assert struct.unpack("<d", f.read(8))[0] == run.pt_energies()[0]
+    assert run.pt_energies()[0] < e[0]
     # final determinant set and |c| of examples/h8_631g/dets
     coeffs, occs = helpers.read_golden_dets(os.path.join(golden_dir, "h8_631g", "dets"), 16)
     gold = {tuple(o): abs(c) for c, o in zip(coeffs, occs)}
@@ -268,7 +273,8 @@ def test_threaded_oracle_matches_serial(tmp_path, golden_dir, monkeypatch):
     partial vectors, as the reference's MPI_Reduce sums per-rank ones)."""
     g = _golden(golden_dir)
     fcidump = os.path.join(golden_dir, "h8_631g", "FCIDUMP")
-    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\n" % (fcidump, tmp_path)
+    # (the variational step only: nPTiter 0 turns the input's stochastic PT off)
+    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nnPTiter 0\n" % (fcidump, tmp_path)
     _write(tmp_path / "dice.dat", inp)
     monkeypatch.setenv("ORACLE_THREADS", "1")
     serial = OracleRun(str(tmp_path / "dice.dat"), str(tmp_path / "log1"))
@@ -473,8 +479,8 @@ def test_run_writes_dets_bin_and_all_determinants(tmp_path, golden_dir, monkeypa
     determinant."""
     g = _golden(golden_dir)
     fcidump = os.path.join(golden_dir, "h8_631g", "FCIDUMP")
-    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nwriteBestDeterminants 101\nprintAllDeterminants\n" % (
-        fcidump, tmp_path)
+    inp = g["h8_631g"]["input"] + "\norbitals %s\nprefix %s\nwriteBestDeterminants 101\nprintAllDeterminants\n" \
+        "nPTiter 0\n" % (fcidump, tmp_path)
     _write(tmp_path / "dice.dat", inp)
     monkeypatch.chdir(tmp_path)
     run = OracleRun(str(tmp_path / "dice.dat"), str(tmp_path / "log"))
@@ -558,3 +564,57 @@ def test_restart_converged_moves_to_pt(tmp_path, golden_dir):
     assert abs(rb.pt_energies()[0] - (ra.energies()[0] + pt)) < 1e-12
     with open(tmp_path / "shci.e", "rb") as f:
         assert struct.unpack("<d", f.read(8))[0] == rb.pt_energies()[0]
+
+
+# ---- semistochastic EN-PT2 (SHCI/SHCIbasics.cpp:55-452) ----
+
+def test_pt_semistochastic_identity_at_equal_thresholds(tmp_path):
+    """With epsilon2Large = epsilon2 every sampled term is also a
+    deterministic one, so each iteration's estimate is the deterministic PT
+    energy exactly and the standard error is 0 at the fifth iteration (the
+    reference's first error estimate)."""
+    o, dets, ci, e0 = _pt_system(tmp_path, 2e-2)
+    det = o.pt_deterministic(dets, ci, e0, 1e-4)[0]
+    r, per, it, err = o.pt_semistochastic(dets, ci, e0, 1e-4, 1e-4, 200, 50, 1e-12, 7)
+    assert it == 5 and err == 0.0
+    assert np.all(per == det) and r == det
+
+
+def test_pt_semistochastic_is_unbiased(tmp_path):
+    """The sampled estimator is unbiased: over 400 iterations of 200 draws its
+    mean is within four standard errors of the deterministic PT energy at
+    epsilon2, with (epsilon2Large 1e-3) and without (1000) a deterministic
+    part; the deterministic part shrinks the spread."""
+    o, dets, ci, e0 = _pt_system(tmp_path, 2e-2)
+    det = o.pt_deterministic(dets, ci, e0, 1e-4)[0]
+    spread = []
+    for large in (1000.0, 1e-3):
+        _, per, it, _ = o.pt_semistochastic(dets, ci, e0, 1e-4, large, 200, 400, 1e-12, 7)
+        assert it == 400
+        se = per.std(ddof=1) / np.sqrt(len(per))
+        assert abs(per.mean() - det) < 4 * se, (large, per.mean(), det, se)
+        spread.append(se)
+    assert spread[1] < spread[0]
+
+
+def test_run_semistochastic_pt(tmp_path):
+    """sampleN with nPTiter != 0 and no `deterministic`: the run's PT step is
+    the semistochastic one (SHCI/SHCI.cpp:722-741); its log carries the
+    reference's lines and shci.e = E_var + the mean correction."""
+    from dice_amd import synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 5, 8, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 2e-2)], 3,
+                      extra=["nPTiter 200", "epsilon2 1e-4", "epsilon2Large 1e-3", "targetError 6e-4",
+                             "prefix %s" % tmp_path])
+    run = OracleRun(inp, str(tmp_path / "log"))
+    log = open(tmp_path / "log").read()
+    for line in ("Performing (semi)stochastic PT for state:   0", "1/ Deterministic calculation with epsilon2=0.001",
+                 "2/ Stochastic calculation with epsilon2=0.0001", "Semistochastic PT calculation converged"):
+        assert line in log
+    ept = run.pt_energies()[0]
+    assert "PTEnergy: %18.10f +/- " % ept in log
+    with open(tmp_path / "shci.e", "rb") as f:
+        assert struct.unpack("<d", f.read(8))[0] == ept
+    assert -0.07 < ept - run.energies()[0] < -0.04
