@@ -21,8 +21,9 @@ this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
 
 pt_deterministic: the deterministic EN-PT2 step (after the variational one)
-on the final space at eps2 = 1e-4, and GPU vs CPU oracle on the first
-iteration's space at eps2 = 1e-3.
+on the final space at eps2 = 1e-4, the semistochastic one (deterministic
+part at 1e-4, 50 sampled iterations down to 1e-6), and GPU vs CPU oracle on
+the first iteration's space at eps2 = 1e-3.
 
 cpu_baseline: the CPU oracle (a restatement of the reference algorithm whose
 rank-parallel loops run on OpenMP threads, up to 16) on a bounded sample of
@@ -46,6 +47,7 @@ EPS1 = 5e-3
 NITER = 6
 PT_EPS2 = 1e-4
 PT_SAMPLE_EPS2 = 1e-3  # keeps the CPU oracle's PT sample to tens of seconds
+PT_STOCH_EPS2 = 1e-6
 
 
 def traffic_bytes(precision):
@@ -156,9 +158,18 @@ def pt_measure(shci, full, sample, log, args):
     t0 = time.time()
     ept, psi1, next_ = S.pt_deterministic(ci, e - core, PT_EPS2)
     gpu_s = time.time() - t0
+    # semistochastic: the deterministic part at PT_EPS2, 50 sampled iterations
+    # of 200 draws down to PT_STOCH_EPS2
+    t0 = time.time()
+    sval, per, its, serr = S.pt_semistochastic(ci, e - core, PT_STOCH_EPS2, PT_EPS2, 200, 50, -1.0, 1)
+    stoch_s = time.time() - t0
     S.close()
     res = {"eps2": PT_EPS2, "ndets": len(dets), "seconds": round(gpu_s, 4), "E_PT": ept, "E_var_plus_PT": e + ept,
-           "external_dets": next_, "external_per_s": round(next_ / gpu_s, 1)}
+           "external_dets": next_, "external_per_s": round(next_ / gpu_s, 1),
+           "semistochastic": {"eps2": PT_STOCH_EPS2, "eps2_large": PT_EPS2, "sample_n": 200, "iterations": its,
+                              "seconds": round(stoch_s, 3),
+                              "seconds_per_iteration": round((stoch_s - gpu_s) / max(its, 1), 5),
+                              "E_PT_mean": float(per.mean()) if len(per) else None, "std_error": serr}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import OracleRun, OracleSystem
