@@ -166,6 +166,135 @@ bool readCheckpoint(const std::string& path, Checkpoint& c, std::string& err) {
   return ok;
 }
 
+// RDM text files: spatial1RDM / spin1RDM as save1RDM writes them
+// (SHCI/SHCIrdm.cpp:355-400), spatialRDM / spinRDM as saveRDM
+// (SHCI/SHCIrdm.cpp:477-526); the Boost archives beside them are not written.
+void save1Rdm(const std::string& prefix, int r1, int r2, int norb, const std::vector<double>& g1,
+              const std::vector<double>* s1) {
+  char name[64];
+  snprintf(name, sizeof name, "/spatial1RDM.%d.%d.txt", r1, r2);
+  if (FILE* f = fopen((prefix + name).c_str(), "w")) {
+    fprintf(f, "%d\n", norb);
+    for (int p = 0; p < norb; p++)
+      for (int q = 0; q < norb; q++) {
+        const double v = g1[(size_t)p * norb + q];
+        if (std::fabs(v) > 1.e-15) fprintf(f, "%3d   %3d   %16.12g\n", p, q, v);
+      }
+    fclose(f);
+  }
+  if (!s1) return;
+  const int nso = 2 * norb;
+  snprintf(name, sizeof name, "/spin1RDM.%d.%d.txt", r1, r2);
+  if (FILE* f = fopen((prefix + name).c_str(), "w")) {
+    fprintf(f, "%d\n", nso);
+    for (int p = 0; p < nso; p++)
+      for (int q = 0; q < nso; q++) {
+        const double v = (*s1)[(size_t)p * nso + q];
+        if (std::fabs(v) > 1.e-6) fprintf(f, "%3d   %3d   %10.8g\n", p, q, v);
+      }
+    fclose(f);
+  }
+}
+
+void save2Rdm(const std::string& prefix, int r1, int r2, int norb, const std::vector<double>& g2,
+              const std::vector<double>* s2) {
+  char name[64];
+  snprintf(name, sizeof name, "/spatialRDM.%d.%d.txt", r1, r2);
+  const size_t n = norb;
+  if (FILE* f = fopen((prefix + name).c_str(), "w")) {
+    fprintf(f, "%d\n", norb);
+    for (size_t p = 0; p < n; p++)
+      for (size_t q = 0; q < n; q++)
+        for (size_t r = 0; r < n; r++)
+          for (size_t t = 0; t < n; t++) {
+            const double v = g2[((p * n + q) * n + r) * n + t];
+            if (std::fabs(v) > 1.e-15)
+              fprintf(f, "%3d   %3d   %3d   %3d   %16.12g\n", (int)p, (int)q, (int)r, (int)t, v);
+          }
+    fclose(f);
+  }
+  if (!s2) return;
+  const size_t nso = 2 * n, npair = nso * (nso + 1) / 2;
+  snprintf(name, sizeof name, "/spinRDM.%d.%d.txt", r1, r2);
+  if (FILE* f = fopen((prefix + name).c_str(), "w")) {
+    for (size_t a = 1; a < nso; a++)
+      for (size_t b = 0; b < a; b++)
+        for (size_t c = 1; c < nso; c++)
+          for (size_t d = 0; d < c; d++) {
+            const double v = (*s2)[(a * (a + 1) / 2 + b) * npair + c * (c + 1) / 2 + d];
+            if (std::fabs(v) > 1.e-6) fprintf(f, "%3d %3d %3d %3d %16.14g\n", (int)a, (int)b, (int)c, (int)d, v);
+          }
+    fclose(f);
+  }
+}
+
+// ComputeEnergyFromSpatialRDM (SHCI/SHCIrdm.cpp:1188-1245): the one-body part
+// from the 1-RDM that the 2-RDM implies, gamma(p,q) = sum_r G(p r, q r) /
+// (nelec - 1)
+double energyFromSpatialRdm(const HostSystem& hs, const std::vector<double>& g2, FILE* log) {
+  const int n = hs.norb;
+  double onebody = 0.0, twobody = 0.0;
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++) {
+      double g = 0.0;
+      for (int r = 0; r < n; r++) g += g2[(((size_t)p * n + r) * n + q) * n + r] / (1. * hs.nelec - 1.);
+      onebody += hs.h1[(size_t)p * n + q] * g;
+    }
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++)
+      for (int r = 0; r < n; r++)
+        for (int t = 0; t < n; t++)
+          twobody += 0.5 * g2[(((size_t)p * n + q) * n + r) * n + t] * hs.i2(2 * p, 2 * r, 2 * q, 2 * t);
+  const double e = hs.coreE + onebody + twobody;
+  if (log) {
+    fprintf(log, "E(one-body) from 2RDM: %18.10f\n", onebody);
+    fprintf(log, "E(two-body) from 2RDM: %18.10f\n", twobody);
+    fprintf(log, "E from 2RDM:           %18.10f\n", e);
+  }
+  return e;
+}
+
+// the RDM step at the end of the variational iterations
+// (SHCI/SHCIbasics.cpp:1236-1340)
+void rdmStep(Engine& E, const HostSchedule& schd, FILE* log) {
+  const std::string& prefix = schd.prefix[0];
+  const int norb = E.hs.norb, nroots = E.nroots;
+  const double* ci = E.ci.as<double>();
+  std::vector<double> g1, g2, s1, s2;
+  if (schd.doOneRDM) {
+    if (log) fprintf(log, "\nCalculating 1-RDM\n");
+    for (int i = 0; i < nroots; i++) {
+      const double* c = ci + (int64_t)i * E.n;
+      E.rdm(c, c, g1, g2, schd.doSpinOneRDM ? &s1 : nullptr, nullptr);
+      save1Rdm(prefix, i, i, norb, g1, schd.doSpinOneRDM ? &s1 : nullptr);
+    }
+  }
+  if (!(schd.doRDM || schd.doTransitionRDM)) return;
+  if (log) fprintf(log, "\nCalculating 2-RDM\n");
+  std::vector<double>* sp2 = schd.doSpinRDM ? &s2 : nullptr;
+  for (int i = 0; i < nroots; i++) {
+    const double* ciI = ci + (int64_t)i * E.n;
+    if (E.direct) {  // the reference's direct-mode branch: the 2-RDM only
+      E.rdm(ciI, ciI, g1, g2, nullptr, sp2);
+      energyFromSpatialRdm(E.hs, g2, log);
+      save2Rdm(prefix, i, i, norb, g2, sp2);
+      continue;
+    }
+    if (schd.doTransitionRDM)
+      for (int j = 0; j < i; j++) {
+        E.rdm(ciI, ci + (int64_t)j * E.n, g1, g2, schd.doSpinOneRDM ? &s1 : nullptr, sp2);
+        save1Rdm(prefix, i, j, norb, g1, schd.doSpinOneRDM ? &s1 : nullptr);
+        energyFromSpatialRdm(E.hs, g2, log);
+        save2Rdm(prefix, i, j, norb, g2, sp2);
+      }
+    E.rdm(ciI, ciI, g1, g2, schd.doSpinOneRDM ? &s1 : nullptr, sp2);
+    energyFromSpatialRdm(E.hs, g2, log);
+    save2Rdm(prefix, i, i, norb, g2, sp2);
+    save1Rdm(prefix, i, i, norb, g1, schd.doSpinOneRDM ? &s1 : nullptr);
+  }
+  if (log) fflush(log);
+}
+
 shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, const shci_run_options& opt) {
   Engine& E = h->eng;
   const int precision = opt.precision;
@@ -189,10 +318,18 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", err.c_str());
     return fail(h, SHCI_BAD_INPUT, err);
   }
-  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchLowest || schd.doLCC) {
+  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchLowest || schd.doLCC ||
+      schd.doResponse || schd.doHigherRDM) {
     std::string m =
         "keyword outside the variational path implemented here (Treversal, diskdavidson, dosoc, "
-        "searchForLowestEnergyDet, dolcc)";
+        "searchForLowestEnergyDet, dolcc, doresponse, DoThreeRDM, DoFourRDM)";
+    if (log) fprintf(log, "%s\n", m.c_str());
+    return fail(h, SHCI_UNSUPPORTED, m);
+  }
+  if (schd.doRDM && !schd.rdmUnrelaxed && !schd.stochastic && schd.nblocks == 1) {
+    // SHCI/SHCI.cpp:759-806 would add the PT correction to the 2-RDM
+    std::string m = "DoRDM with deterministic PT asks for the relaxed PT 2-RDM, which is not implemented here "
+                    "(relaxedRDM keeps the variational RDMs)";
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
   }
@@ -406,6 +543,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");
       if (schd.io) saveCheckpoint(iter);
+      rdmStep(E, schd, log);
       break;
     }
     if (schd.io) saveCheckpoint(iter);
@@ -524,6 +662,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       fprintf(log, "\n\n**************************************************************\n");
       fprintf(log, "PERTURBATION THEORY STEP  \n");
       fprintf(log, "**************************************************************\n");
+      if (schd.doRDM) fprintf(log, "(We cannot perform PT RDM with stochastic PT. Disabling RDM.)\n\n");
     }
     std::vector<double> ept(E0.size());
     for (size_t root = 0; root < E0.size(); root++)
@@ -822,6 +961,28 @@ shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, d
     dc.reserve((size_t)std::max<int64_t>(n, 1) * 8);
     if (n) HIP_CHECK(hipMemcpyAsync(dc.p, ci, n * 8, hipMemcpyHostToDevice, E.stream));
     *e_pt = E.ptDeterministic(dc.as<double>(), e0, eps2, psi1_norm, n_external);
+    return SHCI_OK;
+  });
+}
+
+shci_status shci_rdm(shci_handle* h, const double* bra, const double* ket, double* one_rdm, double* two_rdm,
+                     double* spin_one_rdm, double* spin_two_rdm) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (!bra || !ket || !one_rdm || !two_rdm) throw std::invalid_argument("null argument");
+    const int64_t n = E.n;
+    if (n < 1) throw std::invalid_argument("no determinants held");
+    DevMem db, dk;
+    db.reserve((size_t)n * 8);
+    dk.reserve((size_t)n * 8);
+    HIP_CHECK(hipMemcpyAsync(db.p, bra, n * 8, hipMemcpyHostToDevice, E.stream));
+    HIP_CHECK(hipMemcpyAsync(dk.p, ket, n * 8, hipMemcpyHostToDevice, E.stream));
+    std::vector<double> g1, g2, s1, s2;
+    E.rdm(db.as<double>(), dk.as<double>(), g1, g2, spin_one_rdm ? &s1 : nullptr, spin_two_rdm ? &s2 : nullptr);
+    std::copy(g1.begin(), g1.end(), one_rdm);
+    std::copy(g2.begin(), g2.end(), two_rdm);
+    if (spin_one_rdm) std::copy(s1.begin(), s1.end(), spin_one_rdm);
+    if (spin_two_rdm) std::copy(s2.begin(), s2.end(), spin_two_rdm);
     return SHCI_OK;
   });
 }

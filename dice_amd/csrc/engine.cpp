@@ -302,6 +302,46 @@ double Engine::ptSemistochastic(const double* dCi, double e0, double eps2, doubl
   return avg;
 }
 
+void Engine::rdm(const double* dBra, const double* dKet, std::vector<double>& g1, std::vector<double>& g2,
+                 std::vector<double>* s1, std::vector<double>* s2) {
+  const int nso = hs.nso, norb = nso / 2;
+  if (direct) {  // the connections are not stored in direct mode: build them
+    direct = false;
+    hamRows = -1;
+    buildHamiltonian();
+    direct = true;
+  } else {
+    buildHamiltonian();
+  }
+  const size_t n1 = (size_t)norb * norb, n2 = n1 * n1, ns1 = (size_t)nso * nso;
+  const size_t npair = (size_t)nso * (nso + 1) / 2, ns2 = npair * npair;
+  auto zeroed = [&](DevMem& m, size_t count) {
+    m.reserve(std::max<size_t>(count, 1) * 8, stream);
+    HIP_CHECK(hipMemsetAsync(m.p, 0, count * 8, stream));
+    return m.as<double>();
+  };
+  double* g1d = zeroed(rdmG1, n1);
+  double* g2d = zeroed(rdmG2, n2);
+  double* s1d = s1 ? zeroed(rdmS1, ns1) : nullptr;
+  double* s2d = s2 ? zeroed(rdmS2, ns2) : nullptr;
+  rdmOcc.reserve((size_t)64 * hs.W * 64 * hs.W * 8, stream);
+  launchRdm(norb, hs.W, dets.as<uint64_t>(), rowPtr.as<int64_t>(), cols.as<int32_t>(), dBra, dKet, n, g1d, g2d, s1d,
+            s2d, rdmOcc.as<double>(), stream);
+  g1.resize(n1);
+  g2.resize(n2);
+  HIP_CHECK(hipMemcpyAsync(g1.data(), g1d, n1 * 8, hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipMemcpyAsync(g2.data(), g2d, n2 * 8, hipMemcpyDeviceToHost, stream));
+  if (s1) {
+    s1->resize(ns1);
+    HIP_CHECK(hipMemcpyAsync(s1->data(), s1d, ns1 * 8, hipMemcpyDeviceToHost, stream));
+  }
+  if (s2) {
+    s2->resize(ns2);
+    HIP_CHECK(hipMemcpyAsync(s2->data(), s2d, ns2 * 8, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
 void Engine::ensureTable(int64_t need) {
   uint64_t want = 1024;
   while (want < (uint64_t)need * 2) want <<= 1;

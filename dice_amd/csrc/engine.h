@@ -150,6 +150,16 @@ struct Engine {
                           double targetError, double coreE, int root, FILE* log, double startofCalc, double* error,
                           int* iters, std::vector<double>* perIter);
 
+  // RDMs (SHCI/SHCIrdm.cpp:863-1100) of <bra| ... |ket> (device vectors of
+  // n values) over the current determinants and their connections (built
+  // when direct mode does not store them), copied to the host: the spatial
+  // g1 (norb^2) and g2 (norb^4, g2[((P*n+Q)*n+R)*n+S] = the reference's
+  // s2RDM(P*n+Q, R*n+S)) and, when asked, the spin-orbital s1 (nso^2) and s2
+  // (pair-indexed, pair(p,q) = p(p+1)/2 + q)
+  DevMem rdmG1, rdmG2, rdmS1, rdmS2, rdmOcc;
+  void rdm(const double* dBra, const double* dKet, std::vector<double>& g1, std::vector<double>& g2,
+           std::vector<double>* s1 = nullptr, std::vector<double>* s2 = nullptr);
+
   // last run
   std::vector<IterRecord> iters;
   std::vector<double> energies;

@@ -293,8 +293,14 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
     else if (same(key, "printbestdeterminants")) schd.printBestDeterminants = iarg(1);
     else if (same(key, "printalldeterminants")) schd.printAllDeterminants = true;
     else if (same(key, "writebestdeterminants")) schd.writeBestDeterminants = iarg(1);
-    else if (same(key, "dordm") || same(key, "dospinrdm")) schd.doRDM = true;
+    else if (same(key, "dordm")) schd.doRDM = true;
+    else if (same(key, "dospinrdm")) schd.doRDM = schd.doSpinRDM = true;
     else if (same(key, "DoOneRDM")) schd.doOneRDM = true;
+    else if (same(key, "DoSpinOneRDM")) schd.doSpinOneRDM = true;
+    else if (same(key, "TransitionRDM")) schd.doTransitionRDM = true;
+    else if (same(key, "relaxedRDM")) schd.rdmUnrelaxed = true;  // (sic: SHCI/input.cpp:187-188)
+    else if (same(key, "doresponse")) schd.doResponse = true;
+    else if (same(key, "DoThreeRDM") || same(key, "DoFourRDM")) schd.doHigherRDM = true;
     else if (same(key, "Treversal")) {
       schd.Trev = iarg(1);
       if (!(schd.Trev == 0 || schd.Trev == 1 || schd.Trev == -1)) {
@@ -344,15 +350,13 @@ bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err
         st = tokens(line, " \t");
       }
     } else if (same(key, "maxiter")) maxiter = iarg(1);
-    else if (same(key, "relaxedRDM") || same(key, "num_thrds") || same(key, "extrapolate") ||
-             same(key, "algorithm") || same(key, "doresponse") || same(key, "maxseniority") ||
+    else if (same(key, "num_thrds") || same(key, "extrapolate") ||
+             same(key, "algorithm") || same(key, "maxseniority") ||
              same(key, "maxexcitation") || same(key, "SenioAndExc") || same(key, "dogtensor") ||
              same(key, "dosocqdpt") || same(key, "socmultiplier") ||
-             same(key, "onlyperturbative") || same(key, "DoSpinOneRDM") ||
-             same(key, "TransitionRDM") || same(key, "quasiq") ||
+             same(key, "onlyperturbative") || same(key, "quasiq") ||
              same(key, "cdfciTol") || same(key, "excitation") || same(key, "nvirt") || same(key, "eps") ||
-             same(key, "pointGroup") || same(key, "spin") || same(key, "irrep") || same(key, "DoThreeRDM") ||
-             same(key, "DoFourRDM") || same(key, "cdfciIter") || same(key, "reportInterval") ||
+             same(key, "pointGroup") || same(key, "spin") || same(key, "irrep") || same(key, "cdfciIter") || same(key, "reportInterval") ||
              same(key, "z_threshold") || same(key, "cdfciSample")) {
       // accepted; affects only steps after the variational one
     } else {

@@ -53,7 +53,8 @@ struct HostSchedule {
   int nblocks = 1;
   int Trev = 0, davidsonType = 0;
   bool restart = false, fullrestart = false, doSOC = false, searchLowest = false, doLCC = false;
-  bool doRDM = false, doOneRDM = false;
+  bool doRDM = false, doOneRDM = false, doSpinRDM = false, doSpinOneRDM = false, doTransitionRDM = false;
+  bool rdmUnrelaxed = false, doResponse = false, doHigherRDM = false;
   int cdfciOn = 1000;
   // extensions of this implementation (not reference keywords; set by the
   // executable's command line): Hamiltonian value precision and block size

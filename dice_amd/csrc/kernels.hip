@@ -572,6 +572,209 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
 }
 
 // ---------------------------------------------------------------------------
+// RDMs of <bra| ... |ket> (the quantities of SHCI/SHCIrdm.cpp:863-1100):
+//   g1[P*n + R]                 = sum_s   <a+_{P s} a_{R s}>
+//   g2[((P*n + Q)*n + R)*n + S] = sum_s,t <a+_{P s} a+_{Q t} a_{S t} a_{R s}>
+// and optionally the spin-orbital ones
+//   s1[p*nso + r]               = <a+_p a_r>
+//   s2[pair(p,q)*npair + pair(r,s)] = <a+_p a+_q a_s a_r>, p > q, r > s,
+//   pair(p,q) = p(p+1)/2 + q, npair = nso(nso+1)/2
+// over the stored connections (each unordered pair once, from the J < I
+// entries of the full symmetric CSR: an element with bra[I] ket[J] and its
+// transpose with bra[J] ket[I]) and the diagonal; FP64 atomics.
+
+// parity of the occupied spin-orbitals of d below k
+template <int W>
+__device__ __forceinline__ int belowParity(const uint64_t* d, int k) {
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < W; w++) {
+    const int lo = 64 * w;
+    if (k >= lo + 64)
+      c += __popcll(d[w]);
+    else if (k > lo)
+      c += __popcll(d[w] & ((1ull << (k - lo)) - 1ull));
+  }
+  return c & 1;
+}
+
+struct RdmOut {
+  double *g1, *g2, *s1, *s2;  // s1 / s2 may be null
+  int n, nso;
+  int64_t npair;
+  __device__ __forceinline__ void add2(int P, int Q, int R, int S, double v) const {
+    atomicAdd(g2 + (((int64_t)P * n + Q) * n + R) * n + S, v);
+  }
+  __device__ __forceinline__ void spin2(int p, int q, int r, int s, double v) const {
+    // p > q, r > s
+    atomicAdd(s2 + ((int64_t)p * (p + 1) / 2 + q) * npair + ((int64_t)r * (r + 1) / 2 + s), v);
+  }
+};
+
+// diagonal part: M[p*nso + q] = sum_I bra[I] ket[I] occ_p(I) occ_q(I), a
+// rank-n product of the occupation matrix; thread t of a workgroup owns row
+// p = t / TPR and QPT consecutive columns of it in registers over the
+// workgroup's slice of determinants, then adds them to M
+template <int W>
+__global__ void __launch_bounds__(kBlock) kRdmOcc(const uint64_t* dets, const double* bra, const double* ket, int64_t n,
+                                                  double* M) {
+  constexpr int QPT = W == 2 ? 64 : 16, TPR = 64 * W / QPT, NSO = 64 * W;
+  const int p = threadIdx.x / TPR, h = threadIdx.x % TPR;
+  const int qw = (h * QPT) >> 6, qs = (h * QPT) & 63;
+  double acc[QPT];
+#pragma unroll
+  for (int j = 0; j < QPT; j++) acc[j] = 0.0;
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b = (int64_t)blockIdx.x * per, e = b + per < n ? b + per : n;
+  for (int64_t I = b; I < e; I++) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[I * W + w];
+    if (!((d[p >> 6] >> (p & 63)) & 1ull)) continue;
+    const double c2 = bra[I] * ket[I];
+    const uint64_t row = d[qw] >> qs;
+#pragma unroll
+    for (int j = 0; j < QPT; j++)
+      if ((row >> j) & 1ull) acc[j] += c2;
+  }
+#pragma unroll
+  for (int j = 0; j < QPT; j++)
+    if (acc[j] != 0.0) atomicAdd(M + (int64_t)p * NSO + h * QPT + j, acc[j]);
+}
+
+// scatters M (NSO x NSO) into the RDMs: <a+_p a_p> = n_p,
+// <a+_p a+_q a_q a_p> = n_p n_q and <a+_p a+_q a_p a_q> = -n_p n_q (p != q)
+__global__ void kRdmOccScatter(RdmOut o, const double* M, int NSO) {
+  const int nso = o.nso;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nso * nso; t += gridDim.x * blockDim.x) {
+    const int p = t / nso, q = t % nso;
+    const double v = M[(int64_t)p * NSO + q];
+    if (v == 0.0) continue;
+    if (p == q) {
+      atomicAdd(o.g1 + (int64_t)(p >> 1) * o.n + (p >> 1), v);
+      if (o.s1) atomicAdd(o.s1 + (int64_t)p * nso + p, v);
+      continue;
+    }
+    o.add2(p >> 1, q >> 1, p >> 1, q >> 1, v);
+    if (((p ^ q) & 1) == 0) o.add2(p >> 1, q >> 1, q >> 1, p >> 1, -v);
+    if (o.s2 && p > q) o.spin2(p, q, p, q, v);
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) kRdm(RdmOut o, const uint64_t* dets, const int64_t* rp, const int32_t* cols,
+                                               const double* bra, const double* ket, int64_t ndets) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int nso = o.nso, norb = o.n;
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < ndets; I += (int64_t)gridDim.x * kWaves) {
+    uint64_t dI[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
+    const double bI = bra[I], kI = ket[I];
+    // connections J < I
+    const int64_t b = rp[I], e = rp[I + 1];
+    for (int64_t p = b + lane; p < e; p += 64) {
+      const int64_t J = cols[p];
+      if (J >= I) continue;
+      uint64_t dJ[W];
+#pragma unroll
+      for (int w = 0; w < W; w++) dJ[w] = dets[J * W + w];
+      int cre[2] = {0, 0}, des[2] = {0, 0}, nc = 0, nd = 0;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        uint64_t x = dI[w] & ~dJ[w], y = dJ[w] & ~dI[w];
+        while (x) {
+          const int q = 64 * w + __ffsll((unsigned long long)x) - 1;
+          x &= x - 1;
+          if (nc < 2) cre[nc] = q;
+          nc++;
+        }
+        while (y) {
+          const int q = 64 * w + __ffsll((unsigned long long)y) - 1;
+          y &= y - 1;
+          if (nd < 2) des[nd] = q;
+          nd++;
+        }
+      }
+      if (nc == 1 && nd == 1) {
+        // I = a+_a a_i J: sign = parity of the occupied of J strictly between
+        const int a = cre[0], i = des[0];
+        const int lo = a < i ? a : i, hi = a < i ? i : a;
+        const double sg = (countBetween<W>(dJ, lo, hi) & 1) ? -1.0 : 1.0;
+        const double vE = sg * bI * ket[J], vT = sg * bra[J] * kI;
+        const int A = a >> 1, Ii = i >> 1;
+        atomicAdd(o.g1 + (int64_t)A * norb + Ii, vE);
+        atomicAdd(o.g1 + (int64_t)Ii * norb + A, vT);
+        if (o.s1) {
+          atomicAdd(o.s1 + (int64_t)a * nso + i, vE);
+          atomicAdd(o.s1 + (int64_t)i * nso + a, vT);
+        }
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          uint64_t common = dJ[w] & dI[w];
+          while (common) {
+            const int k = 64 * w + __ffsll((unsigned long long)common) - 1;
+            common &= common - 1;
+            const int K = k >> 1;
+            o.add2(A, K, Ii, K, vE);
+            o.add2(K, A, K, Ii, vE);
+            o.add2(Ii, K, A, K, vT);
+            o.add2(K, Ii, K, A, vT);
+            if (((k ^ a) & 1) == 0) {
+              o.add2(A, K, K, Ii, -vE);
+              o.add2(K, A, Ii, K, -vE);
+              o.add2(K, Ii, A, K, -vT);
+              o.add2(Ii, K, K, A, -vT);
+            }
+            if (o.s2) {
+              // <a+_max(k,a) a+_min(k,a) a_min(k,i) a_max(k,i)>: one swap
+              // when k lies between a and i
+              const double f = (k > lo && k < hi) ? -1.0 : 1.0;
+              const int ah = k > a ? k : a, al = k > a ? a : k, ih = k > i ? k : i, il = k > i ? i : k;
+              o.spin2(ah, al, ih, il, f * vE);
+              o.spin2(ih, il, ah, al, f * vT);
+            }
+          }
+        }
+      } else if (nc == 2 && nd == 2) {
+        // I = a+_a a+_b a_j a_i J (a < b, i < j): sign from applying the
+        // operators one by one
+        const int a = cre[0], bb = cre[1], i = des[0], j = des[1];
+        uint64_t d[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) d[w] = dJ[w];
+        int par = belowParity<W>(d, i);
+        d[i >> 6] ^= 1ull << (i & 63);
+        par += belowParity<W>(d, j);
+        d[j >> 6] ^= 1ull << (j & 63);
+        par += belowParity<W>(d, bb);
+        d[bb >> 6] ^= 1ull << (bb & 63);
+        par += belowParity<W>(d, a);
+        const double sg = (par & 1) ? -1.0 : 1.0;
+        const double vE = sg * bI * ket[J], vT = sg * bra[J] * kI;
+        const int A = a >> 1, B = bb >> 1, Ii = i >> 1, Jj = j >> 1;
+        if (((a ^ i) & 1) == 0 && ((bb ^ j) & 1) == 0) {
+          o.add2(A, B, Ii, Jj, vE);
+          o.add2(B, A, Jj, Ii, vE);
+          o.add2(Ii, Jj, A, B, vT);
+          o.add2(Jj, Ii, B, A, vT);
+        }
+        if (((a ^ j) & 1) == 0 && ((bb ^ i) & 1) == 0) {
+          o.add2(A, B, Jj, Ii, -vE);
+          o.add2(B, A, Ii, Jj, -vE);
+          o.add2(Jj, Ii, A, B, -vT);
+          o.add2(Ii, Jj, B, A, -vT);
+        }
+        if (o.s2) {  // <a+_b a+_a a_i a_j> = <a+_a a+_b a_j a_i>
+          o.spin2(bb, a, j, i, vE);
+          o.spin2(j, i, bb, a, vT);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // learned screening: features, targets and the exact test of ranked records
 
 // X[r * din + f]: f < nso occupation bit f of the source determinant; then
@@ -1829,6 +2032,22 @@ void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_
   DISPATCH_W(S.W, kPtStochReduce<WW><<<nb, kBlock, 0, s>>>(S, keys, idx, num, num2, pres, m, e0, nmc,
                                                            partial.as<double>()));
   kFinalSum<<<2, kBlock, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const int32_t* cols, const double* bra,
+               const double* ket, int64_t n, double* g1, double* g2, double* s1, double* s2, double* occScratch,
+               hipStream_t s) {
+  if (n <= 0) return;
+  if (norb > 32 * W) throw HipError("launchRdm: more orbitals than the determinant words hold");
+  RdmOut o{g1, g2, s1, s2, norb, 2 * norb, (int64_t)(2 * norb) * (2 * norb + 1) / 2};
+  const int NSO = 64 * W;
+  HIP_CHECK(hipMemsetAsync(occScratch, 0, (size_t)NSO * NSO * 8, s));
+  const unsigned go = (unsigned)std::min<int64_t>(std::max<int64_t>(n / 64, 1), 2048);
+  DISPATCH_W(W, kRdmOcc<WW><<<go, kBlock, 0, s>>>(dets, bra, ket, n, occScratch));
+  kRdmOccScatter<<<gridFor((int64_t)o.nso * o.nso), kBlock, 0, s>>>(o, occScratch, NSO);
+  const unsigned g = gridFor(n, kWaves, 1 << 16);
+  DISPATCH_W(W, kRdm<WW><<<g, kBlock, 0, s>>>(o, dets, rp, cols, bra, ket, n));
   HIP_CHECK(hipGetLastError());
 }
 

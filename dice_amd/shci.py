@@ -93,6 +93,7 @@ def library():
         "shci_pt_deterministic": ([P, pD, D, D, pD, pD, pI64], I32),
         "shci_pt_semistochastic": ([P, pD, D, D, D, I32, I32, D, ctypes.c_uint, pD, pD, I32,
                                     ctypes.POINTER(ctypes.c_int), pD], I32),
+        "shci_rdm": ([P, pD, pD, pD, pD, pD, pD], I32),
         "shci_run": ([ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(P)], I32),
         "shci_run_ex": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RunOptions), ctypes.POINTER(P)], I32),
         "shci_run_iteration_phases": ([P, I32, pD, pD, pD, pI64], I32),
@@ -280,6 +281,27 @@ class System:
                                                     ctypes.byref(psi), ctypes.byref(nx)))
         return e.value, psi.value, nx.value
 
+
+    def rdm(self, bra, ket=None, spin=False):
+        """RDMs of <bra| ... |ket> over the held determinants (SHCIrdm::EvaluateRDM
+        and EvaluateOneRDM, SHCI/SHCIrdm.cpp:863-1100): (g1[P,R], g2[P,Q,R,S]) with
+        g2[P,Q,R,S] = sum_st <a+_Ps a+_Qt a_St a_Rs> (the reference's
+        s2RDM(P*n+Q, R*n+S)); with spin, also the spin-orbital 1-RDM and the
+        pair-indexed spin 2-RDM (npair x npair, pair(p,q) = p(p+1)/2 + q, p > q)."""
+        b = np.ascontiguousarray(bra, dtype=np.float64)
+        k = b if ket is None else np.ascontiguousarray(ket, dtype=np.float64)
+        if len(b) != self.num_dets() or len(k) != len(b):
+            raise ValueError("need one coefficient per held determinant")
+        n = self.norb
+        nso, npair = 2 * n, n * (2 * n + 1)
+        g1, g2 = np.zeros((n, n)), np.zeros((n,) * 4)
+        s1 = np.zeros((nso, nso)) if spin else None
+        s2 = np.zeros((npair, npair)) if spin else None
+        self._check(self._lib.shci_rdm(self._h, _p(b, ctypes.c_double), _p(k, ctypes.c_double),
+                                       _p(g1, ctypes.c_double), _p(g2, ctypes.c_double),
+                                       _p(s1, ctypes.c_double) if spin else None,
+                                       _p(s2, ctypes.c_double) if spin else None))
+        return (g1, g2, s1, s2) if spin else (g1, g2)
 
     def pt_semistochastic(self, ci, e0, eps2, eps2_large, sample_n, n_iter, target_error, seed):
         """Semistochastic EN-PT2 of one root (SHCI/SHCIbasics.cpp:55-452): returns

@@ -131,6 +131,19 @@ shci_status shci_davidson(shci_handle* h, double* x0, int nroots, int max_copies
  * of shci_add_restriction (the "restrict" keyword), not ncore / nact. */
 shci_status shci_pt_deterministic(shci_handle* h, const double* ci, double e0, double eps2, double* e_pt,
                                   double* psi1_norm, int64_t* n_external);
+/* RDMs of <bra| ... |ket> over the held determinants (bra, ket: n host
+ * coefficients each; SHCIrdm::EvaluateOneRDM and SHCIrdm::EvaluateRDM,
+ * SHCI/SHCIrdm.cpp:996-1100 and :863-995, whose connections this call
+ * builds as shci_build_hamiltonian does):
+ *   one_rdm[P*n+R]           = sum_s <a+_{P s} a_{R s}>          (n = orbitals)
+ *   two_rdm[((P*n+Q)*n+R)*n+S] = sum_s,t <a+_{P s} a+_{Q t} a_{S t} a_{R s}>
+ *                              (the reference's s2RDM(P*n+Q, R*n+S))
+ * and, when not null, spin_one_rdm[p*2n+r] = <a+_p a_r> over spin-orbitals
+ * and spin_two_rdm[pair(p,q)*npair + pair(r,s)] = <a+_p a+_q a_s a_r> for
+ * p > q, r > s (pair(p,q) = p(p+1)/2 + q, npair = 2n(2n+1)/2; the other
+ * entries are zero), the reference's twoRDM of DoSpinRDM. */
+shci_status shci_rdm(shci_handle* h, const double* bra, const double* ket, double* one_rdm, double* two_rdm,
+                     double* spin_one_rdm, double* spin_two_rdm);
 /* Semistochastic EN-PT2 of one root over the held determinants
  * (SHCIbasics::DoPerturbativeStochastic2SingleListDoubleEpsilon2AllTogether,
  * SHCI/SHCIbasics.cpp:55-452, with the alias sampling of

@@ -26,6 +26,14 @@ COPIES = {
     "h2o/FCIDUMP": "tests/SHCI/integrals/h2o_FCIDUMP_symm",
     "h2h2/FCIDUMP": "tests/SHCI/integrals/H2H2_FCIDUMP_nosym",
     "multislater_rhf/dets.bin": "tests/DQMC/multislater_rhf/dets.bin",
+    "h2he_rdm/pyscf_1RDM.npy": "tests/SHCI/H2He_rdm/pyscf_1RDM.npy",
+    "h2he_rdm/pyscf_2RDM.npy": "tests/SHCI/H2He_rdm/pyscf_2RDM.npy",
+    "h2co_rdm/FCIDUMP": "tests/SHCI/integrals/h2co_FCIDUMP_nosymm",
+    "h2co_rdm/pyscf_1RDM.npy": "tests/SHCI/h2co_rdm/pyscf_1RDM.npy",
+    "h2co_rdm/pyscf_2RDM.npy": "tests/SHCI/h2co_rdm/pyscf_2RDM.npy",
+    # (tests/SHCI/o2_stoc uses the o2_det FCIDUMP, byte for byte)
+    "o2_stoc/trusted2RDM.txt": "tests/SHCI/o2_stoc/trusted2RDM.txt",
+    "ueg/FCIDUMP": "tests/SHCI/integrals/UEG_FCIDUMP",
 }
 ENERGIES = {
     "restart": "tests/SHCI/restart/trusted_hci.e",
@@ -58,6 +66,10 @@ def main():
     # inputs of the reference tests, kept as text data
     gold["restart_input3"] = open(os.path.join(REF, "tests/SHCI/restart/input3.dat")).read()
     # (tests/SHCI/fullrestart uses the restart test's FCIDUMP, byte for byte)
+    gold["h2he_rdm_input"] = open(os.path.join(REF, "tests/SHCI/H2He_rdm/input.dat")).read()
+    gold["h2co_rdm_input"] = open(os.path.join(REF, "tests/SHCI/h2co_rdm/test_input.dat")).read()
+    gold["o2_stoc_input"] = open(os.path.join(REF, "tests/SHCI/o2_stoc/input.dat")).read()
+    gold["spin1rdm_input"] = open(os.path.join(REF, "tests/SHCI/spin1rdm/input.dat")).read()
     for name in ("restart/input2", "fullrestart/input2", "fullrestart/input3"):
         gold[name.replace("/", "_")] = open(os.path.join(REF, "tests/SHCI/%s.dat" % name)).read()
     gold["n2_RAS_input"] = open(os.path.join(REF, "tests/SHCI/n2_RAS/input.dat")).read()

@@ -276,3 +276,28 @@ def mlp_loss_grad(params, din, hidden, X, t, bf16=True):
     loss = L.oref_mlp_loss_grad(_p(P, ctypes.c_float), din, hidden, _p(Xc, ctypes.c_float), _p(tc, ctypes.c_float),
                                 len(Xc), 1 if bf16 else 0, _p(g, ctypes.c_double))
     return loss, g
+
+
+def spatial_integrals(o):
+    """(h1 n x n, eri n^4 chemist (ij|kl), core) of an OracleSystem."""
+    L = o.L
+    n = L.oref_norb(o.h)
+    nso = 2 * n
+    I1 = np.zeros(nso * nso)
+    I2 = np.zeros(L.oref_i2_size(o.h))
+    L.oref_get_integrals(o.h, _p(I1, ctypes.c_double), _p(I2, ctypes.c_double), None, None)
+    h1 = I1.reshape(nso, nso)[0::2, 0::2].copy()
+
+    def pair(i, j):
+        return max(i, j) * (max(i, j) + 1) // 2 + min(i, j)
+
+    eri = np.zeros((n,) * 4)
+    for i in range(n):
+        for j in range(n):
+            ij = pair(i, j)
+            for k in range(n):
+                for l in range(n):
+                    kl = pair(k, l)
+                    A, B = max(ij, kl), min(ij, kl)
+                    eri[i, j, k, l] = I2[A * (A + 1) // 2 + B]
+    return h1, eri, L.oref_core_energy(o.h)

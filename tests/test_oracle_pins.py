@@ -618,3 +618,103 @@ def test_run_semistochastic_pt(tmp_path):
     with open(tmp_path / "shci.e", "rb") as f:
         assert struct.unpack("<d", f.read(8))[0] == ept
     assert -0.07 < ept - run.energies()[0] < -0.04
+
+
+# ---- RDMs (SHCI/SHCIrdm.cpp) ----
+
+def _rdm_ref():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "rdm_ref", os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle", "rdm_ref.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_rdm_oracle_matches_pyscf_fci(tmp_path, golden_dir):
+    """tests/SHCI/H2He_rdm: H2H2 (6 orbitals, 2+2 electrons) selected at eps1
+    1e-10 is the full space; the oracle's spatial 1- and 2-RDMs of the
+    variational state equal PySCF's FCI RDMs shipped with the test
+    (pyscf_1RDM.npy, pyscf_2RDM.npy, the 2-RDM transposed (0,2,1,3) as the
+    reference's test_twopdm.py does) to 1e-9 per element, their traces are
+    N and N(N-1), and the energy evaluated from them is E_var (1e-10 Ha)."""
+    from oracle_bind import spatial_integrals
+    rdm = _rdm_ref()
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "h2h2", "FCIDUMP")
+    text = "\n".join(l for l in g["h2he_rdm_input"].splitlines() if not l.strip().startswith("orbitals"))
+    _write(tmp_path / "in.dat", text + "\norbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+    run = OracleRun(str(tmp_path / "in.dat"), str(tmp_path / "log"))
+    o = OracleSystem(fcidump)
+    g1, g2 = rdm.rdms(run.dets(o.words), run.ci()[:, 0], 6)
+    ref1 = np.load(os.path.join(golden_dir, "h2he_rdm", "pyscf_1RDM.npy"), allow_pickle=False)
+    ref2 = np.load(os.path.join(golden_dir, "h2he_rdm", "pyscf_2RDM.npy"), allow_pickle=False).transpose(0, 2, 1, 3)
+    assert np.max(np.abs(g1 - ref1)) < 1e-9 and np.max(np.abs(g2 - ref2)) < 1e-9
+    assert abs(np.trace(g1) - 4) < 1e-12
+    assert abs(np.einsum("pqpq->", g2) - 12) < 1e-12
+    h1, eri, core = spatial_integrals(o)
+    assert abs(rdm.energy(g1, g2, h1, eri, core) - run.energies()[0]) < 1e-10
+    # spin-resolved forms of the same state: N_alpha = N_beta = 2, the
+    # antisymmetry of G, and <S^2> = Sz^2 + Sz + N_beta - sum_pq
+    # <a+_{q b} a+_{p a} a_{q a} a_{p b}> = 0 for this singlet
+    s1, G = rdm.spin_rdms(run.dets(o.words), run.ci()[:, 0], 6)
+    assert abs(np.trace(s1[0::2, 0::2]) - 2) < 1e-12 and abs(np.trace(s1[1::2, 1::2]) - 2) < 1e-12
+    assert np.max(np.abs(G + G.transpose(1, 0, 2, 3))) < 1e-14 and np.max(np.abs(G + G.transpose(0, 1, 3, 2))) < 1e-14
+    s2 = 0.0 + 0.0 + 2.0
+    for p in range(6):
+        for q in range(6):
+            s2 -= G[2 * q + 1, 2 * p, 2 * p + 1, 2 * q]
+    assert abs(s2) < 1e-9
+
+
+def _h2h2_subset(n, seed):
+    import itertools
+    dets = []
+    for a in itertools.combinations(range(6), 2):
+        for b in itertools.combinations(range(6), 2):
+            dets.append(sum(1 << (2 * x) for x in a) | sum(1 << (2 * x + 1) for x in b))
+    d = sort_dets(np.array(dets, dtype=np.int64).reshape(-1, 1))
+    rng = np.random.default_rng(seed)
+    return d[np.sort(rng.choice(len(d), n, replace=False))], rng
+
+
+def test_rdm_oracle_constructions_agree_energy_identity(golden_dir):
+    """On 90 of the 225 H2H2 determinants with unrelated random bra and ket:
+    the oracle's two constructions (overlaps of the states the annihilators
+    leave, and all four operators applied to each determinant) agree; the
+    contraction sum h gamma + 1/2 sum Gamma[PQRS] (PR|QS) equals bra . (H ket)
+    with H from the oracle's Slater-Condon sigma (no core energy) to 1e-12
+    relative; swapping bra and ket transposes both RDMs."""
+    from oracle_bind import spatial_integrals
+    rdm = _rdm_ref()
+    d, rng = _h2h2_subset(90, 2)
+    bra, ket = rng.standard_normal(len(d)), rng.standard_normal(len(d))
+    a1, aG = rdm.spin_rdms(d, bra, 6, ket)
+    b1, bG = rdm.spin_rdms_brute(d, bra, 6, ket)
+    assert np.max(np.abs(a1 - b1)) < 1e-12 and np.max(np.abs(aG - bG)) < 1e-12
+    o = OracleSystem(os.path.join(golden_dir, "h2h2", "FCIDUMP"))
+    o.csr_append(d)
+    g1, g2 = rdm.rdms(d, bra, 6, ket)
+    h1, eri, _ = spatial_integrals(o)
+    e = rdm.energy(g1, g2, h1, eri, 0.0)
+    assert abs(e - bra @ o.sigma(ket)) <= 1e-12 * abs(e)
+    t1, t2 = rdm.rdms(d, ket, 6, bra)
+    assert np.max(np.abs(t1 - g1.T)) < 1e-13 and np.max(np.abs(t2 - g2.transpose(2, 3, 0, 1))) < 1e-13
+
+
+def test_rdm_oracle_matches_o2_trusted_2rdm(tmp_path, golden_dir):
+    """tests/SHCI/o2_stoc: the spatial 2-RDM of the O2 variational state
+    (schedule 1e-4 / 1e-5) equals the reference's trusted2RDM.txt to the
+    reference test's bar (test_twopdm.py: L2 norm of the difference / 8^4
+    below 1e-8), and element-wise to 1e-7 (the file keeps 8 digits)."""
+    rdm = _rdm_ref()
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "o2", "FCIDUMP")
+    text = g["o2_stoc_input"].replace("nPTiter 200", "nPTiter 0")
+    _write(tmp_path / "in.dat", text + "\norbitals %s\nprefix %s\n" % (fcidump, tmp_path))
+    run = OracleRun(str(tmp_path / "in.dat"), str(tmp_path / "log"))
+    o = OracleSystem(fcidump)
+    _, g2 = rdm.rdms(run.dets(o.words), run.ci()[:, 0], 8)
+    ref = rdm.read_dice_2rdm(os.path.join(golden_dir, "o2_stoc", "trusted2RDM.txt"))
+    assert np.linalg.norm(g2 - ref) / g2.size < 1e-8
+    assert np.max(np.abs(g2 - ref)) < 1e-7
