@@ -23,7 +23,10 @@ profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) wh
 pt_deterministic: the deterministic EN-PT2 step (after the variational one)
 on the final space at eps2 = 1e-4, the semistochastic one (deterministic
 part at 1e-4, 50 sampled iterations down to 1e-6), and GPU vs CPU oracle on
-the first iteration's space at eps2 = 1e-3.
+the first iteration's space at eps2 = 1e-3.  rdm (inside it): the spatial
+1- and 2-RDMs of the final state (one call with the connections stored), and
+GPU vs the C restatement of EvaluateOneRDM / EvaluateRDM (one thread) on the
+first iteration's space.
 
 cpu_baseline: the CPU oracle (a restatement of the reference algorithm whose
 rank-parallel loops run on OpenMP threads, up to 16) on a bounded sample of
@@ -163,13 +166,22 @@ def pt_measure(shci, full, sample, log, args):
     t0 = time.time()
     sval, per, its, serr = S.pt_semistochastic(ci, e - core, PT_STOCH_EPS2, PT_EPS2, 200, 50, -1.0, 1)
     stoch_s = time.time() - t0
+    # spatial 1- and 2-RDMs of the final state (DoRDM): the first call builds
+    # the connections of the held determinants, the second is timed
+    S.rdm(ci)
+    t0 = time.time()
+    g1, g2 = S.rdm(ci)
+    rdm_s = time.time() - t0
     S.close()
     res = {"eps2": PT_EPS2, "ndets": len(dets), "seconds": round(gpu_s, 4), "E_PT": ept, "E_var_plus_PT": e + ept,
            "external_dets": next_, "external_per_s": round(next_ / gpu_s, 1),
            "semistochastic": {"eps2": PT_STOCH_EPS2, "eps2_large": PT_EPS2, "sample_n": 200, "iterations": its,
                               "seconds": round(stoch_s, 3),
                               "seconds_per_iteration": round((stoch_s - gpu_s) / max(its, 1), 5),
-                              "E_PT_mean": float(per.mean()) if len(per) else None, "std_error": serr}}
+                              "E_PT_mean": float(per.mean()) if len(per) else None, "std_error": serr},
+           "rdm": {"what": "spatial 1- and 2-RDM of the final state (DoRDM), one call with the connections stored",
+                   "seconds": round(rdm_s, 4), "trace_1rdm": float(g1.trace()),
+                   "trace_2rdm": float(g2.reshape(g1.size, g1.size).trace())}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import OracleRun, OracleSystem
@@ -188,11 +200,25 @@ def pt_measure(shci, full, sample, log, args):
         t0 = time.time()
         gpu_pt = G.pt_deterministic(sc, se, PT_SAMPLE_EPS2)
         gpu_sample_s = time.time() - t0
+        # RDMs of the same sample: the reference's EvaluateOneRDM / EvaluateRDM
+        # restated in C (one thread, as one reference rank) vs the GPU
+        G.set_dets(sd)
+        G.rdm(sc)
+        t0 = time.time()
+        gg1, gg2 = G.rdm(sc)
+        gpu_rdm_s = time.time() - t0
         G.close()
+        o.csr_append(sd)
+        t0 = time.time()
+        og1, og2 = o.rdm(sc)
+        cpu_rdm_s = time.time() - t0
         res["cpu_sample"] = {"sample": "first variational iteration's space (%d determinants), eps2 %.0e"
                                        % (len(sd), PT_SAMPLE_EPS2),
                              "cpu_s": round(cpu_s, 3), "cores": threads, "kind": "port", "gpu_s": round(gpu_sample_s, 4),
                              "E_PT_cpu": cpu_pt[0], "E_PT_gpu": gpu_pt[0], "external_dets": cpu_pt[2]}
+        res["rdm"]["cpu_sample"] = {"sample": "the same %d determinants" % len(sd), "cpu_s": round(cpu_rdm_s, 3),
+                                    "cores": 1, "kind": "port", "gpu_s": round(gpu_rdm_s, 4),
+                                    "max_abs_diff_2rdm": float(abs(gg2 - og2).max())}
     print("# PT eps2 %.0e: %.3f s, %d external determinants, E_PT %.10f" % (PT_EPS2, gpu_s, next_, ept),
           file=sys.stderr, flush=True)
     return res
@@ -208,6 +234,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="time only the --precision form")
     ap.add_argument("--davidson-block", action="store_true",
                     help="the --precision form with the block Davidson and hipGraph replay (batching form)")
+    ap.add_argument("--post-variational", action="store_true",
+                    help="only the steps after the variational one (PT, RDMs) on the final space, printed as "
+                         "their own JSON object (for profiling them)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the variational step on a single MI355X "
@@ -219,6 +248,9 @@ def main():
     full, sample = workload_inputs(work)
     log = os.path.join(work, "gpu.log")
     primary = shci.PREC_FP32 if args.precision == "fp32" else shci.PREC_FP64
+    if args.post_variational:
+        print(json.dumps({"post_variational": pt_measure(shci, full, sample, log, args)}))
+        return
     results = {}
     forms = [primary] if args.no_forms else [primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]
     block = dict(davidson_block=True, hip_graphs=True) if args.davidson_block else {}

@@ -974,11 +974,14 @@ shci_status shci_rdm(shci_handle* h, const double* bra, const double* ket, doubl
     if (n < 1) throw std::invalid_argument("no determinants held");
     DevMem db, dk;
     db.reserve((size_t)n * 8);
-    dk.reserve((size_t)n * 8);
     HIP_CHECK(hipMemcpyAsync(db.p, bra, n * 8, hipMemcpyHostToDevice, E.stream));
-    HIP_CHECK(hipMemcpyAsync(dk.p, ket, n * 8, hipMemcpyHostToDevice, E.stream));
+    if (ket != bra) {  // (the same vector on both sides lets the kernels use hermiticity)
+      dk.reserve((size_t)n * 8);
+      HIP_CHECK(hipMemcpyAsync(dk.p, ket, n * 8, hipMemcpyHostToDevice, E.stream));
+    }
+    const double* dKet = ket != bra ? dk.as<double>() : db.as<double>();
     std::vector<double> g1, g2, s1, s2;
-    E.rdm(db.as<double>(), dk.as<double>(), g1, g2, spin_one_rdm ? &s1 : nullptr, spin_two_rdm ? &s2 : nullptr);
+    E.rdm(db.as<double>(), dKet, g1, g2, spin_one_rdm ? &s1 : nullptr, spin_two_rdm ? &s2 : nullptr);
     std::copy(g1.begin(), g1.end(), one_rdm);
     std::copy(g2.begin(), g2.end(), two_rdm);
     if (spin_one_rdm) std::copy(s1.begin(), s1.end(), spin_one_rdm);

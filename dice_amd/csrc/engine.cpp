@@ -321,12 +321,13 @@ void Engine::rdm(const double* dBra, const double* dKet, std::vector<double>& g1
     return m.as<double>();
   };
   double* g1d = zeroed(rdmG1, n1);
-  double* g2d = zeroed(rdmG2, n2);
+  rdmG2.reserve(n2 * 8, stream);
+  double* g2d = rdmG2.as<double>();
   double* s1d = s1 ? zeroed(rdmS1, ns1) : nullptr;
   double* s2d = s2 ? zeroed(rdmS2, ns2) : nullptr;
-  rdmOcc.reserve((size_t)64 * hs.W * 64 * hs.W * 8, stream);
+  rdmScratch.reserve(rdmScratchDoubles(norb, hs.W) * 8, stream);
   launchRdm(norb, hs.W, dets.as<uint64_t>(), rowPtr.as<int64_t>(), cols.as<int32_t>(), dBra, dKet, n, g1d, g2d, s1d,
-            s2d, rdmOcc.as<double>(), stream);
+            s2d, rdmScratch.as<double>(), stream);
   g1.resize(n1);
   g2.resize(n2);
   HIP_CHECK(hipMemcpyAsync(g1.data(), g1d, n1 * 8, hipMemcpyDeviceToHost, stream));

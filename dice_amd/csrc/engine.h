@@ -156,7 +156,7 @@ struct Engine {
   // g1 (norb^2) and g2 (norb^4, g2[((P*n+Q)*n+R)*n+S] = the reference's
   // s2RDM(P*n+Q, R*n+S)) and, when asked, the spin-orbital s1 (nso^2) and s2
   // (pair-indexed, pair(p,q) = p(p+1)/2 + q)
-  DevMem rdmG1, rdmG2, rdmS1, rdmS2, rdmOcc;
+  DevMem rdmG1, rdmG2, rdmS1, rdmS2, rdmScratch;
   void rdm(const double* dBra, const double* dKet, std::vector<double>& g1, std::vector<double>& g2,
            std::vector<double>* s1 = nullptr, std::vector<double>* s2 = nullptr);
 

@@ -53,10 +53,12 @@ void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial
 // g1 (norb^2) and g2 (norb^4) and, when not null, the spin-orbital s1
-// (nso^2) and pair-indexed s2 ((nso(nso+1)/2)^2); the caller zeroes them.
-// occScratch holds (64 W)^2 doubles.
+// (nso^2) and pair-indexed s2 ((nso(nso+1)/2)^2).  The caller zeroes g1, s1
+// and s2; g2 is overwritten.  scratch holds rdmScratchDoubles(norb, W)
+// doubles.
+size_t rdmScratchDoubles(int norb, int W);
 void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const int32_t* cols, const double* bra,
-               const double* ket, int64_t n, double* g1, double* g2, double* s1, double* s2, double* occScratch,
+               const double* ket, int64_t n, double* g1, double* g2, double* s1, double* s2, double* scratch,
                hipStream_t s);
 // m candidates sorted by determinant (payload idx into num): out[0..2]
 // (device) = E_PT, psi1 norm, number of external determinants

@@ -579,9 +579,20 @@ __global__ void __launch_bounds__(kBlock) kPtGen(DevSystem S, const uint64_t* de
 //   s1[p*nso + r]               = <a+_p a_r>
 //   s2[pair(p,q)*npair + pair(r,s)] = <a+_p a+_q a_s a_r>, p > q, r > s,
 //   pair(p,q) = p(p+1)/2 + q, npair = nso(nso+1)/2
-// over the stored connections (each unordered pair once, from the J < I
-// entries of the full symmetric CSR: an element with bra[I] ket[J] and its
-// transpose with bra[J] ket[I]) and the diagonal; FP64 atomics.
+// Off-diagonal pairs come from the J < I entries of the full symmetric CSR
+// (an element with bra[I] ket[J] and its transpose with bra[J] ket[I]); the
+// diagonal is a rank-n product of the occupation matrix.
+//
+// g2 is assembled from three accumulators, so that each connection costs
+// few FP64 atomics:
+//   A[P,Q,R,S]  one element of every exchange pair G[PQRS] = G[QPSR] (and,
+//               when bra = ket, of every hermitian pair G[PQRS] = G[RSPQ]);
+//               G = A + A[QPSR] (+ the hermitian images)
+//   Z[X,Y,K]    single excitations X <- Y times the number of common
+//               occupied spin-orbitals of spatial orbital K; it adds to
+//               G[X,K,Y,K] and G[K,X,K,Y]
+//   Zs[X,Y,K]   the same over the common spin-orbitals with the spin of the
+//               excitation; it subtracts from G[X,K,K,Y] and G[K,X,Y,K]
 
 // parity of the occupied spin-orbitals of d below k
 template <int W>
@@ -600,8 +611,13 @@ __device__ __forceinline__ int belowParity(const uint64_t* d, int k) {
 
 struct RdmOut {
   double *g1, *g2, *s1, *s2;  // s1 / s2 may be null
+  double *A, *Z, *Zs;         // accumulators of g2
   int n, nso;
   int64_t npair;
+  bool herm;  // bra == ket: only one element of each hermitian pair is accumulated
+  __device__ __forceinline__ void acc(int P, int Q, int R, int S, double v) const {
+    atomicAdd(A + (((int64_t)P * n + Q) * n + R) * n + S, v);
+  }
   __device__ __forceinline__ void add2(int P, int Q, int R, int S, double v) const {
     atomicAdd(g2 + (((int64_t)P * n + Q) * n + R) * n + S, v);
   }
@@ -671,7 +687,6 @@ __global__ void __launch_bounds__(kBlock) kRdm(RdmOut o, const uint64_t* dets, c
 #pragma unroll
     for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
     const double bI = bra[I], kI = ket[I];
-    // connections J < I
     const int64_t b = rp[I], e = rp[I + 1];
     for (int64_t p = b + lane; p < e; p += 64) {
       const int64_t J = cols[p];
@@ -709,24 +724,35 @@ __global__ void __launch_bounds__(kBlock) kRdm(RdmOut o, const uint64_t* dets, c
           atomicAdd(o.s1 + (int64_t)a * nso + i, vE);
           atomicAdd(o.s1 + (int64_t)i * nso + a, vT);
         }
+        // common occupied spin-orbitals, per spatial orbital: count and
+        // count with the excitation's spin
+        const int64_t zE = ((int64_t)A * norb + Ii) * norb, zT = ((int64_t)Ii * norb + A) * norb;
+        const int sa = a & 1;
 #pragma unroll
         for (int w = 0; w < W; w++) {
-          uint64_t common = dJ[w] & dI[w];
-          while (common) {
-            const int k = 64 * w + __ffsll((unsigned long long)common) - 1;
-            common &= common - 1;
-            const int K = k >> 1;
-            o.add2(A, K, Ii, K, vE);
-            o.add2(K, A, K, Ii, vE);
-            o.add2(Ii, K, A, K, vT);
-            o.add2(K, Ii, K, A, vT);
-            if (((k ^ a) & 1) == 0) {
-              o.add2(A, K, K, Ii, -vE);
-              o.add2(K, A, Ii, K, -vE);
-              o.add2(K, Ii, A, K, -vT);
-              o.add2(Ii, K, K, A, -vT);
+          const uint64_t common = dJ[w] & dI[w];
+          uint64_t sp = (common | (common >> 1)) & 0x5555555555555555ull;
+          while (sp) {
+            const int bit = __ffsll((unsigned long long)sp) - 1;
+            sp &= sp - 1;
+            const int K = (64 * w + bit) >> 1;
+            const uint64_t pairBits = (common >> bit) & 3ull;
+            const double cnt = (double)__popcll(pairBits);
+            atomicAdd(o.Z + zE + K, cnt * vE);
+            if (!o.herm) atomicAdd(o.Z + zT + K, cnt * vT);
+            if ((pairBits >> sa) & 1ull) {
+              atomicAdd(o.Zs + zE + K, vE);
+              if (!o.herm) atomicAdd(o.Zs + zT + K, vT);
             }
-            if (o.s2) {
+          }
+        }
+        if (o.s2) {
+#pragma unroll
+          for (int w = 0; w < W; w++) {
+            uint64_t common = dJ[w] & dI[w];
+            while (common) {
+              const int k = 64 * w + __ffsll((unsigned long long)common) - 1;
+              common &= common - 1;
               // <a+_max(k,a) a+_min(k,a) a_min(k,i) a_max(k,i)>: one swap
               // when k lies between a and i
               const double f = (k > lo && k < hi) ? -1.0 : 1.0;
@@ -753,17 +779,15 @@ __global__ void __launch_bounds__(kBlock) kRdm(RdmOut o, const uint64_t* dets, c
         const double sg = (par & 1) ? -1.0 : 1.0;
         const double vE = sg * bI * ket[J], vT = sg * bra[J] * kI;
         const int A = a >> 1, B = bb >> 1, Ii = i >> 1, Jj = j >> 1;
+        // G[A,B,I,J] += vE (exchange image G[B,A,J,I]), G[I,J,A,B] += vT;
+        // G[A,B,J,I] -= vE (image G[B,A,I,J]), G[J,I,A,B] -= vT
         if (((a ^ i) & 1) == 0 && ((bb ^ j) & 1) == 0) {
-          o.add2(A, B, Ii, Jj, vE);
-          o.add2(B, A, Jj, Ii, vE);
-          o.add2(Ii, Jj, A, B, vT);
-          o.add2(Jj, Ii, B, A, vT);
+          o.acc(A, B, Ii, Jj, vE);
+          if (!o.herm) o.acc(Ii, Jj, A, B, vT);
         }
         if (((a ^ j) & 1) == 0 && ((bb ^ i) & 1) == 0) {
-          o.add2(A, B, Jj, Ii, -vE);
-          o.add2(B, A, Ii, Jj, -vE);
-          o.add2(Jj, Ii, A, B, -vT);
-          o.add2(Ii, Jj, B, A, -vT);
+          o.acc(A, B, Jj, Ii, -vE);
+          if (!o.herm) o.acc(Jj, Ii, A, B, -vT);
         }
         if (o.s2) {  // <a+_b a+_a a_i a_j> = <a+_a a+_b a_j a_i>
           o.spin2(bb, a, j, i, vE);
@@ -771,6 +795,27 @@ __global__ void __launch_bounds__(kBlock) kRdm(RdmOut o, const uint64_t* dets, c
         }
       }
     }
+  }
+}
+
+// g2 from the accumulators (one thread per element; overwrites g2)
+__global__ void kRdmFinish(RdmOut o) {
+  const int n = o.n;
+  const int64_t n2 = (int64_t)n * n, n4 = n2 * n2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += (int64_t)gridDim.x * blockDim.x) {
+    const int S = (int)(t % n), R = (int)((t / n) % n), Q = (int)((t / n2) % n), P = (int)(t / (n2 * n));
+    auto at = [&](int p, int q, int r, int s) { return o.A[(((int64_t)p * n + q) * n + r) * n + s]; };
+    auto z = [&](const double* Z, int x, int y, int k) {
+      const double v = Z[((int64_t)x * n + y) * n + k];
+      return o.herm ? v + Z[((int64_t)y * n + x) * n + k] : v;
+    };
+    double g = at(P, Q, R, S) + at(Q, P, S, R);
+    if (o.herm) g += at(R, S, P, Q) + at(S, R, Q, P);
+    if (Q == S) g += z(o.Z, P, R, Q);
+    if (P == R) g += z(o.Z, Q, S, P);
+    if (Q == R) g -= z(o.Zs, P, S, Q);
+    if (P == S) g -= z(o.Zs, Q, R, P);
+    o.g2[t] = g;
   }
 }
 
@@ -1509,6 +1554,9 @@ __global__ void __launch_bounds__(kBlock) kSpmmPerm(const int64_t* __restrict__ 
 // Davidson vector kernels
 
 constexpr int kRedBlocks = 512;
+// the PT reductions evaluate E_a of every unique external determinant: enough
+// workgroups to keep every SIMD busy with latency-bound work
+constexpr int kPtRedBlocks = 8192;
 
 // per-block partial Gram entries, then a fixed-order final sum
 template <int MA, int MB>
@@ -1718,10 +1766,34 @@ __global__ void __launch_bounds__(kBlock) kFinalSum(const double* partial, int n
 // determinants (sorted, generation order kept), a1/a2 = sums of num / num2,
 // b1/b2 = the same over the terms present at epsilon2Large; per-block partial
 // sums of (|a1|^2 Nmc/(Nmc-1) - a2) / (E_a - E0) and of the b version.
-template <int W>
+// The energy tables of E_a in LDS: hdiag (n), direct (n^2), exchange (n^2)
+// when they fit (LDS_TAB), else read from global memory.
+constexpr size_t kPtTabMaxBytes = 64 * 1024;
+inline size_t ptTabBytes(int n) { return ((size_t)n + 2 * (size_t)n * n) * sizeof(double); }
+
+template <int W, bool LDS_TAB>
+__device__ __forceinline__ double ptEnergy(const DevSystem& S, const double* tab, const uint64_t* d) {
+  if (LDS_TAB) return detEnergyTab<W>(tab, tab + S.norb, tab + S.norb + S.norb * S.norb, S.norb, d);
+  return detEnergy<W>(S, d);
+}
+
+template <bool LDS_TAB>
+__device__ __forceinline__ double* ptStageTables(const DevSystem& S) {
+  extern __shared__ double tab[];
+  if (LDS_TAB) {
+    const int n = S.norb, nn = n * n;
+    for (int k = threadIdx.x; k < n + 2 * nn; k += blockDim.x)
+      tab[k] = k < n ? S.h1[k * n + k] : (k < n + nn ? S.direct[k - n] : S.exch[k - n - nn]);
+    __syncthreads();
+  }
+  return tab;
+}
+
+template <int W, bool LDS_TAB>
 __global__ void __launch_bounds__(kBlock) kPtStochReduce(DevSystem S, const uint64_t* keys, const uint32_t* idx,
                                                          const double* num, const double* num2, const uint8_t* pres,
                                                          int64_t m, double e0, double nmc, double* partial) {
+  const double* tab = ptStageTables<LDS_TAB>(S);
   double sA = 0.0, sB = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     if (i > 0 && detEqual<W>(keys + i * W, keys + (i - 1) * W)) continue;
@@ -1738,7 +1810,7 @@ __global__ void __launch_bounds__(kBlock) kPtStochReduce(DevSystem S, const uint
     uint64_t d[W];
 #pragma unroll
     for (int w = 0; w < W; w++) d[w] = keys[i * W + w];
-    const double den = __dsub_rn(detEnergy<W>(S, d), e0);
+    const double den = __dsub_rn(ptEnergy<W, LDS_TAB>(S, tab, d), e0);
     sA = __dadd_rn(sA, __ddiv_rn(__dsub_rn(__ddiv_rn(__dmul_rn(__dmul_rn(fabs(a1), fabs(a1)), nmc), nmc - 1.0), a2),
                                  den));
     sB = __dadd_rn(sB, __ddiv_rn(__dsub_rn(__ddiv_rn(__dmul_rn(__dmul_rn(fabs(b1), fabs(b1)), nmc), nmc - 1.0), b2),
@@ -1761,9 +1833,10 @@ __global__ void kIota(uint32_t* v, int64_t n) {
 // numerators in that order (the reference's merge) and contributes
 // num^2 / (E0 - E_a) and (num / (E0 - E_a))^2; per-block partial sums of
 // (E_PT, psi1 norm, number of external determinants) in fixed order.
-template <int W>
+template <int W, bool LDS_TAB>
 __global__ void __launch_bounds__(kBlock) kPtReduce(DevSystem S, const uint64_t* keys, const uint32_t* idx,
                                                     const double* num, int64_t m, double e0, double* partial) {
+  const double* tab = ptStageTables<LDS_TAB>(S);
   double sPt = 0.0, sPsi = 0.0, sCnt = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     if (i > 0 && detEqual<W>(keys + i * W, keys + (i - 1) * W)) continue;
@@ -1772,7 +1845,7 @@ __global__ void __launch_bounds__(kBlock) kPtReduce(DevSystem S, const uint64_t*
     uint64_t d[W];
 #pragma unroll
     for (int w = 0; w < W; w++) d[w] = keys[i * W + w];
-    const double den = __dsub_rn(e0, detEnergy<W>(S, d));
+    const double den = __dsub_rn(e0, ptEnergy<W, LDS_TAB>(S, tab, d));
     sPt = __dadd_rn(sPt, __ddiv_rn(__dmul_rn(fabs(sum), fabs(sum)), den));
     const double r = __ddiv_rn(sum, den);
     sPsi = __dadd_rn(sPsi, __dmul_rn(fabs(r), fabs(r)));
@@ -2027,27 +2100,43 @@ void launchPtGenStoch(const DevSystem& S, const uint64_t* dets, const uint32_t* 
 void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num,
                          const double* num2, const uint8_t* pres, int64_t m, double e0, double nmc, DevMem& partial,
                          double* out, hipStream_t s) {
-  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + kBlock - 1) / kBlock));
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kPtRedBlocks, (m + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * 2 * sizeof(double));
-  DISPATCH_W(S.W, kPtStochReduce<WW><<<nb, kBlock, 0, s>>>(S, keys, idx, num, num2, pres, m, e0, nmc,
-                                                           partial.as<double>()));
+  const size_t tb = ptTabBytes(S.norb);
+  if (tb <= kPtTabMaxBytes)
+    DISPATCH_W(S.W, kPtStochReduce<WW, true><<<nb, kBlock, tb, s>>>(S, keys, idx, num, num2, pres, m, e0, nmc,
+                                                                    partial.as<double>()));
+  else
+    DISPATCH_W(S.W, kPtStochReduce<WW, false><<<nb, kBlock, 0, s>>>(S, keys, idx, num, num2, pres, m, e0, nmc,
+                                                                     partial.as<double>()));
   kFinalSum<<<2, kBlock, 0, s>>>(partial.as<double>(), nb, 2, 2, out);
   HIP_CHECK(hipGetLastError());
 }
 
+size_t rdmScratchDoubles(int norb, int W) {
+  const size_t n = norb, NSO = 64 * W;
+  return NSO * NSO + n * n * n * n + 2 * n * n * n;
+}
+
 void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const int32_t* cols, const double* bra,
-               const double* ket, int64_t n, double* g1, double* g2, double* s1, double* s2, double* occScratch,
+               const double* ket, int64_t n, double* g1, double* g2, double* s1, double* s2, double* scratch,
                hipStream_t s) {
-  if (n <= 0) return;
   if (norb > 32 * W) throw HipError("launchRdm: more orbitals than the determinant words hold");
-  RdmOut o{g1, g2, s1, s2, norb, 2 * norb, (int64_t)(2 * norb) * (2 * norb + 1) / 2};
   const int NSO = 64 * W;
-  HIP_CHECK(hipMemsetAsync(occScratch, 0, (size_t)NSO * NSO * 8, s));
-  const unsigned go = (unsigned)std::min<int64_t>(std::max<int64_t>(n / 64, 1), 2048);
-  DISPATCH_W(W, kRdmOcc<WW><<<go, kBlock, 0, s>>>(dets, bra, ket, n, occScratch));
-  kRdmOccScatter<<<gridFor((int64_t)o.nso * o.nso), kBlock, 0, s>>>(o, occScratch, NSO);
-  const unsigned g = gridFor(n, kWaves, 1 << 16);
-  DISPATCH_W(W, kRdm<WW><<<g, kBlock, 0, s>>>(o, dets, rp, cols, bra, ket, n));
+  const int64_t nn = norb, n3 = nn * nn * nn;
+  double* M = scratch;
+  RdmOut o{g1, g2, s1, s2, M + (int64_t)NSO * NSO, M + (int64_t)NSO * NSO + n3 * nn,
+           M + (int64_t)NSO * NSO + n3 * nn + n3, norb, 2 * norb, (int64_t)(2 * norb) * (2 * norb + 1) / 2,
+           bra == ket};
+  HIP_CHECK(hipMemsetAsync(scratch, 0, rdmScratchDoubles(norb, W) * 8, s));
+  if (n > 0) {
+    const unsigned go = (unsigned)std::min<int64_t>(std::max<int64_t>(n / 64, 1), 2048);
+    DISPATCH_W(W, kRdmOcc<WW><<<go, kBlock, 0, s>>>(dets, bra, ket, n, M));
+    const unsigned g = gridFor(n, kWaves, 1 << 16);
+    DISPATCH_W(W, kRdm<WW><<<g, kBlock, 0, s>>>(o, dets, rp, cols, bra, ket, n));
+  }
+  kRdmFinish<<<gridFor(n3 * nn), kBlock, 0, s>>>(o);
+  kRdmOccScatter<<<gridFor((int64_t)o.nso * o.nso), kBlock, 0, s>>>(o, M, NSO);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -2059,9 +2148,13 @@ void launchIota(uint32_t* v, int64_t n, hipStream_t s) {
 
 void launchPtReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num, int64_t m,
                     double e0, DevMem& partial, double* out, hipStream_t s) {
-  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + kBlock - 1) / kBlock));
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kPtRedBlocks, (m + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * 3 * sizeof(double));
-  DISPATCH_W(S.W, kPtReduce<WW><<<nb, kBlock, 0, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
+  const size_t tb = ptTabBytes(S.norb);
+  if (tb <= kPtTabMaxBytes)
+    DISPATCH_W(S.W, kPtReduce<WW, true><<<nb, kBlock, tb, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
+  else
+    DISPATCH_W(S.W, kPtReduce<WW, false><<<nb, kBlock, 0, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
   kFinalSum<<<3, kBlock, 0, s>>>(partial.as<double>(), nb, 3, 3, out);
   HIP_CHECK(hipGetLastError());
 }

@@ -110,6 +110,37 @@ __device__ double detEnergy(const DevSystem& S, const uint64_t* d) {
   return e;
 }
 
+// detEnergy with the tables from elsewhere (e.g. LDS): hdiag[K] = h1(K,K),
+// D / X = the direct / exchange tables (n x n); the same operations in the
+// same order, so the result is identical
+template <int W>
+__device__ double detEnergyTab(const double* hdiag, const double* D, const double* X, int n, const uint64_t* d) {
+  double e = 0.0;
+#pragma unroll
+  for (int w = 0; w < W; w++) {
+    uint64_t bi = d[w];
+    while (bi) {
+      int I = 64 * w + __ffsll((unsigned long long)bi) - 1;
+      bi &= bi - 1;
+      e = __dadd_rn(e, hdiag[I >> 1]);
+      uint64_t bj = bi;
+      int wj = w;
+      while (true) {
+        while (bj) {
+          int J = 64 * wj + __ffsll((unsigned long long)bj) - 1;
+          bj &= bj - 1;
+          e = __dadd_rn(e, D[(I >> 1) * n + (J >> 1)]);
+          if (((I ^ J) & 1) == 0) e = __dsub_rn(e, X[(I >> 1) * n + (J >> 1)]);
+        }
+        wj++;
+        if (wj >= W) break;
+        bj = d[wj];
+      }
+    }
+  }
+  return e;
+}
+
 // member Hij_1Excite(a, i) on ket (SHCI/Determinants.cpp:472-505)
 template <int W>
 __device__ double hijSingle(const DevSystem& S, const uint64_t* ket, int a, int i) {

@@ -446,6 +446,8 @@ static void parity(const Det& d, int start, int end, double& par) {
   if (getocc(d, start)) par *= -1.;
 }
 
+static void parityDouble(Det d, int i, int j, int a, int b, double& sgn);
+
 static void occList(const Det& d, int W, int* closed, int& nclosed) {
   nclosed = 0;
   for (int i = 0; i < W; i++) {
@@ -1140,6 +1142,96 @@ void hmultDirect(const System& s, const std::vector<Det>& dets, const SparseHam&
     }
   }
   for (size_t j = 0; j < n; j++) y[j] = yt[j];
+}
+
+// ---------------------------------------------------------------------------
+// RDMs: SHCIrdm::EvaluateOneRDM / EvaluateRDM (SHCI/SHCIrdm.cpp:863-1100)
+// over the stored rows (row i: i itself, then every connected J < i), each
+// connection's excitation as Hij(Dets[J], Dets[i]) encodes it
+// (Determinants.cpp:564-631: creation = in J not in i, destruction = in i
+// not in J, both ascending), with the reference's sign rules and its
+// spatial scatter populateSpatialRDM (SHCIrdm.cpp:846-860).
+void evaluateRdm(const System& s, const std::vector<Det>& dets, const SparseHam& H, const double* bra,
+                 const double* ket, double* g1, double* g2, double* s1, double* s2) {
+  const int nso = s.nso, n = s.norb;
+  const size_t nn = (size_t)n * n, npair = (size_t)nso * (nso + 1) / 2;
+  auto spatial = [&](int i, int j, int k, int l, double v) {
+    const size_t I = i / 2, J = j / 2, K = k / 2, L = l / 2;
+    if (i % 2 == l % 2 && j % 2 == k % 2) {
+      g2[(I * n + J) * nn + L * n + K] -= v;
+      g2[(J * n + I) * nn + K * n + L] -= v;
+    }
+    if (i % 2 == k % 2 && l % 2 == j % 2) {
+      g2[(I * n + J) * nn + K * n + L] += v;
+      g2[(J * n + I) * nn + L * n + K] += v;
+    }
+  };
+  auto spin2 = [&](size_t a, size_t b, size_t c, size_t d, double v) {  // twoRDM(a(a+1)/2+b, c(c+1)/2+d)
+    if (s2) s2[(a * (a + 1) / 2 + b) * npair + c * (c + 1) / 2 + d] += v;
+  };
+  for (size_t i = 0; i < H.connections.size(); i++) {
+    int closed[128], nc;
+    occList(dets[i], s.W, closed, nc);
+    const double cc = bra[i] * ket[i];
+    for (int n1 = 0; n1 < nc; n1++) {  // <Di| Gamma |Di>
+      const int o1 = closed[n1];
+      g1[(size_t)(o1 / 2) * n + o1 / 2] += cc;
+      if (s1) s1[(size_t)o1 * nso + o1] += cc;
+      for (int n2 = 0; n2 < n1; n2++) {
+        const int o2 = closed[n2];
+        spin2(o1, o2, o1, o2, cc);
+        spatial(o1, o2, o1, o2, cc);
+      }
+    }
+    for (size_t jj = 1; jj < H.connections[i].size(); jj++) {
+      const size_t J = (size_t)H.connections[i][jj];
+      int cre[2] = {-1, -1}, des[2] = {-1, -1}, ncre = 0, ndes = 0;
+      for (int w = 0; w < s.W; w++) {
+        const uint64_t u = (uint64_t)(dets[J].r[w] ^ dets[i].r[w]);
+        uint64_t b = u & (uint64_t)dets[J].r[w], k = u & (uint64_t)dets[i].r[w];
+        for (; b; b &= b - 1) {
+          if (ncre < 2) cre[ncre] = w * 64 + __builtin_ctzll(b);
+          ncre++;
+        }
+        for (; k; k &= k - 1) {
+          if (ndes < 2) des[ndes] = w * 64 + __builtin_ctzll(k);
+          ndes++;
+        }
+      }
+      const double vE = bra[J] * ket[i], vT = ket[J] * bra[i];
+      if (ncre == 1) {
+        const int c0 = cre[0], d0 = des[0];
+        double sgn = 1.0;
+        parity(dets[i], std::min(c0, d0), std::max(c0, d0), sgn);
+        g1[(size_t)(c0 / 2) * n + d0 / 2] += sgn * vE;
+        g1[(size_t)(d0 / 2) * n + c0 / 2] += sgn * vT;
+        if (s1) {
+          s1[(size_t)c0 * nso + d0] += sgn * vE;
+          s1[(size_t)d0 * nso + c0] += sgn * vT;
+        }
+        for (int n1 = 0; n1 < nc; n1++) {
+          const int cl = closed[n1];
+          if (cl == d0 || cl == c0) continue;
+          double sg = 1.0;
+          parity(dets[i], std::min(d0, c0), std::max(d0, c0), sg);
+          const int a = std::max(cl, c0), b = std::min(cl, c0), I = std::max(cl, d0), J2 = std::min(cl, d0);
+          if (!((cl > c0 && cl > d0) || (cl < c0 && cl < d0))) sg *= -1.;
+          spin2(a, b, I, J2, sg * vE);
+          spin2(I, J2, a, b, sg * vT);
+          spatial(a, b, I, J2, sg * vE);
+          spatial(I, J2, a, b, sg * vT);
+        }
+      } else if (ncre == 2) {
+        const int c0 = cre[0], c1 = cre[1], d0 = des[0], d1 = des[1];
+        double sgn = 1.0;
+        parityDouble(dets[i], d1, d0, c1, c0, sgn);
+        spin2(c1, c0, d1, d0, sgn * vE);
+        spin2(d1, d0, c1, c0, sgn * vT);
+        spatial(c1, c0, d1, d0, sgn * vE);
+        spatial(d1, d0, c1, c0, sgn * vT);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

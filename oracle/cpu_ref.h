@@ -155,6 +155,13 @@ void symEig(int m, const double* A, double* w, double* V);
 // taken from the stored rows of H
 void hmultDirect(const System& s, const std::vector<Det>& dets, const SparseHam& H, const double* x, double* y,
                  size_t n);
+// SHCIrdm::EvaluateOneRDM / EvaluateRDM (SHCI/SHCIrdm.cpp:863-1100) of
+// <bra| ... |ket> over the stored rows of H: g1 (norb^2), g2 (norb^4,
+// g2[((P*n+Q)*n+R)*n+S] = s2RDM(P*n+Q, R*n+S)) and, when not null, the
+// spin-orbital s1 (nso^2) and the reference's twoRDM s2 (npair^2, pair(p,q)
+// = p(p+1)/2 + q); the caller zeroes them.
+void evaluateRdm(const System& s, const std::vector<Det>& dets, const SparseHam& H, const double* bra,
+                 const double* ket, double* g1, double* g2, double* s1, double* s2);
 // davidson (SHCI/Davidson.cpp:111-348) on the product `op`; directRules
 // selects davidsonDirect's restart size, iteration cap and progress lines
 // (SHCI/Davidson.cpp:353-590)

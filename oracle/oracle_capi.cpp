@@ -1,5 +1,6 @@
 // C entry points of the CPU oracle, for the test suite and bench.py's
 // cpu_baseline leg (loaded with ctypes).  TEST INFRASTRUCTURE ONLY.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -189,6 +190,17 @@ void oref_sigma(void* p, const double* x, double* y) {
 void oref_sigma_direct(void* p, const double* x, double* y) {
   Handle* h = (Handle*)p;
   hmultDirect(h->s, h->dets, h->H, x, y, h->H.connections.size());
+}
+// RDMs of <bra| ... |ket> over the determinants appended with oref_csr_append
+// (zeroes the outputs first; s1 / s2 may be null).
+void oref_rdm(void* p, const double* bra, const double* ket, double* g1, double* g2, double* s1, double* s2) {
+  Handle* h = (Handle*)p;
+  const size_t n = h->s.norb, nso = h->s.nso, npair = nso * (nso + 1) / 2;
+  std::fill(g1, g1 + n * n, 0.0);
+  std::fill(g2, g2 + n * n * n * n, 0.0);
+  if (s1) std::fill(s1, s1 + nso * nso, 0.0);
+  if (s2) std::fill(s2, s2 + npair * npair, 0.0);
+  evaluateRdm(h->s, h->dets, h->H, bra, ket, g1, g2, s1, s2);
 }
 // Davidson on the stored H (x0: nroots columns of length n, in/out).
 int oref_davidson(void* p, double* x0, int nroots, const double* diag, int maxCopies, double tol, double* eroots) {

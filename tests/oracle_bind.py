@@ -52,6 +52,7 @@ def load():
         "oref_csr_get": ([P, pI64, ctypes.POINTER(ctypes.c_int32), pD], None),
         "oref_sigma": ([P, pD, pD], None),
         "oref_sigma_direct": ([P, pD, pD], None),
+        "oref_rdm": ([P, pD, pD, pD, pD, pD, pD], None),
         "oref_davidson": ([P, pD, I32, pD, I32, D, pD], I32),
         "oref_sym_eig": ([I32, pD, pD, pD], None),
         "oref_run": ([ctypes.c_char_p, ctypes.c_char_p, I32], P),
@@ -159,6 +160,22 @@ class OracleSystem:
         y = np.zeros_like(x)
         self.L.oref_sigma_direct(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
         return y
+
+    def rdm(self, bra, ket=None, spin=False):
+        """SHCIrdm::EvaluateOneRDM / EvaluateRDM over the appended determinants:
+        (g1, g2[P,Q,R,S]) and with spin the spin-orbital 1-RDM and the
+        pair-indexed spin 2-RDM."""
+        b = np.ascontiguousarray(bra, dtype=np.float64)
+        k = b if ket is None else np.ascontiguousarray(ket, dtype=np.float64)
+        n, nso = self.norb, 2 * self.norb
+        npair = nso * (nso + 1) // 2
+        g1, g2 = np.zeros((n, n)), np.zeros((n,) * 4)
+        s1 = np.zeros((nso, nso)) if spin else None
+        s2 = np.zeros((npair, npair)) if spin else None
+        self.L.oref_rdm(self.h, _p(b, ctypes.c_double), _p(k, ctypes.c_double), _p(g1, ctypes.c_double),
+                        _p(g2, ctypes.c_double), _p(s1, ctypes.c_double) if spin else None,
+                        _p(s2, ctypes.c_double) if spin else None)
+        return (g1, g2, s1, s2) if spin else (g1, g2)
 
     def add_pt_restriction(self, min_elec, max_elec, spatial_orbs):
         o = np.ascontiguousarray(spatial_orbs, dtype=np.int32)
