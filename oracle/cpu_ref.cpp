@@ -537,6 +537,141 @@ double hij(const System& s, const Det& bra, const Det& ket) {
 }
 
 // ---------------------------------------------------------------------------
+// time reversal (Determinants.h:245-301, Determinants.cpp:32-121,
+// SHCImakeHamiltonian.cpp:411-483)
+static const uint64_t kEvenBits = 0x5555555555555555ull, kOddBits = 0xAAAAAAAAAAAAAAAAull;
+static Det flipAlphaBeta(const Det& d, int W) {
+  Det f;
+  for (int i = 0; i < W; i++) {
+    const uint64_t r = (uint64_t)d.r[i];
+    f.r[i] = (int64_t)(((r & kEvenBits) << 1) + ((r & kOddBits) >> 1));
+  }
+  return f;
+}
+static bool hasUnpairedElectrons(const Det& d, int W) {
+  for (int i = W - 1; i >= 0; i--) {
+    const uint64_t r = (uint64_t)d.r[i];
+    if (((r & kEvenBits) << 1) != (r & kOddBits)) return true;
+  }
+  return false;
+}
+// the reference compares each word with its flip as unsigned 64-bit values,
+// from the highest word down
+static bool isStandard(const Det& d, int W) {
+  if (!hasUnpairedElectrons(d, W)) return true;
+  for (int i = W - 1; i >= 0; i--) {
+    const uint64_t r = (uint64_t)d.r[i], f = ((r & kEvenBits) << 1) + ((r & kOddBits) >> 1);
+    if (r < f) return false;
+    if (r > f) return true;
+  }
+  return true;
+}
+static void makeStandard(Det& d, int W) {
+  if (!isStandard(d, W)) d = flipAlphaBeta(d, W);
+}
+static double parityOfFlipAlphaBeta(const Det& d, int W) {
+  int numpaired = 0, nbeta = 0;
+  for (int i = 0; i < W; i++) {
+    const uint64_t r = (uint64_t)d.r[i];
+    numpaired += bitCount((int64_t)(((r & kEvenBits) << 1) & r));
+    nbeta += bitCount((int64_t)(r & kOddBits));
+  }
+  const double p = numpaired % 2 == 0 ? 1.0 : -1.0;
+  return nbeta % 2 == 0 ? p : -p;
+}
+static bool connectedDets(const Det& a, const Det& b, int W) {  // excitation distance <= 2
+  int nd = 0;
+  for (int i = 0; i < W; i++) nd += bitCount(a.r[i] ^ b.r[i]);
+  return nd <= 4;
+}
+
+// updateHijForTReversal(hij, dj, dk), Determinants.cpp:32-121: the element
+// between the time-reversal adapted functions of dj and dk (as the reference
+// writes it, without the Treversal sign, which enters only when the state is
+// unpacked)
+static void updateHijForTReversal(const System& s, double& h, const Det& dj, const Det& dk) {
+  const int W = s.W;
+  const bool uj = hasUnpairedElectrons(dj, W), uk = hasUnpairedElectrons(dk, W);
+  if (!uj && uk) {
+    const Det c = flipAlphaBeta(dk, W);
+    if (!connectedDets(c, dj, W)) return;
+    const double parity = parityOfFlipAlphaBeta(dk, W);
+    const double hc = hij(s, dj, c);
+    h = (h + parity * hc) / std::pow(2., 0.5);
+  } else if (uj && !uk) {
+    const Det c = flipAlphaBeta(dj, W);
+    if (!connectedDets(c, dk, W)) return;
+    const double parity = parityOfFlipAlphaBeta(dj, W);
+    const double hc = hij(s, c, dk);
+    h = (h + parity * hc) / std::pow(2., 0.5);
+  } else if (uj && uk) {
+    const Det ck = flipAlphaBeta(dk, W);
+    if (!connectedDets(ck, dj, W)) return;
+    const double parityk = parityOfFlipAlphaBeta(dk, W);
+    const double hc = hij(s, dj, ck);
+    h = h + parityk * hc;
+  }
+}
+
+// fixForTreversal (SHCImakeHamiltonian.cpp:411-483) on the element hij =
+// Hij(D_|J|, D_|I|) of a pair found through the helpers, where a negative
+// index stands for the flipped copy of an open-shell determinant
+static void fixForTreversal(const System& s, const std::vector<Det>& dets, int DetI, int DetJ, double& h) {
+  const int W = s.W;
+  if (DetI < 0) {
+    h = 0.0;
+    return;
+  }
+  const Det& dJ = dets[std::abs(DetJ) - 1];
+  const Det& dI = dets[std::abs(DetI) - 1];
+  const bool uJ = hasUnpairedElectrons(dJ, W), uI = hasUnpairedElectrons(dI, W);
+  if (!uJ && !uI) return;
+  if (!uJ || DetJ > 0) {
+    updateHijForTReversal(s, h, dJ, dI);
+    return;
+  }
+  if (connectedDets(dJ, dI, W)) {
+    h = 0.0;  // the pair is also found through D_J itself
+    return;
+  }
+  const Det fj = flipAlphaBeta(dJ, W);
+  double parity = parityOfFlipAlphaBeta(dJ, W);
+  h = parity * hij(s, fj, dI);
+  if (uI) {
+    const Det fi = flipAlphaBeta(dI, W);
+    parity = parityOfFlipAlphaBeta(dI, W);
+    h += parity * hij(s, dJ, fi);
+    h /= 2.0;
+  } else {
+    h /= std::pow(2.0, 0.5);
+  }
+}
+
+// unpackTrevState, SHCIbasics.cpp:730-776: every open-shell determinant's
+// flipped partner is appended (in determinant order) with coefficient
+// Treversal * parity * c / sqrt(2), and c becomes c / sqrt(2)
+static void unpackTrevState(std::vector<Det>& dets, std::vector<std::vector<double>>& ci, int W, int trev) {
+  const size_t oldLen = dets.size();
+  std::vector<Det> add;
+  std::vector<size_t> src;
+  for (size_t i = 0; i < oldLen; i++)
+    if (hasUnpairedElectrons(dets[i], W)) {
+      add.push_back(flipAlphaBeta(dets[i], W));
+      src.push_back(i);
+    }
+  dets.insert(dets.end(), add.begin(), add.end());
+  for (auto& c : ci) {
+    c.resize(dets.size());
+    for (size_t k = 0; k < src.size(); k++) {
+      const size_t i = src[k];
+      const double v = c[i];
+      c[i] = v / std::sqrt(2.0);
+      c[oldLen + k] = trev * parityOfFlipAlphaBeta(dets[i], W) * v / std::sqrt(2.0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // OccRestrictions.cpp
 static bool oneElecAllowed(const Restriction& r, int cur, int i, int a) {
   int elec = cur;
@@ -987,6 +1122,15 @@ void populateHelpers(const System& s, Helpers& h, const std::vector<Det>& dets, 
     h.AlphaMajorToDet[ia].push_back(i + 1);
     h.BetaMajorToAlpha[ib].push_back(ia);
     h.BetaMajorToDet[ib].push_back(i + 1);
+    if (s.trev != 0 && hasUnpairedElectrons(dets[i], s.W)) {
+      // the flipped copy, under index -(i+1) (SHCImakeHamiltonian.cpp:344-351)
+      int fb = findOrInsert(h.BetaN, da, h.BetaMajorToAlpha, h.BetaMajorToDet, h.SinglesFromBeta, s.norb);
+      int fa = findOrInsert(h.AlphaN, db, h.AlphaMajorToBeta, h.AlphaMajorToDet, h.SinglesFromAlpha, s.norb);
+      h.AlphaMajorToBeta[fa].push_back(fb);
+      h.AlphaMajorToDet[fa].push_back(-(i + 1));
+      h.BetaMajorToAlpha[fb].push_back(fa);
+      h.BetaMajorToDet[fb].push_back(-(i + 1));
+    }
   }
   auto sortMajor = [](std::vector<int>& key, std::vector<int>& det) {
     std::vector<std::pair<int, int>> p(key.size());
@@ -1024,12 +1168,21 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
   int end = (int)dets.size();
   for (int k = start; k < end; k++) {
     H.connections.push_back(std::vector<int>(1, k));
-    H.Helements.push_back(std::vector<double>(1, energy(s, dets[k])));
+    double e = energy(s, dets[k]);
+    if (s.trev != 0) updateHijForTReversal(s, e, dets[k], dets[k]);
+    H.Helements.push_back(std::vector<double>(1, e));
   }
+  // (with time reversal, DetJ < 0 is the flipped copy of determinant -DetJ)
   auto add = [&](int DetI, int DetJ) {
-    double v = hij(s, dets[DetJ - 1], dets[DetI - 1]);
-    H.connections[DetI - 1].push_back(DetJ - 1);
+    double v = hij(s, dets[std::abs(DetJ) - 1], dets[DetI - 1]);
+    if (s.trev != 0) fixForTreversal(s, dets, DetI, DetJ, v);
+    H.connections[DetI - 1].push_back(std::abs(DetJ) - 1);
     H.Helements[DetI - 1].push_back(v);
+  };
+  auto doubleApart = [&](int DetI, int DetJ) {
+    Det di = dets[DetI - 1];
+    if (DetJ < 0) di = flipAlphaBeta(di, s.W);
+    return excitationDistance(dets[std::abs(DetJ) - 1], di, s.W) == 2;
   };
   // each determinant belongs to one alpha string, so the rows written by
   // different alpha strings are disjoint
@@ -1038,7 +1191,7 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
   for (size_t i = 0; i < h.AlphaMajorToBeta.size(); i++) {
     for (size_t ii = 0; ii < h.AlphaMajorToBeta[i].size(); ii++) {
       int Astring = (int)i, Bstring = h.AlphaMajorToBeta[i][ii], DetI = h.AlphaMajorToDet[i][ii];
-      if (DetI - 1 < start) continue;
+      if (DetI < 0 || DetI - 1 < start) continue;
       const std::vector<int>& bToA = h.BetaMajorToAlpha[Bstring];
       const std::vector<int>& bToDet = h.BetaMajorToDet[Bstring];
       // singles from Astring
@@ -1046,7 +1199,7 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
         int index = binarySearch(bToA.data(), 0, (int)bToA.size() - 1, Asingle);
         if (index != -1) {
           int DetJ = bToDet[index];
-          if (DetJ >= DetI) continue;
+          if (std::abs(DetJ) >= DetI) continue;
           add(DetI, DetJ);
         }
       }
@@ -1063,7 +1216,7 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
           SearchStartIndex = index;
           if (index < AlphaToBetaLen && aToB[index] == Bsingle) {
             int DetJ = aToDet[SearchStartIndex];
-            if (DetJ >= DetI) continue;
+            if (std::abs(DetJ) >= DetI) continue;
             add(DetI, DetJ);
           }
         }
@@ -1075,21 +1228,21 @@ void makeHRows(const System& s, const Helpers& h, const std::vector<Det>& dets, 
         int index = binarySearch(aToB0.data(), 0, (int)aToB0.size() - 1, Bsingle);
         if (index != -1) {
           int DetJ = aToDet0[index];
-          if (DetJ >= DetI) continue;
+          if (std::abs(DetJ) >= DetI) continue;
           add(DetI, DetJ);
         }
       }
       // double beta excitation
       for (size_t j = 0; j < aToDet0.size(); j++) {
         int DetJ = aToDet0[j];
-        if (DetJ >= DetI) continue;
-        if (excitationDistance(dets[DetJ - 1], dets[DetI - 1], s.W) == 2) add(DetI, DetJ);
+        if (std::abs(DetJ) >= DetI) continue;
+        if (doubleApart(DetI, DetJ)) add(DetI, DetJ);
       }
       // double alpha excitation
       for (size_t j = 0; j < bToDet.size(); j++) {
         int DetJ = bToDet[j];
-        if (DetJ >= DetI) continue;
-        if (excitationDistance(dets[DetJ - 1], dets[DetI - 1], s.W) == 2) add(DetI, DetJ);
+        if (std::abs(DetJ) >= DetI) continue;
+        if (doubleApart(DetI, DetJ)) add(DetI, DetJ);
       }
     }
   }
@@ -1494,9 +1647,9 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     if (log) fprintf(log, "%s\n", err.c_str());
     return;
   }
-  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchForLowestEnergyDet || schd.doLCC) {
+  if (schd.davidsonType == 2 || schd.doSOC || schd.searchForLowestEnergyDet || schd.doLCC) {
     res.status = 2;
-    res.error = "keyword outside the restated variational path (Treversal/disk/soc/search/lcc)";
+    res.error = "keyword outside the restated variational path (disk/soc/search/lcc)";
     if (log) fprintf(log, "%s\n", res.error.c_str());
     return;
   }
@@ -1515,6 +1668,11 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     return;
   }
   buildHeatBath(s);
+  if (schd.occupied[0].size() % 2 != 0 && schd.Trev != 0) {  // SHCI.cpp:189-195
+    if (log) fprintf(log, "\nCannot use time reversal symmetry for odd electron system.\n");
+    schd.Trev = 0;
+  }
+  s.trev = schd.Trev;
   int W = s.W;
   auto less = [W](const Det& x, const Det& y) { return detLess(x, y, W); };
   auto eq = [W](const Det& x, const Det& y) { return detEq(x, y, W); };
@@ -1537,6 +1695,7 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       }
       setocc(Dets[d], o, true);
     }
+    if (s.trev != 0) makeStandard(Dets[d], W);
     for (size_t i = 0; i < d; i++)
       if (eq(Dets[d], Dets[i])) {
         res.status = 1;
@@ -1573,13 +1732,21 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
   SparseHam H;
   std::vector<double> diag;
   double e0 = energy(s, Dets[0]);
+  if (s.trev != 0) updateHijForTReversal(s, e0, Dets[0], Dets[0]);
   std::vector<double> E0(nroots, std::fabs(e0));
   populateHelpers(s, helpers, Dets, 0);
   makeHRows(s, helpers, Dets, 0, H);
   if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
   // directdavidson: the same product with every element evaluated on the fly
   const bool direct = schd.davidsonType == 1;
-  auto directOp = [&](const double* x, double* y) { hmultDirect(s, Dets, H, x, y, (size_t)DetsSize); };
+  // (with time reversal the elements are those of the stored rows, which
+  // HmultDirect recomputes with the same fixForTreversal)
+  auto directOp = [&](const double* x, double* y) {
+    if (s.trev != 0)
+      hmult(H, x, y, (size_t)DetsSize);
+    else
+      hmultDirect(s, Dets, H, x, y, (size_t)DetsSize);
+  };
   // checkpoint: the role of writeVariationalResult / readVariationalResult
   // (SHCIbasics.cpp:1481-1575) in the drop-in's own format: <prefix>/shci_var.bkp,
   // "SHCIVAR1", int32 W, nroots, iter, converged, int64 n, E0[nroots] (no
@@ -1648,6 +1815,7 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
         (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
       if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
       fromConverged = true;
+      if (s.trev != 0) unpackTrevState(Dets, ci, W, s.trev);
     }
   }
   for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
@@ -1672,8 +1840,17 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       }
       for (auto& f : found) newDets.insert(newDets.end(), f.begin(), f.end());
     }
+    if (s.trev != 0)  // SHCIbasics.cpp:984-996
+      for (Det& d : newDets) makeStandard(d, W);
     std::sort(newDets.begin(), newDets.end(), less);
     newDets.erase(std::unique(newDets.begin(), newDets.end(), eq), newDets.end());
+    if (s.trev != 0)
+      newDets.erase(std::remove_if(newDets.begin(), newDets.end(),
+                                   [&](const Det& d) {
+                                     return std::binary_search(SortedDets.begin(), SortedDets.begin() + SortedDetsSize,
+                                                               d, less);
+                                   }),
+                    newDets.end());
     std::vector<std::vector<double>> X0(nroots, std::vector<double>(DetsSize + newDets.size(), 0.0));
     for (int i = 0; i < nroots; i++) std::copy(ci[i].begin(), ci[i].end(), X0[i].begin());
     Dets.insert(Dets.end(), newDets.begin(), newDets.end());
@@ -1734,6 +1911,7 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
       }
       if (log) fprintf(log, "Exiting variational iterations\n");
       saveCheckpoint(iter);
+      if (s.trev != 0) unpackTrevState(Dets, ci, W, s.trev);
       break;
     }
     saveCheckpoint(iter);
@@ -1747,6 +1925,7 @@ void runVariational(const std::string& inputPath, RunResult& res, FILE* log, boo
     if (log && !fromConverged) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
   }
   if (log && !fromConverged) fprintf(log, "\n");
+  DetsSize = (int)Dets.size();  // (unpacked with time reversal)
   res.energies = E0;
   res.dets = Dets;
   res.ci = ci;

@@ -87,6 +87,7 @@ struct System {
   int W = 1;     // 64-bit words per determinant (EffDetLen)
   int nelec = 0;
   double coreE = 0.0;
+  int trev = 0;  // Treversal (Determinant::Trev): the basis is time-reversal adapted
   std::vector<int> irrep;
   std::vector<double> I1;  // nso x nso, row-major
   std::vector<double> I2;  // packed pair-of-pairs triangle

@@ -41,6 +41,9 @@ ENERGIES = {
     "o2_det_var_plus_pt": "tests/SHCI/o2_det/trusted_hci.e",
     "o2_det_direct": "tests/SHCI/o2_det_direct/trusted_hci.e",
     "fullrestart": "tests/SHCI/fullrestart/trusted_hci.e",
+    "restart_trev": "tests/SHCI/restart_trev/trusted_hci.e",
+    "o2_det_trev": "tests/SHCI/o2_det_trev/trusted_hci.e",
+    "o2_det_trev_direct": "tests/SHCI/o2_det_trev_direct/trusted_hci.e",
 }
 
 
@@ -70,7 +73,10 @@ def main():
     gold["h2co_rdm_input"] = open(os.path.join(REF, "tests/SHCI/h2co_rdm/test_input.dat")).read()
     gold["o2_stoc_input"] = open(os.path.join(REF, "tests/SHCI/o2_stoc/input.dat")).read()
     gold["spin1rdm_input"] = open(os.path.join(REF, "tests/SHCI/spin1rdm/input.dat")).read()
-    for name in ("restart/input2", "fullrestart/input2", "fullrestart/input3"):
+    # (tests/SHCI/restart_trev and the o2 time-reversal tests use the restart
+    # and o2_det FCIDUMPs, byte for byte)
+    for name in ("restart/input2", "fullrestart/input2", "fullrestart/input3", "restart_trev/input2",
+                 "restart_trev/input3", "o2_det_trev/input", "o2_det_trev_direct/input"):
         gold[name.replace("/", "_")] = open(os.path.join(REF, "tests/SHCI/%s.dat" % name)).read()
     gold["n2_RAS_input"] = open(os.path.join(REF, "tests/SHCI/n2_RAS/input.dat")).read()
     with open(os.path.join(HERE, "golden.json"), "w") as f:

@@ -753,3 +753,101 @@ def test_rdm_oracle_matches_o2_trusted_2rdm(tmp_path, golden_dir):
     ref = rdm.read_dice_2rdm(os.path.join(golden_dir, "o2_stoc", "trusted2RDM.txt"))
     assert np.linalg.norm(g2 - ref) / g2.size < 1e-8
     assert np.max(np.abs(g2 - ref)) < 1e-7
+
+
+# ---- time reversal (Treversal; Determinants.cpp:32-121, SHCImakeHamiltonian.cpp:344-483,
+#      SHCIbasics.cpp:730-776,984-996) ----
+
+def _reference_input(g, name, fcidump, prefix):
+    text = "\n".join(l for l in g[name].splitlines()
+                     if not l.strip().lower().startswith(("prefix", "orbitals")))
+    return text + "\norbitals %s\nprefix %s\n" % (fcidump, prefix)
+
+
+@pytest.mark.parametrize("case", ["o2_det_trev", "o2_det_trev_direct"])
+def test_trev_o2_trusted_energy(case, tmp_path, golden_dir):
+    """tests/SHCI/o2_det_trev and o2_det_trev_direct as the reference runs
+    them (Treversal 1, deterministic PT, the second with directdavidson):
+    E_var + E_PT reproduces trusted_hci.e (1e-9 and 2e-9 Ha); the packed
+    space holds 70 standard determinants, which unpack to 112."""
+    g = _golden(golden_dir)
+    _write(tmp_path / "in.dat", _reference_input(g, case + "_input", os.path.join(golden_dir, "o2", "FCIDUMP"),
+                                                 tmp_path))
+    r = OracleRun(str(tmp_path / "in.dat"), str(tmp_path / "log"))
+    tol = 1e-9 if case == "o2_det_trev" else 2e-9
+    assert abs(r.pt_energies()[0] - g["trusted_energy"][case]) < tol
+    if case == "o2_det_trev":
+        assert r.iterations()[-1]["ndets"] == 70 and len(r.dets(1)) == 112
+
+
+@pytest.mark.slow
+def test_trev_restart_chain_trusted(tmp_path, golden_dir):
+    """tests/SHCI/restart_trev (N2, Treversal 1): input2 writes the
+    checkpoint of the packed space, input3 restarts and continues to eps1
+    1e-4; shci.e reproduces trusted_hci.e to 1e-9 Ha.  The same chain
+    without Treversal ends 3.8e-4 Ha higher (the packed coefficients of
+    open-shell determinants are sqrt(2) larger in the screening), so this
+    pins the time-reversal path, not only the energy."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "restart", "FCIDUMP")
+    for k, name in enumerate(("restart_trev_input2", "restart_trev_input3")):
+        _write(tmp_path / ("in%d.dat" % k), _reference_input(g, name, fcidump, tmp_path))
+        OracleRun(str(tmp_path / ("in%d.dat" % k)), str(tmp_path / ("log%d" % k)))
+    with open(tmp_path / "shci.e", "rb") as f:
+        e = struct.unpack("<d", f.read(8))[0]
+    assert abs(e - g["trusted_energy"]["restart_trev"]) < 1e-9, e
+
+
+def test_trev_sector_eigenvalues(tmp_path, golden_dir):
+    """H2H2 from HF at eps1 1e-10 (its whole symmetry block, 32
+    determinants) with Treversal 1 and four roots: the four energies are the
+    lowest eigenvalues of the dense H of the unpacked space restricted to the
+    +1 eigenspace of the spin flip F|D> = p_D |flip(D)> (1e-9), the plain run
+    finds a root of the -1 eigenspace that the time-reversal run skips, and
+    the unpacked vectors satisfy F c = c (1e-12)."""
+    g = _golden(golden_dir)
+    fcidump = os.path.join(golden_dir, "h2h2", "FCIDUMP")
+    base = "\n".join(l for l in g["h2he_rdm_input"].splitlines()
+                     if not l.strip().lower().startswith(("orbitals", "nroots", "dothreerdm", "dordm",
+                                                           "doonerdm", "dospinonerdm")))
+    runs = {}
+    for trev in (0, 1):
+        _write(tmp_path / ("in%d.dat" % trev), base + "\norbitals %s\nprefix %s\nnroots 4\n%s"
+               % (fcidump, tmp_path, "Treversal 1\n" if trev else ""))
+        runs[trev] = OracleRun(str(tmp_path / ("in%d.dat" % trev)), str(tmp_path / ("log%d" % trev)))
+    o = OracleSystem(fcidump)
+    d = runs[1].dets(o.words)
+    ints = [int(x[0]) & ((1 << 64) - 1) for x in d]
+    index = {v: k for k, v in enumerate(ints)}
+    n = len(d)
+
+    def flip(x):
+        return ((x & 0x5555555555555555) << 1) | ((x & 0xAAAAAAAAAAAAAAAA) >> 1)
+
+    def parity(x):  # sign of the spin flip: (-1)^(doubly occupied) (-1)^(beta electrons)
+        p = (-1) ** bin(((x & 0x5555555555555555) << 1) & x).count("1")
+        return p * (-1) ** bin(x & 0xAAAAAAAAAAAAAAAA).count("1")
+
+    F = np.zeros((n, n))
+    for k, x in enumerate(ints):
+        F[index[flip(x)], k] = parity(x)
+    o.csr_append(sort_dets(d))
+    sd = sort_dets(d)
+    perm = [index[int(x[0]) & ((1 << 64) - 1)] for x in sd]
+    rp, cols, vals = o.csr()
+    H = np.zeros((n, n))
+    for i in range(n):
+        for k in range(rp[i], rp[i + 1]):
+            a, b = perm[i], perm[cols[k]]
+            H[a, b] = H[b, a] = vals[k]
+    assert np.max(np.abs(F @ F - np.eye(n))) == 0 and np.max(np.abs(H @ F - F @ H)) < 1e-13
+    w, U = np.linalg.eigh((np.eye(n) + F) / 2)
+    B = U[:, w > 0.5]
+    sector = np.linalg.eigvalsh(B.T @ H @ B)
+    core = o.L.oref_core_energy(o.h)
+    assert np.max(np.abs(runs[1].energies() - core - sector[:4])) < 1e-9
+    minus = np.linalg.eigvalsh(H)
+    assert any(abs(e - core - m) < 1e-9 for e in runs[0].energies() for m in minus if
+               all(abs(m - x) > 1e-6 for x in sector))
+    c = runs[1].ci()
+    assert np.max(np.abs(F @ c - c)) < 1e-12
