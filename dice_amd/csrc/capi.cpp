@@ -318,10 +318,10 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", err.c_str());
     return fail(h, SHCI_BAD_INPUT, err);
   }
-  if (schd.Trev != 0 || schd.davidsonType == 2 || schd.doSOC || schd.searchLowest || schd.doLCC ||
-      schd.doResponse || schd.doHigherRDM) {
+  if (schd.davidsonType == 2 || schd.doSOC || schd.searchLowest || schd.doLCC || schd.doResponse ||
+      schd.doHigherRDM) {
     std::string m =
-        "keyword outside the variational path implemented here (Treversal, diskdavidson, dosoc, "
+        "keyword outside the variational path implemented here (diskdavidson, dosoc, "
         "searchForLowestEnergyDet, dolcc, doresponse, DoThreeRDM, DoFourRDM)";
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
@@ -333,8 +333,20 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
   }
+  if (schd.occupied[0].size() % 2 != 0 && schd.Trev != 0) {  // SHCI/SHCI.cpp:189-195
+    if (log) fprintf(log, "\nCannot use time reversal symmetry for odd electron system.\n");
+    schd.Trev = 0;
+  }
+  if (schd.Trev != 0 && (opt.davidson_block || opt.learned)) {
+    std::string m = "Treversal runs with the reference's Davidson and screening only (no block Davidson or "
+                    "learned screening)";
+    if (log) fprintf(log, "%s\n", m.c_str());
+    return fail(h, SHCI_UNSUPPORTED, m);
+  }
   std::srand(schd.randomSeed + 0);
   E.load(schd.integralFile);
+  E.trev = schd.Trev;
+  E.S.trev = schd.Trev != 0;
   E.precision = precision;
   // directdavidson: no stored Hamiltonian (HmultDirect) and davidsonDirect
   // (SHCI/SHCIbasics.cpp:1092,1167-1172,1206-1208)
@@ -372,6 +384,12 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
         return fail(h, SHCI_BAD_INPUT, m);
       }
       hdets[d * W + o / 64] |= (int64_t)(1ull << (o % 64));
+    }
+    if (schd.Trev != 0) {  // SHCI/SHCI.cpp:302
+      if (W == 1)
+        makeStandardForm<1>(reinterpret_cast<uint64_t*>(&hdets[d * W]));
+      else
+        makeStandardForm<2>(reinterpret_cast<uint64_t*>(&hdets[d * W]));
     }
     for (int64_t i = 0; i < d; i++)
       if (std::equal(&hdets[d * W], &hdets[d * W] + W, &hdets[i * W])) {
@@ -462,6 +480,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
         (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
       if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
       fromConverged = true;
+      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:939
     }
   }
   for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
@@ -543,6 +562,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
       if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
       if (log) fprintf(log, "Exiting variational iterations\n");
       if (schd.io) saveCheckpoint(iter);
+      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:1229-1234
       rdmStep(E, schd, log);
       break;
     }

@@ -28,7 +28,78 @@ Engine::~Engine() {
     (void)hipEventDestroy(e.start);
     (void)hipEventDestroy(e.stop);
   }
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream && ownsStream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::initShared(const Engine& parent) {
+  hs = parent.hs;
+  S = parent.S;
+  S.trev = 0;
+  stream = parent.stream;
+  ownsStream = false;
+  precision = parent.precision;
+  relabel = parent.relabel;
+  incremental = false;
+  counter.reserve(64);
+}
+
+void Engine::setDetsDevice(const uint64_t* d, int64_t cnt) {
+  const int W = hs.W;
+  dets.reserve((size_t)std::max<int64_t>(cnt, 1) * W * 8, stream);
+  if (cnt) HIP_CHECK(hipMemcpyAsync(dets.p, d, cnt * W * 8, hipMemcpyDeviceToDevice, stream));
+  n = cnt;
+  slots = 0;
+  ensureTable(std::max<int64_t>(n, 512));
+  nDiag = 0;
+  hamRows = -1;
+  permRows = -1;
+  nNew = 0;
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::buildTrev() {
+  const int W = hs.W;
+  const int64_t nu = launchTrevBasis(dets.as<uint64_t>(), n, W, trevFlags, nullptr, nullptr, nullptr, ws, stream);
+  const int64_t nE = n + nu;
+  DevMem detsE;
+  detsE.reserve((size_t)std::max<int64_t>(nE, 1) * W * 8, stream);
+  trevPartner.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
+  trevParity.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
+  launchTrevBasis(dets.as<uint64_t>(), n, W, trevFlags, detsE.as<uint64_t>(), trevPartner.as<int64_t>(),
+                  trevParity.as<double>(), ws, stream);
+  if (!tE) {
+    tE.reset(new Engine());
+    tE->initShared(*this);
+  }
+  tE->precision = precision;
+  tE->setDetsDevice(detsE.as<uint64_t>(), nE);
+  tE->buildHamiltonian();
+  trevN = n;
+}
+
+void Engine::sigmaTrev(const double* dx, double* dy, int k, int64_t ld, bool exact) {
+  const int64_t nE = tE->n;
+  trevX.reserve((size_t)std::max<int64_t>(nE * k, 1) * 8, stream);
+  trevY.reserve((size_t)std::max<int64_t>(nE * k, 1) * 8, stream);
+  launchTrevExpand(dx, n, k, ld, trevPartner.as<int64_t>(), trevParity.as<double>(), trevX.as<double>(), nE, stream);
+  tE->sigma(trevX.as<double>(), trevY.as<double>(), k, nE, exact);
+  launchTrevContract(trevY.as<double>(), nE, n, k, trevPartner.as<int64_t>(), trevParity.as<double>(), dy, ld,
+                     stream);
+}
+
+void Engine::unpackTrev(int sign) {
+  if (trevN != n) buildTrev();
+  const int64_t nE = tE->n;
+  DevMem out;
+  out.reserve((size_t)std::max<int64_t>(nE * nroots, 1) * 8, stream);
+  launchTrevUnpack(ci.as<double>(), n, nroots, n, trevPartner.as<int64_t>(), trevParity.as<double>(), sign,
+                   out.as<double>(), nE, stream);
+  ci.swap(out);
+  trev = 0;
+  S.trev = 0;
+  setDetsDevice(tE->dets.as<uint64_t>(), nE);
+  tE.reset();
+  trevN = -1;
 }
 
 void Engine::load(const std::string& fcidump) {
@@ -130,7 +201,7 @@ double Engine::ptDeterministic(const double* dCi, double e0, double eps2, double
     const int64_t total = countPass(1, 0);
     // keys and their sort scratch, numerator, 32-bit payload and its scratch
     const size_t perCand = (size_t)16 * W + 16;
-    const size_t budget = std::min<size_t>(deviceFreeBytes() / 3, size_t(32) << 30);
+    const size_t budget = std::min<size_t>(deviceFreeBytes() / 3, size_t(96) << 30);
     const int64_t maxPer = std::max<int64_t>(1 << 20, std::min<int64_t>((int64_t)(budget / perCand), INT32_MAX));
     nparts = (uint32_t)std::max<int64_t>(1, (total + maxPer - 1) / maxPer);
     // SHCI_PT_PARTITIONS: at least this many partitions
@@ -453,6 +524,11 @@ void Engine::buildHamiltonian() {
   if (nDiag < n) launchEnergy(S, dets.as<uint64_t>() + nDiag * hs.W, n - nDiag, diag.as<double>() + nDiag, stream);
   nDiag = n;
   if (hamRows == n) return;
+  if (trev) {  // the adapted basis (also in direct mode: the E Hamiltonian is stored)
+    buildTrev();
+    hamRows = n;
+    return;
+  }
   if (direct) {  // HmultDirect: sigma evaluates H from the string tables
     buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
     for (DevMem* m : {&rowPtr, &cols, &vals, &vals32, &valsLo, &rowPtrNew, &colsNew, &valsNew, &vals32New,
@@ -890,6 +966,11 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     ev->bytes = (double)nnzOff * ((f32 ? (exact ? 8 : 4) : 8) + 4) + (double)(n + 1) * 8 + (double)n * 8 +
                 2.0 * k * n * 8;
     HIP_CHECK(hipEventRecord(ev->start, stream));
+  }
+  if (trev) {
+    sigmaTrev(dx, dy, k, ld, exact);
+    if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+    return;
   }
   if (direct) {
     launchSigmaDirect(S, dets.as<uint64_t>(), T, diag.as<double>(), dx, dy, k, ld, stream);

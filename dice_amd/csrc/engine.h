@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -160,6 +161,23 @@ struct Engine {
   void rdm(const double* dBra, const double* dKet, std::vector<double>& g1, std::vector<double>& g2,
            std::vector<double>* s1 = nullptr, std::vector<double>* s2 = nullptr);
 
+  // time reversal (Treversal, SHCI/Determinants.cpp:32-121,
+  // SHCI/SHCImakeHamiltonian.cpp:344-483): `dets` holds standard forms and
+  // the Davidson works in the adapted basis; sigma = U^T H_E U with H_E the
+  // plain Hamiltonian of E = dets + the flipped copies of the open-shell ones,
+  // held by the sub-engine tE (same system tables and stream).  The
+  // preconditioner keeps the plain diagonal energies, as the reference's.
+  int trev = 0;
+  std::unique_ptr<Engine> tE;
+  DevMem trevFlags, trevPartner, trevParity, trevX, trevY;
+  int64_t trevN = -1;  // n the adapted basis was built for
+  void buildTrev();
+  void sigmaTrev(const double* dx, double* dy, int k, int64_t ld, bool exact);
+  // unpackTrevState (SHCI/SHCIbasics.cpp:730-776): dets and ci become the
+  // plain ones (Treversal sign sign); time reversal is off afterwards
+  void unpackTrev(int sign);
+  void setDetsDevice(const uint64_t* d, int64_t cnt);
+
   // last run
   std::vector<IterRecord> iters;
   std::vector<double> energies;
@@ -183,6 +201,9 @@ struct Engine {
 
   explicit Engine();
   ~Engine();
+  bool ownsStream = true;
+  // an engine on the system tables and stream of parent (owned by parent)
+  void initShared(const Engine& parent);
   void load(const std::string& fcidump);  // throws
   void uploadRestrictions();
   void setDets(const int64_t* h, int64_t cnt);

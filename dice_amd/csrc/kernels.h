@@ -49,6 +49,20 @@ void launchPtGenStoch(const DevSystem& S, const uint64_t* dets, const uint32_t* 
 void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num,
                          const double* num2, const uint8_t* pres, int64_t m, double e0, double nmc, DevMem& partial,
                          double* out, hipStream_t s);
+// time reversal: flags + scan of the open-shell standard determinants; with
+// detsE, the basis E (n + returned count determinants), each determinant's
+// partner index in E (-1 when closed-shell) and flip parity.  Returns the
+// number of open-shell determinants (synchronises).
+int64_t launchTrevBasis(const uint64_t* dets, int64_t n, int W, DevMem& flags, uint64_t* detsE, int64_t* partner,
+                        double* parity, Workspace& ws, hipStream_t s);
+// xE = U x and y = U^T yE for k columns (U: adapted basis -> E)
+void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner, const double* parity,
+                      double* xE, int64_t ldE, hipStream_t s);
+void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner, const double* parity,
+                        double* y, int64_t ld, hipStream_t s);
+// the unpacked coefficients in E (Treversal sign trev)
+void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
+                      const double* parity, int trev, double* out, int64_t ldE, hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial

@@ -113,9 +113,14 @@ __global__ void kCmax(const double* ci, int nroots, int64_t ld, int64_t n, doubl
 // emit one candidate per lane with `want`, skipping determinants already in
 // the variational space; wave-aggregated append
 template <int W>
-__device__ __forceinline__ void emitCandidate(bool want, const uint64_t* cand, const int32_t* table, uint64_t mask,
+__device__ __forceinline__ void emitCandidate(bool want, const uint64_t* candIn, const int32_t* table, uint64_t mask,
                                               const uint64_t* dets, uint64_t* out, unsigned long long* counter,
-                                              int64_t cap) {
+                                              int64_t cap, bool trev = false) {
+  // with time reversal the space holds standard forms (SHCI/SHCIbasics.cpp:984-996)
+  uint64_t cand[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) cand[w] = candIn[w];
+  if (trev) makeStandardForm<W>(cand);
   bool keep = want && !hashContains<W>(table, mask, dets, cand);
   uint64_t m = __ballot(keep);
   if (m == 0) return;
@@ -216,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
           }
         }
       }
-      emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
+      emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap, S.trev);
     }
     // ---- double excitations
     if (MODE == 1) continue;
@@ -315,7 +320,7 @@ __global__ void __launch_bounds__(kBlock) kScreen(DevSystem S, const uint64_t* d
         if (MODE == 2)
           emitRecord(want, recTmp, rec, counter, cap);
         else
-          emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap);
+          emitCandidate<W>(want, cand, table, mask, dets, out, counter, cap, S.trev);
       }
       __syncwarp();
     }
@@ -820,6 +825,93 @@ __global__ void kRdmFinish(RdmOut o) {
 }
 
 // ---------------------------------------------------------------------------
+// time reversal: the adapted basis of the standard determinants S and the
+// plain basis E = S followed by the flipped copies of the open-shell ones
+// (in determinant order).  Basis function of I: |I> when I is closed-shell,
+// else (|I> + p_I |flip I>) / sqrt 2, p_I = the flip parity (the Hamiltonian
+// of SHCI/Determinants.cpp:32-121 and SHCI/SHCImakeHamiltonian.cpp:411-483,
+// which leaves out the Treversal sign).
+
+template <int W>
+__global__ void kTrevMark(const uint64_t* dets, int64_t n, int64_t* flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = dets[i * W + w];
+    flag[i] = hasUnpaired<W>(d) ? 1 : 0;
+  }
+}
+
+// after an exclusive scan of the flags: E list, partner index (-1 for
+// closed-shell) and flip parity per standard determinant
+template <int W>
+__global__ void kTrevBuild(const uint64_t* dets, int64_t n, const int64_t* pos, uint64_t* detsE, int64_t* partner,
+                           double* parity) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      d[w] = dets[i * W + w];
+      detsE[i * W + w] = d[w];
+    }
+    const bool u = pos[i + 1] != pos[i];
+    parity[i] = flipParity<W>(d);
+    partner[i] = u ? n + pos[i] : -1;
+    if (u) {
+#pragma unroll
+      for (int w = 0; w < W; w++) detsE[(n + pos[i]) * W + w] = flipWord(d[w]);
+    }
+  }
+}
+
+// xE = U x (k columns, ld / ldE)
+__global__ void kTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner,
+                            const double* parity, double* xE, int64_t ldE) {
+  const double r2 = 1.0 / sqrt(2.0);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * k; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, c = t / n;
+    const double v = x[c * ld + i];
+    const int64_t q = partner[i];
+    if (q < 0) {
+      xE[c * ldE + i] = v;
+    } else {
+      xE[c * ldE + i] = v * r2;
+      xE[c * ldE + q] = parity[i] * v * r2;
+    }
+  }
+}
+
+// y = U^T yE
+__global__ void kTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner,
+                              const double* parity, double* y, int64_t ld) {
+  const double r2 = 1.0 / sqrt(2.0);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * k; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, c = t / n;
+    const int64_t q = partner[i];
+    y[c * ld + i] = q < 0 ? yE[c * ldE + i] : (yE[c * ldE + i] + parity[i] * yE[c * ldE + q]) * r2;
+  }
+}
+
+// unpackTrevState (SHCI/SHCIbasics.cpp:730-776): c_I / sqrt 2 and
+// Treversal p_I c_I / sqrt 2 for the partner, per root (ld n -> ldE)
+__global__ void kTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
+                            const double* parity, int trev, double* out, int64_t ldE) {
+  const double s2 = sqrt(2.0);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * nroots;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, c = t / n;
+    const double v = ci[c * ld + i];
+    const int64_t q = partner[i];
+    if (q < 0) {
+      out[c * ldE + i] = v;
+    } else {
+      out[c * ldE + i] = v / s2;
+      out[c * ldE + q] = trev * parity[i] * v / s2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // learned screening: features, targets and the exact test of ranked records
 
 // X[r * din + f]: f < nso occupation bit f of the source determinant; then
@@ -887,7 +979,7 @@ __global__ void __launch_bounds__(kBlock) kCandExact(const CandRecord* rec, int6
       atomicAdd(&counts[0], (unsigned long long)__popcll(ms));
       atomicAdd(&counts[1], (unsigned long long)__popcll(mp));
     }
-    emitCandidate<W>(pass, cand, table, mask, dets, out, counter, cap);
+    emitCandidate<W>(pass, cand, table, mask, dets, out, counter, cap);  // (no time reversal in this form)
   }
 }
 
@@ -2137,6 +2229,42 @@ void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const i
   }
   kRdmFinish<<<gridFor(n3 * nn), kBlock, 0, s>>>(o);
   kRdmOccScatter<<<gridFor((int64_t)o.nso * o.nso), kBlock, 0, s>>>(o, M, NSO);
+  HIP_CHECK(hipGetLastError());
+}
+
+int64_t launchTrevBasis(const uint64_t* dets, int64_t n, int W, DevMem& flags, uint64_t* detsE, int64_t* partner,
+                        double* parity, Workspace& ws, hipStream_t s) {
+  flags.reserve((size_t)(n + 1) * 8, s);
+  int64_t* f = flags.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(f + n, 0, 8, s));
+  DISPATCH_W(W, kTrevMark<WW><<<gridFor(n), kBlock, 0, s>>>(dets, n, f));
+  exclusiveScanI64(f, n + 1, ws, s);
+  int64_t nu = 0;
+  HIP_CHECK(hipMemcpyAsync(&nu, f + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (detsE) DISPATCH_W(W, kTrevBuild<WW><<<gridFor(n), kBlock, 0, s>>>(dets, n, f, detsE, partner, parity));
+  HIP_CHECK(hipGetLastError());
+  return nu;
+}
+
+void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner, const double* parity,
+                      double* xE, int64_t ldE, hipStream_t s) {
+  if (n <= 0) return;
+  kTrevExpand<<<gridFor(n * k), kBlock, 0, s>>>(x, n, k, ld, partner, parity, xE, ldE);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner, const double* parity,
+                        double* y, int64_t ld, hipStream_t s) {
+  if (n <= 0) return;
+  kTrevContract<<<gridFor(n * k), kBlock, 0, s>>>(yE, ldE, n, k, partner, parity, y, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
+                      const double* parity, int trev, double* out, int64_t ldE, hipStream_t s) {
+  if (n <= 0) return;
+  kTrevUnpack<<<gridFor(n * nroots), kBlock, 0, s>>>(ci, n, nroots, ld, partner, parity, trev, out, ldE);
   HIP_CHECK(hipGetLastError());
 }
 

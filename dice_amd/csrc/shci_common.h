@@ -36,7 +36,59 @@ struct DevSystem {
   int nrestrict;
   int rmin[SHCI_MAX_RESTRICT], rmax[SHCI_MAX_RESTRICT];
   const int8_t* rmult;  // nrestrict x nso
+  // time reversal (Treversal): the variational space holds standard forms
+  int trev;
 };
+
+// ---------------------------------------------------------------------------
+// time reversal (SHCI/Determinants.h:245-301): spin flip swaps the alpha and
+// beta bit of every spatial orbital
+
+__host__ __device__ __forceinline__ uint64_t flipWord(uint64_t r) {
+  return ((r & 0x5555555555555555ull) << 1) + ((r & 0xAAAAAAAAAAAAAAAAull) >> 1);
+}
+
+template <int W>
+__host__ __device__ __forceinline__ bool hasUnpaired(const uint64_t* d) {
+  bool u = false;
+#pragma unroll
+  for (int w = 0; w < W; w++) u |= ((d[w] & 0x5555555555555555ull) << 1) != (d[w] & 0xAAAAAAAAAAAAAAAAull);
+  return u;
+}
+
+// standard form: of a determinant and its flip, the one that is larger as
+// unsigned words compared from the highest word down (Determinant::isStandard)
+template <int W>
+__host__ __device__ __forceinline__ void makeStandardForm(uint64_t* d) {
+  if (!hasUnpaired<W>(d)) return;
+  bool flip = false;
+#pragma unroll
+  for (int w = W - 1; w >= 0; w--) {
+    const uint64_t f = flipWord(d[w]);
+    if (d[w] != f) {
+      flip = d[w] < f;
+      break;
+    }
+  }
+  if (flip) {
+#pragma unroll
+    for (int w = 0; w < W; w++) d[w] = flipWord(d[w]);
+  }
+}
+
+// Determinant::parityOfFlipAlphaBeta: (-1)^(doubly occupied orbitals) times
+// (-1)^(beta electrons)
+template <int W>
+__host__ __device__ __forceinline__ double flipParity(const uint64_t* d) {
+  int paired = 0, beta = 0;
+#pragma unroll
+  for (int w = 0; w < W; w++) {
+    paired += __builtin_popcountll(((d[w] & 0x5555555555555555ull) << 1) & d[w]);
+    beta += __builtin_popcountll(d[w] & 0xAAAAAAAAAAAAAAAAull);
+  }
+  const double p = (paired & 1) ? -1.0 : 1.0;
+  return (beta & 1) ? -p : p;
+}
 
 // ---------------------------------------------------------------------------
 // bit helpers
