@@ -56,7 +56,9 @@ def test_nocc_count_mismatch(tmp_path):
 
 
 def test_outside_path_keywords_refused(tmp_path):
-    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\nTreversal 1\n", 5,
+    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\ndolcc\n", 5,
+                "outside the variational path")
+    _run_expect(tmp_path, "nocc 2\n0 1\nend\nschedule\n0 1e-3\nend\nmaxiter 2\nDoThreeRDM\n", 5,
                 "outside the variational path")
 
 
