@@ -1490,10 +1490,14 @@ __global__ void __launch_bounds__(kBlock) kPermRows(const int64_t* rowPtr, const
 // x[r * ld + perm[m]] for r < k; K > 1 (SpMM): interleaved, xP[m * K + r]
 // (zero for k <= r < K).  Also zeroes the work counter of a persistent sigma
 // launch that follows on the same stream.
+// the persistent sigma's work counters: 8 words kSchedStride apart (the
+// counter buffer holds 64 words)
+constexpr int kSchedStride = 8;
+
 template <typename XT, int K>
 __global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, int64_t n, XT* xP,
                             unsigned* counter) {
-  if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+  if (counter && blockIdx.x == 0 && threadIdx.x < 8) counter[kSchedStride * threadIdx.x] = 0u;
   for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
     const int64_t I = perm[m];
     if (K == 1) {
@@ -1522,13 +1526,32 @@ struct ChunkSched {
       }
       return false;
     }
-    __shared__ unsigned sNext;
+    // XCD-aware: workgroups are dispatched round-robin over the 8 XCDs, so
+    // blockIdx mod 8 names the XCD.  Each XCD takes the chunks of its own
+    // eighth of the rows first (consecutive alpha groups, whose x gathers
+    // then share its L2), then helps the next XCDs; one counter per eighth
+    // (kSchedCounters words apart, cleared by the gather kernel).  Results
+    // do not depend on which workgroup takes a chunk.
+    __shared__ int64_t sCur;
+    const int x = blockIdx.x & 7;
     __syncthreads();
-    if (threadIdx.x == 0) sNext = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+      sCur = -1;
+      for (; off < 8; off++) {
+        const int t = (x + off) & 7;
+        const int64_t lo = nChunks * t / 8, hi = nChunks * (t + 1) / 8;
+        const int64_t c = lo + (int64_t)atomicAdd(counter + kSchedStride * t, 1u);
+        if (c < hi) {
+          sCur = c;
+          break;
+        }
+      }
+    }
     __syncthreads();
-    cur = sNext;
-    return cur < nChunks;
+    cur = sCur;
+    return cur >= 0;
   }
+  int off = 0;  // (thread 0) eighths already exhausted
 };
 
 // y[r*ld + perm[m]] = diag .* x + sum over row m of the copy (gathers from
