@@ -103,3 +103,23 @@ def test_run_trev_roots_and_unpacked_state(tmp_path):
     log = open(tmp_path / "gpu" / "log").read()
     es = [float(b.split()[0]) for b in log.split("E from 2RDM:")[1:]]
     assert len(es) == 4 and np.max(np.abs(np.array(es) - r["gpu"].energies())) < 1e-8
+
+
+def test_run_trev_fp32_form_and_refusals(tmp_path):
+    """The mixed-precision form with time reversal ends at the FP64 run's
+    energy (1e-9 Ha: the final Davidson refines with hi + lo values) on the
+    same determinants; the block Davidson and the learned screening are
+    refused with Treversal."""
+    from dice_amd import shci
+    g = _gold()
+    fcidump = os.path.join(GOLD, "o2", "FCIDUMP")
+    text = _input(g["o2_det_trev_input"], fcidump, tmp_path, drop=("deterministic",), extra="nPTiter 0\n")
+    p = tmp_path / "in.dat"
+    p.write_text(text)
+    r64 = shci.do_variational(str(p), str(tmp_path / "log64"))
+    r32 = shci.do_variational(str(p), str(tmp_path / "log32"), precision=shci.PREC_FP32)
+    assert abs(r32.energies()[0] - r64.energies()[0]) < 1e-9
+    np.testing.assert_array_equal(r32.dets(), r64.dets())
+    for kw in (dict(davidson_block=True), dict(learned=True)):
+        with pytest.raises(shci.ShciError):
+            shci.do_variational(str(p), str(tmp_path / "logx"), **kw)
