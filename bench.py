@@ -240,7 +240,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the variational step on a single MI355X "
-              "(see the sharding notes in DESIGN.md); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+              "(the row-block sharding over RCCL is a design, not built); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from dice_amd import shci
