@@ -57,22 +57,43 @@ void Engine::setDetsDevice(const uint64_t* d, int64_t cnt) {
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+// The plain basis E grows in blocks, one per call: the determinants added
+// since the last call, then the flipped copies of the open-shell ones among
+// them, so the sub-engine's Hamiltonian is extended incrementally like the
+// plain one (the E order is not the unpacked order; unpackTrev builds that).
 void Engine::buildTrev() {
   const int W = hs.W;
-  const int64_t nu = launchTrevBasis(dets.as<uint64_t>(), n, W, trevFlags, nullptr, nullptr, nullptr, ws, stream);
-  const int64_t nE = n + nu;
-  DevMem detsE;
-  detsE.reserve((size_t)std::max<int64_t>(nE, 1) * W * 8, stream);
-  trevPartner.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
-  trevParity.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
-  launchTrevBasis(dets.as<uint64_t>(), n, W, trevFlags, detsE.as<uint64_t>(), trevPartner.as<int64_t>(),
-                  trevParity.as<double>(), ws, stream);
-  if (!tE) {
+  const bool fresh = !tE || trevN < 0 || trevN > n;
+  const int64_t i0 = fresh ? 0 : trevN;
+  if (fresh) {
     tE.reset(new Engine());
     tE->initShared(*this);
+    tE->incremental = true;
   }
   tE->precision = precision;
-  tE->setDetsDevice(detsE.as<uint64_t>(), nE);
+  if (i0 < n) {
+    const int64_t cnt = n - i0;
+    const int64_t nu = launchTrevBasis(dets.as<uint64_t>(), i0, n, W, trevFlags, nullptr, 0, nullptr, nullptr, nullptr,
+                                       ws, stream);
+    const size_t keep = (size_t)std::max<int64_t>(n, 1) * 8;
+    trevEIdx.reserve(keep, stream, true);
+    trevPartner.reserve(keep, stream, true);
+    trevParity.reserve(keep, stream, true);
+    DevMem block;
+    block.reserve((size_t)(cnt + nu) * W * 8, stream);
+    const int64_t base = fresh ? 0 : tE->n;
+    launchTrevBasis(dets.as<uint64_t>(), i0, n, W, trevFlags, block.as<uint64_t>(), base, trevEIdx.as<int64_t>(),
+                    trevPartner.as<int64_t>(), trevParity.as<double>(), ws, stream);
+    if (fresh) {
+      tE->setDetsDevice(block.as<uint64_t>(), cnt + nu);
+    } else {
+      tE->newDets.reserve((size_t)(cnt + nu) * W * 8, stream);
+      HIP_CHECK(hipMemcpyAsync(tE->newDets.p, block.p, (size_t)(cnt + nu) * W * 8, hipMemcpyDeviceToDevice, stream));
+      tE->nNew = cnt + nu;
+      tE->appendNew();
+      HIP_CHECK(hipStreamSynchronize(stream));
+    }
+  }
   tE->buildHamiltonian();
   trevN = n;
 }
@@ -81,25 +102,36 @@ void Engine::sigmaTrev(const double* dx, double* dy, int k, int64_t ld, bool exa
   const int64_t nE = tE->n;
   trevX.reserve((size_t)std::max<int64_t>(nE * k, 1) * 8, stream);
   trevY.reserve((size_t)std::max<int64_t>(nE * k, 1) * 8, stream);
-  launchTrevExpand(dx, n, k, ld, trevPartner.as<int64_t>(), trevParity.as<double>(), trevX.as<double>(), nE, stream);
+  launchTrevExpand(dx, n, k, ld, trevEIdx.as<int64_t>(), trevPartner.as<int64_t>(), trevParity.as<double>(),
+                   trevX.as<double>(), nE, stream);
   tE->sigma(trevX.as<double>(), trevY.as<double>(), k, nE, exact);
-  launchTrevContract(trevY.as<double>(), nE, n, k, trevPartner.as<int64_t>(), trevParity.as<double>(), dy, ld,
-                     stream);
+  launchTrevContract(trevY.as<double>(), nE, n, k, trevEIdx.as<int64_t>(), trevPartner.as<int64_t>(),
+                     trevParity.as<double>(), dy, ld, stream);
 }
 
 void Engine::unpackTrev(int sign) {
-  if (trevN != n) buildTrev();
-  const int64_t nE = tE->n;
-  DevMem out;
+  // the reference's order: the determinants, then the flipped copies of the
+  // open-shell ones in determinant order
+  const int W = hs.W;
+  const int64_t nu = launchTrevBasis(dets.as<uint64_t>(), 0, n, W, trevFlags, nullptr, 0, nullptr, nullptr, nullptr,
+                                     ws, stream);
+  const int64_t nE = n + nu;
+  DevMem detsU, eIdxU, partnerU, parityU, out;
+  detsU.reserve((size_t)std::max<int64_t>(nE, 1) * W * 8, stream);
+  eIdxU.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
+  partnerU.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
+  parityU.reserve((size_t)std::max<int64_t>(n, 1) * 8, stream);
+  launchTrevBasis(dets.as<uint64_t>(), 0, n, W, trevFlags, detsU.as<uint64_t>(), 0, eIdxU.as<int64_t>(),
+                  partnerU.as<int64_t>(), parityU.as<double>(), ws, stream);
   out.reserve((size_t)std::max<int64_t>(nE * nroots, 1) * 8, stream);
-  launchTrevUnpack(ci.as<double>(), n, nroots, n, trevPartner.as<int64_t>(), trevParity.as<double>(), sign,
+  launchTrevUnpack(ci.as<double>(), n, nroots, n, partnerU.as<int64_t>(), parityU.as<double>(), sign,
                    out.as<double>(), nE, stream);
   ci.swap(out);
   trev = 0;
   S.trev = 0;
-  setDetsDevice(tE->dets.as<uint64_t>(), nE);
   tE.reset();
   trevN = -1;
+  setDetsDevice(detsU.as<uint64_t>(), nE);
 }
 
 void Engine::load(const std::string& fcidump) {
@@ -435,6 +467,7 @@ void Engine::setDets(const int64_t* h, int64_t cnt) {
   hamRows = -1;
   permRows = -1;
   nNew = 0;
+  trevN = -1;
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 

@@ -169,7 +169,7 @@ struct Engine {
   // preconditioner keeps the plain diagonal energies, as the reference's.
   int trev = 0;
   std::unique_ptr<Engine> tE;
-  DevMem trevFlags, trevPartner, trevParity, trevX, trevY;
+  DevMem trevFlags, trevEIdx, trevPartner, trevParity, trevX, trevY;
   int64_t trevN = -1;  // n the adapted basis was built for
   void buildTrev();
   void sigmaTrev(const double* dx, double* dy, int k, int64_t ld, bool exact);

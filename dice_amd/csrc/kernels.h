@@ -49,17 +49,19 @@ void launchPtGenStoch(const DevSystem& S, const uint64_t* dets, const uint32_t* 
 void launchPtStochReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num,
                          const double* num2, const uint8_t* pres, int64_t m, double e0, double nmc, DevMem& partial,
                          double* out, hipStream_t s);
-// time reversal: flags + scan of the open-shell standard determinants; with
-// detsE, the basis E (n + returned count determinants), each determinant's
-// partner index in E (-1 when closed-shell) and flip parity.  Returns the
-// number of open-shell determinants (synchronises).
-int64_t launchTrevBasis(const uint64_t* dets, int64_t n, int W, DevMem& flags, uint64_t* detsE, int64_t* partner,
-                        double* parity, Workspace& ws, hipStream_t s);
+// time reversal: flags + scan of the open-shell determinants among
+// dets[i0, i1); with block, the block of the plain basis E for them (the
+// i1 - i0 determinants, then the flipped copies of the open-shell ones) to
+// be placed at E index base, and per determinant its E index, its partner's
+// (-1 when closed-shell) and its flip parity.  Returns the number of
+// open-shell determinants (synchronises).
+int64_t launchTrevBasis(const uint64_t* dets, int64_t i0, int64_t i1, int W, DevMem& flags, uint64_t* block,
+                        int64_t base, int64_t* eIdx, int64_t* partner, double* parity, Workspace& ws, hipStream_t s);
 // xE = U x and y = U^T yE for k columns (U: adapted basis -> E)
-void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner, const double* parity,
-                      double* xE, int64_t ldE, hipStream_t s);
-void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner, const double* parity,
-                        double* y, int64_t ld, hipStream_t s);
+void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* eIdx, const int64_t* partner,
+                      const double* parity, double* xE, int64_t ldE, hipStream_t s);
+void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* eIdx, const int64_t* partner,
+                        const double* parity, double* y, int64_t ld, hipStream_t s);
 // the unpacked coefficients in E (Treversal sign trev)
 void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
                       const double* parity, int trev, double* out, int64_t ldE, hipStream_t s);

@@ -833,62 +833,67 @@ __global__ void kRdmFinish(RdmOut o) {
 // which leaves out the Treversal sign).
 
 template <int W>
-__global__ void kTrevMark(const uint64_t* dets, int64_t n, int64_t* flag) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+__global__ void kTrevMark(const uint64_t* dets, int64_t i0, int64_t i1, int64_t* flag) {
+  for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += (int64_t)gridDim.x * blockDim.x) {
     uint64_t d[W];
 #pragma unroll
     for (int w = 0; w < W; w++) d[w] = dets[i * W + w];
-    flag[i] = hasUnpaired<W>(d) ? 1 : 0;
+    flag[i - i0] = hasUnpaired<W>(d) ? 1 : 0;
   }
 }
 
-// after an exclusive scan of the flags: E list, partner index (-1 for
-// closed-shell) and flip parity per standard determinant
+// After an exclusive scan of the flags of [i0, i1): the block of E for these
+// determinants (the cnt = i1 - i0 determinants, then the flipped copies of
+// the open-shell ones), placed at E index base; each determinant's index in
+// E, its partner's (-1 when closed-shell) and its flip parity
 template <int W>
-__global__ void kTrevBuild(const uint64_t* dets, int64_t n, const int64_t* pos, uint64_t* detsE, int64_t* partner,
-                           double* parity) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+__global__ void kTrevBlock(const uint64_t* dets, int64_t i0, int64_t i1, const int64_t* pos, int64_t base,
+                           uint64_t* block, int64_t* eIdx, int64_t* partner, double* parity) {
+  const int64_t cnt = i1 - i0;
+  for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i - i0;
     uint64_t d[W];
 #pragma unroll
     for (int w = 0; w < W; w++) {
       d[w] = dets[i * W + w];
-      detsE[i * W + w] = d[w];
+      block[k * W + w] = d[w];
     }
-    const bool u = pos[i + 1] != pos[i];
+    const bool u = pos[k + 1] != pos[k];
+    eIdx[i] = base + k;
     parity[i] = flipParity<W>(d);
-    partner[i] = u ? n + pos[i] : -1;
+    partner[i] = u ? base + cnt + pos[k] : -1;
     if (u) {
 #pragma unroll
-      for (int w = 0; w < W; w++) detsE[(n + pos[i]) * W + w] = flipWord(d[w]);
+      for (int w = 0; w < W; w++) block[(cnt + pos[k]) * W + w] = flipWord(d[w]);
     }
   }
 }
 
 // xE = U x (k columns, ld / ldE)
-__global__ void kTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner,
-                            const double* parity, double* xE, int64_t ldE) {
+__global__ void kTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* eIdx,
+                            const int64_t* partner, const double* parity, double* xE, int64_t ldE) {
   const double r2 = 1.0 / sqrt(2.0);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * k; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % n, c = t / n;
     const double v = x[c * ld + i];
-    const int64_t q = partner[i];
+    const int64_t q = partner[i], e = eIdx[i];
     if (q < 0) {
-      xE[c * ldE + i] = v;
+      xE[c * ldE + e] = v;
     } else {
-      xE[c * ldE + i] = v * r2;
+      xE[c * ldE + e] = v * r2;
       xE[c * ldE + q] = parity[i] * v * r2;
     }
   }
 }
 
 // y = U^T yE
-__global__ void kTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner,
-                              const double* parity, double* y, int64_t ld) {
+__global__ void kTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* eIdx,
+                              const int64_t* partner, const double* parity, double* y, int64_t ld) {
   const double r2 = 1.0 / sqrt(2.0);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * k; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % n, c = t / n;
-    const int64_t q = partner[i];
-    y[c * ld + i] = q < 0 ? yE[c * ldE + i] : (yE[c * ldE + i] + parity[i] * yE[c * ldE + q]) * r2;
+    const int64_t q = partner[i], e = eIdx[i];
+    y[c * ld + i] = q < 0 ? yE[c * ldE + e] : (yE[c * ldE + e] + parity[i] * yE[c * ldE + q]) * r2;
   }
 }
 
@@ -2255,32 +2260,35 @@ void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const i
   HIP_CHECK(hipGetLastError());
 }
 
-int64_t launchTrevBasis(const uint64_t* dets, int64_t n, int W, DevMem& flags, uint64_t* detsE, int64_t* partner,
-                        double* parity, Workspace& ws, hipStream_t s) {
-  flags.reserve((size_t)(n + 1) * 8, s);
+int64_t launchTrevBasis(const uint64_t* dets, int64_t i0, int64_t i1, int W, DevMem& flags, uint64_t* block,
+                        int64_t base, int64_t* eIdx, int64_t* partner, double* parity, Workspace& ws, hipStream_t s) {
+  const int64_t cnt = i1 - i0;
+  if (cnt <= 0) return 0;
+  flags.reserve((size_t)(cnt + 1) * 8, s);
   int64_t* f = flags.as<int64_t>();
-  HIP_CHECK(hipMemsetAsync(f + n, 0, 8, s));
-  DISPATCH_W(W, kTrevMark<WW><<<gridFor(n), kBlock, 0, s>>>(dets, n, f));
-  exclusiveScanI64(f, n + 1, ws, s);
+  HIP_CHECK(hipMemsetAsync(f + cnt, 0, 8, s));
+  DISPATCH_W(W, kTrevMark<WW><<<gridFor(cnt), kBlock, 0, s>>>(dets, i0, i1, f));
+  exclusiveScanI64(f, cnt + 1, ws, s);
   int64_t nu = 0;
-  HIP_CHECK(hipMemcpyAsync(&nu, f + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&nu, f + cnt, 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  if (detsE) DISPATCH_W(W, kTrevBuild<WW><<<gridFor(n), kBlock, 0, s>>>(dets, n, f, detsE, partner, parity));
+  if (block)
+    DISPATCH_W(W, kTrevBlock<WW><<<gridFor(cnt), kBlock, 0, s>>>(dets, i0, i1, f, base, block, eIdx, partner, parity));
   HIP_CHECK(hipGetLastError());
   return nu;
 }
 
-void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* partner, const double* parity,
-                      double* xE, int64_t ldE, hipStream_t s) {
+void launchTrevExpand(const double* x, int64_t n, int k, int64_t ld, const int64_t* eIdx, const int64_t* partner,
+                      const double* parity, double* xE, int64_t ldE, hipStream_t s) {
   if (n <= 0) return;
-  kTrevExpand<<<gridFor(n * k), kBlock, 0, s>>>(x, n, k, ld, partner, parity, xE, ldE);
+  kTrevExpand<<<gridFor(n * k), kBlock, 0, s>>>(x, n, k, ld, eIdx, partner, parity, xE, ldE);
   HIP_CHECK(hipGetLastError());
 }
 
-void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* partner, const double* parity,
-                        double* y, int64_t ld, hipStream_t s) {
+void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const int64_t* eIdx, const int64_t* partner,
+                        const double* parity, double* y, int64_t ld, hipStream_t s) {
   if (n <= 0) return;
-  kTrevContract<<<gridFor(n * k), kBlock, 0, s>>>(yE, ldE, n, k, partner, parity, y, ld);
+  kTrevContract<<<gridFor(n * k), kBlock, 0, s>>>(yE, ldE, n, k, eIdx, partner, parity, y, ld);
   HIP_CHECK(hipGetLastError());
 }
 
