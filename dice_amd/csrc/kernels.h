@@ -133,6 +133,8 @@ struct StringTables {
   DevMem aMajorDet, bMajorDet;  // det index per group member (u32)
   DevMem aMajorB, bMajorA;      // partner string per group member (u64)
   DevMem aSinglesStart, aSingles;  // alpha strings one excitation apart (int64 / u32)
+  DevMem bSinglesStart, bSingles;  // beta strings one excitation apart (int64 / u32)
+  DevMem aSortB, aSortDet;         // alpha-major groups with members by beta string: beta (u64), det (u32)
   DevMem key, val;              // scratch
 };
 void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, StringTables& T, Workspace& ws,

@@ -1058,6 +1058,15 @@ __global__ void kGroupKeys(const uint32_t* major, int64_t n, uint64_t* key, uint
   }
 }
 
+// key = major * nminor + minor (members of a group ordered by minor)
+__global__ void kGroupKeys2(const uint32_t* major, const uint32_t* minor, int64_t n, uint64_t nminor, uint64_t* key,
+                            uint32_t* val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    key[i] = (uint64_t)major[i] * nminor + (uint64_t)minor[i];
+    val[i] = (uint32_t)i;
+  }
+}
+
 __global__ void kGroupStarts(const uint64_t* key, int64_t n, uint64_t nminor, int64_t ngroups, int64_t* start,
                              const uint32_t* val, const uint64_t* partnerStr, uint64_t* memberPartner) {
   for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m <= n; m += (int64_t)gridDim.x * blockDim.x) {
@@ -1114,6 +1123,11 @@ struct RowTables {
   const uint64_t* bMajorA;
   const int64_t* aSinglesStart;
   const uint32_t* aSingles;
+  const int64_t* bSinglesStart;
+  const uint32_t* bSingles;
+  const uint64_t* sortedB;
+  const uint64_t* aSortB;
+  const uint32_t* aSortDet;
   int64_t n;
 };
 
@@ -1180,7 +1194,7 @@ __device__ __forceinline__ int64_t firstDetAtLeast(const uint32_t* a, int64_t lo
 // jmin skips a prefix of each group (binary search).
 template <class Sink>
 __device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint32_t jmin, int wave, int lane,
-                                             int (*sExcl)[65], int64_t (*sBeg)[64], Sink& sink) {
+                                             int (*sExcl)[65], int64_t (*sBeg)[64], int (*sLook)[64], Sink& sink) {
   const uint64_t astr = T.alpha[I], bstr = T.beta[I];
   const uint32_t ai = T.aIdx[I], bi = T.bIdx[I];
   {
@@ -1214,17 +1228,31 @@ __device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint
     }
   }
   {
+    // alpha single x beta single: per alpha single group either scan its
+    // members for a beta one excitation from the row's (the members in
+    // determinant order, from the first at or after jmin), or, when the
+    // group is large, look up each of the row's beta singles in the group's
+    // beta-ordered members (binary search)
     int64_t s0 = T.aSinglesStart[ai], s1 = T.aSinglesStart[ai + 1];
+    const int64_t bs0 = T.bSinglesStart[bi];
+    const int nbs = (int)(T.bSinglesStart[bi + 1] - bs0);
     for (int64_t c0 = s0; c0 < s1; c0 += 64) {
       int64_t c = c0 + lane;
-      int len = 0;
+      int len = 0, look = 0, gsz = 0;
       int64_t beg = 0;
       if (c < s1) {
         uint32_t sa = T.aSingles[c];
-        beg = T.aStart[sa];
-        int64_t end = T.aStart[sa + 1];
-        if (jmin) beg = firstDetAtLeast(T.aMajorDet, beg, end, jmin);
-        len = (int)(end - beg);
+        const int64_t g0 = T.aStart[sa], g1 = T.aStart[sa + 1];
+        beg = g0;
+        if (jmin) beg = firstDetAtLeast(T.aMajorDet, beg, g1, jmin);
+        len = (int)(g1 - beg);
+        gsz = (int)(g1 - g0);
+        const int steps = 66 - __clzll((unsigned long long)gsz);  // binary-search depth + 2
+        if ((int64_t)nbs * steps < len) {
+          look = 1;
+          len = nbs;
+          beg = g0;
+        }
       }
       int inc = len;
 #pragma unroll
@@ -1234,6 +1262,7 @@ __device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint
       }
       sExcl[wave][lane] = inc - len;
       sBeg[wave][lane] = beg;
+      sLook[wave][lane] = look ? gsz : -1;
       int total = __shfl(inc, 63, 64);
       __syncwarp();
       for (int k0 = 0; k0 < total; k0 += 64) {
@@ -1249,9 +1278,27 @@ __device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint
             else
               hi = mid - 1;
           }
-          int64_t m = sBeg[wave][lo] + (k - sExcl[wave][lo]);
-          h = __popcll(T.aMajorB[m] ^ bstr) == 2;
-          J = T.aMajorDet[m];
+          const int o = k - sExcl[wave][lo];
+          const int g = sLook[wave][lo];
+          if (g < 0) {
+            int64_t m = sBeg[wave][lo] + o;
+            h = __popcll(T.aMajorB[m] ^ bstr) == 2;
+            J = T.aMajorDet[m];
+          } else {
+            const uint64_t want = T.sortedB[T.bSingles[bs0 + o]];
+            int64_t a = sBeg[wave][lo], b = a + g;  // [a, b)
+            while (a < b) {
+              const int64_t mid = (a + b) >> 1;
+              if (T.aSortB[mid] < want)
+                a = mid + 1;
+              else
+                b = mid;
+            }
+            if (a < sBeg[wave][lo] + g && T.aSortB[a] == want) {
+              J = T.aSortDet[a];
+              h = J >= jmin;
+            }
+          }
         }
         sink.hit(h, J);
       }
@@ -1268,6 +1315,7 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
                                                    double* vals, float* hi, float* lo) {
   __shared__ int sExcl[kWaves][65];
   __shared__ int64_t sBeg[kWaves][64];
+  __shared__ int sLook[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < T.n; I += (int64_t)gridDim.x * kWaves) {
     uint64_t dI[W];
@@ -1285,7 +1333,7 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     sink.lo = lo;
     sink.pos = FILL ? fillStart[I] : 0;
     sink.cnt = 0;
-    enumerateRow(T, I, jmin, wave, lane, sExcl, sBeg, sink);
+    enumerateRow(T, I, jmin, wave, lane, sExcl, sBeg, sLook, sink);
     if (!FILL) {
       int c = sink.cnt;
 #pragma unroll
@@ -1326,6 +1374,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaDirect(DevSystem S, const uint64
                                                        int64_t ld) {
   __shared__ int sExcl[kWaves][65];
   __shared__ int64_t sBeg[kWaves][64];
+  __shared__ int sLook[kWaves][64];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < T.n; I += (int64_t)gridDim.x * kWaves) {
     uint64_t dI[W];
@@ -1340,7 +1389,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaDirect(DevSystem S, const uint64
     sink.k = k;
 #pragma unroll
     for (int r = 0; r < 8; r++) sink.acc[r] = 0.0;
-    enumerateRow(T, I, 0u, wave, lane, sExcl, sBeg, sink);
+    enumerateRow(T, I, 0u, wave, lane, sExcl, sBeg, sLook, sink);
 #pragma unroll
     for (int r = 0; r < 8; r++) {
       if (r >= k) break;
@@ -2424,6 +2473,33 @@ void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, Stri
   kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)n, T.nB,
                                                   T.bStart.as<int64_t>(), T.val.as<uint32_t>(),
                                                   T.alpha.as<uint64_t>(), T.bMajorA.as<uint64_t>());
+  // alpha-major groups with members ordered by beta string (for lookups of
+  // a row's beta singles in large groups): key = aIdx * nB + bIdx (the
+  // group starts it writes equal the ones above)
+  T.aSortB.reserve(n * 8);
+  T.aSortDet.reserve(n * 4);
+  kGroupKeys2<<<gridFor(n), kBlock, 0, s>>>(T.aIdx.as<uint32_t>(), T.bIdx.as<uint32_t>(), n, (uint64_t)T.nB,
+                                             T.key.as<uint64_t>(), T.val.as<uint32_t>());
+  radixSortKeys(T.key.as<uint64_t>(), T.val.as<uint32_t>(), n, 1, bitsFor((uint64_t)T.nA * T.nB), false, ws, s);
+  HIP_CHECK(hipMemcpyAsync(T.aSortDet.p, T.val.p, n * 4, hipMemcpyDeviceToDevice, s));
+  kGroupStarts<<<gridFor(n + 1), kBlock, 0, s>>>(T.key.as<uint64_t>(), n, (uint64_t)T.nB, T.nA,
+                                                  T.aStart.as<int64_t>(), T.val.as<uint32_t>(),
+                                                  T.beta.as<uint64_t>(), T.aSortB.as<uint64_t>());
+  // beta single-excitation adjacency
+  {
+    T.bSinglesStart.reserve((T.nB + 1) * 8);
+    int64_t* bc = T.bSinglesStart.as<int64_t>();
+    HIP_CHECK(hipMemsetAsync(bc + T.nB, 0, 8, s));
+    kStringSingles<false><<<gridFor(T.nB), kBlock, 0, s>>>(T.sortedB.as<uint64_t>(), T.nB, norb, nullptr, bc,
+                                                            nullptr);
+    exclusiveScanI64(bc, T.nB + 1, ws, s);
+    int64_t tb = 0;
+    HIP_CHECK(hipMemcpyAsync(&tb, bc + T.nB, 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    T.bSingles.reserve(std::max<int64_t>(tb, 1) * 4);
+    kStringSingles<true><<<gridFor(T.nB), kBlock, 0, s>>>(T.sortedB.as<uint64_t>(), T.nB, norb, bc, nullptr,
+                                                           T.bSingles.as<uint32_t>());
+  }
   // alpha single-excitation adjacency
   T.aSinglesStart.reserve((T.nA + 1) * 8);
   int64_t* cnt = T.aSinglesStart.as<int64_t>();
@@ -2454,6 +2530,11 @@ static RowTables rowTables(const StringTables& T) {
   r.bMajorA = T.bMajorA.as<uint64_t>();
   r.aSinglesStart = T.aSinglesStart.as<int64_t>();
   r.aSingles = T.aSingles.as<uint32_t>();
+  r.bSinglesStart = T.bSinglesStart.as<int64_t>();
+  r.bSingles = T.bSingles.as<uint32_t>();
+  r.sortedB = T.sortedB.as<uint64_t>();
+  r.aSortB = T.aSortB.as<uint64_t>();
+  r.aSortDet = T.aSortDet.as<uint32_t>();
   r.n = T.n;
   return r;
 }
