@@ -878,7 +878,7 @@ std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& 
     sigma(col(V, m0), col(S, m0), b, N, exactSigma);
     launchGramTiled(V, m, S, m, N, N, G, dvPart, s);
     launchSymmetrizeUpper(G, m, A, s);
-    launchJacobi(A, m, w, C, s);
+    launchJacobiRR(A, m, w, C, s);
     launchSubMatrix(C, m, m, nr, Cn, s);
     launchGemmTS(V, m, Cn, nr, N, N, X, N, 1.0, 0.0, s);
     launchGemmTS(S, m, Cn, nr, N, N, HX, N, 1.0, 0.0, s);

@@ -250,5 +250,8 @@ void launchSymmetrizeUpper(const double* G, int m, double* A, hipStream_t s);
 // symmetric eigen-decomposition of the m x m matrix (row-major) by cyclic
 // Jacobi, eigenvalues ascending in w, eigenvectors in the columns of V
 void launchJacobi(const double* A, int m, double* w, double* V, hipStream_t s);
+// the same with the rotations of a sweep in round-robin order (disjoint pairs
+// at once): eigenpairs equal to rounding, fewer barriers (block Davidson)
+void launchJacobiRR(const double* A, int m, double* w, double* V, hipStream_t s);
 
 }  // namespace shci

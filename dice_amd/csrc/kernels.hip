@@ -2175,6 +2175,126 @@ __global__ void __launch_bounds__(64) kJacobi(const double* A, int m, double* w,
   }
 }
 
+// The same eigenproblem with the rotations of a sweep in round-robin
+// (tournament) order: the m'/2 disjoint pairs of a step (m' = m rounded up
+// to even) are rotated at once, m'-1 steps per sweep, so a sweep needs
+// m'-1 rounds of barriers instead of m(m-1)/2.  Another rotation order than
+// kJacobi's, so the eigenpairs agree with it to rounding, not bit for bit;
+// used by the block Davidson.
+__global__ void __launch_bounds__(64) kJacobiRR(const double* A, int m, double* w, double* V) {
+  __shared__ double a[kJacobiMax * kJacobiMax], v[kJacobiMax * kJacobiMax];
+  __shared__ double rc[kJacobiMax / 2], rs[kJacobiMax / 2];
+  __shared__ int rp[kJacobiMax / 2], rq[kJacobiMax / 2];
+  __shared__ double sOff;
+  __shared__ int order[kJacobiMax];
+  const int lane = threadIdx.x;
+  const int mm = (m + 1) & ~1, half = mm / 2;
+  for (int q = lane; q < m * m; q += 64) {
+    a[q] = A[q];
+    v[q] = 0.0;
+  }
+  __syncthreads();
+  if (lane < m) v[lane * m + lane] = 1.0;
+  __syncthreads();
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double part = 0.0;
+    for (int t = lane; t < m * m; t += 64) {
+      const int p = t / m, q = t % m;
+      if (q > p) part += fabs(a[t]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) sOff = part;
+    __syncthreads();
+    if (sOff == 0.0) break;
+    for (int step = 0; step < mm - 1; step++) {
+      if (lane < half) {
+        // round-robin: player 0 fixed, the others rotate by step
+        const int i = lane, j = mm - 1 - lane;
+        int p = i == 0 ? 0 : (i - 1 + step) % (mm - 1) + 1;
+        int q = (j - 1 + step) % (mm - 1) + 1;
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, sn = 0.0;
+        int pp = -1;
+        if (q < m) {
+          const double apq = a[p * m + q], app = a[p * m + p], aqq = a[q * m + q];
+          const double g = 100.0 * fabs(apq);
+          if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
+            pp = -2;  // negligible: zero it without a rotation
+          } else if (apq != 0.0) {
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+            pp = p;
+          }
+        }
+        rp[lane] = pp == -2 ? -2 - p : pp;
+        rq[lane] = q;
+        rc[lane] = c;
+        rs[lane] = sn;
+      }
+      __syncthreads();
+      // columns p, q of every row k (a), then rows p, q and columns of v
+      for (int t = lane; t < half * m; t += 64) {
+        const int r = t / m, k = t % m;
+        const int p = rp[r], q = rq[r];
+        if (p < 0) continue;
+        const double c = rc[r], sn = rs[r];
+        const double akp = a[k * m + p], akq = a[k * m + q];
+        a[k * m + p] = c * akp - sn * akq;
+        a[k * m + q] = sn * akp + c * akq;
+      }
+      __syncthreads();
+      for (int t = lane; t < half * m; t += 64) {
+        const int r = t / m, k = t % m;
+        const int p = rp[r], q = rq[r];
+        if (p < 0) continue;
+        const double c = rc[r], sn = rs[r];
+        const double apk = a[p * m + k], aqk = a[q * m + k];
+        a[p * m + k] = c * apk - sn * aqk;
+        a[q * m + k] = sn * apk + c * aqk;
+        const double vkp = v[k * m + p], vkq = v[k * m + q];
+        v[k * m + p] = c * vkp - sn * vkq;
+        v[k * m + q] = sn * vkp + c * vkq;
+      }
+      __syncthreads();
+      if (lane < half) {
+        int p = rp[lane];
+        const int q = rq[lane];
+        if (p <= -2) p = -2 - p;  // negligible pair
+        if (p >= 0 && q < m) {
+          a[p * m + q] = 0.0;
+          a[q * m + p] = 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    for (int i = 0; i < m; i++) order[i] = i;
+    for (int i = 1; i < m; i++) {  // stable insertion sort by eigenvalue
+      const int o = order[i];
+      int j = i;
+      while (j > 0 && a[order[j - 1] * m + order[j - 1]] > a[o * m + o]) {
+        order[j] = order[j - 1];
+        j--;
+      }
+      order[j] = o;
+    }
+  }
+  __syncthreads();
+  if (lane < m) w[lane] = a[order[lane] * m + order[lane]];
+  for (int q = lane; q < m * m; q += 64) {
+    const int k = q / m, j = q % m;
+    V[k * m + j] = v[k * m + order[j]];
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2878,6 +2998,12 @@ void launchAxpy(double* x, const double* y, double alpha, int64_t n, hipStream_t
 
 void launchSymmetrizeUpper(const double* G, int m, double* A, hipStream_t s) {
   kSymUpper<<<1, 256, 0, s>>>(G, m, A);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchJacobiRR(const double* A, int m, double* w, double* V, hipStream_t s) {
+  if (m > kJacobiMax) throw HipError("launchJacobiRR: subspace too large");
+  kJacobiRR<<<1, 64, 0, s>>>(A, m, w, V);
   HIP_CHECK(hipGetLastError());
 }
 
