@@ -123,6 +123,37 @@ def test_hamiltonian_rows_and_sigma(name, n, systems):
     assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
+def test_hamiltonian_rows_large_alpha_groups(tmp_path):
+    """Alpha groups far larger than a row's beta singles (16 orbitals, one
+    alpha and eight beta electrons: three alpha strings x all 12,870 beta
+    strings), where the build looks the row's beta singles up in the
+    beta-ordered members of each alpha group one excitation away instead of
+    scanning them: rows and values equal the oracle's, and sigma agrees."""
+    from dice_amd import synth
+    import itertools
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 7, 16, 1, 8, 24)
+    dets = []
+    for a in (0, 1, 2):
+        for b in itertools.combinations(range(16), 8):
+            dets.append((1 << (2 * a)) | sum(1 << (2 * x + 1) for x in b))
+    dets = np.array(dets, dtype=np.int64).reshape(-1, 1)
+    g, o = _gpu(path), OracleSystem(path)
+    g.set_dets(dets)
+    rp, cols, vals = g.hamiltonian_rows()
+    o.csr_append(dets)
+    orp, ocols, ovals = o.csr()
+    np.testing.assert_array_equal(np.diff(rp), np.diff(orp))
+    for i in range(len(dets)):
+        a, b = rp[i], rp[i + 1]
+        order = np.argsort(ocols[orp[i] + 1:orp[i + 1]], kind="stable")
+        np.testing.assert_array_equal(cols[a + 1:b], ocols[orp[i] + 1:orp[i + 1]][order])
+        np.testing.assert_array_equal(vals[a + 1:b], ovals[orp[i] + 1:orp[i + 1]][order])
+    x = np.random.default_rng(5).normal(size=len(dets))
+    y, yo = g.hmult(x), o.sigma(x)
+    assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
+
+
 # (system, precision, screened sources, eps1 of the two rounds): a few
 # thousand to a few tens of thousands of new determinants per round
 @pytest.mark.parametrize("name,prec,nsrc,eps", [("h8", 0, 60, (3e-3, 1.5e-3)), ("w2", 0, 5, (1e-1, 5e-2)),
