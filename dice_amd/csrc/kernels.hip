@@ -1942,6 +1942,8 @@ inline size_t ptTabBytes(int n) { return ((size_t)n + 2 * (size_t)n * n) * sizeo
 
 template <int W, bool LDS_TAB>
 __device__ __forceinline__ double ptEnergy(const DevSystem& S, const double* tab, const uint64_t* d) {
+  // (the same additions in the same order as Energy(): E_a is bit-identical,
+  // which matters for external determinants whose E0 - E_a is small)
   if (LDS_TAB) return detEnergyTab<W>(tab, tab + S.norb, tab + S.norb + S.norb * S.norb, S.norb, d);
   return detEnergy<W>(S, d);
 }
