@@ -251,8 +251,8 @@ double Engine::ptDeterministic(const double* dCi, double e0, double eps2, double
                   ptKeys.as<uint64_t>(), ptNum.as<double>(), stream);
       launchIota(ptIdx.as<uint32_t>(), m, stream);
       radixSortKeys(ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), m, W, hs.nso, true, ws, stream);
-      launchPtReduce(P, ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), ptNum.as<double>(), m, e0, ptPartial,
-                     ptOut.as<double>(), stream);
+      launchPtReduce(P, ptKeys.as<uint64_t>(), ptIdx.as<uint32_t>(), ptNum.as<double>(), m, e0, ptPartial, ptStartCnt,
+                     ptStarts, ws, ptOut.as<double>(), stream);
       HIP_CHECK(hipMemcpyAsync(host, ptOut.p, 3 * 8, hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
       pt += host[0];

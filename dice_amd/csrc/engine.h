@@ -135,7 +135,7 @@ struct Engine {
   // deterministic EN-PT2 (SHCI/SHCIbasics.cpp:453-728) of root vector dCi
   // (device, n values) with variational energy e0 (no core energy): E_PT;
   // candidates are processed in hash partitions sized to the free memory
-  DevMem ptCounts, ptKeys, ptNum, ptIdx, ptOut, ptPartial;
+  DevMem ptCounts, ptKeys, ptNum, ptIdx, ptOut, ptPartial, ptStartCnt, ptStarts;
   double ptDeterministic(const double* dCi, double e0, double eps2, double* psi1Norm, int64_t* nExternal,
                          int* nPartitions = nullptr);
   DevSystem ptSystem() const;

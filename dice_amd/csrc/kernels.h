@@ -79,7 +79,8 @@ void launchRdm(int norb, int W, const uint64_t* dets, const int64_t* rp, const i
 // m candidates sorted by determinant (payload idx into num): out[0..2]
 // (device) = E_PT, psi1 norm, number of external determinants
 void launchPtReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num, int64_t m,
-                    double e0, DevMem& partial, double* out, hipStream_t s);
+                    double e0, DevMem& partial, DevMem& counts, DevMem& starts, Workspace& ws, double* out,
+                    hipStream_t s);
 void launchScreen(const DevSystem& S, const uint64_t* dets, int64_t begin, int64_t end, const double* cmax,
                   double eps1, int ncore, int nact, const int32_t* table, uint64_t slots, uint64_t* out,
                   unsigned long long* counter, int64_t cap, hipStream_t s);

@@ -2004,13 +2004,58 @@ __global__ void kIota(uint32_t* v, int64_t n) {
 // numerators in that order (the reference's merge) and contributes
 // num^2 / (E0 - E_a) and (num / (E0 - E_a))^2; per-block partial sums of
 // (E_PT, psi1 norm, number of external determinants) in fixed order.
+
+// This is evaluated over the run starts only (compacted in order first), so
+// that every lane of the E_a evaluation works on a unique determinant
+// instead of one lane in ~6 (runs average six candidates on the 1.18M-
+// determinant space).
+constexpr int kStartChunk = kBlock * 16;
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) kPtCountStarts(const uint64_t* keys, int64_t m, int64_t* counts) {
+  const int64_t b0 = (int64_t)blockIdx.x * kStartChunk;
+  const int64_t b1 = b0 + kStartChunk < m ? b0 + kStartChunk : m;
+  int c = 0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+    if (i == 0 || !detEqual<W>(keys + i * W, keys + (i - 1) * W)) c++;
+  const double t = blockSum((double)c);
+  if (threadIdx.x == 0) counts[blockIdx.x] = (int64_t)t;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) kPtWriteStarts(const uint64_t* keys, int64_t m, const int64_t* offs,
+                                                         uint32_t* starts) {
+  __shared__ int sWave[kWaves];
+  __shared__ int64_t sBase;
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int64_t b0 = (int64_t)blockIdx.x * kStartChunk;
+  const int64_t b1 = b0 + kStartChunk < m ? b0 + kStartChunk : m;
+  if (threadIdx.x == 0) sBase = offs[blockIdx.x];
+  __syncthreads();
+  for (int64_t t0 = b0; t0 < b1; t0 += blockDim.x) {
+    const int64_t i = t0 + threadIdx.x;
+    const bool f = i < b1 && (i == 0 || !detEqual<W>(keys + i * W, keys + (i - 1) * W));
+    const uint64_t bal = __ballot(f);
+    if (lane == 0) sWave[wave] = __popcll(bal);
+    __syncthreads();
+    int64_t base = sBase;
+    for (int w = 0; w < wave; w++) base += sWave[w];
+    if (f) starts[base + __popcll(bal & lanemaskLt())] = (uint32_t)i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 0; w < kWaves; w++) sBase += sWave[w];
+    __syncthreads();
+  }
+}
+
 template <int W, bool LDS_TAB>
-__global__ void __launch_bounds__(kBlock) kPtReduce(DevSystem S, const uint64_t* keys, const uint32_t* idx,
-                                                    const double* num, int64_t m, double e0, double* partial) {
+__global__ void __launch_bounds__(kBlock) kPtReduceStarts(DevSystem S, const uint64_t* keys, const uint32_t* idx,
+                                                          const double* num, const uint32_t* starts, int64_t nu,
+                                                          int64_t m, double e0, double* partial) {
   const double* tab = ptStageTables<LDS_TAB>(S);
   double sPt = 0.0, sPsi = 0.0, sCnt = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-    if (i > 0 && detEqual<W>(keys + i * W, keys + (i - 1) * W)) continue;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = starts[u];
     double sum = num[idx[i]];
     for (int64_t j = i + 1; j < m && detEqual<W>(keys + j * W, keys + i * W); j++) sum = __dadd_rn(sum, num[idx[j]]);
     uint64_t d[W];
@@ -2477,14 +2522,33 @@ void launchIota(uint32_t* v, int64_t n, hipStream_t s) {
 }
 
 void launchPtReduce(const DevSystem& S, const uint64_t* keys, const uint32_t* idx, const double* num, int64_t m,
-                    double e0, DevMem& partial, double* out, hipStream_t s) {
-  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kPtRedBlocks, (m + kBlock - 1) / kBlock));
+                    double e0, DevMem& partial, DevMem& counts, DevMem& starts, Workspace& ws, double* out,
+                    hipStream_t s) {
+  if (m <= 0) {
+    HIP_CHECK(hipMemsetAsync(out, 0, 3 * sizeof(double), s));
+    return;
+  }
+  // run starts, compacted in order
+  const int64_t nbS = (m + kStartChunk - 1) / kStartChunk;
+  counts.reserve((size_t)(nbS + 1) * 8, s);
+  int64_t* cnt = counts.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(cnt + nbS, 0, 8, s));
+  DISPATCH_W(S.W, kPtCountStarts<WW><<<(unsigned)nbS, kBlock, 0, s>>>(keys, m, cnt));
+  exclusiveScanI64(cnt, nbS + 1, ws, s);
+  int64_t nu = 0;
+  HIP_CHECK(hipMemcpyAsync(&nu, cnt + nbS, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  starts.reserve((size_t)std::max<int64_t>(nu, 1) * 4, s);
+  DISPATCH_W(S.W, kPtWriteStarts<WW><<<(unsigned)nbS, kBlock, 0, s>>>(keys, m, cnt, starts.as<uint32_t>()));
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kPtRedBlocks, (nu + kBlock - 1) / kBlock));
   partial.reserve((size_t)nb * 3 * sizeof(double));
   const size_t tb = ptTabBytes(S.norb);
   if (tb <= kPtTabMaxBytes)
-    DISPATCH_W(S.W, kPtReduce<WW, true><<<nb, kBlock, tb, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
+    DISPATCH_W(S.W, kPtReduceStarts<WW, true><<<nb, kBlock, tb, s>>>(S, keys, idx, num, starts.as<uint32_t>(), nu, m,
+                                                                     e0, partial.as<double>()));
   else
-    DISPATCH_W(S.W, kPtReduce<WW, false><<<nb, kBlock, 0, s>>>(S, keys, idx, num, m, e0, partial.as<double>()));
+    DISPATCH_W(S.W, kPtReduceStarts<WW, false><<<nb, kBlock, 0, s>>>(S, keys, idx, num, starts.as<uint32_t>(), nu,
+                                                                      m, e0, partial.as<double>()));
   kFinalSum<<<3, kBlock, 0, s>>>(partial.as<double>(), nb, 3, 3, out);
   HIP_CHECK(hipGetLastError());
 }
