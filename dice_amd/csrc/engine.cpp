@@ -171,6 +171,8 @@ void Engine::load(const std::string& fcidump) {
   // SHCI_SIGMA_LAYOUT=external keeps sigma on H in determinant order
   const char* layout = std::getenv("SHCI_SIGMA_LAYOUT");
   relabel = !(layout && std::strcmp(layout, "external") == 0);
+  // SHCI_SIGMA_LAYOUT=staged adds the staged form of the relabelled copy
+  staged = layout && std::strcmp(layout, "staged") == 0;
   uploadRestrictions();
   counter.reserve(64);
   HIP_CHECK(hipStreamSynchronize(stream));
@@ -690,6 +692,7 @@ void Engine::buildHamiltonian() {
 // the sigma kernel; skipped when it does not fit beside H.
 void Engine::buildPermuted() {
   permRows = -1;
+  stagedRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
   const size_t need = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4;
@@ -719,6 +722,19 @@ void Engine::buildPermuted() {
                       rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
   }
   permRows = n;
+  if (staged && !f32) {
+    const size_t items = (size_t)(n + 3) / 4;
+    const size_t extra = (size_t)nnzOff * 6 + items * 4;
+    freeB = deviceFreeBytes(&totalB);
+    if (extra + (size_t)n * 64 < freeB + stLidx.bytes + stUniq.bytes + stCount.bytes) {
+      stLidx.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 2);
+      stUniq.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+      stCount.reserve(items * 4);
+      launchStageBuild(rp, permCols.as<int32_t>(), n, stLidx.as<uint16_t>(), stUniq.as<uint32_t>(),
+                       stCount.as<int32_t>(), stream);
+      stagedRows = n;
+    }
+  }
   // re-tune when the space has doubled since the last tuning (small spaces
   // keep the default shape: their sigma is too short to time reliably)
   if (autotune && n >= 20000 && n >= 2 * tunedRows) tuneSigma();
@@ -1016,6 +1032,10 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
+  else if (perm && k == 1 && stagedRows == n)
+    launchSigmaStaged(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), stLidx.as<uint16_t>(),
+                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(), n,
+                      dx, dy, permX, sigmaCounter.as<unsigned>(), stream);
   else if (perm)
     launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);

@@ -62,6 +62,11 @@ struct Engine {
   bool relabel = true;
   DevMem permPos, permRp, permCols, permVals, permX;
   int64_t permRows = -1;
+  // staged form of the FP64 relabelled copy (distinct columns per work item,
+  // 16-bit ranks per entry); valid when stagedRows == n
+  bool staged = false;
+  DevMem stLidx, stUniq, stCount;
+  int64_t stagedRows = -1;
   int64_t nnzOff = 0, hamRows = -1;
 
   // Davidson scratch

@@ -65,6 +65,16 @@ void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const i
 // the unpacked coefficients in E (Treversal sign trev)
 void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
                       const double* parity, int trev, double* out, int64_t ldE, hipStream_t s);
+// staged layout of the relabelled FP64 CSR (per work item of consecutive
+// rows: the sorted distinct columns in uniq at the item's first entry, and
+// per entry the 16-bit rank of its column; count = distinct columns, or -1
+// for items too large to stage), and the sigma over it (one vector, x and y
+// in determinant order; same results as the relabelled sigma)
+void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_t* lidx, uint32_t* uniq,
+                      int32_t* count, hipStream_t s);
+void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* vals, const uint16_t* lidx,
+                       const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
+                       int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter, hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial
