@@ -173,6 +173,9 @@ void Engine::load(const std::string& fcidump) {
   relabel = !(layout && std::strcmp(layout, "external") == 0);
   // SHCI_SIGMA_LAYOUT=staged adds the staged form of the relabelled copy
   staged = layout && std::strcmp(layout, "staged") == 0;
+  // SHCI_SIGMA_LAYOUT=alpha keeps every relabelled column alpha-major (no
+  // beta-major copy of x)
+  split = !(layout && std::strcmp(layout, "alpha") == 0);
   uploadRestrictions();
   counter.reserve(64);
   HIP_CHECK(hipStreamSynchronize(stream));
@@ -693,6 +696,7 @@ void Engine::buildHamiltonian() {
 void Engine::buildPermuted() {
   permRows = -1;
   stagedRows = -1;
+  splitRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
   const size_t need = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4;
@@ -722,6 +726,13 @@ void Engine::buildPermuted() {
                       rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
   }
   permRows = n;
+  if (split && 2 * n < 0x7fffffffLL) {
+    bPos.reserve((size_t)n * 4);
+    launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);
+    launchSplitCols(rp, permCols.as<int32_t>(), perm, T.bIdx.as<uint32_t>(), bPos.as<uint32_t>(), n,
+                    permCols.as<int32_t>(), stream);
+    splitRows = n;
+  }
   if (staged && !f32) {
     const size_t items = (size_t)(n + 3) / 4;
     const size_t extra = (size_t)nnzOff * 6 + items * 4;
@@ -837,7 +848,8 @@ std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& 
   reserveKeep(bdSmall, (size_t)8 * 4096 * 8);
   reserveKeep(dvPart, (size_t)16 * 256 * 64 * 8);  // largest Gram: 16 tiles x 256 blocks x 8 x 8
   reserveKeep(dvScal, 64 * 8);
-  reserveKeep(permX, (size_t)N * 8 * (precision == 1 && !exactSigma ? 4 : 8));
+  // x in alpha- and beta-major order, up to 8 interleaved vectors
+  reserveKeep(permX, (size_t)2 * N * 8 * (precision == 1 && !exactSigma ? 4 : 8));
   reserveKeep(sigmaCounter, 256);
   if (!bdHost) HIP_CHECK(hipHostMalloc((void**)&bdHost, 64 * 8, hipHostMallocDefault));
   double* sm = bdSmall.as<double>();
@@ -1029,16 +1041,17 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   const bool perm = permRows == n && !(f32 && exact);
   const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
   if (perm) sigmaCounter.reserve(256);
+  const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
   if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
   else if (perm && k == 1 && stagedRows == n)
     launchSigmaStaged(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), stLidx.as<uint16_t>(),
-                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(), n,
+                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, n,
                       dx, dy, permX, sigmaCounter.as<unsigned>(), stream);
   else if (perm)
     launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
   else if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
                    diag.as<double>(), n, dx, dy, k, ld, stream);

@@ -67,6 +67,12 @@ struct Engine {
   bool staged = false;
   DevMem stLidx, stUniq, stCount;
   int64_t stagedRows = -1;
+  // split columns: in the relabelled copy, columns whose determinant shares
+  // the row's beta string point into a beta-major copy of x (n + position);
+  // valid when splitRows == n
+  bool split = true;
+  DevMem bPos;
+  int64_t splitRows = -1;
   int64_t nnzOff = 0, hamRows = -1;
 
   // Davidson scratch

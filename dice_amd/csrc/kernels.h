@@ -74,7 +74,14 @@ void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_
                       int32_t* count, hipStream_t s);
 void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* vals, const uint16_t* lidx,
                        const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
-                       int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter, hipStream_t s);
+                       const uint32_t* bPerm, int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter,
+                       hipStream_t s);
+// columns of the relabelled copy for the split gather from [x alpha-major |
+// x beta-major]: entries whose determinant has the row's beta string take
+// n + its beta-major position bPos[J], the others keep theirs (out may be
+// colsP)
+void launchSplitCols(const int64_t* rpP, const int32_t* colsP, const uint32_t* perm, const uint32_t* bIdx,
+                     const uint32_t* bPos, int64_t n, int32_t* out, hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial
@@ -216,15 +223,17 @@ struct SigmaConfig {
 };
 // y = H x through the copy; x, y in determinant order (column-major, ld),
 // y must not overlap x; k > 1 runs as SpMM over up to 8 interleaved
-// vectors at a time.  xP is scratch for x in alpha-major order (FP64, or
-// FP32 for the FP32-value form); counter (device, one unsigned) is needed
-// by the persistent configuration.
+// vectors at a time.  xP is scratch for x in alpha-major order followed, when
+// bPerm (the beta-major determinant order) is given, by x in beta-major
+// order, for columns rewritten by launchSplitCols (FP64, or FP32 for the
+// FP32-value form); counter (device, one unsigned) is needed by the
+// persistent configuration.
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
-                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
+                        const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
+                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
-                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
+                        const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
+                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -1540,22 +1540,45 @@ __global__ void __launch_bounds__(kBlock) kPermRows(const int64_t* rowPtr, const
   }
 }
 
-// x in alpha-major order for the sigma kernels: K == 1: xP[r * n + m] =
-// x[r * ld + perm[m]] for r < k; K > 1 (SpMM): interleaved, xP[m * K + r]
-// (zero for k <= r < K).  Also zeroes the work counter of a persistent sigma
-// launch that follows on the same stream.
+// columns of the relabelled copy for a gather from [x alpha-major | x
+// beta-major]: an entry whose determinant has the row's beta string takes
+// n + its position in the beta-major order (the row's same-beta columns all
+// lie in one beta group there), every other entry keeps its alpha-major
+// position
+// (in place: out may be colsP)
+__global__ void __launch_bounds__(kBlock) kSplitCols(const int64_t* __restrict__ rpP, const int32_t* colsP,
+                                                     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ bIdx,
+                                                     const uint32_t* __restrict__ bPos, int64_t n, int32_t* out) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int64_t m = (int64_t)blockIdx.x * kWaves + wave; m < n; m += (int64_t)gridDim.x * kWaves) {
+    const uint32_t bi = bIdx[perm[m]];
+    for (int64_t p = rpP[m] + lane; p < rpP[m + 1]; p += 64) {
+      const int32_t c = colsP[p];
+      const uint32_t J = perm[c];
+      out[p] = bIdx[J] == bi ? (int32_t)(n + bPos[J]) : c;
+    }
+  }
+}
+
+// x in alpha-major order for the sigma kernels, followed (bPerm given) by
+// x in beta-major order: row m < nx of the copy holds x[perm[m]] and row
+// n + m' holds x[bPerm[m']], nx = 2n (n without bPerm); K == 1: xP[r * nx +
+// m] for r < k; K > 1 (SpMM): interleaved, xP[m * K + r] (zero for k <= r <
+// K).  Also zeroes the work counter of a persistent sigma launch that
+// follows on the same stream.
 // the persistent sigma's work counters: 8 words kSchedStride apart (the
 // counter buffer holds 64 words)
 constexpr int kSchedStride = 8;
 
 template <typename XT, int K>
-__global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, int64_t n, XT* xP,
-                            unsigned* counter) {
+__global__ void kGatherPerm(const double* x, int k, int64_t ld, const uint32_t* perm, const uint32_t* bPerm, int64_t n,
+                            XT* xP, unsigned* counter) {
   if (counter && blockIdx.x == 0 && threadIdx.x < 8) counter[kSchedStride * threadIdx.x] = 0u;
-  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t I = perm[m];
+  const int64_t nx = bPerm ? 2 * n : n;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nx; m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t I = m < n ? perm[m] : bPerm[m - n];
     if (K == 1) {
-      for (int r = 0; r < k; r++) xP[r * n + m] = (XT)x[r * ld + I];
+      for (int r = 0; r < k; r++) xP[r * nx + m] = (XT)x[r * ld + I];
     } else {
 #pragma unroll
       for (int r = 0; r < K; r++) xP[m * K + r] = r < k ? (XT)x[r * ld + I] : (XT)0;
@@ -1609,12 +1632,12 @@ struct ChunkSched {
 };
 
 // y[r*ld + perm[m]] = diag .* x + sum over row m of the copy (gathers from
-// xP, one vector at a time)
+// xP, one vector at a time, vectors nx apart)
 template <typename VT, typename XT, int CHUNK, bool PERSIST>
 __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
                                                      const VT* __restrict__ vals, const double* __restrict__ diag,
                                                      const uint32_t* __restrict__ perm, int64_t n,
-                                                     const XT* __restrict__ xP, const double* __restrict__ x,
+                                                     const XT* __restrict__ xP, int64_t nx, const double* __restrict__ x,
                                                      double* __restrict__ y, int k, int64_t ld, unsigned* counter) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   constexpr int U = 8;
@@ -1627,7 +1650,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__
       const int64_t I = perm[m];
       const double d = diag[I];
       for (int r = 0; r < k; r++) {
-        const XT* xr = xP + r * n;
+        const XT* xr = xP + r * nx;
         double acc[U];
 #pragma unroll
         for (int u = 0; u < U; u++) acc[u] = 0.0;
@@ -2665,6 +2688,14 @@ void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const
 
 static int computeUnits();
 
+void launchSplitCols(const int64_t* rpP, const int32_t* colsP, const uint32_t* perm, const uint32_t* bIdx,
+                     const uint32_t* bPos, int64_t n, int32_t* out, hipStream_t s) {
+  if (n <= 0) return;
+  if (2 * n > 0x7fffffffLL) throw HipError("too many rows for the split sigma columns");
+  kSplitCols<<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rpP, colsP, perm, bIdx, bPos, n, out);
+  HIP_CHECK(hipGetLastError());
+}
+
 void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_t* lidx, uint32_t* uniq,
                       int32_t* count, hipStream_t s) {
   if (n <= 0) return;
@@ -2676,10 +2707,12 @@ void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_
 
 void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* vals, const uint16_t* lidx,
                        const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
-                       int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter, hipStream_t s) {
+                       const uint32_t* bPerm, int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter,
+                       hipStream_t s) {
   if (n <= 0) return;
-  xP.reserve((size_t)n * sizeof(double));
-  kGatherPerm<double, 1><<<gridFor(n), kBlock, 0, s>>>(x, 1, n, perm, n, xP.as<double>(), counter);
+  const int64_t nx = bPerm ? 2 * n : n;
+  xP.reserve((size_t)nx * sizeof(double));
+  kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xP.as<double>(), counter);
   const int64_t items = (n + kStageRows - 1) / kStageRows;
   const unsigned grid = (unsigned)std::min<int64_t>(items, (int64_t)computeUnits() * 8);
   kSigmaStaged<true><<<grid, kBlock, 0, s>>>(rp, cols, vals, lidx, uniq, count, diag, perm, n, xP.as<double>(), x, y,
@@ -3060,8 +3093,9 @@ static int computeUnits() {
 
 template <typename VT, typename XT, int CHUNK, bool PERSIST>
 static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
-                           const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                           unsigned* counter, hipStream_t s) {
+                           const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
+                           int64_t ld, DevMem& xP, unsigned* counter, hipStream_t s) {
+  const int64_t nx = bPerm ? 2 * n : n;
   const int64_t chunks = (n + CHUNK - 1) / CHUNK;
   if (chunks > 0x7fffffffLL) throw HipError("too many rows for the relabelled sigma");
   unsigned grid = (unsigned)chunks;
@@ -3073,17 +3107,17 @@ static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* val
     auto spmm = [&](auto KT) {
       constexpr int K = decltype(KT)::value;
       const int take = kk < K ? kk : K;
-      xP.reserve((size_t)n * K * sizeof(XT));
-      kGatherPerm<XT, K><<<gridFor(n), kBlock, 0, s>>>(xc, take, ld, perm, n, xP.as<XT>(), counter);
+      xP.reserve((size_t)nx * K * sizeof(XT));
+      kGatherPerm<XT, K><<<gridFor(nx), kBlock, 0, s>>>(xc, take, ld, perm, bPerm, n, xP.as<XT>(), counter);
       kSpmmPerm<VT, XT, K, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), xc,
                                                                      yc, take, ld, counter);
       return take;
     };
     if (kk == 1) {
-      xP.reserve((size_t)n * sizeof(XT));
-      kGatherPerm<XT, 1><<<gridFor(n), kBlock, 0, s>>>(xc, 1, ld, perm, n, xP.as<XT>(), counter);
-      kSigmaPerm<VT, XT, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), xc, yc,
-                                                                   1, ld, counter);
+      xP.reserve((size_t)nx * sizeof(XT));
+      kGatherPerm<XT, 1><<<gridFor(nx), kBlock, 0, s>>>(xc, 1, ld, perm, bPerm, n, xP.as<XT>(), counter);
+      kSigmaPerm<VT, XT, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), nx, xc,
+                                                                   yc, 1, ld, counter);
       c0 += 1;
     } else if (kk == 2) {
       c0 += spmm(std::integral_constant<int, 2>());
@@ -3098,33 +3132,33 @@ static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* val
 
 template <typename VT, typename XT>
 static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
-                             const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld,
+                             const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k, int64_t ld,
                              DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
   if (n <= 0 || k <= 0) return;
   if (cfg.persistent && !counter) throw HipError("persistent sigma needs a work counter");
   switch (cfg.chunk * 2 + (cfg.persistent ? 1 : 0)) {
-    case 8: sigmaPermChunk<VT, XT, 4, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 9: sigmaPermChunk<VT, XT, 4, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 16: sigmaPermChunk<VT, XT, 8, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 17: sigmaPermChunk<VT, XT, 8, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 32: sigmaPermChunk<VT, XT, 16, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 33: sigmaPermChunk<VT, XT, 16, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 64: sigmaPermChunk<VT, XT, 32, false>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
-    case 65: sigmaPermChunk<VT, XT, 32, true>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, counter, s); break;
+    case 8: sigmaPermChunk<VT, XT, 4, false>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 9: sigmaPermChunk<VT, XT, 4, true>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 16: sigmaPermChunk<VT, XT, 8, false>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 17: sigmaPermChunk<VT, XT, 8, true>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 32: sigmaPermChunk<VT, XT, 16, false>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 33: sigmaPermChunk<VT, XT, 16, true>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 64: sigmaPermChunk<VT, XT, 32, false>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
+    case 65: sigmaPermChunk<VT, XT, 32, true>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, counter, s); break;
     default: throw HipError("unsupported sigma configuration");
   }
 }
 
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
-                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
-  launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, cfg, counter, s);
+                        const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
+                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+  launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
-                        const uint32_t* perm, int64_t n, const double* x, double* y, int k, int64_t ld, DevMem& xP,
-                        const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
-  launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, n, x, y, k, ld, xP, cfg, counter, s);
+                        const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
+                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+  launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,

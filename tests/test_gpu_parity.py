@@ -190,23 +190,28 @@ def test_incremental_hamiltonian(name, prec, nsrc, eps, systems):
 
 @pytest.mark.parametrize("name,prec", [("c0", 0), ("hub", 0), ("w2", 0), ("c0", 1)])
 def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
-    """Sigma through the alpha-major relabelled copy of H (the default) equals
-    sigma over H in determinant order: to 1e-12 relative with FP64 values;
-    with FP32 values the relabelled path also rounds x to FP32 (1e-6)."""
+    """Sigma through the alpha-major relabelled copy of H (the default, with
+    same-beta columns gathered from a beta-major copy of x) equals sigma over
+    H in determinant order: to 1e-12 relative with FP64 values; with FP32
+    values the relabelled path also rounds x to FP32 (1e-6).  The columns
+    kept all alpha-major give the same sums bit for bit, for one vector and
+    for the interleaved SpMM of two."""
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = []
-    for layout in ("relabelled", "external"):
-        if layout == "external":
-            monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "external")
+    for layout in ("relabelled", "external", "alpha"):
+        if layout != "relabelled":
+            monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
         g.set_precision(prec)
         g.set_dets(dets)
         g.make_hamiltonian()
-        out.append(g.hmult(x))
+        out.append((g.hmult(x), g.hmult(x[:, 0].copy())))
     tol = 1e-12 if prec == 0 else 1e-6
-    assert np.max(np.abs(out[0] - out[1])) <= tol * np.max(np.abs(out[1]))
+    assert np.max(np.abs(out[0][0] - out[1][0])) <= tol * np.max(np.abs(out[1][0]))
+    np.testing.assert_array_equal(out[0][0], out[2][0])
+    np.testing.assert_array_equal(out[0][1], out[2][1])
 
 
 @pytest.mark.parametrize("name", ["c0", "hub", "w2"])
