@@ -716,23 +716,24 @@ void Engine::buildPermuted() {
   launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
   exclusiveScanI64(rp, n + 1, ws, stream);
   permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-  if (f32) {
-    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(), n,
-                      rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
-  } else {
-    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
-    launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(), n,
-                      rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
-  }
-  permRows = n;
-  if (split && 2 * n < 0x7fffffffLL) {
+  const bool doSplit = split && 2 * n < 0x7fffffffLL;
+  if (doSplit) {
     bPos.reserve((size_t)n * 4);
     launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);
-    launchSplitCols(rp, permCols.as<int32_t>(), perm, T.bIdx.as<uint32_t>(), bPos.as<uint32_t>(), n,
-                    permCols.as<int32_t>(), stream);
-    splitRows = n;
   }
+  const uint32_t* bIdx = doSplit ? T.bIdx.as<uint32_t>() : nullptr;
+  const uint32_t* bp = doSplit ? bPos.as<uint32_t>() : nullptr;
+  if (f32) {
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(), bIdx,
+                      bp, n, rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
+  } else {
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
+    launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(), bIdx,
+                      bp, n, rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
+  }
+  permRows = n;
+  if (doSplit) splitRows = n;
   if (staged && !f32) {
     const size_t items = (size_t)(n + 3) / 4;
     const size_t extra = (size_t)nnzOff * 6 + items * 4;
@@ -1044,14 +1045,16 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
   if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(),
+                       stream);
   else if (perm && k == 1 && stagedRows == n)
     launchSigmaStaged(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), stLidx.as<uint16_t>(),
-                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, n,
-                      dx, dy, permX, sigmaCounter.as<unsigned>(), stream);
+                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(),
+                      bPerm, n, dx, dy, permX, sigmaCounter.as<unsigned>(), stream);
   else if (perm)
     launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(), stream);
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(),
+                       stream);
   else if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
                    diag.as<double>(), n, dx, dy, k, ld, stream);

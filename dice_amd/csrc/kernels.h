@@ -76,12 +76,6 @@ void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* val
                        const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
                        const uint32_t* bPerm, int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter,
                        hipStream_t s);
-// columns of the relabelled copy for the split gather from [x alpha-major |
-// x beta-major]: entries whose determinant has the row's beta string take
-// n + its beta-major position bPos[J], the others keep theirs (out may be
-// colsP)
-void launchSplitCols(const int64_t* rpP, const int32_t* colsP, const uint32_t* perm, const uint32_t* bIdx,
-                     const uint32_t* bPos, int64_t n, int32_t* out, hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial
@@ -205,13 +199,17 @@ void launchSplitHiLo(const double* v, float* hi, float* lo, int64_t n, hipStream
 void launchInversePerm(const uint32_t* perm, int64_t n, uint32_t* pos, hipStream_t s);
 // len[m] = length of row perm[m]
 void launchPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int64_t n, int64_t* len, hipStream_t s);
-// row m of the copy = row perm[m] with columns pos[J]
+// row m of the copy = row perm[m] with columns pos[J]; with bIdx (per
+// determinant beta string index) and bPos (position in the beta-major order,
+// inverse of StringTables::bMajorDet), columns J with the row's beta string
+// become n + bPos[J], for the sigma's gather from [x alpha-major | x
+// beta-major] (n < 2^30)
 void launchPermRowsF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const uint32_t* perm,
-                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, double* valsP,
-                       hipStream_t s);
+                       const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos, int64_t n,
+                       const int64_t* rpP, int32_t* colsP, double* valsP, hipStream_t s);
 void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const uint32_t* perm,
-                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, float* valsP,
-                       hipStream_t s);
+                       const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos, int64_t n,
+                       const int64_t* rpP, int32_t* colsP, float* valsP, hipStream_t s);
 // Launch shape of the relabelled sigma: rows per work item (4, 8, 16 or 32)
 // and static (one workgroup per work item) or persistent (workgroups pull
 // work items from an atomic counter).  The default is the fastest shape
@@ -225,7 +223,7 @@ struct SigmaConfig {
 // y must not overlap x; k > 1 runs as SpMM over up to 8 interleaved
 // vectors at a time.  xP is scratch for x in alpha-major order followed, when
 // bPerm (the beta-major determinant order) is given, by x in beta-major
-// order, for columns rewritten by launchSplitCols (FP64, or FP32 for the
+// order, for columns split by launchPermRows (FP64, or FP32 for the
 // FP32-value form); counter (device, one unsigned) is needed by the
 // persistent configuration.
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,

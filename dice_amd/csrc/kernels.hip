@@ -1524,38 +1524,25 @@ __global__ void kPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int
   }
 }
 
-// row m of the copy = row perm[m] of H with columns mapped through pos
+// row m of the copy = row perm[m] of H with columns mapped through pos; with
+// bIdx / bPos given, a column whose determinant has the row's beta string
+// maps to n + bPos[J] instead (its place in a beta-major copy of x that
+// follows the alpha-major one: a row's same-beta columns all lie in one beta
+// group there, so their gathers share lines)
 template <typename T1>
 __global__ void __launch_bounds__(kBlock) kPermRows(const int64_t* rowPtr, const int32_t* cols, const T1* vals,
-                                                    const uint32_t* perm, const uint32_t* pos, int64_t n,
+                                                    const uint32_t* perm, const uint32_t* pos,
+                                                    const uint32_t* bIdx, const uint32_t* bPos, int64_t n,
                                                     const int64_t* rpP, int32_t* colsP, T1* valsP) {
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   for (int64_t m = (int64_t)blockIdx.x * kWaves + wave; m < n; m += (int64_t)gridDim.x * kWaves) {
     const int64_t I = perm[m];
     const int64_t b = rowPtr[I], e = rowPtr[I + 1], d = rpP[m];
+    const uint32_t bi = bIdx ? bIdx[I] : 0u;
     for (int64_t p = b + lane; p < e; p += 64) {
-      colsP[d + (p - b)] = (int32_t)pos[__builtin_nontemporal_load(cols + p)];
+      const int32_t J = __builtin_nontemporal_load(cols + p);
+      colsP[d + (p - b)] = bIdx && bIdx[J] == bi ? (int32_t)(n + bPos[J]) : (int32_t)pos[J];
       valsP[d + (p - b)] = __builtin_nontemporal_load(vals + p);
-    }
-  }
-}
-
-// columns of the relabelled copy for a gather from [x alpha-major | x
-// beta-major]: an entry whose determinant has the row's beta string takes
-// n + its position in the beta-major order (the row's same-beta columns all
-// lie in one beta group there), every other entry keeps its alpha-major
-// position
-// (in place: out may be colsP)
-__global__ void __launch_bounds__(kBlock) kSplitCols(const int64_t* __restrict__ rpP, const int32_t* colsP,
-                                                     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ bIdx,
-                                                     const uint32_t* __restrict__ bPos, int64_t n, int32_t* out) {
-  const int wave = threadIdx.x >> 6, lane = __lane_id();
-  for (int64_t m = (int64_t)blockIdx.x * kWaves + wave; m < n; m += (int64_t)gridDim.x * kWaves) {
-    const uint32_t bi = bIdx[perm[m]];
-    for (int64_t p = rpP[m] + lane; p < rpP[m + 1]; p += 64) {
-      const int32_t c = colsP[p];
-      const uint32_t J = perm[c];
-      out[p] = bIdx[J] == bi ? (int32_t)(n + bPos[J]) : c;
     }
   }
 }
@@ -2688,14 +2675,6 @@ void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const
 
 static int computeUnits();
 
-void launchSplitCols(const int64_t* rpP, const int32_t* colsP, const uint32_t* perm, const uint32_t* bIdx,
-                     const uint32_t* bPos, int64_t n, int32_t* out, hipStream_t s) {
-  if (n <= 0) return;
-  if (2 * n > 0x7fffffffLL) throw HipError("too many rows for the split sigma columns");
-  kSplitCols<<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rpP, colsP, perm, bIdx, bPos, n, out);
-  HIP_CHECK(hipGetLastError());
-}
-
 void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_t* lidx, uint32_t* uniq,
                       int32_t* count, hipStream_t s) {
   if (n <= 0) return;
@@ -3063,20 +3042,22 @@ void launchPermRowLengths(const int64_t* rowPtr, const uint32_t* perm, int64_t n
 }
 
 void launchPermRowsF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const uint32_t* perm,
-                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, double* valsP,
-                       hipStream_t s) {
+                       const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos, int64_t n,
+                       const int64_t* rpP, int32_t* colsP, double* valsP, hipStream_t s) {
   if (n <= 0) return;
-  kPermRows<double><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, n, rpP, colsP,
-                                                                     valsP);
+  if (bIdx && 2 * n > 0x7fffffffLL) throw HipError("too many rows for the split relabelled columns");
+  kPermRows<double><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, bIdx, bPos, n, rpP,
+                                                                     colsP, valsP);
   HIP_CHECK(hipGetLastError());
 }
 
 void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const uint32_t* perm,
-                       const uint32_t* pos, int64_t n, const int64_t* rpP, int32_t* colsP, float* valsP,
-                       hipStream_t s) {
+                       const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos, int64_t n,
+                       const int64_t* rpP, int32_t* colsP, float* valsP, hipStream_t s) {
   if (n <= 0) return;
-  kPermRows<float><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, n, rpP, colsP,
-                                                                    valsP);
+  if (bIdx && 2 * n > 0x7fffffffLL) throw HipError("too many rows for the split relabelled columns");
+  kPermRows<float><<<gridFor(n, kWaves, 1 << 16), kBlock, 0, s>>>(rowPtr, cols, vals, perm, pos, bIdx, bPos, n, rpP,
+                                                                    colsP, valsP);
   HIP_CHECK(hipGetLastError());
 }
 
