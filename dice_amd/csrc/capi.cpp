@@ -100,7 +100,7 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   // evaluation threshold (default 0.8)
   if (const char* m = std::getenv("SHCI_LEARNED_MARGIN")) E.learnedMargin = std::atof(m);
   if (!E.autotune) {  // the shapes measured fastest on the benchmark matrices
-    E.sigmaCfg[0] = SigmaConfig{8, true};
+    E.sigmaCfg[0] = SigmaConfig{4, true};
     E.sigmaCfg[1] = SigmaConfig{8, false};
     E.tunedRows = 0;
   }

@@ -212,9 +212,9 @@ void launchPermRowsF32(const int64_t* rowPtr, const int32_t* cols, const float* 
                        const int64_t* rpP, int32_t* colsP, float* valsP, hipStream_t s);
 // Launch shape of the relabelled sigma: rows per work item (4, 8, 16 or 32)
 // and static (one workgroup per work item) or persistent (workgroups pull
-// work items from an atomic counter).  The default is the fastest shape
-// measured on the 1.2M-determinant benchmark matrix (8 rows, persistent:
-// 7.6 ms against 8.3 ms for 16 rows, static).
+// work items from an atomic counter).  The struct's default is 8 rows,
+// persistent (7.6 ms against 8.3 ms for 16 rows, static, on the
+// 1.2M-determinant benchmark matrix with every column alpha-major).
 struct SigmaConfig {
   int chunk = 8;
   bool persistent = true;
