@@ -343,7 +343,7 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
     if (log) fprintf(log, "%s\n", m.c_str());
     return fail(h, SHCI_UNSUPPORTED, m);
   }
-  std::srand(schd.randomSeed + 0);
+  E.rng.reseed((unsigned)(schd.randomSeed + 0));  // srand(seed + rank), SHCI/SHCI.cpp:199
   E.load(schd.integralFile);
   E.trev = schd.Trev;
   E.S.trev = schd.Trev != 0;
@@ -1022,7 +1022,7 @@ shci_status shci_pt_semistochastic(shci_handle* h, const double* ci, double e0, 
     dc.reserve((size_t)std::max<int64_t>(n, 1) * 8);
     if (n) HIP_CHECK(hipMemcpyAsync(dc.p, ci, n * 8, hipMemcpyHostToDevice, E.stream));
     HIP_CHECK(hipStreamSynchronize(E.stream));
-    std::srand(seed);
+    E.rng.reseed(seed);
     std::vector<double> per;
     *e_pt = E.ptSemistochastic(dc.as<double>(), e0, eps2, eps2_large, sample_n, n_iter, target_error, 0.0, 0,
                                nullptr, 0.0, error, iterations, &per);

@@ -39,6 +39,11 @@ void Engine::initShared(const Engine& parent) {
   ownsStream = false;
   precision = parent.precision;
   relabel = parent.relabel;
+  split = parent.split;
+  autotune = parent.autotune;
+  davidsonMode = parent.davidsonMode;
+  sigmaCfg[0] = parent.sigmaCfg[0];
+  sigmaCfg[1] = parent.sigmaCfg[1];
   incremental = false;
   counter.reserve(64);
 }
@@ -171,8 +176,6 @@ void Engine::load(const std::string& fcidump) {
   // SHCI_SIGMA_LAYOUT=external keeps sigma on H in determinant order
   const char* layout = std::getenv("SHCI_SIGMA_LAYOUT");
   relabel = !(layout && std::strcmp(layout, "external") == 0);
-  // SHCI_SIGMA_LAYOUT=staged adds the staged form of the relabelled copy
-  staged = layout && std::strcmp(layout, "staged") == 0;
   // SHCI_SIGMA_LAYOUT=alpha keeps every relabelled column alpha-major (no
   // beta-major copy of x)
   split = !(layout && std::strcmp(layout, "alpha") == 0);
@@ -333,8 +336,11 @@ double Engine::ptSemistochastic(const double* dCi, double e0, double eps2, doubl
     std::fill(sample.begin(), sample.end(), -1);
     int distinct = 0;
     for (int index = 0; index < Nmc; index++) {
-      int detIndex = (int)std::floor(1. * ((double)std::rand() / (RAND_MAX)) * N);
-      const double r = ((double)std::rand() / (RAND_MAX));
+      // floor(r / RAND_MAX * N) (SHCIsampledeterminants.cpp:160), clamped:
+      // r == RAND_MAX would index one past the alias and probability tables
+      int detIndex = (int)std::floor(1. * ((double)rng.next() / (LibcRand::kMax)) * N);
+      if (detIndex >= N) detIndex = N - 1;
+      const double r = ((double)rng.next() / (LibcRand::kMax));
       if (r >= prob[detIndex]) detIndex = alias[detIndex];
       auto it = std::find(sample.begin(), sample.end(), detIndex);
       if (it == sample.end()) {
@@ -695,14 +701,20 @@ void Engine::buildHamiltonian() {
 // the sigma kernel; skipped when it does not fit beside H.
 void Engine::buildPermuted() {
   permRows = -1;
-  stagedRows = -1;
   splitRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
-  const size_t need = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4;
+  // the copy (columns, values, row offsets, inverse permutation) with 64
+  // bytes per determinant of slack, and the permuted x of the sigma calls:
+  // up to 8 interleaved vectors in the block Davidson, in alpha-major order
+  // and, with split columns, beta-major order too (plus bPos)
+  const size_t xt = f32 ? 4 : 8, kmax = davidsonMode == 1 ? 8 : 1;
+  const size_t base = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4 + (size_t)n * 64;
+  const size_t xPlain = (size_t)n * kmax * xt, xSplit = 2 * (size_t)n * kmax * xt + (size_t)n * 4;
   size_t freeB = 0, totalB = 0;
   freeB = deviceFreeBytes(&totalB);
-  if (need + (size_t)n * 64 > freeB + permCols.bytes + permVals.bytes) {
+  const size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes;
+  if (base + xPlain > have) {
     permCols.release();
     permVals.release();
     return;
@@ -716,7 +728,8 @@ void Engine::buildPermuted() {
   launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
   exclusiveScanI64(rp, n + 1, ws, stream);
   permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-  const bool doSplit = split && 2 * n < 0x7fffffffLL;
+  // split columns only when their x copy fits too
+  const bool doSplit = split && 2 * n < 0x7fffffffLL && base + xSplit <= have;
   if (doSplit) {
     bPos.reserve((size_t)n * 4);
     launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);
@@ -734,19 +747,6 @@ void Engine::buildPermuted() {
   }
   permRows = n;
   if (doSplit) splitRows = n;
-  if (staged && !f32) {
-    const size_t items = (size_t)(n + 3) / 4;
-    const size_t extra = (size_t)nnzOff * 6 + items * 4;
-    freeB = deviceFreeBytes(&totalB);
-    if (extra + (size_t)n * 64 < freeB + stLidx.bytes + stUniq.bytes + stCount.bytes) {
-      stLidx.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 2);
-      stUniq.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-      stCount.reserve(items * 4);
-      launchStageBuild(rp, permCols.as<int32_t>(), n, stLidx.as<uint16_t>(), stUniq.as<uint32_t>(),
-                       stCount.as<int32_t>(), stream);
-      stagedRows = n;
-    }
-  }
   // re-tune when the space has doubled since the last tuning (small spaces
   // keep the default shape: their sigma is too short to time reliably)
   if (autotune && n >= 20000 && n >= 2 * tunedRows) tuneSigma();
@@ -944,7 +944,7 @@ std::vector<double> Engine::davidsonBlock(double* dX0, int nr, double tol, int& 
   // setRandom), orthonormalised
   auto randomColumn = [&](double* x) {
     std::vector<double> h(N);
-    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + 2.0 * double(std::rand()) / double(RAND_MAX);
+    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + 2.0 * double(rng.next()) / double(LibcRand::kMax);
     HIP_CHECK(hipMemcpyAsync(x, h.data(), N * 8, hipMemcpyHostToDevice, s));
     sync();
   };
@@ -1043,17 +1043,14 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
   if (perm) sigmaCounter.reserve(256);
   const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
+  const XScratch xs{&permX};
   if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(),
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, sigmaCounter.as<unsigned>(),
                        stream);
-  else if (perm && k == 1 && stagedRows == n)
-    launchSigmaStaged(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), stLidx.as<uint16_t>(),
-                      stUniq.as<uint32_t>(), stCount.as<int32_t>(), diag.as<double>(), T.aMajorDet.as<uint32_t>(),
-                      bPerm, n, dx, dy, permX, sigmaCounter.as<unsigned>(), stream);
   else if (perm)
     launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
-                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, permX, cfg, sigmaCounter.as<unsigned>(),
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, sigmaCounter.as<unsigned>(),
                        stream);
   else if (f32)
     launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), exact ? valsLo.as<float>() : nullptr,
@@ -1111,7 +1108,7 @@ std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double 
   };
   auto setRandom = [&](double* x) {
     std::vector<double> h(N);
-    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + (1.0 - (-1.0)) * double(std::rand()) / double(RAND_MAX);
+    for (int64_t k = 0; k < N; k++) h[k] = -1.0 + (1.0 - (-1.0)) * double(rng.next()) / double(LibcRand::kMax);
     HIP_CHECK(hipMemcpyAsync(x, h.data(), N * 8, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
   };

@@ -34,6 +34,9 @@ struct Engine {
   int ncore = 0, nact = 1000000;
   std::vector<Restriction> restrictions;
   std::vector<Restriction> restrictionsPT;  // the PT space's ("restrict", not "restrictv")
+  // the C library's rand() sequence for the random Davidson guesses and the
+  // PT alias sampling, seeded like the reference's srand(seed)
+  LibcRand rng;
 
   // system tables
   DevMem dH1, dI2, dDirect, dExch, dTsame, dTopp, dSingles, dHbVal[2], dHbPair[2], dHbStart[2], dRmult, dRmultPt;
@@ -62,11 +65,6 @@ struct Engine {
   bool relabel = true;
   DevMem permPos, permRp, permCols, permVals, permX;
   int64_t permRows = -1;
-  // staged form of the FP64 relabelled copy (distinct columns per work item,
-  // 16-bit ranks per entry); valid when stagedRows == n
-  bool staged = false;
-  DevMem stLidx, stUniq, stCount;
-  int64_t stagedRows = -1;
   // split columns: in the relabelled copy, columns whose determinant shares
   // the row's beta string point into a beta-major copy of x (n + position);
   // valid when splitRows == n

@@ -10,6 +10,41 @@
 
 namespace shci {
 
+// The C library's rand() sequence (glibc's additive feedback generator,
+// TYPE_3: 31 words, separation 3, seeded by the Park-Miller minimal
+// standard generator through Schrage's method and 310 discarded outputs),
+// held privately so that nothing else in the process (the GPU runtime
+// included) can advance it between the reference's srand(seed) and its
+// draws (SHCI/SHCI.cpp:199; alias sampling, SHCI/SHCIsampledeterminants.cpp:148-176;
+// random Davidson guesses, eigen/Eigen/src/Core/MathFunctions.h:825-834).
+struct LibcRand {
+  static const int kMax = 2147483647;  // RAND_MAX
+  int32_t state[31];
+  int f = 3, r = 0;
+  explicit LibcRand(unsigned seed = 1) { reseed(seed); }
+  void reseed(unsigned seed) {
+    if (seed == 0) seed = 1;
+    state[0] = (int32_t)seed;
+    int32_t word = (int32_t)seed;
+    for (int i = 1; i < 31; i++) {
+      const long hi = word / 127773, lo = word % 127773;
+      word = (int32_t)(16807 * lo - 2836 * hi);
+      if (word < 0) word += 2147483647;
+      state[i] = word;
+    }
+    f = 3;
+    r = 0;
+    for (int i = 0; i < 310; i++) (void)next();
+  }
+  int next() {
+    const uint32_t val = (uint32_t)state[f] + (uint32_t)state[r];
+    state[f] = (int32_t)val;
+    f = f + 1 == 31 ? 0 : f + 1;
+    r = r + 1 == 31 ? 0 : r + 1;
+    return (int)(val >> 1);
+  }
+};
+
 struct HostSystem {
   int norb = 0, nso = 0, W = 1, nelec = 0;
   double coreE = 0.0;

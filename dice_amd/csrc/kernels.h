@@ -65,17 +65,6 @@ void launchTrevContract(const double* yE, int64_t ldE, int64_t n, int k, const i
 // the unpacked coefficients in E (Treversal sign trev)
 void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const int64_t* partner,
                       const double* parity, int trev, double* out, int64_t ldE, hipStream_t s);
-// staged layout of the relabelled FP64 CSR (per work item of consecutive
-// rows: the sorted distinct columns in uniq at the item's first entry, and
-// per entry the 16-bit rank of its column; count = distinct columns, or -1
-// for items too large to stage), and the sigma over it (one vector, x and y
-// in determinant order; same results as the relabelled sigma)
-void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_t* lidx, uint32_t* uniq,
-                      int32_t* count, hipStream_t s);
-void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* vals, const uint16_t* lidx,
-                       const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
-                       const uint32_t* bPerm, int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter,
-                       hipStream_t s);
 void launchIota(uint32_t* v, int64_t n, hipStream_t s);
 // RDMs of <bra| ... |ket> over the n determinants and their stored
 // connections (full symmetric CSR rp / cols), accumulated into the spatial
@@ -219,19 +208,31 @@ struct SigmaConfig {
   int chunk = 8;
   bool persistent = true;
 };
+// Scratch for the permuted x of the relabelled sigma: a buffer grown on
+// demand (mem), or a fixed device region of fixedBytes (the allocation-free
+// entry point; a larger request throws).
+struct XScratch {
+  DevMem* mem = nullptr;
+  void* fixed = nullptr;
+  size_t fixedBytes = 0;
+  void* get(size_t bytes) const;
+};
+// bytes of permuted x the relabelled sigma of k vectors needs (the largest
+// pass: 1, 2, 4 or 8 interleaved vectors), xBytes = sizeof of its x type
+size_t sigmaPermScratchBytes(int64_t n, bool split, int k, size_t xBytes);
 // y = H x through the copy; x, y in determinant order (column-major, ld),
 // y must not overlap x; k > 1 runs as SpMM over up to 8 interleaved
 // vectors at a time.  xP is scratch for x in alpha-major order followed, when
 // bPerm (the beta-major determinant order) is given, by x in beta-major
 // order, for columns split by launchPermRows (FP64, or FP32 for the
-// FP32-value form); counter (device, one unsigned) is needed by the
-// persistent configuration.
+// FP32-value form); counter (device, 64 unsigned words) is needed by the
+// persistent configuration and is cleared by the launch itself.
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
-                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
+                        int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
-                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
+                        int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <string>
 
 #include "kernels.h"
 
@@ -1665,154 +1666,6 @@ __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__
   }
 }
 
-// Staged layout of the relabelled CSR (FP64, one vector): per work item of
-// kStageRows consecutive rows, the distinct columns its entries touch
-// (sorted, uniq[first entry of the item ...]) and, per entry, the 16-bit
-// rank of its column among them.  Consecutive rows of the alpha-major order
-// share most of their x lines (16.7 entries per distinct 128-byte line over
-// eight rows on a 27,685-determinant space, against 2.8 within one
-// wavefront's 64 entries), so the sigma kernel gathers each distinct x once
-// per work item into LDS and the entries read x from LDS.  Items with more
-// than kStageMaxEntries entries keep the per-row gathers (count -1).
-constexpr int kStageRows = 4;
-constexpr int kStageMaxEntries = 8192;
-constexpr int kStageMaxUniq = 4096;  // 32 KB of x per workgroup
-
-__global__ void __launch_bounds__(kBlock) kStageBuild(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                      int64_t n, uint16_t* __restrict__ lidx,
-                                                      uint32_t* __restrict__ uniq, int32_t* __restrict__ count) {
-  __shared__ uint64_t key[kStageMaxEntries];
-  __shared__ int sTot[kWaves + 1];
-  const int wave = threadIdx.x >> 6, lane = __lane_id();
-  const int64_t items = (n + kStageRows - 1) / kStageRows;
-  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const int64_t r0 = it * kStageRows, r1 = r0 + kStageRows < n ? r0 + kStageRows : n;
-    const int64_t b = rp[r0], e = rp[r1];
-    const int E = (int)(e - b);
-    if (E > kStageMaxEntries || E == 0) {
-      if (threadIdx.x == 0) count[it] = E == 0 ? 0 : -1;
-      continue;
-    }
-    int P = 1;
-    while (P < E) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-      key[i] = i < E ? ((uint64_t)(uint32_t)cols[b + i] << 16) | (uint64_t)i : ~0ull;
-    __syncthreads();
-    // bitonic sort of P keys (column, entry offset)
-    for (int k = 2; k <= P; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-          const int l = i ^ j;
-          if (l > i) {
-            const uint64_t a = key[i], c = key[l];
-            const bool up = (i & k) == 0;
-            if ((a > c) == up) {
-              key[i] = c;
-              key[l] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    // rank of each distinct column: flags, block prefix sum in chunks of kBlock
-    int base = 0;
-    for (int c0 = 0; c0 < E; c0 += kBlock) {
-      const int i = c0 + threadIdx.x;
-      const bool first = i < E && (i == 0 || (key[i] >> 16) != (key[i - 1] >> 16));
-      const uint64_t bal = __ballot(first);
-      if (lane == 0) sTot[wave] = __popcll(bal);
-      __syncthreads();
-      int off = base;
-      for (int w = 0; w < wave; w++) off += sTot[w];
-      int tot = 0;
-      for (int w = 0; w < kWaves; w++) tot += sTot[w];
-      const int rank = off + __popcll(bal & lanemaskLt()) - (first ? 0 : 1);
-      if (i < E) {
-        lidx[b + (int)(key[i] & 0xFFFF)] = (uint16_t)rank;
-        if (first) uniq[b + rank] = (uint32_t)(key[i] >> 16);
-      }
-      __syncthreads();
-      base += tot;
-    }
-    if (threadIdx.x == 0) count[it] = base <= kStageMaxUniq ? base : -1;
-    __syncthreads();
-  }
-}
-
-// sigma = H x on the staged layout (FP64 values, FP64 x in relabelled order):
-// per work item, the distinct x values into LDS, then one wavefront per row
-// with the same per-lane accumulation order as kSigmaPerm (results are
-// identical); items without a staging gather per row.
-template <bool PERSIST>
-__global__ void __launch_bounds__(kBlock) kSigmaStaged(const int64_t* __restrict__ rp, const int32_t* __restrict__ cols,
-                                                       const double* __restrict__ vals,
-                                                       const uint16_t* __restrict__ lidx,
-                                                       const uint32_t* __restrict__ uniq,
-                                                       const int32_t* __restrict__ count,
-                                                       const double* __restrict__ diag,
-                                                       const uint32_t* __restrict__ perm, int64_t n,
-                                                       const double* __restrict__ xP, const double* __restrict__ x,
-                                                       double* __restrict__ y, unsigned* counter) {
-  __shared__ double xs[kStageMaxUniq];
-  const int wave = threadIdx.x >> 6, lane = __lane_id();
-  constexpr int U = 8;
-  ChunkSched<kStageRows, PERSIST> sched{(n + kStageRows - 1) / kStageRows, counter, -1};
-  while (sched.next()) {
-    const int64_t it = sched.cur;
-    const int64_t r0 = it * kStageRows;
-    const int64_t r1 = r0 + kStageRows < n ? r0 + kStageRows : n;
-    const int cnt = count[it];
-    const int64_t base = rp[r0];
-    if (cnt > 0) {
-      for (int k = threadIdx.x; k < cnt; k += blockDim.x) xs[k] = xP[uniq[base + k]];
-    }
-    __syncthreads();
-    for (int64_t m = r0 + wave; m < r1; m += kWaves) {
-      const int64_t b = rp[m], e = rp[m + 1];
-      const int64_t I = perm[m];
-      const double d = diag[I];
-      double acc[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) acc[u] = 0.0;
-      int64_t p = b + lane;
-      if (cnt >= 0) {
-        for (; p + 64 * (U - 1) < e; p += 64 * U) {
-          uint16_t c[U];
-          double v[U];
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            c[u] = __builtin_nontemporal_load(lidx + p + 64 * u);
-            v[u] = __builtin_nontemporal_load(vals + p + 64 * u);
-          }
-#pragma unroll
-          for (int u = 0; u < U; u++) acc[u] = fma(v[u], xs[c[u]], acc[u]);
-        }
-        for (; p < e; p += 64) acc[0] = fma(vals[p], xs[lidx[p]], acc[0]);
-      } else {
-        for (; p + 64 * (U - 1) < e; p += 64 * U) {
-          int32_t c[U];
-          double v[U];
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
-            v[u] = __builtin_nontemporal_load(vals + p + 64 * u);
-          }
-#pragma unroll
-          for (int u = 0; u < U; u++) acc[u] = fma(v[u], xP[c[u]], acc[u]);
-        }
-        for (; p < e; p += 64) acc[0] = fma(vals[p], xP[cols[p]], acc[0]);
-      }
-      double sum = 0.0;
-#pragma unroll
-      for (int u = 0; u < U; u++) sum += acc[u];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-      if (lane == 0) y[I] = fma(d, x[I], sum);
-    }
-    __syncthreads();
-  }
-}
-
 // SpMM: K interleaved vectors per gather (one K*sizeof(XT)-byte load per
 // stored element serves all K products); rows read once.
 template <typename XT, int K>
@@ -2675,30 +2528,6 @@ void launchTrevUnpack(const double* ci, int64_t n, int nroots, int64_t ld, const
 
 static int computeUnits();
 
-void launchStageBuild(const int64_t* rp, const int32_t* cols, int64_t n, uint16_t* lidx, uint32_t* uniq,
-                      int32_t* count, hipStream_t s) {
-  if (n <= 0) return;
-  const int64_t items = (n + kStageRows - 1) / kStageRows;
-  kStageBuild<<<(unsigned)std::min<int64_t>(items, (int64_t)computeUnits() * 4), kBlock, 0, s>>>(rp, cols, n, lidx,
-                                                                                                  uniq, count);
-  HIP_CHECK(hipGetLastError());
-}
-
-void launchSigmaStaged(const int64_t* rp, const int32_t* cols, const double* vals, const uint16_t* lidx,
-                       const uint32_t* uniq, const int32_t* count, const double* diag, const uint32_t* perm,
-                       const uint32_t* bPerm, int64_t n, const double* x, double* y, DevMem& xP, unsigned* counter,
-                       hipStream_t s) {
-  if (n <= 0) return;
-  const int64_t nx = bPerm ? 2 * n : n;
-  xP.reserve((size_t)nx * sizeof(double));
-  kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xP.as<double>(), counter);
-  const int64_t items = (n + kStageRows - 1) / kStageRows;
-  const unsigned grid = (unsigned)std::min<int64_t>(items, (int64_t)computeUnits() * 8);
-  kSigmaStaged<true><<<grid, kBlock, 0, s>>>(rp, cols, vals, lidx, uniq, count, diag, perm, n, xP.as<double>(), x, y,
-                                             counter);
-  HIP_CHECK(hipGetLastError());
-}
-
 void launchIota(uint32_t* v, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   kIota<<<gridFor(n, kBlock, 8192), kBlock, 0, s>>>(v, n);
@@ -3072,10 +2901,26 @@ static int computeUnits() {
   return cus;
 }
 
+void* XScratch::get(size_t bytes) const {
+  if (mem) {
+    mem->reserve(bytes);
+    return mem->p;
+  }
+  if (bytes > fixedBytes)
+    throw HipError("sigma needs " + std::to_string(bytes) + " bytes of x scratch, more than the fixed " +
+                   std::to_string(fixedBytes));
+  return fixed;
+}
+
+size_t sigmaPermScratchBytes(int64_t n, bool split, int k, size_t xBytes) {
+  const int K = k <= 1 ? 1 : k == 2 ? 2 : k <= 4 ? 4 : 8;
+  return (size_t)(split ? 2 * n : n) * K * xBytes;
+}
+
 template <typename VT, typename XT, int CHUNK, bool PERSIST>
 static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
                            const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
-                           int64_t ld, DevMem& xP, unsigned* counter, hipStream_t s) {
+                           int64_t ld, const XScratch& xP, unsigned* counter, hipStream_t s) {
   const int64_t nx = bPerm ? 2 * n : n;
   const int64_t chunks = (n + CHUNK - 1) / CHUNK;
   if (chunks > 0x7fffffffLL) throw HipError("too many rows for the relabelled sigma");
@@ -3088,17 +2933,17 @@ static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* val
     auto spmm = [&](auto KT) {
       constexpr int K = decltype(KT)::value;
       const int take = kk < K ? kk : K;
-      xP.reserve((size_t)nx * K * sizeof(XT));
-      kGatherPerm<XT, K><<<gridFor(nx), kBlock, 0, s>>>(xc, take, ld, perm, bPerm, n, xP.as<XT>(), counter);
-      kSpmmPerm<VT, XT, K, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), xc,
-                                                                     yc, take, ld, counter);
+      XT* xs = (XT*)xP.get((size_t)nx * K * sizeof(XT));
+      kGatherPerm<XT, K><<<gridFor(nx), kBlock, 0, s>>>(xc, take, ld, perm, bPerm, n, xs, counter);
+      kSpmmPerm<VT, XT, K, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xs, xc, yc, take,
+                                                                     ld, counter);
       return take;
     };
     if (kk == 1) {
-      xP.reserve((size_t)nx * sizeof(XT));
-      kGatherPerm<XT, 1><<<gridFor(nx), kBlock, 0, s>>>(xc, 1, ld, perm, bPerm, n, xP.as<XT>(), counter);
-      kSigmaPerm<VT, XT, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xP.as<XT>(), nx, xc,
-                                                                   yc, 1, ld, counter);
+      XT* xs = (XT*)xP.get((size_t)nx * sizeof(XT));
+      kGatherPerm<XT, 1><<<gridFor(nx), kBlock, 0, s>>>(xc, 1, ld, perm, bPerm, n, xs, counter);
+      kSigmaPerm<VT, XT, CHUNK, PERSIST><<<grid, kBlock, 0, s>>>(rp, cols, vals, diag, perm, n, xs, nx, xc, yc, 1,
+                                                                   ld, counter);
       c0 += 1;
     } else if (kk == 2) {
       c0 += spmm(std::integral_constant<int, 2>());
@@ -3114,7 +2959,7 @@ static void sigmaPermChunk(const int64_t* rp, const int32_t* cols, const VT* val
 template <typename VT, typename XT>
 static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* vals, const double* diag,
                              const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k, int64_t ld,
-                             DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+                             const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
   if (n <= 0 || k <= 0) return;
   if (cfg.persistent && !counter) throw HipError("persistent sigma needs a work counter");
   switch (cfg.chunk * 2 + (cfg.persistent ? 1 : 0)) {
@@ -3132,13 +2977,13 @@ static void launchSigmaPermT(const int64_t* rp, const int32_t* cols, const VT* v
 
 void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* vals, const double* diag,
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
-                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+                        int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
   launchSigmaPermT<double, double>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
-                        int64_t ld, DevMem& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
+                        int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
   launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, cfg, counter, s);
 }
 

@@ -925,6 +925,15 @@ static void aliasSetup(const double* ci, int n, double cumulative, std::vector<i
   }
 }
 
+// The determinant index of one alias draw from a raw rand() value r
+// (SHCIsampledeterminants.cpp:160): floor(r / RAND_MAX * n), clamped to
+// n - 1.  The reference leaves r == RAND_MAX at index n, one past its alias
+// and probability tables; the clamp keeps every other draw unchanged.
+int aliasDrawIndex(int r, int n) {
+  const int i = (int)std::floor(1. * ((double)r / (RAND_MAX)) * n);
+  return i < n ? i : n - 1;
+}
+
 // sample_N2_alias (SHCIsampledeterminants.cpp:148-176): Nmc draws with
 // replacement; distinct determinants in first-drawn order with weights
 // cumulative * sign(c) * (times drawn) / Nmc; returns the distinct count
@@ -933,7 +942,7 @@ static int sampleAlias(const double* ci, int n, double cumulative, std::vector<i
   const int niter = (int)sample.size();
   int sampleIndex = 0;
   for (int index = 0; index < niter; index++) {
-    int detIndex = (int)std::floor(1. * ((double)rand() / (RAND_MAX)) * n);
+    int detIndex = aliasDrawIndex(rand(), n);
     double rand_no = ((double)rand() / (RAND_MAX));
     if (rand_no >= prob[detIndex]) detIndex = alias[detIndex];
     auto it = std::find(sample.begin(), sample.end(), detIndex);

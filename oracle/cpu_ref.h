@@ -193,6 +193,10 @@ double ptDeterministic(const System& s, const Schedule& schd, const std::vector<
 // function returns (the average energy correction, or the unaveraged sum
 // when nPTiter runs out); *error / *iters receive the last standard error and
 // the iteration count.
+// index of one alias draw from a raw rand() value r: floor(r / RAND_MAX * n)
+// clamped to n - 1 (SHCIsampledeterminants.cpp:160 without its r == RAND_MAX
+// overrun)
+int aliasDrawIndex(int r, int n);
 double ptSemistochastic(const System& s, const Schedule& schd, const std::vector<Det>& dets, const double* ci,
                         double E0, double coreE, int root, FILE* log, double startofCalc, double* error = nullptr,
                         int* iters = nullptr, std::vector<double>* perIter = nullptr);

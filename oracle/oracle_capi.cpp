@@ -219,6 +219,8 @@ int oref_davidson(void* p, double* x0, int nroots, const double* diag, int maxCo
 }
 
 void oref_sym_eig(int m, const double* A, double* w, double* V) { symEig(m, A, w, V); }
+int oref_alias_draw_index(int r, int n) { return aliasDrawIndex(r, n); }
+int oref_rand_max() { return RAND_MAX; }
 
 // ---- whole variational runs (input.dat driven) -----------------------------
 void* oref_run(const char* input_path, const char* log_path, int record_iters) {

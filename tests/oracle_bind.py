@@ -55,6 +55,8 @@ def load():
         "oref_rdm": ([P, pD, pD, pD, pD, pD, pD], None),
         "oref_davidson": ([P, pD, I32, pD, I32, D, pD], I32),
         "oref_sym_eig": ([I32, pD, pD, pD], None),
+        "oref_alias_draw_index": ([I32, I32], I32),
+        "oref_rand_max": ([], I32),
         "oref_run": ([ctypes.c_char_p, ctypes.c_char_p, I32], P),
         "oref_run_status": ([P], I32),
         "oref_run_error": ([P], ctypes.c_char_p),

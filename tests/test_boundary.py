@@ -73,3 +73,35 @@ def test_synthetic_fcidump_format(tmp_path):
     first = lines[4]
     assert len(first.split()[0]) == 18 and first[:20].strip() == first.split()[0]
     assert lines[-1].split()[1:] == ["0", "0", "0", "0"]
+
+
+def test_private_rand_sequence_equals_the_c_library(tmp_path):
+    """The library draws the reference's rand() sequence (alias sampling of
+    the semistochastic PT, random Davidson guesses) from a private copy of
+    the C library's generator, so that other code in the process cannot
+    advance it between srand(seed) and the draws: its outputs equal
+    srand(seed); rand() for seeds including 0 and ones above 2^31."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cstdlib>
+#include "host_io.h"
+int main() {
+  const unsigned seeds[] = {0u, 1u, 7u, 12345u, 2147483647u, 2147483648u, 4294967295u, 1764763200u};
+  long bad = 0;
+  for (unsigned s : seeds) {
+    srand(s);
+    shci::LibcRand g(s);
+    for (int i = 0; i < 20000; i++)
+      if (rand() != g.next()) bad++;
+  }
+  std::printf("%ld %d\n", bad, RAND_MAX == shci::LibcRand::kMax);
+  return 0;
+}
+''')
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "dice_amd", "csrc"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out == ["0", "1"]

@@ -214,30 +214,6 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     np.testing.assert_array_equal(out[0][1], out[2][1])
 
 
-@pytest.mark.parametrize("name", ["c0", "hub", "w2"])
-def test_staged_sigma_bitwise(name, systems, monkeypatch):
-    """The staged form of the relabelled copy (distinct x values of each
-    four-row work item gathered once into LDS, 16-bit ranks per entry) keeps
-    every row's summation order: one-vector sigma is bit-identical to the
-    relabelled sigma, also after the space grows."""
-    path = systems[name][0]
-    dets = _space(name, systems, 3000, 1)
-    x = np.random.default_rng(6).normal(size=len(dets))
-    out = []
-    for layout in ("relabelled", "staged"):
-        if layout == "staged":
-            monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "staged")
-        g = _gpu(path)
-        g.set_dets(dets[: len(dets) // 2])
-        g.make_hamiltonian()
-        y0 = g.hmult(x[: len(dets) // 2])
-        g.set_dets(dets)
-        g.make_hamiltonian()
-        out.append((y0, g.hmult(x)))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
-
-
 @pytest.mark.parametrize("name,nroots", [("c0", 1), ("hub", 2)])
 def test_davidson_energies(name, nroots, systems):
     path = systems[name][0]

@@ -580,6 +580,22 @@ def test_pt_semistochastic_identity_at_equal_thresholds(tmp_path):
     assert np.all(per == det) and r == det
 
 
+def test_alias_draw_index_stays_in_range(oracle):
+    """The alias draw's index floor(r / RAND_MAX * N)
+    (SHCI/SHCIsampledeterminants.cpp:160) reaches N when rand() returns
+    RAND_MAX, one past the alias and probability tables; the oracle (and the
+    HIP path's host sampler, which uses the same clamp) maps that draw to
+    N - 1 and leaves every other draw where the reference puts it."""
+    rmax = oracle.oref_rand_max()
+    for n in (1, 2, 7, 1000, 1177638):
+        assert oracle.oref_alias_draw_index(rmax, n) == n - 1
+        assert oracle.oref_alias_draw_index(0, n) == 0
+        for r in (1, rmax // 3, rmax // 2, rmax - 1):
+            want = int(np.floor(1.0 * (r / rmax) * n))
+            assert want < n
+            assert oracle.oref_alias_draw_index(r, n) == want
+
+
 def test_pt_semistochastic_is_unbiased(tmp_path):
     """The sampled estimator is unbiased: over 400 iterations of 200 draws its
     mean is within four standard errors of the deterministic PT energy at
