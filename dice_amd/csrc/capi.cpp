@@ -932,6 +932,29 @@ shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, in
   });
 }
 
+shci_status shci_sigma_device_noalloc(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld) {
+  return guarded(h, [&] {
+    Engine& E = h->eng;
+    if (E.hamRows != E.n) throw std::invalid_argument("Hamiltonian not built for the held determinants");
+    if (k < 1 || ld < E.n) throw std::invalid_argument("need k >= 1 and ld >= number of determinants");
+    const char *xb = (const char*)d_x, *xe = xb + ((k - 1) * ld + E.n) * 8;
+    const char *yb = (const char*)d_y, *ye = yb + ((k - 1) * ld + E.n) * 8;
+    if (xb < ye && yb < xe) throw std::invalid_argument("d_y overlaps d_x");
+    if (E.trev || E.direct)
+      return fail(h, SHCI_UNSUPPORTED, "shci_sigma_device_noalloc: not available in time-reversal or direct mode");
+    size_t cap = 0;
+    (void)fixedSigmaScratch(&cap);
+    const size_t need = E.sigmaFixedScratchNeed(k);
+    if (need > cap - 256)
+      return fail(h, SHCI_UNSUPPORTED,
+                  "shci_sigma_device_noalloc: " + std::to_string(E.n) + " determinants x " + std::to_string(k) +
+                      " vectors need " + std::to_string(need) + " bytes of scratch, more than its fixed " +
+                      std::to_string(cap - 256) + " (use shci_sigma_device)");
+    E.sigmaFixed(d_x, d_y, k, ld);
+    return SHCI_OK;
+  });
+}
+
 shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k) {
   return guarded(h, [&] {
     Engine& E = h->eng;

@@ -181,6 +181,8 @@ void Engine::load(const std::string& fcidump) {
   split = !(layout && std::strcmp(layout, "alpha") == 0);
   uploadRestrictions();
   counter.reserve(64);
+  // the allocation-free sigma's fixed scratch: its address on this device
+  (void)fixedSigmaScratch(nullptr);
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -1059,6 +1061,39 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k, ld,
                    stream);
   if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+}
+
+size_t Engine::sigmaFixedScratchNeed(int k) const {
+  if (permRows != n) return 0;  // the unrelabelled sigma needs no scratch
+  return sigmaPermScratchBytes(n, splitRows == n, k, precision == 1 ? 4 : 8);
+}
+
+// sigma as sigma(dx, dy, k, ld, false) with the fixed scratch region in place
+// of the handle's buffers: no allocation, no profiling events
+void Engine::sigmaFixed(const double* dx, double* dy, int k, int64_t ld) {
+  if (trev || direct) throw std::invalid_argument("the allocation-free sigma does not run in time-reversal or direct mode");
+  const bool f32 = precision == 1;
+  if (permRows != n) {
+    if (f32)
+      launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), nullptr, diag.as<double>(), n, dx,
+                     dy, k, ld, stream);
+    else
+      launchSigmaF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k,
+                     ld, stream);
+    return;
+  }
+  size_t cap = 0;
+  unsigned char* base = (unsigned char*)fixedSigmaScratch(&cap);
+  const XScratch xs{nullptr, base + 256, cap - 256};
+  const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
+  const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
+  unsigned* ctr = (unsigned*)base;
+  if (f32)
+    launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, ctr, stream);
+  else
+    launchSigmaPermF64(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<double>(), diag.as<double>(),
+                       T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, ctr, stream);
 }
 
 void Engine::readSigmaProfile(int64_t& calls, double& ms, double& bytes) {

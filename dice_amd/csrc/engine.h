@@ -224,6 +224,10 @@ struct Engine {
   void buildHamiltonian();
   void buildPermuted();
   void sigma(const double* dx, double* dy, int k, int64_t ld, bool exact);
+  // sigma(dx, dy, k, ld, false) on the fixed device scratch (no allocation);
+  // sigmaFixedScratchNeed = its bytes of permuted x (0: none needed)
+  void sigmaFixed(const double* dx, double* dy, int k, int64_t ld);
+  size_t sigmaFixedScratchNeed(int k) const;
   std::vector<double> davidson(double* dX0, int nr, int maxCopies, double tol, int& numIter, bool exactSigma);
   int64_t lowerNnz() const { return nnzOff / 2 + n; }
 };

@@ -233,6 +233,10 @@ void launchSigmaPermF64(const int64_t* rp, const int32_t* cols, const double* va
 void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* vals, const double* diag,
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
                         int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s);
+// The fixed scratch region of the allocation-free sigma on the current
+// device: its device address (looked up once per device with
+// hipGetSymbolAddress) and size.  Every caller on one device shares it.
+void* fixedSigmaScratch(size_t* bytes);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "kernels.h"
@@ -2915,6 +2916,30 @@ void* XScratch::get(size_t bytes) const {
 size_t sigmaPermScratchBytes(int64_t n, bool split, int k, size_t xBytes) {
   const int K = k <= 1 ? 1 : k == 2 ? 2 : k <= 4 ? 4 : 8;
   return (size_t)(split ? 2 * n : n) * K * xBytes;
+}
+
+// Fixed scratch of the allocation-free sigma: 256 bytes of work counters,
+// then up to 64 MiB of permuted x (one FP64 vector of up to 4,194,304
+// determinants with split columns, or n * vectors per pass up to that).
+// Each GPU holds its own copy for the life of the process.
+constexpr size_t kFixedScratchBytes = ((size_t)64 << 20) + 256;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+void* fixedSigmaScratch(size_t* bytes) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw HipError("device index out of range for the fixed sigma scratch");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the fixed sigma scratch");
+    addr[dev] = p;
+  }
+  if (bytes) *bytes = kFixedScratchBytes;
+  return addr[dev];
 }
 
 template <typename VT, typename XT, int CHUNK, bool PERSIST>

@@ -89,6 +89,7 @@ def library():
         "shci_get_hamiltonian_rows": ([P, I64, pI64, pI32, pD, I64], I32),
         "shci_sigma": ([P, pD, pD, I32], I32),
         "shci_sigma_device": ([P, P, P, I32, I64], I32),
+        "shci_sigma_device_noalloc": ([P, P, P, I32, I64], I32),
         "shci_davidson": ([P, pD, I32, I32, D, pD, ctypes.POINTER(ctypes.c_int)], I32),
         "shci_pt_deterministic": ([P, pD, D, D, pD, pD, pI64], I32),
         "shci_pt_semistochastic": ([P, pD, D, D, D, I32, I32, D, ctypes.c_uint, pD, pD, I32,

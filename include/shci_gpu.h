@@ -113,6 +113,18 @@ shci_status shci_sigma(shci_handle* h, const double* x, double* y, int k);
  * that keep vectors resident; returns after enqueueing.  ld >= n; d_y must
  * not overlap d_x (SHCI_BAD_INPUT). */
 shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld);
+/* shci_sigma_device without any device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * Hamiltonian must already be built (shci_build_hamiltonian).  The permuted
+ * x and work counters live in one fixed scratch region per GPU (64 MiB of x:
+ * with split columns n * vectors-per-pass up to 4,194,304 in FP64, where a
+ * pass takes 1, 2, 4 or 8 of the k vectors), held until the process ends;
+ * a larger request returns SHCI_UNSUPPORTED before anything is enqueued, as
+ * do time-reversal and direct mode.  Every call on one GPU shares that
+ * region: calls that overlap on one GPU (other handles or streams) race.
+ * Results are those of shci_sigma_device, bit for bit. */
+shci_status shci_sigma_device_noalloc(shci_handle* h, const double* d_x, double* d_y, int k, int64_t ld);
 /* Davidson with the reference's semantics (SHCI/Davidson.cpp:111-348):
  * x0 host n x nroots column-major guess in, eigenvectors out; energies out
  * (coreE not added); *iterations = Davidson iteration count.  With the
