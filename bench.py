@@ -140,10 +140,108 @@ def batching_form(shci, args):
             "sigma_GBps": round(R["nbytes"] / (R["ms"] * 1e-3) / 1e9, 1) if R["ms"] else None,
             "iterations": R["per_iter"],
         }
+    # the block Davidson's sigma alone: k = 4 interleaved vectors as one SpMM
+    # over the relabelled copy of the final Hamiltonian (wall time of
+    # shci_sigma_device, x permutation included); algorithmic bytes as for
+    # the one-vector sigma with 2 k N vector values
+    r = shci.do_variational(inp, log, davidson_block=True, autotune=True, hip_graphs=True)
+    n, nnz_lower = r.iterations()[-1]["ndets"], r.phases()[-1][3]
+    t = time_sigma_device(shci, r._h, n, 4, 20)
+    del r
+    nbytes = 2 * (nnz_lower - n) * 12 + (n + 1) * 8 + n * 8 + 2 * 4 * n * 8
+    res["block_davidson"]["spmm_k4"] = {"ndets": n, "nnz_stored": 2 * (nnz_lower - n), "ms": round(1e3 * t, 4),
+                                        "bytes": nbytes, "GBps": round(nbytes / t / 1e9, 1),
+                                        "frac_of_hbm_peak": round(nbytes / t / 1e9 / PEAK_HBM_GBS, 4)}
+    print("# Hubbard SpMM k=4: %.3f ms, %.1f GB/s" % (1e3 * t, nbytes / t / 1e9), file=sys.stderr, flush=True)
     res["workload"] = ("6x6 periodic Hubbard, t=1, U=4, 18a+18b, eps1 2e-4, 4 roots, 3 iterations "
                        "(the fourth would select 12.6M more determinants)")
     res["davidson_speedup"] = round(res["reference_davidson"]["davidson_s"] / res["block_davidson"]["davidson_s"], 3)
     return res
+
+
+def hip_runtime():
+    import ctypes
+    H = ctypes.CDLL("libamdhip64.so")
+    H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    H.hipFree.argtypes = [ctypes.c_void_p]
+    H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return H
+
+
+def time_sigma_device(shci, handle, n, k, reps):
+    """Mean wall time (s) of shci_sigma_device with k vectors on the handle's
+    Hamiltonian (device vectors, synchronised around the repetitions; the x
+    permutation kernel included)."""
+    import ctypes
+    import numpy as np
+    L, H = shci.library(), hip_runtime()
+    dx, dy = ctypes.c_void_p(), ctypes.c_void_p()
+    assert H.hipMalloc(ctypes.byref(dx), n * k * 8) == 0 and H.hipMalloc(ctypes.byref(dy), n * k * 8) == 0
+    try:
+        x = np.random.default_rng(1).normal(size=n * k)
+        H.hipMemcpy(dx, x.ctypes.data, n * k * 8, 1)
+        assert L.shci_sigma_device(handle, dx, dy, k, n) == 0
+        H.hipDeviceSynchronize()
+        t0 = time.time()
+        for _ in range(reps):
+            assert L.shci_sigma_device(handle, dx, dy, k, n) == 0
+        H.hipDeviceSynchronize()
+        return (time.time() - t0) / reps
+    finally:
+        H.hipFree(dx)
+        H.hipFree(dy)
+
+
+def host_threads():
+    """Host threads for the oracle's rank-parallel loops: the box's CPU share
+    (OMP_NUM_THREADS, set per GPU), else the host's core count."""
+    return max(1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
+
+
+def cpu_final_space(shci, full, args):
+    """The reference's algorithm (the oracle, rank-parallel loops on the host
+    threads) on the final space of the workload: its Hamiltonian build
+    (MakeHfromSMHelpers2) and its sigma (Hmult2), against a from-scratch GPU
+    build of the same space and the GPU sigma."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import OracleSystem
+    threads = host_threads()
+    os.environ["ORACLE_THREADS"] = str(threads)
+    r = shci.do_variational(full, os.path.join(os.path.dirname(full), "gpu.log"))
+    dets = r.dets()
+    n = len(dets)
+    fcidump = os.path.join(os.path.dirname(full), "FCIDUMP")
+    G = shci.System.from_fcidump(fcidump)
+    G.set_dets(dets)
+    t0 = time.time()
+    nnz_lower = G.make_hamiltonian()
+    gpu_build = time.time() - t0
+    gpu_sigma = time_sigma_device(shci, G._h, n, 1, 20)
+    G.close()
+    del r
+    o = OracleSystem(fcidump)
+    t0 = time.time()
+    o.csr_append(dets)
+    cpu_build = time.time() - t0
+    x = np.random.default_rng(2).normal(size=n)
+    o.sigma(x)
+    t0 = time.time()
+    reps = 3
+    for _ in range(reps):
+        o.sigma(x)
+    cpu_sigma = (time.time() - t0) / reps
+    del o
+    print("# final space %d dets: build CPU %.2f s / GPU %.3f s; sigma CPU %.3f s / GPU %.2f ms (%d threads)"
+          % (n, cpu_build, gpu_build, cpu_sigma, 1e3 * gpu_sigma, threads), file=sys.stderr, flush=True)
+    return {"what": "the reference's Hamiltonian build (MakeHfromSMHelpers2, rows round-robin) and sigma "
+                    "(Hmult2, lower-triangle rows applied symmetrically, per-thread partial vectors summed as the "
+                    "per-rank ones) restated in the oracle, on the workload's final space, against a from-scratch "
+                    "GPU build of the same space and one GPU sigma (shci_sigma_device, x permutation included)",
+            "ndets": n, "nnz_lower": nnz_lower, "cores": threads, "host_nproc": os.cpu_count(),
+            "cpu_build_s": round(cpu_build, 3), "gpu_build_s": round(gpu_build, 4),
+            "cpu_sigma_s": round(cpu_sigma, 4), "gpu_sigma_s": round(gpu_sigma, 6),
+            "sigma_speedup": round(cpu_sigma / gpu_sigma, 1), "build_speedup": round(cpu_build / gpu_build, 1)}
 
 
 def pt_measure(shci, full, sample, log, args):
@@ -185,7 +283,7 @@ def pt_measure(shci, full, sample, log, args):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import OracleRun, OracleSystem
-        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), os.cpu_count() or 1))
+        threads = host_threads()
         os.environ["ORACLE_THREADS"] = str(threads)
         fcidump = os.path.join(os.path.dirname(full), "FCIDUMP")
         run = OracleRun(sample, None)
@@ -363,7 +461,7 @@ def main():
         from oracle_bind import OracleRun
         # host threads for the oracle's rank-parallel loops (rows of H,
         # screened determinants, sigma rows): the box's CPU share
-        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), os.cpu_count() or 1))
+        threads = host_threads()
         os.environ["ORACLE_THREADS"] = str(threads)
         cpu = []
         for _ in range(3):
@@ -383,6 +481,8 @@ def main():
                       "threads in place of MPI ranks; GPU time of the same sample: %.3f s"
                       % (EPS1, threads, gpu_sample),
             "gpu_same_sample_s": round(gpu_sample, 4),
+            "host_nproc": os.cpu_count(),
+            "final_space": cpu_final_space(shci, full, args),
         }
     print(json.dumps(out))
 

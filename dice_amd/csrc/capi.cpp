@@ -89,10 +89,18 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   E.davidsonMode = opt.davidson_block ? 1 : 0;
   E.autotune = opt.autotune != 0;
   E.useGraphs = opt.hip_graphs != 0;
+  if (opt.half_storage && !E.lowerOnly) {  // from the next build on, rebuilt in half storage
+    E.lowerOnly = true;
+    E.hamRows = -1;
+  }
   E.learned = opt.learned != 0;
+  // 1: gated (the ranking runs only where it is predicted to pay); 2: the
+  // untrained control and 3: the trained network, both always ranking
+  E.learnedGate = opt.learned == 1;
   if (E.direct != (opt.direct_sigma != 0)) {  // switching between stored and direct H: rebuild
     E.hamRows = -1;
     E.permRows = -1;
+    E.tiledRows = -1;
   }
   E.direct = opt.direct_sigma != 0;
   E.learnedControl = opt.learned == 2;
@@ -295,6 +303,184 @@ void rdmStep(Engine& E, const HostSchedule& schd, FILE* log) {
   if (log) fflush(log);
 }
 
+// The variational iterations (SHCIbasics::DoVariational,
+// SHCI/SHCIbasics.cpp:783-1378) from the held determinants (diagonal built)
+// with the coefficients already in E.ci (n x nroots, column-major): the
+// reference's table on log, per-iteration records, checkpoints and restart as
+// schd asks, the final tight Davidson and the RDM step; E.energies = the
+// energies with the core energy added.  fromConverged: a restart found a
+// converged checkpoint (the iterations were skipped).
+shci_status runIterations(shci_handle* h, HostSchedule& schd, FILE* log, double startofCalc, double e0init,
+                          bool& fromConverged) {
+  Engine& E = h->eng;
+  const HostSystem& hs = E.hs;
+  const int W = hs.W;
+  const int nroots = schd.nroots;
+  if (log) {
+    fprintf(log, "\n\n**************************************************************\n");
+    fprintf(log, "VARIATIONAL STEP  \n");
+    fprintf(log, "**************************************************************\n");
+  }
+  const double coreEbkp = hs.coreE;
+  std::vector<double> E0(nroots, e0init);
+  E.iters.clear();
+  if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
+  DevMem X0;
+  const std::string ckptPath = schd.prefix[0] + "/shci_var.bkp";
+  auto saveCheckpoint = [&](int iter) {  // SHCI/SHCIbasics.cpp:1224-1227,1345-1352 (always "converged")
+    Checkpoint c;
+    c.W = W;
+    c.nroots = nroots;
+    c.iter = iter;
+    c.converged = 1;
+    c.n = E.n;
+    c.E0 = E0;
+    c.dets.resize(c.n * W);
+    c.ci.resize(c.n * nroots);
+    HIP_CHECK(hipMemcpy(c.dets.data(), E.dets.p, c.n * W * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(c.ci.data(), E.ci.p, c.n * nroots * 8, hipMemcpyDeviceToHost));
+    if (!writeCheckpoint(ckptPath, c) && log) fprintf(log, "could not write the checkpoint %s\n", ckptPath.c_str());
+  };
+  // restart from the checkpoint (SHCI/SHCIbasics.cpp:871-941)
+  int iterstart = 0;
+  fromConverged = false;
+  if (schd.restart || schd.fullrestart) {
+    Checkpoint c;
+    std::string e;
+    if (!readCheckpoint(ckptPath, c, e) || c.W != W || c.nroots != nroots) {
+      if (e.empty()) e = "the checkpoint " + ckptPath + " does not match this system (words or roots)";
+      if (log) fprintf(log, "%s\n", e.c_str());
+      return fail(h, SHCI_BAD_INPUT, e);
+    }
+    E.setDets(c.dets.data(), c.n);
+    E.buildHamiltonian();
+    E.ci.reserve((size_t)c.n * nroots * 8);
+    HIP_CHECK(hipMemcpy(E.ci.p, c.ci.data(), c.n * nroots * 8, hipMemcpyHostToDevice));
+    E0 = c.E0;
+    const int last = (int)schd.epsilon1.size() - 1;
+    iterstart = c.iter;
+    for (int i = 0; i < nroots; i++)
+      if (log)
+        fprintf(log, "%4d %4d  %10.2e  %10lld -   %18.10f  %10.2f\n", iterstart, i,
+                schd.epsilon1[std::min(iterstart, last)], (long long)c.n, E0[i] + coreEbkp, wallTime() - startofCalc);
+    iterstart = schd.fullrestart ? 0 : iterstart + 1;
+    // (a full restart re-runs the schedule: the reference's test would read
+    // epsilon1[-1] there)
+    if (!schd.fullrestart && c.converged &&
+        (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
+      if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
+      fromConverged = true;
+      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:939
+    }
+  }
+  for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
+    const double eps1 = schd.epsilon1[iter];
+    const double t0 = wallTime();
+    const int64_t nold = E.n;
+    E.cmax.reserve((size_t)std::max<int64_t>(nold, 1) * 8);
+    launchCmax(E.ci.as<double>(), nroots, nold, nold, E.cmax.as<double>(), E.stream);
+    int64_t nnew = -1;
+    if (E.learned && E.mlpReady && iter >= E.learnedLogIters) {
+      nnew = E.screenLearned(E.cmax.as<double>(), eps1, iter);
+      // once the ranking would not save evaluations, later iterations
+      // screen in full without trying it again
+      if (nnew < 0) E.mlpReady = false;
+    }
+    const double tScreenStart = wallTime();
+    if (nnew < 0) nnew = E.screen(E.cmax.as<double>(), eps1);
+    const double tExactScreen = wallTime() - tScreenStart;
+    if (E.learned && iter < E.learnedLogIters) {
+      E.learnedLog(E.cmax.as<double>(), eps1);
+      if (iter == E.learnedLogIters - 1) {
+        // gated form: train and rank only when ranking is predicted to cost
+        // less than the exact screen it replaces
+        if (E.learnedGate && !E.learnedWorthIt(tExactScreen, nold, iter)) {
+          E.learned = false;
+        } else {
+          E.learnedTrain();
+        }
+      }
+    }
+    if (std::getenv("SHCI_TIMING"))
+      fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);
+    E.appendNew();
+    const int64_t N = E.n;
+    if (E.direct && log) fprintf(log, "New size of determinant space %8lld\n", (long long)N);
+    // X0 = [ci; 0]
+    X0.reserve((size_t)N * nroots * 8);
+    HIP_CHECK(hipMemsetAsync(X0.p, 0, (size_t)N * nroots * 8, E.stream));
+    for (int r = 0; r < nroots; r++)
+      HIP_CHECK(hipMemcpyAsync(X0.as<double>() + (int64_t)r * N, E.ci.as<double>() + (int64_t)r * nold, nold * 8,
+                               hipMemcpyDeviceToDevice, E.stream));
+    (void)nnew;
+    HIP_CHECK(hipStreamSynchronize(E.stream));
+    const double t1 = wallTime();
+    E.buildHamiltonian();
+    const double t2 = wallTime();
+    const double prevE0 = iter == 0 ? -10.0 : E0[0];
+    int numIter = 0;
+    const int looseCopies = E.direct ? nroots + 2 : nroots + 4;
+    E0 = E.davidsonMode == 1 ? E.davidsonBlock(X0.as<double>(), nroots, schd.davidsonTolLoose, numIter, false)
+                             : E.davidson(X0.as<double>(), nroots, looseCopies, schd.davidsonTolLoose, numIter, false);
+    if ((int)E0.size() != nroots) {
+      if (log) fprintf(log, "%s\n", E.err.c_str());
+      return fail(h, SHCI_NO_CONVERGENCE, E.err);
+    }
+    const double t3 = wallTime();
+    const double t = t3 - startofCalc;
+    if (log) {
+      fprintf(log, "%4d %4d  %10.2e  %10lld", iter, 0, eps1, (long long)N);
+      fprintf(log, "   %18.10f  %9d  %10.2f\n", E0[0] + coreEbkp, numIter, t);
+      for (int i = 1; i < nroots; i++)
+        fprintf(log, "%4d %4d  %10.2e  %10lld   %18.10f  %9d  %10.2f\n", iter, i, eps1, (long long)N,
+                E0[i] + coreEbkp, numIter, t);
+      if (nroots > 1) fprintf(log, "\n");
+      fflush(log);
+    }
+    IterRecord rec{iter, eps1, N, {}, numIter, t};
+    rec.tScreen = t1 - t0;
+    rec.tHam = t2 - t1;
+    rec.tDavidson = t3 - t2;
+    rec.tIter = t3 - t0;
+    rec.nnzLower = E.lowerNnz();
+    if (std::getenv("SHCI_TIMING"))
+      fprintf(stderr, "# iter %d: n=%lld nnz_lower=%lld screen %.3fs hamiltonian %.3fs davidson %.3fs (%d it)\n", iter,
+              (long long)N, (long long)rec.nnzLower, rec.tScreen, rec.tHam, rec.tDavidson, numIter);
+    for (double e : E0) rec.e0.push_back(e + coreEbkp);
+    E.iters.push_back(rec);
+    // ci = X0
+    E.ci.reserve((size_t)N * nroots * 8);
+    HIP_CHECK(hipMemcpyAsync(E.ci.p, X0.p, (size_t)N * nroots * 8, hipMemcpyDeviceToDevice, E.stream));
+    if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
+      if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
+      const double tf0 = wallTime();
+      if (E.direct) E.davidsonPrint = log;  // davidsonDirect(..., print = true)
+      E0 = E.davidsonMode == 1 ? E.davidsonBlock(E.ci.as<double>(), nroots, schd.davidsonTol, numIter, true)
+                               : E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
+      E.davidsonPrint = nullptr;
+      E.tFinalDavidson = wallTime() - tf0;
+      if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
+      if (log) fprintf(log, "Exiting variational iterations\n");
+      if (schd.io) saveCheckpoint(iter);
+      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:1229-1234
+      rdmStep(E, schd, log);
+      break;
+    }
+    if (schd.io) saveCheckpoint(iter);
+  }
+  if (log && !fromConverged) {
+    fprintf(log, "\nVariational calculation result\n");
+    fprintf(log, "%4s %18s  %10s\n", "Root", "Energy", "Time(s)");
+  }
+  for (size_t i = 0; i < E0.size(); i++) {
+    E0[i] += coreEbkp;
+    if (log && !fromConverged) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
+  }
+  if (log && !fromConverged) fprintf(log, "\n");
+  E.energies = E0;
+  return SHCI_OK;
+}
+
 shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, const shci_run_options& opt) {
   Engine& E = h->eng;
   const int precision = opt.precision;
@@ -426,158 +612,13 @@ shci_status runVariational(shci_handle* h, const char* inputPath, FILE* log, con
   }
   E.ci.reserve(hci.size() * 8);
   HIP_CHECK(hipMemcpy(E.ci.p, hci.data(), hci.size() * 8, hipMemcpyHostToDevice));
-  if (log) {
-    fprintf(log, "\n\n**************************************************************\n");
-    fprintf(log, "VARIATIONAL STEP  \n");
-    fprintf(log, "**************************************************************\n");
-  }
-  const double coreEbkp = hs.coreE;
-  std::vector<double> E0(nroots, std::fabs(eref[0]));
-  E.iters.clear();
-  if (log) fprintf(log, "Iter Root       Eps1   #Var. Det.               Energy   #Davidson     Time(s)\n");
-  DevMem X0;
-  const std::string ckptPath = schd.prefix[0] + "/shci_var.bkp";
-  auto saveCheckpoint = [&](int iter) {  // SHCI/SHCIbasics.cpp:1224-1227,1345-1352 (always "converged")
-    Checkpoint c;
-    c.W = W;
-    c.nroots = nroots;
-    c.iter = iter;
-    c.converged = 1;
-    c.n = E.n;
-    c.E0 = E0;
-    c.dets.resize(c.n * W);
-    c.ci.resize(c.n * nroots);
-    HIP_CHECK(hipMemcpy(c.dets.data(), E.dets.p, c.n * W * 8, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(c.ci.data(), E.ci.p, c.n * nroots * 8, hipMemcpyDeviceToHost));
-    if (!writeCheckpoint(ckptPath, c) && log) fprintf(log, "could not write the checkpoint %s\n", ckptPath.c_str());
-  };
-  // restart from the checkpoint (SHCI/SHCIbasics.cpp:871-941)
-  int iterstart = 0;
   bool fromConverged = false;
-  if (schd.restart || schd.fullrestart) {
-    Checkpoint c;
-    std::string e;
-    if (!readCheckpoint(ckptPath, c, e) || c.W != W || c.nroots != nroots) {
-      if (e.empty()) e = "the checkpoint " + ckptPath + " does not match this system (words or roots)";
-      if (log) fprintf(log, "%s\n", e.c_str());
-      return fail(h, SHCI_BAD_INPUT, e);
-    }
-    E.setDets(c.dets.data(), c.n);
-    E.buildHamiltonian();
-    E.ci.reserve((size_t)c.n * nroots * 8);
-    HIP_CHECK(hipMemcpy(E.ci.p, c.ci.data(), c.n * nroots * 8, hipMemcpyHostToDevice));
-    E0 = c.E0;
-    const int last = (int)schd.epsilon1.size() - 1;
-    iterstart = c.iter;
-    for (int i = 0; i < nroots; i++)
-      if (log)
-        fprintf(log, "%4d %4d  %10.2e  %10lld -   %18.10f  %10.2f\n", iterstart, i,
-                schd.epsilon1[std::min(iterstart, last)], (long long)c.n, E0[i] + coreEbkp, wallTime() - startofCalc);
-    iterstart = schd.fullrestart ? 0 : iterstart + 1;
-    // (a full restart re-runs the schedule: the reference's test would read
-    // epsilon1[-1] there)
-    if (!schd.fullrestart && c.converged &&
-        (iterstart - 1 > last || schd.epsilon1[iterstart - 1] <= schd.epsilon1[last])) {
-      if (log) fprintf(log, "# restarting from a converged calculation, moving to perturbative part.!!\n");
-      fromConverged = true;
-      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:939
-    }
+  {
+    const shci_status st = runIterations(h, schd, log, startofCalc, std::fabs(eref[0]), fromConverged);
+    if (st != SHCI_OK) return st;
   }
-  for (int iter = iterstart; !fromConverged && iter < (int)schd.epsilon1.size(); iter++) {
-    const double eps1 = schd.epsilon1[iter];
-    const double t0 = wallTime();
-    const int64_t nold = E.n;
-    E.cmax.reserve((size_t)std::max<int64_t>(nold, 1) * 8);
-    launchCmax(E.ci.as<double>(), nroots, nold, nold, E.cmax.as<double>(), E.stream);
-    int64_t nnew = -1;
-    if (E.learned && E.mlpReady && iter >= E.learnedLogIters) {
-      nnew = E.screenLearned(E.cmax.as<double>(), eps1, iter);
-      // once the ranking would not save evaluations, later iterations
-      // screen in full without trying it again
-      if (nnew < 0) E.mlpReady = false;
-    }
-    if (nnew < 0) nnew = E.screen(E.cmax.as<double>(), eps1);
-    if (E.learned && iter < E.learnedLogIters) {
-      E.learnedLog(E.cmax.as<double>(), eps1);
-      if (iter == E.learnedLogIters - 1) E.learnedTrain();
-    }
-    if (std::getenv("SHCI_TIMING"))
-      fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);
-    E.appendNew();
-    const int64_t N = E.n;
-    if (E.direct && log) fprintf(log, "New size of determinant space %8lld\n", (long long)N);
-    // X0 = [ci; 0]
-    X0.reserve((size_t)N * nroots * 8);
-    HIP_CHECK(hipMemsetAsync(X0.p, 0, (size_t)N * nroots * 8, E.stream));
-    for (int r = 0; r < nroots; r++)
-      HIP_CHECK(hipMemcpyAsync(X0.as<double>() + (int64_t)r * N, E.ci.as<double>() + (int64_t)r * nold, nold * 8,
-                               hipMemcpyDeviceToDevice, E.stream));
-    (void)nnew;
-    HIP_CHECK(hipStreamSynchronize(E.stream));
-    const double t1 = wallTime();
-    E.buildHamiltonian();
-    const double t2 = wallTime();
-    const double prevE0 = iter == 0 ? -10.0 : E0[0];
-    int numIter = 0;
-    const int looseCopies = E.direct ? nroots + 2 : nroots + 4;
-    E0 = E.davidsonMode == 1 ? E.davidsonBlock(X0.as<double>(), nroots, schd.davidsonTolLoose, numIter, false)
-                             : E.davidson(X0.as<double>(), nroots, looseCopies, schd.davidsonTolLoose, numIter, false);
-    if ((int)E0.size() != nroots) {
-      if (log) fprintf(log, "%s\n", E.err.c_str());
-      return fail(h, SHCI_NO_CONVERGENCE, E.err);
-    }
-    const double t3 = wallTime();
-    const double t = t3 - startofCalc;
-    if (log) {
-      fprintf(log, "%4d %4d  %10.2e  %10lld", iter, 0, eps1, (long long)N);
-      fprintf(log, "   %18.10f  %9d  %10.2f\n", E0[0] + coreEbkp, numIter, t);
-      for (int i = 1; i < nroots; i++)
-        fprintf(log, "%4d %4d  %10.2e  %10lld   %18.10f  %9d  %10.2f\n", iter, i, eps1, (long long)N,
-                E0[i] + coreEbkp, numIter, t);
-      if (nroots > 1) fprintf(log, "\n");
-      fflush(log);
-    }
-    IterRecord rec{iter, eps1, N, {}, numIter, t};
-    rec.tScreen = t1 - t0;
-    rec.tHam = t2 - t1;
-    rec.tDavidson = t3 - t2;
-    rec.tIter = t3 - t0;
-    rec.nnzLower = E.lowerNnz();
-    if (std::getenv("SHCI_TIMING"))
-      fprintf(stderr, "# iter %d: n=%lld nnz_lower=%lld screen %.3fs hamiltonian %.3fs davidson %.3fs (%d it)\n", iter,
-              (long long)N, (long long)rec.nnzLower, rec.tScreen, rec.tHam, rec.tDavidson, numIter);
-    for (double e : E0) rec.e0.push_back(e + coreEbkp);
-    E.iters.push_back(rec);
-    // ci = X0
-    E.ci.reserve((size_t)N * nroots * 8);
-    HIP_CHECK(hipMemcpyAsync(E.ci.p, X0.p, (size_t)N * nroots * 8, hipMemcpyDeviceToDevice, E.stream));
-    if (std::fabs(E0[0] - prevE0) < schd.dE || iter == (int)schd.epsilon1.size() - 1) {
-      if (log) fprintf(log, "Performing final tight davidson with tol: %g\n", schd.davidsonTol);
-      const double tf0 = wallTime();
-      if (E.direct) E.davidsonPrint = log;  // davidsonDirect(..., print = true)
-      E0 = E.davidsonMode == 1 ? E.davidsonBlock(E.ci.as<double>(), nroots, schd.davidsonTol, numIter, true)
-                               : E.davidson(E.ci.as<double>(), nroots, nroots + 4, schd.davidsonTol, numIter, true);
-      E.davidsonPrint = nullptr;
-      E.tFinalDavidson = wallTime() - tf0;
-      if ((int)E0.size() != nroots) return fail(h, SHCI_NO_CONVERGENCE, E.err);
-      if (log) fprintf(log, "Exiting variational iterations\n");
-      if (schd.io) saveCheckpoint(iter);
-      if (E.trev) E.unpackTrev(schd.Trev);  // SHCI/SHCIbasics.cpp:1229-1234
-      rdmStep(E, schd, log);
-      break;
-    }
-    if (schd.io) saveCheckpoint(iter);
-  }
-  if (log && !fromConverged) {
-    fprintf(log, "\nVariational calculation result\n");
-    fprintf(log, "%4s %18s  %10s\n", "Root", "Energy", "Time(s)");
-  }
-  for (size_t i = 0; i < E0.size(); i++) {
-    E0[i] += coreEbkp;
-    if (log && !fromConverged) fprintf(log, "%4d  %18.10f  %10.2f\n", (int)i, E0[i], wallTime() - startofCalc);
-  }
-  if (log && !fromConverged) fprintf(log, "\n");
-  E.energies = E0;
+  std::vector<double> E0 = E.energies;
+  const double coreEbkp = hs.coreE;
   {  // SHCI/SHCI.cpp:437-446
     std::string efile = schd.prefix[0] + "/shci.e";
     FILE* f = fopen(efile.c_str(), "wb");
@@ -726,6 +767,94 @@ shci_status shci_create(const char* fcidump_path, shci_handle** out) {
   return SHCI_OK;
 }
 
+shci_status shci_create_from_integrals(int norb, int nelec, const double* h1, const double* eri8, double core_energy,
+                                      const int* orbsym, shci_handle** out) {
+  if (!out) return fail(nullptr, SHCI_BAD_INPUT, "null argument");
+  *out = nullptr;
+  HostSystem sys;
+  std::string err;
+  if (!integralsFromArrays(norb, nelec, h1, eri8, core_energy, orbsym, sys, err))
+    return fail(nullptr, SHCI_BAD_INPUT, err);
+  std::unique_ptr<shci_handle> h;
+  try {
+    h.reset(new shci_handle());
+  } catch (const std::exception& e) {
+    return fail(nullptr, SHCI_HIP_ERROR, e.what());
+  }
+  shci_status st = guarded(h.get(), [&] {
+    h->eng.loadSystem(std::move(sys));
+    return SHCI_OK;
+  });
+  if (st != SHCI_OK) {
+    g_createError = h->err;
+    return st;
+  }
+  *out = h.release();
+  return SHCI_OK;
+}
+
+shci_status shci_do_variational(shci_handle* h, const shci_var_params* p, const int64_t* dets, int64_t ndets,
+                                const double* ci, double* energies) {
+  if (!h) return fail(nullptr, SHCI_BAD_INPUT, "null handle");
+  return guarded(h, [&] {
+    if (!p || !dets || !energies) throw std::invalid_argument("null argument");
+    if (p->nroots < 1 || p->n_iter < 1 || !p->eps1) throw std::invalid_argument("need nroots >= 1 and an eps1 list");
+    if (ndets < 1) throw std::invalid_argument("need at least one determinant");
+    if (!(p->davidson_tol > 0) || !(p->davidson_tol_loose > 0) || !(p->dE > 0))
+      throw std::invalid_argument("tolerances must be positive");
+    Engine& E = h->eng;
+    if (E.trev) throw std::invalid_argument("time reversal runs through shci_run (Treversal keyword)");
+    HostSchedule schd;
+    schd.nroots = p->nroots;
+    schd.epsilon1.assign(p->eps1, p->eps1 + p->n_iter);
+    schd.davidsonTol = p->davidson_tol;
+    schd.davidsonTolLoose = p->davidson_tol_loose;
+    schd.dE = p->dE;
+    schd.randomSeed = p->seed;
+    schd.io = false;
+    schd.prefix.assign(1, ".");
+    E.rng.reseed(p->seed);
+    E.ncore = p->ncore;
+    E.nact = p->nact;
+    E.learnedRecs.clear();
+    E.sampX.clear();
+    E.sampT.clear();
+    E.trainSamples = 0;
+    E.mlpReady = false;
+    E.resetSigmaProfile();
+    E.ptEnergies.clear();
+    E.davidsonDirectRules = false;
+    E.davidsonPrint = nullptr;
+    E.setDets(dets, ndets);
+    E.buildHamiltonian();
+    const int nroots = p->nroots;
+    E.nroots = nroots;
+    std::vector<double> hci((size_t)ndets * nroots, 0.0);
+    if (ci) {
+      std::copy(ci, ci + hci.size(), hci.begin());
+    } else {  // SHCI/SHCI.cpp:395-399: every determinant 1, the lowest-energy one + ndets, normalised
+      std::vector<double> d(ndets);
+      HIP_CHECK(hipMemcpy(d.data(), E.diag.p, (size_t)ndets * 8, hipMemcpyDeviceToHost));
+      const int64_t lowest = std::min_element(d.begin(), d.end()) - d.begin();
+      for (int64_t j = 0; j < ndets; j++) hci[j] = 1.0;
+      hci[lowest] += (double)ndets;
+      double s = 0.0;
+      for (int64_t j = 0; j < ndets; j++) s += hci[j] * hci[j];
+      s = std::sqrt(s);
+      for (int64_t j = 0; j < ndets; j++) hci[j] /= s;
+    }
+    E.ci.reserve(hci.size() * 8);
+    HIP_CHECK(hipMemcpy(E.ci.p, hci.data(), hci.size() * 8, hipMemcpyHostToDevice));
+    double d0 = 0.0;
+    HIP_CHECK(hipMemcpy(&d0, E.diag.p, 8, hipMemcpyDeviceToHost));
+    bool fromConverged = false;
+    const shci_status st = runIterations(h, schd, nullptr, wallTime(), std::fabs(d0), fromConverged);
+    if (st != SHCI_OK) return st;
+    for (int r = 0; r < nroots; r++) energies[r] = E.energies[r];
+    return SHCI_OK;
+  });
+}
+
 shci_status shci_destroy(shci_handle* h) {
   delete h;
   return SHCI_OK;
@@ -743,6 +872,7 @@ shci_status shci_set_precision(shci_handle* h, int precision) {
   h->eng.precision = precision;
   h->eng.hamRows = -1;
   h->eng.permRows = -1;
+  h->eng.tiledRows = -1;
   return SHCI_OK;
 }
 

@@ -40,6 +40,10 @@ void Engine::initShared(const Engine& parent) {
   precision = parent.precision;
   relabel = parent.relabel;
   split = parent.split;
+  tiledOn = parent.tiledOn;
+  tiledStep = parent.tiledStep;
+  tiledPersistent = parent.tiledPersistent;
+  lowerOnly = parent.lowerOnly;
   autotune = parent.autotune;
   davidsonMode = parent.davidsonMode;
   sigmaCfg[0] = parent.sigmaCfg[0];
@@ -58,6 +62,7 @@ void Engine::setDetsDevice(const uint64_t* d, int64_t cnt) {
   nDiag = 0;
   hamRows = -1;
   permRows = -1;
+  tiledRows = -1;
   nNew = 0;
   HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -141,7 +146,13 @@ void Engine::unpackTrev(int sign) {
 
 void Engine::load(const std::string& fcidump) {
   std::string e;
-  if (!readFcidump(fcidump, hs, e)) throw std::invalid_argument(e);
+  HostSystem sys;
+  if (!readFcidump(fcidump, sys, e)) throw std::invalid_argument(e);
+  loadSystem(std::move(sys));
+}
+
+void Engine::loadSystem(HostSystem&& sys) {
+  hs = std::move(sys);
   buildScreeningTables(hs);
   if (!stream) HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   upload(dH1, hs.h1, stream);
@@ -179,6 +190,19 @@ void Engine::load(const std::string& fcidump) {
   // SHCI_SIGMA_LAYOUT=alpha keeps every relabelled column alpha-major (no
   // beta-major copy of x)
   split = !(layout && std::strcmp(layout, "alpha") == 0);
+  // SHCI_SIGMA_LAYOUT=tiled: the one-vector sigma on the tiled copy of the
+  // relabelled Hamiltonian instead of the row-ordered one
+  tiledOn = layout && std::strcmp(layout, "tiled") == 0;
+  // SHCI_SIGMA_PERSISTENT=0/1: static or persistent workgroups for the
+  // one-vector sigma (the autotuner, when on, decides per Hamiltonian)
+  if (const char* p = std::getenv("SHCI_SIGMA_PERSISTENT")) sigmaCfg[0].persistent = std::atoi(p) != 0;
+  // SHCI_SIGMA_TILE_STEP=8/16: instructions per wave per step of the tiled sigma
+  if (const char* p = std::getenv("SHCI_SIGMA_TILE_STEP")) tiledStep = std::atoi(p) >= 16 ? 16 : 8;
+  // SHCI_SIGMA_TILE_PERSISTENT=1: persistent workgroups for the tiled sigma
+  if (const char* p = std::getenv("SHCI_SIGMA_TILE_PERSISTENT")) tiledPersistent = std::atoi(p) != 0;
+  // SHCI_STORAGE=half: half storage from the first build
+  const char* storage = std::getenv("SHCI_STORAGE");
+  if (storage && std::strcmp(storage, "half") == 0) lowerOnly = true;
   uploadRestrictions();
   counter.reserve(64);
   // the allocation-free sigma's fixed scratch: its address on this device
@@ -479,6 +503,7 @@ void Engine::setDets(const int64_t* h, int64_t cnt) {
   nDiag = 0;
   hamRows = -1;
   permRows = -1;
+  tiledRows = -1;
   nNew = 0;
   trevN = -1;
   HIP_CHECK(hipStreamSynchronize(stream));
@@ -582,11 +607,17 @@ void Engine::buildHamiltonian() {
       m->release();
     nnzOff = 0;
     permRows = -1;
+    tiledRows = -1;
+    tiled.release();
     hamRows = n;
     HIP_CHECK(hipStreamSynchronize(stream));
     return;
   }
   const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
+  if (lowerOnly) {
+    buildLower(nOld);
+    return;
+  }
   // SHCI_TIMING: after the build, print per sub-phase the host time spent
   // issuing it and the GPU time between its stream events (no added
   // synchronisation, so the timed build runs as an untimed one does)
@@ -617,6 +648,9 @@ void Engine::buildHamiltonian() {
   int64_t nnzNew = 0;
   HIP_CHECK(hipMemcpyAsync(&nnzNew, rp + n, 8, hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
+  if (std::getenv("SHCI_TIMING"))
+    std::fprintf(stderr, "# build n=%lld: %lld off-diagonal elements (both triangles)\n", (long long)n,
+                 (long long)nnzNew);
   mark("count");
   const bool f32 = precision == 1;
   // FP64: cols + values; FP32 form: cols + hi + lo (the FP32 remainder lets
@@ -627,6 +661,23 @@ void Engine::buildHamiltonian() {
     freeB = deviceFreeBytes(&totalB);
     const size_t need = (size_t)nnzNew * perEntry;
     const size_t reusable = colsNew.bytes + valsNew.bytes + vals32New.bytes + valsLoNew.bytes;
+    // the two-triangle Hamiltonian with a half-size slack for the next
+    // build does not fit, while half storage would: switch to half storage
+    // for good (rebuilt from scratch)
+    const size_t have = freeB + reusable + cols.bytes + vals.bytes + vals32.bytes + valsLo.bytes + permCols.bytes +
+                        permVals.bytes + tiled.idx.bytes + tiled.vals.bytes;
+    if (need > freeB + reusable && (size_t)nnzNew / 2 * perEntry * 3 / 2 <= have) {
+      for (DevMem* m : {&cols, &vals, &vals32, &valsLo, &colsNew, &valsNew, &vals32New, &valsLoNew, &permCols,
+                        &permVals, &rowPtrNew})
+        m->release();
+      tiled.release();
+      permRows = -1;
+      tiledRows = -1;
+      lowerOnly = true;
+      hamRows = -1;
+      buildLower(0);
+      return;
+    }
     if (need > freeB + reusable)
       throw HipError("out of memory: the Hamiltonian of " + std::to_string(n) + " determinants has " +
                      std::to_string(nnzNew) + " off-diagonal elements (" + std::to_string(need >> 30) +
@@ -699,55 +750,135 @@ void Engine::buildHamiltonian() {
   }
 }
 
-// Alpha-major relabelled copy of H (FP64 values, or the FP32 hi part) for
-// the sigma kernel; skipped when it does not fit beside H.
+// Half storage: the rows of the determinants added since the last build
+// (all rows when nOld = 0), each with its connections J < I, appended in
+// place to the stored ones (old rows never gain an element: new
+// determinants have larger indices).
+void Engine::buildLower(int64_t nOld) {
+  const bool f32 = precision == 1;
+  buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
+  rowPtrNew.reserve((size_t)(n + 1) * 8);
+  int64_t* cnt = rowPtrNew.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(cnt + n, 0, 8, stream));
+  launchCsrCount(S, dets.as<uint64_t>(), T, nOld, cnt, stream, true);
+  exclusiveScanI64(cnt, n + 1, ws, stream);
+  rowPtr.reserve((size_t)(n + 1) * 8, stream, nOld > 0);
+  int64_t* rp = rowPtr.as<int64_t>();
+  launchLowerRowPtr(cnt, nOld, n, rp, stream);
+  int64_t total = 0;
+  HIP_CHECK(hipMemcpyAsync(&total, rp + n, 8, hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  if (std::getenv("SHCI_TIMING"))
+    std::fprintf(stderr, "# half-storage build n=%lld: %lld stored off-diagonal elements\n", (long long)n,
+                 (long long)total);
+  const int64_t added = total - (nOld ? nnzOff : 0);
+  {
+    size_t totalB = 0;
+    const size_t freeB = deviceFreeBytes(&totalB);
+    const size_t need = (size_t)added * 12;
+    if (need > freeB)
+      throw HipError("out of memory: the half-stored Hamiltonian of " + std::to_string(n) + " determinants has " +
+                     std::to_string(total) + " off-diagonal elements (" + std::to_string(need >> 30) +
+                     " GiB more needed), " + std::to_string(freeB >> 30) + " GiB free");
+  }
+  cols.reserve((size_t)std::max<int64_t>(total, 1) * 4, stream, nOld > 0);
+  if (f32) {
+    vals32.reserve((size_t)std::max<int64_t>(total, 1) * 4, stream, nOld > 0);
+    valsLo.reserve((size_t)std::max<int64_t>(total, 1) * 4, stream, nOld > 0);
+    launchCsrFillSplit(S, dets.as<uint64_t>(), T, nOld, rp, cols.as<int32_t>(), vals32.as<float>(),
+                       valsLo.as<float>(), stream, true);
+    vals.release();
+  } else {
+    vals.reserve((size_t)std::max<int64_t>(total, 1) * 8, stream, nOld > 0);
+    launchCsrFill(S, dets.as<uint64_t>(), T, nOld, rp, cols.as<int32_t>(), vals.as<double>(), stream, true);
+    vals32.release();
+    valsLo.release();
+  }
+  rowPtrNew.release();
+  nnzOff = total;
+  hamRows = n;
+  permRows = -1;
+  splitRows = -1;
+  tiledRows = -1;
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// Alpha-major relabelled copies of H (FP64 values, or the FP32 hi part) for
+// the sigma kernels: the tiled copy (one vector) and the row-ordered copy
+// (SpMM of the block Davidson, or one vector when the tiled copy is off);
+// each is skipped when it does not fit beside H.
 void Engine::buildPermuted() {
   permRows = -1;
   splitRows = -1;
+  tiledRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
-  // the copy (columns, values, row offsets, inverse permutation) with 64
+  // a copy (columns, values, row offsets, inverse permutation) with 64
   // bytes per determinant of slack, and the permuted x of the sigma calls:
   // up to 8 interleaved vectors in the block Davidson, in alpha-major order
   // and, with split columns, beta-major order too (plus bPos)
   const size_t xt = f32 ? 4 : 8, kmax = davidsonMode == 1 ? 8 : 1;
-  const size_t base = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8 + (size_t)n * 4 + (size_t)n * 64;
+  const size_t rowsCopy = (size_t)nnzOff * (4 + (f32 ? 4 : 8)) + (size_t)(n + 1) * 8;
+  const size_t tiledCopy = tiledHamBytes(n, nnzOff, f32);
+  const size_t base = (size_t)n * 4 + (size_t)n * 64;
   const size_t xPlain = (size_t)n * kmax * xt, xSplit = 2 * (size_t)n * kmax * xt + (size_t)n * 4;
   size_t freeB = 0, totalB = 0;
   freeB = deviceFreeBytes(&totalB);
-  const size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes;
-  if (base + xPlain > have) {
+  size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes + tiled.idx.bytes +
+                tiled.vals.bytes + tiled.inst.bytes;
+  const bool wantTiled = tiledOn, wantRows = !tiledOn || davidsonMode == 1;
+  const size_t copies = (wantTiled ? tiledCopy : 0) + (wantRows ? rowsCopy : 0);
+  if (base + copies + xPlain > have) {
+    // not both: the tiled copy alone, else none
     permCols.release();
     permVals.release();
-    return;
+    if (!wantTiled || base + tiledCopy + xPlain > have) {
+      tiled.release();
+      return;
+    }
   }
+  const bool buildRows = wantRows && base + copies + xPlain <= have;
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
   permPos.reserve((size_t)n * 4);
   launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
-  permRp.reserve((size_t)(n + 1) * 8);
-  int64_t* rp = permRp.as<int64_t>();
-  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
-  launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
-  exclusiveScanI64(rp, n + 1, ws, stream);
-  permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
   // split columns only when their x copy fits too
-  const bool doSplit = split && 2 * n < 0x7fffffffLL && base + xSplit <= have;
+  const size_t used = base + (wantTiled ? tiledCopy : 0) + (buildRows ? rowsCopy : 0);
+  const bool doSplit = split && 2 * n < 0x7fffffffLL && used + xSplit <= have;
   if (doSplit) {
     bPos.reserve((size_t)n * 4);
     launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);
   }
   const uint32_t* bIdx = doSplit ? T.bIdx.as<uint32_t>() : nullptr;
   const uint32_t* bp = doSplit ? bPos.as<uint32_t>() : nullptr;
-  if (f32) {
-    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(), bIdx,
-                      bp, n, rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
-  } else {
-    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
-    launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(), bIdx,
-                      bp, n, rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
+  if (wantTiled) {
+    if (buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
+                      f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(), bIdx, bp, n, nnzOff, ws,
+                      stream))
+      tiledRows = n;
+    else
+      tiled.release();
   }
-  permRows = n;
+  if (buildRows || (wantTiled && tiledRows != n)) {
+    permRp.reserve((size_t)(n + 1) * 8);
+    int64_t* rp = permRp.as<int64_t>();
+    HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+    launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
+    exclusiveScanI64(rp, n + 1, ws, stream);
+    permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    if (f32) {
+      permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+      launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(),
+                        bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
+    } else {
+      permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
+      launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(),
+                        bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
+    }
+    permRows = n;
+  } else {
+    permCols.release();
+    permVals.release();
+  }
   if (doSplit) splitRows = n;
   // re-tune when the space has doubled since the last tuning (small spaces
   // keep the default shape: their sigma is too short to time reliably)
@@ -1041,12 +1172,29 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
     return;
   }
+  if (lowerOnly) {
+    if (f32)
+      launchSigmaLowerF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(),
+                          exact ? valsLo.as<float>() : nullptr, diag.as<double>(), n, dx, dy, k, ld, stream);
+    else
+      launchSigmaLowerF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k,
+                          ld, stream);
+    if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+    return;
+  }
   const bool perm = permRows == n && !(f32 && exact);
   const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
-  if (perm) sigmaCounter.reserve(256);
+  // the tiled copy: one vector, or each of k vectors when the row-ordered
+  // copy for the SpMM is absent
+  const bool tiledSigma = (k == 1 || !perm) && tiledRows == n && !(f32 && exact);
+  if (perm || tiledSigma) sigmaCounter.reserve(256);
   const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
   const XScratch xs{&permX};
-  if (perm && f32)
+  if (tiledSigma)
+    for (int r = 0; r < k; r++)
+      launchSigmaTiled(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, dx + r * ld, dy + r * ld, xs,
+                       tiledPersistent, sigmaCounter.as<unsigned>(), stream, tiledStep);
+  else if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, sigmaCounter.as<unsigned>(),
                        stream);
@@ -1064,8 +1212,10 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
 }
 
 size_t Engine::sigmaFixedScratchNeed(int k) const {
-  if (permRows != n) return 0;  // the unrelabelled sigma needs no scratch
-  return sigmaPermScratchBytes(n, splitRows == n, k, precision == 1 ? 4 : 8);
+  if (lowerOnly) return 0;
+  const bool tiledSigma = (k == 1 || permRows != n) && tiledRows == n;
+  if (permRows != n && !tiledSigma) return 0;  // the unrelabelled sigma needs no scratch
+  return sigmaPermScratchBytes(n, splitRows == n, tiledSigma ? 1 : k, precision == 1 ? 4 : 8);
 }
 
 // sigma as sigma(dx, dy, k, ld, false) with the fixed scratch region in place
@@ -1073,7 +1223,17 @@ size_t Engine::sigmaFixedScratchNeed(int k) const {
 void Engine::sigmaFixed(const double* dx, double* dy, int k, int64_t ld) {
   if (trev || direct) throw std::invalid_argument("the allocation-free sigma does not run in time-reversal or direct mode");
   const bool f32 = precision == 1;
-  if (permRows != n) {
+  if (lowerOnly) {
+    if (f32)
+      launchSigmaLowerF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), nullptr, diag.as<double>(), n,
+                          dx, dy, k, ld, stream);
+    else
+      launchSigmaLowerF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), diag.as<double>(), n, dx, dy, k,
+                          ld, stream);
+    return;
+  }
+  const bool tiledSigma = (k == 1 || permRows != n) && tiledRows == n;
+  if (permRows != n && !tiledSigma) {
     if (f32)
       launchSigmaF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), nullptr, diag.as<double>(), n, dx,
                      dy, k, ld, stream);
@@ -1088,7 +1248,11 @@ void Engine::sigmaFixed(const double* dx, double* dy, int k, int64_t ld) {
   const SigmaConfig& cfg = sigmaCfg[k > 1 ? 1 : 0];
   const uint32_t* bPerm = splitRows == n ? T.bMajorDet.as<uint32_t>() : nullptr;
   unsigned* ctr = (unsigned*)base;
-  if (f32)
+  if (tiledSigma)
+    for (int r = 0; r < k; r++)
+      launchSigmaTiled(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, dx + r * ld, dy + r * ld, xs,
+                       tiledPersistent, ctr, stream, tiledStep);
+  else if (f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, ctr, stream);
   else
@@ -1205,6 +1369,9 @@ std::vector<double> Engine::davidson(double* dX0, int nr, int maxCopies, double 
     } else {
       error = std::sqrt(hres[cr0]);
     }
+    if (numIter % 100 == 0 && numIter > 0 && std::getenv("SHCI_TIMING"))
+      std::fprintf(stderr, "# davidson n=%lld iteration %d root %d error %.3g\n", (long long)N, numIter,
+                   convergedRoot, error);
     if (davidsonPrint) {  // davidsonDirect's progress lines (SHCI/Davidson.cpp:526-530)
       if (numIter == 0) std::fprintf(davidsonPrint, "nIter  Root               Energy                Error\n");
       std::fprintf(davidsonPrint, "%5i  %4i   %18.10g   %18.10g  %10.2f\n", numIter, convergedRoot, w[cr0], error,

@@ -71,7 +71,22 @@ struct Engine {
   bool split = true;
   DevMem bPos;
   int64_t splitRows = -1;
+  // tiled copy for the one-vector sigma (blocks of 256 relabelled rows,
+  // entries grouped by column tile); valid when tiledRows == n.  With it the
+  // row-ordered relabelled copy is kept only for the block Davidson's SpMM.
+  bool tiledOn = false;
+  int tiledStep = 16;           // instructions per wave per step (8 or 16; 5.5 against 6.0 ms)
+  bool tiledPersistent = false;  // one workgroup per block (6.4 ms) beat persistent ones (7.5 ms)
+  TiledHam tiled;
+  int64_t tiledRows = -1;
   int64_t nnzOff = 0, hamRows = -1;
+  // half storage: each pair once, in row max(I, J) (the reference's
+  // lower-triangular rows without the diagonal), grown in place by new rows
+  // only; chosen by SHCI_STORAGE=half, the half_storage option, or when the
+  // two-triangle Hamiltonian does not fit.  Sigma then adds the transposed
+  // half with atomics.
+  bool lowerOnly = false;
+  void buildLower(int64_t nOld);
 
   // Davidson scratch
   DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;
@@ -120,6 +135,16 @@ struct Engine {
   int learnedLogIters = 2;
   double learnedLoosen = 10.0, learnedMargin = 0.8, learnedMaxFraction = 0.5;
   bool learnedControl = false;  // keep the untrained network (control runs)
+  // gate: after the last logging iteration, rank only when the candidate
+  // records per source of that iteration's windows times the ranking cost
+  // per record (learnedNsPerRecord, an optimistic 2 ns) stay below the exact
+  // screen's cost per source in the same iteration; otherwise the form
+  // turns itself off (no training, full screening), recorded as one
+  // fallback record with evaluated = 0
+  bool learnedGate = false;
+  double learnedNsPerRecord = 2.0;
+  int64_t lastLogRecords = 0, lastLogSources = 0;
+  bool learnedWorthIt(double exactScreenSeconds, int64_t sources, int iter);
   int learnedHidden = 256;
   int64_t learnedMaxSamples = (int64_t)1 << 18;
   MlpModel mlp;
@@ -216,6 +241,7 @@ struct Engine {
   // an engine on the system tables and stream of parent (owned by parent)
   void initShared(const Engine& parent);
   void load(const std::string& fcidump);  // throws
+  void loadSystem(HostSystem&& sys);      // integrals already on the host; throws
   void uploadRestrictions();
   void setDets(const int64_t* h, int64_t cnt);
   void ensureTable(int64_t need);
@@ -229,7 +255,7 @@ struct Engine {
   void sigmaFixed(const double* dx, double* dy, int k, int64_t ld);
   size_t sigmaFixedScratchNeed(int k) const;
   std::vector<double> davidson(double* dX0, int nr, int maxCopies, double tol, int& numIter, bool exactSigma);
-  int64_t lowerNnz() const { return nnzOff / 2 + n; }
+  int64_t lowerNnz() const { return (lowerOnly ? nnzOff : nnzOff / 2) + n; }
 };
 
 }  // namespace shci

@@ -131,6 +131,13 @@ bool readFcidump(const std::string& path, HostSystem& s, std::string& err) {
       s.I2[packIndex(p - 1, q - 1, r - 1, u - 1)] = v;
     }
   }
+  finishIntegrals(s);
+  return true;
+}
+
+// maxEntry, Direct(i,j) = (ii|jj), Exchange(i,j) = (ij|ji) (SHCI/integral.cpp:181-205)
+void finishIntegrals(HostSystem& s) {
+  const int norb = s.norb;
   double best = s.I2.empty() ? 0.0 : s.I2[0];
   for (double v : s.I2)
     if (std::fabs(best) < std::fabs(v)) best = v;
@@ -142,6 +149,40 @@ bool readFcidump(const std::string& path, HostSystem& s, std::string& err) {
       s.direct[(size_t)i * norb + j] = s.I2[packIndex(i, i, j, j)];
       s.exch[(size_t)i * norb + j] = s.I2[packIndex(i, j, j, i)];
     }
+}
+
+bool integralsFromArrays(int norb, int nelec, const double* h1, const double* eri8, double coreE, const int* orbsym,
+                         HostSystem& s, std::string& err) {
+  if (norb < 1 || norb > 63) {
+    err = "norb must be between 1 and 63 (at most 126 spin-orbitals are supported)";
+    return false;
+  }
+  if (nelec < 1 || nelec > 2 * norb) {
+    err = "nelec must be between 1 and 2 norb";
+    return false;
+  }
+  if (!h1 || !eri8) {
+    err = "h1 and eri8 are required";
+    return false;
+  }
+  s = HostSystem();
+  s.norb = norb;
+  s.nso = 2 * norb;
+  s.nelec = nelec;
+  s.W = s.nso / 64 + 1;
+  s.irrep.assign(norb, 1);
+  if (orbsym)
+    for (int i = 0; i < norb; i++) s.irrep[i] = orbsym[i];
+  const size_t npair = (size_t)norb * (norb + 1) / 2;
+  s.I2.assign(eri8, eri8 + npair * (npair + 1) / 2);
+  s.h1.assign((size_t)norb * norb, 0.0);
+  for (int p = 0; p < norb; p++)
+    for (int q = 0; q <= p; q++) {  // the lower triangle, mirrored (a FCIDUMP lists each pair once)
+      s.h1[(size_t)p * norb + q] = h1[(size_t)p * norb + q];
+      s.h1[(size_t)q * norb + p] = h1[(size_t)p * norb + q];
+    }
+  s.coreE = coreE;
+  finishIntegrals(s);
   return true;
 }
 

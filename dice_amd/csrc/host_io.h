@@ -99,6 +99,15 @@ struct HostSchedule {
 // Returns false and fills err on failure (messages follow the reference).
 bool readFcidump(const std::string& path, HostSystem& s, std::string& err);
 void buildScreeningTables(HostSystem& s);
+// maxEntry, Direct and Exchange from I2
+void finishIntegrals(HostSystem& s);
+// The system from in-memory integrals: h1 = norb x norb row-major (its lower
+// triangle is used, mirrored, as a FCIDUMP lists each pair once), eri8 = the
+// 8-fold packed (pq|rs) in the reference's twoInt order (index A(A+1)/2 + B
+// of pair indices A >= B, pair(p,q) = p(p+1)/2 + q for p >= q; PySCF's
+// ao2mo.restore(8, ...) layout), orbsym optional (norb irreps, default 1).
+bool integralsFromArrays(int norb, int nelec, const double* h1, const double* eri8, double coreE, const int* orbsym,
+                         HostSystem& s, std::string& err);
 bool readInputFile(const std::string& path, HostSchedule& schd, std::string& err, std::string* echo);
 
 }  // namespace shci

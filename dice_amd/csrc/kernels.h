@@ -146,17 +146,29 @@ void buildStringTables(const DevSystem& S, const uint64_t* dets, int64_t n, Stri
 // that rows I < nOld only take J >= nOld (the columns a Hamiltonian built over
 // the first nOld determinants lacks).  Values are
 // Hij(bra = D_min(I,J), ket = D_max(I,J)).
+// lower = half storage: only the connections J < I of each row (rows
+// I < nOld get none)
 void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
-                    int64_t* rowCount, hipStream_t s);
+                    int64_t* rowCount, hipStream_t s, bool lower = false);
 void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
-                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s);
+                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s, bool lower = false);
 // direct sigma without a stored Hamiltonian (HmultDirect, SHCI/Hmult.h:153-382):
 // y[:, r] = diag .* x[:, r] + sum_J Hij(D_I, D_J) x[J, r]; y must not overlap x
 void launchSigmaDirect(const DevSystem& S, const uint64_t* dets, const StringTables& T, const double* diag,
                        const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // fill with each value stored as FP32 hi = fp32(v) and lo = fp32(v - hi)
 void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
-                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s);
+                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s,
+                        bool lower = false);
+// half storage (each pair once, in row max(I, J)): row offsets of rows
+// [nOld, n] from the exclusive scan cnt of the new rows' lengths, and sigma
+// applying every stored element to both rows (atomic adds for the
+// transposed half; y must not overlap x)
+void launchLowerRowPtr(const int64_t* cnt, int64_t nOld, int64_t n, int64_t* rowPtr, hipStream_t s);
+void launchSigmaLowerF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
+                         const double* x, double* y, int k, int64_t ld, hipStream_t s);
+void launchSigmaLowerF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
+                         const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s);
 // incremental rebuild helpers: cnt[I] += old row length (I < nOld);
 // fs[I] = rowPtr[I] + old row length; old rows copied to the front of the
 // new rows
@@ -237,6 +249,37 @@ void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* val
 // device: its device address (looked up once per device with
 // hipGetSymbolAddress) and size.  Every caller on one device shares it.
 void* fixedSigmaScratch(size_t* bytes);
+
+// Tiled copy of the relabelled Hamiltonian for the one-vector sigma: the
+// alpha-major rows in blocks of 256, each block's entries grouped by tile of
+// 1,024 consecutive permuted-x values and cut into wave instructions of up
+// to 64 entries of one tile (descriptor: entry offset within the block,
+// tile << 7 | count).  Per entry: (row within block) << 16 | (column within
+// tile), and the value (FP64, or the FP32 hi part).
+struct TiledHam {
+  int64_t n = -1, blocks = 0, segments = 0, nnz = 0, instructions = 0;
+  bool f32 = false, split = false;
+  DevMem segBase, segTile, segCnt, segStart, segInst, segTotal, fill, blockE, blockI, idx, vals, inst;
+  void release();
+};
+// device bytes buildTiledHam needs for nnz entries over n rows (upper bound
+// on the per-segment tables)
+size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32);
+// Builds T from H in determinant order (rowPtr, cols, values: vals64 or, for
+// the FP32 form, vals32) through the alpha-major relabelling (perm: position
+// -> determinant, pos: determinant -> position) and, with bIdx / bPos given,
+// the split columns (the beta-major copy of x after the alpha-major one).
+// Returns false, with nothing built, when the permuted x has more tiles than
+// the build's bitmap holds.
+bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
+                   const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
+                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s);
+// y = H x through the tiled copy (one vector, x and y in determinant order;
+// bPerm = the beta-major determinant order when the copy has split columns)
+// (step: instructions per wave per step, 8 or 16)
+void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
+                      const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
+                      hipStream_t s, int step = 8);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

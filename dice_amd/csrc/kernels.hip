@@ -1136,7 +1136,7 @@ struct RowTables {
 // Collects the connections of row I.  Count pass: per-lane counters; fill
 // pass: wave-uniform cursor, positions by ballot (deterministic order).
 // SPLIT stores each FP64 value as FP32 hi = fp32(v) and lo = fp32(v - hi).
-template <int W, bool FILL, bool SPLIT>
+template <int W, bool FILL, bool SPLIT, bool LOWER>
 struct RowSink {
   const DevSystem* S;
   const uint64_t* dets;
@@ -1149,6 +1149,7 @@ struct RowSink {
   int64_t pos;  // wave-uniform write cursor (fill)
   int cnt;      // per-lane count (count)
   __device__ __forceinline__ void hit(bool h, uint32_t J) {
+    if (LOWER) h = h && (int64_t)J < I;  // half storage: row max(I, J) only
     if (!FILL) {
       cnt += h ? 1 : 0;
       return;
@@ -1311,7 +1312,9 @@ __device__ __forceinline__ void enumerateRow(const RowTables& T, int64_t I, uint
 
 // Rows I < nOld enumerate only connections to J >= nOld (the columns the
 // previous Hamiltonian lacks); rows I >= nOld enumerate all connections.
-template <int W, bool FILL, bool SPLIT>
+// LOWER (half storage) keeps only J < I, so rows I < nOld gain nothing and
+// are skipped.
+template <int W, bool FILL, bool SPLIT, bool LOWER>
 __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* dets, RowTables T, int64_t nOld,
                                                    int64_t* rowCount, const int64_t* fillStart, int32_t* cols,
                                                    double* vals, float* hi, float* lo) {
@@ -1323,8 +1326,12 @@ __global__ void __launch_bounds__(kBlock) kCsrRows(DevSystem S, const uint64_t* 
     uint64_t dI[W];
 #pragma unroll
     for (int w = 0; w < W; w++) dI[w] = dets[I * W + w];
+    if (LOWER && I < nOld) {
+      if (!FILL && lane == 0) rowCount[I] = 0;
+      continue;
+    }
     const uint32_t jmin = I >= nOld ? 0u : (uint32_t)nOld;
-    RowSink<W, FILL, SPLIT> sink;
+    RowSink<W, FILL, SPLIT, LOWER> sink;
     sink.S = &S;
     sink.dets = dets;
     sink.dI = dI;
@@ -1664,6 +1671,422 @@ __global__ void __launch_bounds__(kBlock) kSigmaPerm(const int64_t* __restrict__
         if (lane == 0) y[r * ld + I] = fma(d, x[r * ld + I], sum);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Half storage: each unordered pair {I, J} stored once, in row max(I, J)
+// (the reference's lower-triangular rows, SHCI/SHCImakeHamiltonian.cpp:549-711,
+// without their diagonal), for spaces whose two-triangle Hamiltonian does not
+// fit.  Sigma applies every stored element twice, as Hmult2 does
+// (SHCI/Hmult.h:110-121): y[I] += h x[J] from a gather and y[J] += h x[I]
+// by an FP64 atomic add at the L2, so its sums are not in a fixed order.
+
+// rows [nOld, n] of a grown half-storage CSR: rowPtr[I] = rowPtr[nOld] +
+// cnt[I], cnt the exclusive scan of new-row lengths (zero below nOld)
+__global__ void kLowerRowPtr(const int64_t* cnt, int64_t nOld, int64_t n, int64_t* rowPtr) {
+  const int64_t base = nOld ? rowPtr[nOld] : 0;
+  for (int64_t I = nOld + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I <= n; I += (int64_t)gridDim.x * blockDim.x)
+    if (I > nOld || nOld == 0) rowPtr[I] = base + cnt[I];
+}
+
+__global__ void kZeroVectors(double* y, int k, int64_t ld, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / n;
+    y[r * ld + (i - r * n)] = 0.0;
+  }
+}
+
+template <typename VT, bool LO>
+__global__ void __launch_bounds__(kBlock) kSigmaLower(const int64_t* __restrict__ rowPtr,
+                                                      const int32_t* __restrict__ cols, const VT* __restrict__ vals,
+                                                      const float* __restrict__ lo, const double* __restrict__ diag,
+                                                      int64_t n, const double* __restrict__ x, double* __restrict__ y,
+                                                      int k, int64_t ld) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  constexpr int U = 4;
+  for (int64_t I = (int64_t)blockIdx.x * kWaves + wave; I < n; I += (int64_t)gridDim.x * kWaves) {
+    const int64_t b = rowPtr[I], e = rowPtr[I + 1];
+    for (int r = 0; r < k; r++) {
+      const double* xr = x + r * ld;
+      double* yr = y + r * ld;
+      const double xi = xr[I];
+      double acc = 0.0;
+      int64_t p = b + lane;
+      for (; p + 64 * (U - 1) < e; p += 64 * U) {
+        int32_t c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          c[u] = __builtin_nontemporal_load(cols + p + 64 * u);
+          v[u] = (double)__builtin_nontemporal_load(vals + p + 64 * u);
+          if (LO) v[u] += (double)__builtin_nontemporal_load(lo + p + 64 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          acc = fma(v[u], xr[c[u]], acc);
+          unsafeAtomicAdd(yr + c[u], v[u] * xi);
+        }
+      }
+      for (; p < e; p += 64) {
+        const int32_t c = cols[p];
+        const double v = (double)vals[p] + (LO ? (double)lo[p] : 0.0);
+        acc = fma(v, xr[c], acc);
+        unsafeAtomicAdd(yr + c, v * xi);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) unsafeAtomicAdd(yr + I, fma(diag[I], xi, acc));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled sigma.  The alpha-major rows are cut into blocks of kTileRows
+// consecutive rows, and each block's entries are regrouped by column tile
+// (kTileCols consecutive values of the permuted x).  One wave instruction
+// takes up to 64 entries of one (block, tile) segment, so its gathers stay
+// within one tile, and the tile's x lines, brought into L1 by the first
+// instruction, serve the rest of the block's entries in that tile.  On the
+// 1.18M-determinant benchmark space a row's entries touch 0.44 distinct
+// 128-byte x lines per entry, a 256-row block's 0.038.  Within a segment
+// the entries are ordered by the wave that placed them (row mod 4), then by
+// row, then in the row's own order, so each row is one contiguous run; an
+// instruction's products are summed per row by a keyed segmented scan
+// across lanes and added into that wave's partial row sums in LDS, and the
+// four partials are added in a fixed order at the end of the block: the
+// result does not depend on scheduling.
+constexpr int kTileRows = 256;
+constexpr int kTileLog = 10;
+constexpr int kTileCols = 1 << kTileLog;
+constexpr int kTileMaxWords = 4096;  // bitmap of up to 131,072 tiles: x of up to 134M values
+constexpr int kTileInstPad = 16;  // zero descriptors after the last one (the largest step of kSigmaTiled)
+constexpr int kTileLdsSegs = 1024;  // blocks with at most this many segments count and fill in LDS
+
+__device__ __forceinline__ uint32_t tileMapCol(int32_t J, uint32_t bi, const uint32_t* __restrict__ pos,
+                                               const uint32_t* __restrict__ bIdx,
+                                               const uint32_t* __restrict__ bPos, int64_t n) {
+  return (bIdx && bIdx[J] == bi) ? (uint32_t)(n + bPos[J]) : pos[J];
+}
+
+// inclusive scan over the block's threads (kBlock); returns the thread's
+// inclusive value and the block total in *total
+template <typename T>
+__device__ __forceinline__ T blockInclusiveScan(T v, T* total) {
+  __shared__ T waveSum[kWaves];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) waveSum[wave] = v;
+  __syncthreads();
+  T before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    if (w < wave) before += waveSum[w];
+    all += waveSum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return v + before;
+}
+
+// marks the tiles the block's entries touch in bits[nwords] (LDS, cleared
+// here); with pre given, pre[k] = set bits before word k; returns the
+// number of touched tiles
+__device__ int tileBitmap(const int64_t* __restrict__ rowPtr, const int32_t* __restrict__ cols,
+                          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pos,
+                          const uint32_t* __restrict__ bIdx, const uint32_t* __restrict__ bPos, int64_t n,
+                          int64_t b, int nwords, uint32_t* bits, uint32_t* pre) {
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  for (int k = threadIdx.x; k < nwords; k += kBlock) bits[k] = 0u;
+  __syncthreads();
+  const int64_t m0 = b * kTileRows;
+  const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
+  for (int r = wave; r < rows; r += kWaves) {
+    const int64_t I = perm[m0 + r];
+    const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
+    const uint32_t bi = bIdx ? bIdx[I] : 0u;
+    for (int64_t p = e0 + lane; p < e1; p += 64) {
+      const uint32_t t = tileMapCol(cols[p], bi, pos, bIdx, bPos, n) >> kTileLog;
+      atomicOr(&bits[t >> 5], 1u << (t & 31));
+    }
+  }
+  __syncthreads();
+  const int per = (nwords + kBlock - 1) / kBlock;
+  const int k0 = threadIdx.x * per, k1 = k0 + per < nwords ? k0 + per : nwords;
+  int local = 0;
+  for (int k = k0; k < k1; k++) local += __popc(bits[k]);
+  int total = 0;
+  int run = blockInclusiveScan(local, &total) - local;
+  if (pre)
+    for (int k = k0; k < k1; k++) {
+      pre[k] = (uint32_t)run;
+      run += __popc(bits[k]);
+    }
+  __syncthreads();
+  return total;
+}
+
+__device__ __forceinline__ uint32_t tileRank(const uint32_t* bits, const uint32_t* pre, uint32_t t) {
+  return pre[t >> 5] + __popc(bits[t >> 5] & ((1u << (t & 31)) - 1u));
+}
+
+// pass 1: touched tiles per block (segCount[b])
+__global__ void __launch_bounds__(kBlock) kTileMark(const int64_t* rowPtr, const int32_t* cols, const uint32_t* perm,
+                                                    const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos,
+                                                    int64_t n, int nwords, int64_t* segCount) {
+  __shared__ uint32_t bits[kTileMaxWords];
+  const int total = tileBitmap(rowPtr, cols, perm, pos, bIdx, bPos, n, blockIdx.x, nwords, bits, nullptr);
+  if (threadIdx.x == 0) segCount[blockIdx.x] = total;
+}
+
+// pass 2: per (segment, wave) entry counts; segment tiles; per segment its
+// entry and instruction offsets within the block and the exclusive prefix
+// of its wave counts; per block its entries and instructions
+__global__ void __launch_bounds__(kBlock) kTileCount(const int64_t* rowPtr, const int32_t* cols, const uint32_t* perm,
+                                                     const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos,
+                                                     int64_t n, int nwords, const int64_t* segBase, uint32_t* segTile,
+                                                     uint32_t* segCnt, uint32_t* segStart, uint32_t* segInst,
+                                                     uint32_t* segTotal, int64_t* blockE, int64_t* blockI) {
+  __shared__ uint32_t bits[kTileMaxWords];
+  __shared__ uint32_t pre[kTileMaxWords];
+  __shared__ uint32_t cntL[kWaves][kTileLdsSegs];
+  const int64_t b = blockIdx.x;
+  tileBitmap(rowPtr, cols, perm, pos, bIdx, bPos, n, b, nwords, bits, pre);
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int64_t base = segBase[b];
+  const bool inLds = segBase[b + 1] - base <= kTileLdsSegs;
+  if (inLds)
+    for (int k = threadIdx.x; k < kWaves * kTileLdsSegs; k += kBlock) (&cntL[0][0])[k] = 0u;
+  __syncthreads();
+  const int64_t m0 = b * kTileRows;
+  const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
+  for (int r = wave; r < rows; r += kWaves) {
+    const int64_t I = perm[m0 + r];
+    const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
+    const uint32_t bi = bIdx ? bIdx[I] : 0u;
+    for (int64_t p = e0 + lane; p < e1; p += 64) {
+      const uint32_t t = tileMapCol(cols[p], bi, pos, bIdx, bPos, n) >> kTileLog;
+      const uint32_t sg = tileRank(bits, pre, t);
+      if (inLds)
+        atomicAdd(&cntL[wave][sg], 1u);
+      else
+        atomicAdd(&segCnt[(base + sg) * 4 + wave], 1u);
+    }
+  }
+  if (inLds) {
+    __syncthreads();
+    const int64_t D = segBase[b + 1] - base;
+    for (int64_t k = threadIdx.x; k < D * 4; k += kBlock) segCnt[base * 4 + k] = cntL[k & 3][k >> 2];
+  }
+  for (int k = threadIdx.x; k < nwords; k += kBlock) {
+    uint32_t w = bits[k];
+    uint32_t j = pre[k];
+    while (w) {
+      segTile[base + j] = ((uint32_t)k << 5) + (uint32_t)(__ffs(w) - 1);
+      j++;
+      w &= w - 1u;
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  const int64_t D = segBase[b + 1] - base;
+  uint64_t carry = 0;  // entries (low 32 bits) and instructions (high 32 bits) so far
+  for (int64_t s0 = 0; s0 < D; s0 += kBlock) {
+    const int64_t s = s0 + threadIdx.x;
+    uint64_t mine = 0;
+    if (s < D) {
+      uint32_t c[4];
+#pragma unroll
+      for (int w = 0; w < 4; w++)
+        c[w] = __hip_atomic_load(&segCnt[(base + s) * 4 + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t tot = c[0] + c[1] + c[2] + c[3];
+      segCnt[(base + s) * 4 + 0] = 0u;
+      segCnt[(base + s) * 4 + 1] = c[0];
+      segCnt[(base + s) * 4 + 2] = c[0] + c[1];
+      segCnt[(base + s) * 4 + 3] = c[0] + c[1] + c[2];
+      segTotal[base + s] = tot;
+      mine = (uint64_t)tot | ((uint64_t)((tot + 63u) / 64u) << 32);
+    }
+    uint64_t chunk = 0;
+    const uint64_t incl = blockInclusiveScan(mine, &chunk);
+    if (s < D) {
+      const uint64_t ex = carry + incl - mine;
+      segStart[base + s] = (uint32_t)ex;
+      segInst[base + s] = (uint32_t)(ex >> 32);
+    }
+    carry += chunk;
+  }
+  if (threadIdx.x == 0) {
+    blockE[b] = (int64_t)(uint32_t)carry;
+    blockI[b] = (int64_t)(carry >> 32);
+  }
+}
+
+// pass 3: entries to their places, and the instruction descriptors
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const int32_t* cols, const VT* vals,
+                                                    const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
+                                                    const uint32_t* bPos, int64_t n, int nwords,
+                                                    const int64_t* segBase, const uint32_t* segTile,
+                                                    const uint32_t* segCnt, const uint32_t* segStart,
+                                                    const uint32_t* segInst, const uint32_t* segTotal, uint32_t* fill,
+                                                    const int64_t* blockE, const int64_t* blockI, uint32_t* tIdx,
+                                                    VT* tVal, uint2* inst) {
+  __shared__ uint32_t bits[kTileMaxWords];
+  __shared__ uint32_t pre[kTileMaxWords];
+  __shared__ uint32_t fillL[kWaves][kTileLdsSegs];
+  const int64_t b = blockIdx.x;
+  tileBitmap(rowPtr, cols, perm, pos, bIdx, bPos, n, b, nwords, bits, pre);
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int64_t base = segBase[b], eb = blockE[b];
+  const bool inLds = segBase[b + 1] - base <= kTileLdsSegs;
+  if (inLds)
+    for (int k = threadIdx.x; k < kWaves * kTileLdsSegs; k += kBlock) (&fillL[0][0])[k] = 0u;
+  __syncthreads();
+  const int64_t m0 = b * kTileRows;
+  const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
+  for (int r = wave; r < rows; r += kWaves) {
+    const int64_t I = perm[m0 + r];
+    const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
+    const uint32_t bi = bIdx ? bIdx[I] : 0u;
+    for (int64_t p0 = e0; p0 < e1; p0 += 64) {
+      const int64_t p = p0 + lane;
+      const bool valid = p < e1;
+      uint32_t c = 0, s = 0;
+      VT v = (VT)0;
+      if (valid) {
+        c = tileMapCol(cols[p], bi, pos, bIdx, bPos, n);
+        s = tileRank(bits, pre, c >> kTileLog);
+        v = vals[p];
+      }
+      // the entry's place among the wave's entries of its segment: the
+      // wave's fill counter of the segment, taken by an atomic add with
+      // return (lanes of one instruction that share a counter receive
+      // consecutive values in lane order)
+      uint32_t got = 0;
+      if (valid) {
+        if (inLds)
+          got = atomicAdd(&fillL[wave][s], 1u);
+        else
+          got = atomicAdd(&fill[(base + s) * 4 + wave], 1u);
+      }
+      if (valid) {
+        const int64_t q = eb + segStart[base + s] + segCnt[(base + s) * 4 + wave] + got;
+        tIdx[q] = ((uint32_t)r << 16) | (c & (uint32_t)(kTileCols - 1));
+        tVal[q] = v;
+      }
+    }
+  }
+  const int64_t D = segBase[b + 1] - base, ib = blockI[b];
+  for (int64_t s = threadIdx.x; s < D; s += kBlock) {
+    const uint32_t tot = segTotal[base + s], st = segStart[base + s], tile = segTile[base + s];
+    const int64_t i0 = ib + segInst[base + s];
+    for (uint32_t j = 0; 64u * j < tot; j++) {
+      const uint32_t cnt = tot - 64u * j < 64u ? tot - 64u * j : 64u;
+      inst[i0 + j] = make_uint2(st + 64u * j, (tile << 7) | cnt);
+    }
+  }
+}
+
+// One step of a keyed inclusive scan across the wave by DPP: lane i adds
+// the value of the lane the DPP control CTRL names (within the rows of 16
+// lanes ROWMASK enables) when that lane has the same key.  Keys form
+// contiguous runs, so the steps row_shr 1, 2, 4, 8, then row_bcast 15 and
+// row_bcast 31 give every lane the sum of its run up to itself.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void keyedScanStep(double& s, int k) {
+  const int ok = __builtin_amdgcn_update_dpp(-1, k, CTRL, ROWMASK, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(s), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(s), CTRL, ROWMASK, 0xf, false);
+  if (ok == k) s += __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double keyedInclusiveScan(double s, int k) {
+  keyedScanStep<0x111, 0xf>(s, k);  // row_shr:1
+  keyedScanStep<0x112, 0xf>(s, k);  // row_shr:2
+  keyedScanStep<0x114, 0xf>(s, k);  // row_shr:4
+  keyedScanStep<0x118, 0xf>(s, k);  // row_shr:8
+  keyedScanStep<0x142, 0xa>(s, k);  // row_bcast:15 into rows 1 and 3
+  keyedScanStep<0x143, 0xc>(s, k);  // row_bcast:31 into rows 2 and 3
+  return s;
+}
+
+// key of the next lane (-1 past lane 63), by DPP wave_shl:1
+__device__ __forceinline__ int nextLaneKey(int k) { return __builtin_amdgcn_update_dpp(-1, k, 0x130, 0xf, 0xf, false); }
+
+// y[perm[m]] = diag .* x + (H x) over the tiled copy, one vector; xP holds x
+// in the permuted order of the copy's columns
+template <typename VT, typename XT, bool PERSIST, int U>
+__global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ inst, const int64_t* __restrict__ blockI,
+                                                      const int64_t* __restrict__ blockE,
+                                                      const uint32_t* __restrict__ tIdx, const VT* __restrict__ tVal,
+                                                      const double* __restrict__ diag, const uint32_t* __restrict__ perm,
+                                                      int64_t n, int64_t nblocks, const XT* __restrict__ xP,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      unsigned* counter) {
+  __shared__ double yw[kWaves][kTileRows];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
+  static_assert(U <= kTileInstPad, "step larger than the descriptor padding");
+  ChunkSched<1, PERSIST> sched{nblocks, counter, -1};
+  while (sched.next()) {
+    const int64_t b = sched.cur;
+    for (int j = lane; j < kTileRows; j += 64) yw[wave][j] = 0.0;
+    const int64_t i0 = blockI[b], i1 = blockI[b + 1], e0 = blockE[b];
+    // the U instructions' loads are issued without branches, all matrix loads
+    // before all gathers (vector loads return in order, so a gather issued
+    // between them would make every later wait include it); absent
+    // instructions and lanes past an instruction's count read the block's
+    // first entry and their tile's first x value, and are dropped
+    // each wave takes U consecutive instructions at a time (one scalar load
+    // of their descriptors; the array has U zero descriptors of padding)
+    for (int64_t i = i0 + (int64_t)wave * U; i < i1; i += (int64_t)kWaves * U) {
+      uint32_t dx[U], dy[U];
+      const uint2* dp = inst + i;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint2 d = dp[u];
+        dx[u] = d.x;
+        dy[u] = i + u < i1 ? d.y : (d.y & ~127u);  // count 0 past the block
+      }
+      uint32_t ix[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool valid = (uint32_t)lane < (dy[u] & 127u);
+        const int64_t p = e0 + (valid ? (int64_t)dx[u] + lane : 0);
+        ix[u] = __builtin_nontemporal_load(tIdx + p);
+        v[u] = (double)__builtin_nontemporal_load(tVal + p);
+      }
+      double xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool valid = (uint32_t)lane < (dy[u] & 127u);
+        const size_t tb = (size_t)(dy[u] >> 7) << kTileLog;
+        xv[u] = (double)xP[tb + (valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u)];
+      }
+      // products summed per row run by the keyed scan; the run's last lane
+      // adds the sum into the wave's partial row (lanes past the count carry
+      // keys of their own and are dropped)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool valid = (uint32_t)lane < (dy[u] & 127u);
+        const int k = valid ? (int)(ix[u] >> 16) : kTileRows + lane;
+        const double sum = keyedInclusiveScan(valid ? v[u] * xv[u] : 0.0, k);
+        if (k < kTileRows && nextLaneKey(k) != k) yw[wave][k] += sum;
+      }
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int64_t m = b * kTileRows + t;
+    if (m < n) {
+      const int64_t I = perm[m];
+      y[I] = fma(diag[I], x[I], ((yw[0][t] + yw[1][t]) + yw[2][t]) + yw[3][t]);
+    }
+    if (!PERSIST) break;
   }
 }
 
@@ -2740,32 +3163,70 @@ static RowTables rowTables(const StringTables& T) {
 }
 
 void launchCsrCount(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld, int64_t* rowCount,
-                    hipStream_t s) {
+                    hipStream_t s, bool lower) {
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, false, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr, nullptr,
-                                                                    nullptr, nullptr, nullptr));
+  if (lower)
+    DISPATCH_W(S.W, kCsrRows<WW, false, false, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr,
+                                                                          nullptr, nullptr, nullptr, nullptr));
+  else
+    DISPATCH_W(S.W, kCsrRows<WW, false, false, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, rowCount, nullptr,
+                                                                           nullptr, nullptr, nullptr, nullptr));
   HIP_CHECK(hipGetLastError());
 }
 
 void launchCsrFill(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
-                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s) {
+                   const int64_t* fillStart, int32_t* cols, double* vals, hipStream_t s, bool lower) {
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, true, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, vals,
-                                                                   nullptr, nullptr));
+  if (lower)
+    DISPATCH_W(S.W, kCsrRows<WW, true, false, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols,
+                                                                         vals, nullptr, nullptr));
+  else
+    DISPATCH_W(S.W, kCsrRows<WW, true, false, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols,
+                                                                          vals, nullptr, nullptr));
   HIP_CHECK(hipGetLastError());
 }
 
 void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTables& T, int64_t nOld,
-                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s) {
+                        const int64_t* fillStart, int32_t* cols, float* hi, float* lo, hipStream_t s, bool lower) {
   if (T.n <= 0) return;
   unsigned g = gridFor(T.n, kWaves, 1 << 16);
   RowTables r = rowTables(T);
-  DISPATCH_W(S.W, kCsrRows<WW, true, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols, nullptr,
-                                                                  hi, lo));
+  if (lower)
+    DISPATCH_W(S.W, kCsrRows<WW, true, true, true><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols,
+                                                                        nullptr, hi, lo));
+  else
+    DISPATCH_W(S.W, kCsrRows<WW, true, true, false><<<g, kBlock, 0, s>>>(S, dets, r, nOld, nullptr, fillStart, cols,
+                                                                         nullptr, hi, lo));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchLowerRowPtr(const int64_t* cnt, int64_t nOld, int64_t n, int64_t* rowPtr, hipStream_t s) {
+  if (n + 1 - nOld <= 0) return;
+  kLowerRowPtr<<<gridFor(n + 1 - nOld), kBlock, 0, s>>>(cnt, nOld, n, rowPtr);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaLowerF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
+                         const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  if (n <= 0 || k <= 0) return;
+  kZeroVectors<<<gridFor(n * k), kBlock, 0, s>>>(y, k, ld, n);
+  kSigmaLower<double, false><<<gridFor(n, kWaves), kBlock, 0, s>>>(rowPtr, cols, vals, nullptr, diag, n, x, y, k, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaLowerF32(const int64_t* rowPtr, const int32_t* cols, const float* vals, const float* lo,
+                         const double* diag, int64_t n, const double* x, double* y, int k, int64_t ld, hipStream_t s) {
+  if (n <= 0 || k <= 0) return;
+  kZeroVectors<<<gridFor(n * k), kBlock, 0, s>>>(y, k, ld, n);
+  if (lo)
+    kSigmaLower<float, true><<<gridFor(n, kWaves), kBlock, 0, s>>>(rowPtr, cols, vals, lo, diag, n, x, y, k, ld);
+  else
+    kSigmaLower<float, false><<<gridFor(n, kWaves), kBlock, 0, s>>>(rowPtr, cols, vals, nullptr, diag, n, x, y, k,
+                                                                    ld);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -3010,6 +3471,143 @@ void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* val
                         const uint32_t* perm, const uint32_t* bPerm, int64_t n, const double* x, double* y, int k,
                         int64_t ld, const XScratch& xP, const SigmaConfig& cfg, unsigned* counter, hipStream_t s) {
   launchSigmaPermT<float, float>(rp, cols, vals, diag, perm, bPerm, n, x, y, k, ld, xP, cfg, counter, s);
+}
+
+void TiledHam::release() {
+  for (DevMem* m : {&segBase, &segTile, &segCnt, &segStart, &segInst, &segTotal, &fill, &blockE, &blockI, &idx, &vals,
+                    &inst})
+    m->release();
+  n = -1;
+}
+
+size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32) {
+  const int64_t blocks = (n + kTileRows - 1) / kTileRows;
+  // entries, one instruction per 32 entries at most on average plus one per
+  // segment, segment tables (48 bytes) with at most one segment per 64
+  // entries or per block
+  const int64_t segs = nnz / 64 + blocks;
+  return (size_t)nnz * (4 + (f32 ? 4 : 8)) + (size_t)(nnz / 32 + segs) * 8 + (size_t)segs * 48 +
+         (size_t)(blocks + 1) * 24;
+}
+
+bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
+                   const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
+                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s) {
+  T.n = -1;
+  if (n <= 0) return false;
+  const int64_t nx = bIdx ? 2 * n : n;
+  const int64_t tiles = (nx + kTileCols - 1) / kTileCols;
+  const int nwords = (int)((tiles + 31) / 32);
+  if (nwords > kTileMaxWords) return false;
+  const int64_t blocks = (n + kTileRows - 1) / kTileRows;
+  if (blocks > 0x7fffffffLL) return false;
+  const unsigned grid = (unsigned)blocks;
+  T.segBase.reserve((size_t)(blocks + 1) * 8);
+  int64_t* segBase = T.segBase.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(segBase + blocks, 0, 8, s));
+  kTileMark<<<grid, kBlock, 0, s>>>(rowPtr, cols, perm, pos, bIdx, bPos, n, nwords, segBase);
+  HIP_CHECK(hipGetLastError());
+  exclusiveScanI64(segBase, blocks + 1, ws, s);
+  int64_t S = 0;
+  HIP_CHECK(hipMemcpyAsync(&S, segBase + blocks, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  const size_t sb = (size_t)std::max<int64_t>(S, 1);
+  T.segTile.reserve(sb * 4);
+  T.segCnt.reserve(sb * 16);
+  T.segStart.reserve(sb * 4);
+  T.segInst.reserve(sb * 4);
+  T.segTotal.reserve(sb * 4);
+  T.fill.reserve(sb * 16);
+  T.blockE.reserve((size_t)(blocks + 1) * 8);
+  T.blockI.reserve((size_t)(blocks + 1) * 8);
+  HIP_CHECK(hipMemsetAsync(T.segCnt.p, 0, sb * 16, s));
+  HIP_CHECK(hipMemsetAsync(T.fill.p, 0, sb * 16, s));
+  int64_t* blockE = T.blockE.as<int64_t>();
+  int64_t* blockI = T.blockI.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(blockE + blocks, 0, 8, s));
+  HIP_CHECK(hipMemsetAsync(blockI + blocks, 0, 8, s));
+  kTileCount<<<grid, kBlock, 0, s>>>(rowPtr, cols, perm, pos, bIdx, bPos, n, nwords, segBase,
+                                     T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(), T.segStart.as<uint32_t>(),
+                                     T.segInst.as<uint32_t>(), T.segTotal.as<uint32_t>(), blockE, blockI);
+  HIP_CHECK(hipGetLastError());
+  exclusiveScanI64(blockE, blocks + 1, ws, s);
+  exclusiveScanI64(blockI, blocks + 1, ws, s);
+  int64_t E = 0, I = 0;
+  HIP_CHECK(hipMemcpyAsync(&E, blockE + blocks, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&I, blockI + blocks, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (E != nnz) throw HipError("tiled Hamiltonian: entry count " + std::to_string(E) + " != " + std::to_string(nnz));
+  const bool f32 = vals64 == nullptr;
+  T.idx.reserve((size_t)std::max<int64_t>(E, 1) * 4);
+  T.vals.reserve((size_t)std::max<int64_t>(E, 1) * (f32 ? 4 : 8));
+  // kSigmaTiled reads up to kTileInstPad descriptors past a block's last
+  // one: zero padding after the last block (count 0, tile 0)
+  T.inst.reserve((size_t)(I + kTileInstPad) * 8);
+  HIP_CHECK(hipMemsetAsync(T.inst.as<uint2>() + I, 0, (size_t)kTileInstPad * 8, s));
+  if (f32)
+    kTileFill<float><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals32, perm, pos, bIdx, bPos, n, nwords, segBase,
+                                             T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
+                                             T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
+                                             T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
+                                             T.idx.as<uint32_t>(), T.vals.as<float>(), T.inst.as<uint2>());
+  else
+    kTileFill<double><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals64, perm, pos, bIdx, bPos, n, nwords, segBase,
+                                              T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
+                                              T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
+                                              T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
+                                              T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>());
+  HIP_CHECK(hipGetLastError());
+  T.n = n;
+  T.blocks = blocks;
+  T.segments = S;
+  T.nnz = E;
+  T.instructions = I;
+  T.f32 = f32;
+  T.split = bIdx != nullptr;
+  return true;
+}
+
+template <typename VT, typename XT, bool PERSIST>
+static void sigmaTiledU(int step, unsigned grid, const TiledHam& T, const double* diag, const uint32_t* perm,
+                        int64_t n, const XT* xs, const double* x, double* y, unsigned* counter, hipStream_t s) {
+  if (step >= 16)
+    kSigmaTiled<VT, XT, PERSIST, 16><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
+                                                             T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+                                                             T.vals.as<VT>(), diag, perm, n, T.blocks, xs, x, y,
+                                                             counter);
+  else
+    kSigmaTiled<VT, XT, PERSIST, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
+                                                            T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+                                                            T.vals.as<VT>(), diag, perm, n, T.blocks, xs, x, y,
+                                                            counter);
+}
+
+void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
+                      const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
+                      hipStream_t s, int step) {
+  const int64_t n = T.n;
+  if (n <= 0) return;
+  if (T.split != (bPerm != nullptr)) throw HipError("tiled sigma: split columns need the beta-major order");
+  const int64_t nx = bPerm ? 2 * n : n;
+  const unsigned grid =
+      (unsigned)(persistent ? std::min<int64_t>(T.blocks, (int64_t)computeUnits() * 8) : T.blocks);
+  if (persistent && !counter) throw HipError("persistent sigma needs a work counter");
+  if (T.f32) {
+    float* xs = (float*)xP.get((size_t)nx * 4);
+    kGatherPerm<float, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, counter);
+    if (persistent)
+      sigmaTiledU<float, float, true>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+    else
+      sigmaTiledU<float, float, false>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+  } else {
+    double* xs = (double*)xP.get((size_t)nx * 8);
+    kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, counter);
+    if (persistent)
+      sigmaTiledU<double, double, true>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+    else
+      sigmaTiledU<double, double, false>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+  }
+  HIP_CHECK(hipGetLastError());
 }
 
 void launchGram(const double* A, int ma, const double* B, int mb, int64_t n, int64_t ld, double* G, DevMem& partial,

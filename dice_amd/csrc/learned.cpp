@@ -46,6 +46,8 @@ void Engine::learnedLog(const double* dCmax, double eps1) {
   lVals.reserve((size_t)perWindow * sizeof(float));
   std::vector<float> hx, ht;
   std::vector<int64_t> idx;
+  lastLogRecords = 0;
+  lastLogSources = 0;
   for (int w = 0; w < windows; w++) {
     const int64_t begin = n > window ? (int64_t)(splitmix(learnedRng) % (uint64_t)(n - window + 1)) : 0;
     HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
@@ -55,6 +57,8 @@ void Engine::learnedLog(const double* dCmax, double eps1) {
     HIP_CHECK(hipMemcpyAsync(&found, counter.p, 8, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     const int64_t total = std::min<int64_t>((int64_t)found, kRecordCap);
+    lastLogRecords += (int64_t)found;
+    lastLogSources += std::min<int64_t>(window, n - begin);
     if (total == 0) continue;
     const int64_t m = std::min(total, perWindow);
     idx.resize(m);
@@ -72,6 +76,23 @@ void Engine::learnedLog(const double* dCmax, double eps1) {
     HIP_CHECK(hipStreamSynchronize(stream));
     trainSamples += total;
   }
+}
+
+bool Engine::learnedWorthIt(double exactScreenSeconds, int64_t sources, int iter) {
+  if (lastLogSources <= 0 || sources <= 0) return false;
+  const double perSource = (double)lastLogRecords / (double)lastLogSources;
+  const double rankSeconds = perSource * (double)sources * learnedNsPerRecord * 1e-9;
+  if (rankSeconds < exactScreenSeconds) return true;
+  LearnedRecord rec;
+  rec.iter = iter;
+  rec.fallback = true;
+  rec.candidates = (int64_t)(perSource * (double)sources);
+  learnedRecs.push_back(rec);
+  if (std::getenv("SHCI_TIMING"))
+    std::fprintf(stderr,
+                 "# learned: off (ranking %.3g records at %.1f ns each, %.3g s, against %.3g s of exact screening)\n",
+                 perSource * (double)sources, learnedNsPerRecord, rankSeconds, exactScreenSeconds);
+  return false;
 }
 
 void Engine::learnedTrain() {

@@ -45,7 +45,15 @@ class RunOptions(ctypes.Structure):
     """shci_run_options of include/shci_gpu.h."""
     _fields_ = [("precision", ctypes.c_int), ("profile_sigma", ctypes.c_int), ("davidson_block", ctypes.c_int),
                 ("autotune", ctypes.c_int), ("hip_graphs", ctypes.c_int), ("learned", ctypes.c_int),
-                ("direct_sigma", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
+                ("direct_sigma", ctypes.c_int), ("half_storage", ctypes.c_int)]
+
+
+class VarParams(ctypes.Structure):
+    """shci_var_params of include/shci_gpu.h."""
+    _fields_ = [("nroots", ctypes.c_int), ("n_iter", ctypes.c_int), ("eps1", ctypes.POINTER(ctypes.c_double)),
+                ("davidson_tol", ctypes.c_double), ("davidson_tol_loose", ctypes.c_double), ("dE", ctypes.c_double),
+                ("seed", ctypes.c_uint), ("ncore", ctypes.c_int), ("nact", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class ShciError(RuntimeError):
@@ -68,6 +76,8 @@ def library():
     sig = {
         "shci_create": ([ctypes.c_char_p, ctypes.POINTER(P)], I32),
         "shci_destroy": ([P], I32),
+        "shci_create_from_integrals": ([I32, I32, pD, pD, D, pI32, ctypes.POINTER(P)], I32),
+        "shci_do_variational": ([P, ctypes.POINTER(VarParams), pI64, I64, pD, pD], I32),
         "shci_last_error": ([P], ctypes.c_char_p),
         "shci_norb": ([P], I32),
         "shci_nelec": ([P], I32),
@@ -142,6 +152,61 @@ class System:
         if st != 0:
             raise ShciError(st, L.shci_last_error(None).decode())
         return cls(h)
+
+    @classmethod
+    def from_integrals(cls, h1, eri8, nelec, core_energy=0.0, orbsym=None):
+        """From integrals in memory: h1 norb x norb, eri8 the 8-fold packed (pq|rs)
+        (PySCF's ao2mo.restore(8, eri, norb) order)."""
+        L = library()
+        h1 = np.ascontiguousarray(h1, dtype=np.float64)
+        eri8 = np.ascontiguousarray(eri8, dtype=np.float64)
+        norb = h1.shape[0]
+        npair = norb * (norb + 1) // 2
+        if h1.shape != (norb, norb) or eri8.size != npair * (npair + 1) // 2:
+            raise ValueError("h1 must be norb x norb and eri8 hold npair(npair+1)/2 values")
+        sym = None if orbsym is None else np.ascontiguousarray(orbsym, dtype=np.int32)
+        h = ctypes.c_void_p()
+        st = L.shci_create_from_integrals(norb, int(nelec), _p(h1, ctypes.c_double), _p(eri8, ctypes.c_double),
+                                          float(core_energy), None if sym is None else _p(sym, ctypes.c_int32),
+                                          ctypes.byref(h))
+        if st != 0:
+            raise ShciError(st, L.shci_last_error(None).decode())
+        return cls(h)
+
+    def variational(self, dets, eps1, nroots=1, ci=None, davidson_tol=5e-5, davidson_tol_loose=5e-5, dE=1e-8,
+                    seed=1, ncore=0, nact=1000000):
+        """SHCIbasics::DoVariational on in-memory data (shci_do_variational): the
+        iterations of the eps1 list from the space `dets`; returns the energies
+        (core energy added).  The final space and records stay on the handle
+        (dets(), run_ci(), run_iterations())."""
+        d = self._dets(dets)
+        eps = np.ascontiguousarray(eps1, dtype=np.float64)
+        p = VarParams()
+        p.nroots, p.n_iter, p.eps1 = nroots, len(eps), _p(eps, ctypes.c_double)
+        p.davidson_tol, p.davidson_tol_loose, p.dE, p.seed = davidson_tol, davidson_tol_loose, dE, seed
+        p.ncore, p.nact = ncore, nact
+        c = None
+        if ci is not None:
+            c = np.asfortranarray(np.asarray(ci, dtype=np.float64).reshape(len(d), nroots))
+        e = np.zeros(nroots)
+        self._check(self._lib.shci_do_variational(self._h, ctypes.byref(p), _p(d, ctypes.c_int64), len(d),
+                                                  None if c is None else _p(c, ctypes.c_double),
+                                                  _p(e, ctypes.c_double)))
+        return e
+
+    def run_iterations(self):
+        return VariationalResult.iterations(self)
+
+    def run_energies(self):
+        return VariationalResult.energies(self)
+
+    def run_ci(self):
+        """Final coefficients of the last run on this handle, n x nroots."""
+        n = self.num_dets()
+        nr = len(self.run_energies())
+        out = np.zeros(n * nr)
+        self._check(self._lib.shci_run_ci(self._h, _p(out, ctypes.c_double), len(out)))
+        return out.reshape(nr, n).T
 
     def close(self):
         if self._h:
@@ -408,7 +473,7 @@ class VariationalResult:
 
 
 def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False, davidson_block=False,
-                   autotune=False, hip_graphs=False, learned=False):
+                   autotune=False, hip_graphs=False, learned=False, half_storage=False):
     """Run the variational step of ``Dice input.dat``; returns a VariationalResult.
 
     precision: PREC_FP64 (the drop-in) or PREC_FP32 (FP32 values, FP64
@@ -423,7 +488,9 @@ def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma
     opt.davidson_block = 1 if davidson_block else 0
     opt.autotune = 1 if autotune else 0
     opt.hip_graphs = 1 if hip_graphs else 0
-    opt.learned = 2 if learned == "control" else (1 if learned else 0)
+    opt.half_storage = 1 if half_storage else 0
+    opt.learned = {"control": 2, "forced": 3}.get(learned, 1 if learned else 0) if isinstance(learned, str) \
+        else (1 if learned else 0)
     st = L.shci_run_ex(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, ctypes.byref(opt),
                        ctypes.byref(h))
     if st != 0:

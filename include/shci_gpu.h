@@ -50,6 +50,18 @@ typedef struct shci_handle shci_handle;
  * twoIntHeatBath(SHM)::constructClass (SHCI/integral.h:125-153,
  * SHCI/integral.cpp:228-389), as called from SHCI/SHCI.cpp:213,248-252. */
 shci_status shci_create(const char* fcidump_path, shci_handle** out);
+/* The same from integrals in memory, as SHCIbasics::DoVariational takes them
+ * (SHCI/SHCIbasics.h:89-93: oneInt, twoInt, irrep, coreE, nelec) instead of
+ * a file: h1 = norb x norb one-electron integrals, row-major (its lower
+ * triangle is used and mirrored, as an FCIDUMP lists each pair once);
+ * eri8 = the two-electron integrals (pq|rs) packed 8-fold in the reference's
+ * twoInt order (SHCI/integral.h:65-102): element A(A+1)/2 + B for pair
+ * indices A >= B, pair(p,q) = p(p+1)/2 + q for p >= q (PySCF's
+ * ao2mo.restore(8, eri, norb) layout), npair(npair+1)/2 values with
+ * npair = norb(norb+1)/2; core_energy = the constant term; orbsym = norb
+ * irreps or NULL (all 1).  norb <= 63. */
+shci_status shci_create_from_integrals(int norb, int nelec, const double* h1, const double* eri8, double core_energy,
+                                      const int* orbsym, shci_handle** out);
 shci_status shci_destroy(shci_handle* h);
 /* last error message of the handle (or of the last failed shci_create when h is NULL) */
 const char* shci_last_error(const shci_handle* h);
@@ -183,6 +195,33 @@ shci_status shci_pt_semistochastic(shci_handle* h, const double* ci, double e0, 
  * inspection. */
 shci_status shci_run(const char* input_path, const char* log_path, int precision, shci_handle** out);
 
+/* Schedule and tolerances of shci_do_variational: the fields of the
+ * reference's schedule (SHCI/input.h:35-194) that DoVariational reads. */
+typedef struct {
+  int nroots;
+  int n_iter;                /* number of variational iterations: entries of eps1 */
+  const double* eps1;        /* eps1 of each iteration (the reference's expanded schd.epsilon1) */
+  double davidson_tol;       /* final tight Davidson (davidsonTol) */
+  double davidson_tol_loose; /* the iterations' Davidson (davidsonTolLoose) */
+  double dE;                 /* convergence of the lowest energy between iterations */
+  unsigned seed;             /* srand(seed) of the random Davidson guesses (SHCI/SHCI.cpp:199) */
+  int ncore, nact;           /* excitation limits (0 and >= norb for none) */
+  int reserved[4];
+} shci_var_params;
+/* SHCIbasics::DoVariational (SHCI/SHCIbasics.cpp:783-1378, declared at
+ * SHCI/SHCIbasics.h:89-93, called from SHCI/SHCI.cpp:422 and
+ * SHCI/MBE.cpp:441-446) on in-memory data: the variational space starts as
+ * dets (ndets x words, unique), with coefficients ci (ndets x nroots
+ * column-major; NULL: every determinant 1 and the lowest-energy one
+ * + ndets, normalised, root columns >= 1 zero, as SHCI/SHCI.cpp:395-399);
+ * the iterations select, build and diagonalise as shci_run does (method
+ * options from shci_set_options; no files, no table, no PT).  energies =
+ * nroots variational energies with the core energy added.  The handle then
+ * holds the final space: shci_num_dets / shci_get_dets, shci_run_ci,
+ * shci_run_iteration. */
+shci_status shci_do_variational(shci_handle* h, const shci_var_params* p, const int64_t* dets, int64_t ndets,
+                                const double* ci, double* energies);
+
 /* Options of shci_run_ex and shci_set_options (zero-initialise, then set
  * fields; all zero = the reference's method in FP64). */
 typedef struct {
@@ -195,18 +234,25 @@ typedef struct {
                          workgroups) by timing on each Hamiltonian whose size has doubled */
   int hip_graphs;     /* nonzero (with davidson_block): replay block-Davidson iterations from captured
                          hipGraphs */
-  int learned;        /* 1: learned screening form (shci_run / shci_run_ex only): the first two
+  int learned;        /* learned screening form (shci_run / shci_run_ex only).  3: the first two
                          iterations screen in full and log training samples, an MLP trained on them then
                          predicts log10(|c| |v|) for the double-excitation candidates (heat-bath value
                          >= eps / 10) and only those predicted above log10(eps1) - 0.8 (environment
                          SHCI_LEARNED_MARGIN) are tested exactly; singles stay exact; when a sample shows
                          that more than half would be tested, that iteration and the later ones screen in
-                         full.  2: the same with the untrained network (control) */
+                         full.  1: the same, gated: after the second logging iteration the form turns
+                         itself off (no training, full screening) unless ranking that iteration's candidate
+                         records at 2 ns each would cost less than its exact screen.  2: as 3 with the
+                         untrained network (control) */
   int direct_sigma;   /* nonzero: no stored Hamiltonian; every sigma evaluates the elements from the string
                          tables (HmultDirect, SHCI/Hmult.h:153-382).  shci_run_ex also turns it on, with
                          davidsonDirect's rules (SHCI/Davidson.cpp:353-590), for the `directdavidson`
                          keyword */
-  int reserved[1];
+  int half_storage;   /* nonzero: each connected pair stored once, in row max(I, J) (the reference's
+                         lower-triangular rows, SHCI/SHCImakeHamiltonian.cpp:549-711), sigma applying
+                         the transposed half by FP64 atomic adds (SHCI/Hmult.h:110-121), for spaces whose
+                         two-triangle Hamiltonian does not fit; a build switches to it by itself when the
+                         two-triangle one does not fit */
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,

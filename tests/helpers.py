@@ -210,3 +210,43 @@ def occ_string(occ, norb):
     o = set(occ)
     return "".join("2" if (2 * i in o and 2 * i + 1 in o) else "a" if 2 * i in o else "b" if 2 * i + 1 in o
                    else "0" for i in range(norb))
+
+
+def read_fcidump_arrays(path):
+    """(norb, nelec, h1 norb x norb, eri8 8-fold packed, core energy, orbsym)
+    from a FCIDUMP, in the in-memory layout of shci_create_from_integrals
+    (lower-triangle pair index A = p(p+1)/2 + q for p >= q, element
+    A(A+1)/2 + B for A >= B)."""
+    import re
+    with open(path) as f:
+        text = f.read()
+    head, body = re.split(r"&END|/\s*\n", text, maxsplit=1, flags=re.I)
+    norb = int(re.search(r"NORB\s*=\s*(\d+)", head, re.I).group(1))
+    nelec = int(re.search(r"NELEC\s*=\s*(\d+)", head, re.I).group(1))
+    m = re.search(r"ORBSYM\s*=\s*([\d,\s]+)", head, re.I)
+    orbsym = [int(x) for x in re.split(r"[,\s]+", m.group(1).strip()) if x][:norb] if m else [1] * norb
+    npair = norb * (norb + 1) // 2
+    h1 = np.zeros((norb, norb))
+    eri8 = np.zeros(npair * (npair + 1) // 2)
+    ecore = 0.0
+
+    def pair(p, q):
+        return p * (p + 1) // 2 + q if p >= q else q * (q + 1) // 2 + p
+
+    for line in body.splitlines():
+        t = line.split()
+        if len(t) != 5:
+            continue
+        v = float(t[0])
+        p, q, r, s = (int(x) for x in t[1:])
+        if p == q == r == s == 0:
+            ecore = v
+        elif r == 0 and s == 0:
+            if q:
+                h1[p - 1, q - 1] = h1[q - 1, p - 1] = v
+        else:
+            A, B = pair(p - 1, q - 1), pair(r - 1, s - 1)
+            if A < B:
+                A, B = B, A
+            eri8[A * (A + 1) // 2 + B] = v
+    return norb, nelec, h1, eri8, ecore, orbsym

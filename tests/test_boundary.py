@@ -105,3 +105,16 @@ int main() {
                     str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     assert out == ["0", "1"]
+
+
+def test_in_memory_create_rejects_bad_arguments():
+    """shci_create_from_integrals checks its arguments before any device call
+    (SHCI_BAD_INPUT)."""
+    import numpy as np
+    h1 = np.eye(4)
+    eri8 = np.zeros(55)  # npair = 10 -> 55 packed values
+    for norb_nelec in ((4, 0), (4, 9)):
+        with pytest.raises(shci.ShciError) as e:
+            shci.System.from_integrals(h1, eri8, norb_nelec[1])
+        assert e.value.status == 1, str(e.value)
+        assert "nelec" in str(e.value)

@@ -1,0 +1,81 @@
+"""Half storage (each connected pair once, in row max(I, J): the reference's
+lower-triangular rows without the diagonal, SHCI/SHCImakeHamiltonian.cpp:549-711),
+the layout for spaces whose two-triangle Hamiltonian does not fit.  Sigma
+applies every stored element to both rows as Hmult2 does
+(SHCI/Hmult.h:110-121), the transposed half by atomic adds.  Bar: the rows
+equal the two-triangle build's lower part bit for bit; sigma equals the
+oracle's to 1e-12 relative (its sums are in no fixed order); runs give the
+oracle's determinant sets and energies to 1e-10 Ha."""
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+from oracle_bind import OracleRun, OracleSystem
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _space(path, occ, n, W):
+    o = OracleSystem(path)
+    dets = helpers.occ_to_det(occ, W)[None, :]
+    eps = 1e-2
+    while len(dets) < n:
+        new = o.screen(dets, np.ones(len(dets)), eps)
+        if len(new):
+            dets = np.concatenate([dets, new])
+        eps *= 0.5
+    return dets[:n]
+
+
+@pytest.mark.parametrize("case", ["c0", "w2"])
+def test_half_storage_rows_and_sigma(case, tmp_path, monkeypatch):
+    from dice_amd import shci, synth
+    path = str(tmp_path / "FCIDUMP")
+    if case == "c0":
+        synth.write_molecular_fcidump(path, 0, 16, 8, 8, 64)
+        occ, W = synth.hf_occupation(8, 8), 1
+    else:
+        synth.write_molecular_fcidump(path, 5, 34, 6, 6, 12)
+        occ, W = synth.hf_occupation(6, 6), 2
+    dets = _space(path, occ, 3000, W)
+    full = shci.System.from_fcidump(path)
+    full.set_dets(dets)
+    rp, cols, vals = full.hamiltonian_rows()
+    monkeypatch.setenv("SHCI_STORAGE", "half")
+    half = shci.System.from_fcidump(path)
+    half.set_dets(dets)
+    hrp, hcols, hvals = half.hamiltonian_rows()
+    np.testing.assert_array_equal(rp, hrp)
+    np.testing.assert_array_equal(cols, hcols)
+    np.testing.assert_array_equal(vals, hvals)
+    o = OracleSystem(path)
+    o.csr_append(dets)
+    x = np.random.default_rng(3).normal(size=(len(dets), 3))
+    y = half.hmult(x)
+    for r in range(3):
+        yo = o.sigma(np.ascontiguousarray(x[:, r]))
+        assert np.max(np.abs(y[:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_half_storage_run_matches_oracle(prec, tmp_path, monkeypatch):
+    """A whole schedule (incremental builds append new rows in place) on a
+    12-orbital system: the oracle's determinant set after every iteration and
+    its energies (FP64); the FP32 form within 1e-4 Ha with the same space."""
+    from dice_amd import shci, synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 9, 12, 4, 4, 32)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(4, 4), [(0, 3e-3), (2, 1e-3)], 5,
+                      extra=["prefix %s" % tmp_path])
+    o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
+    monkeypatch.setenv("SHCI_STORAGE", "half")
+    r = shci.do_variational(inp, str(tmp_path / "gpu.log"), precision=prec)
+    assert [x["ndets"] for x in r.iterations()] == [x["ndets"] for x in o.iterations()]
+    np.testing.assert_array_equal(r.dets(), o.dets(1))
+    tol = 1e-10 if prec == 0 else 1e-4
+    np.testing.assert_allclose(r.energies(), o.energies(), rtol=0, atol=tol)

@@ -190,28 +190,56 @@ def test_incremental_hamiltonian(name, prec, nsrc, eps, systems):
 
 @pytest.mark.parametrize("name,prec", [("c0", 0), ("hub", 0), ("w2", 0), ("c0", 1)])
 def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
-    """Sigma through the alpha-major relabelled copy of H (the default, with
-    same-beta columns gathered from a beta-major copy of x) equals sigma over
-    H in determinant order: to 1e-12 relative with FP64 values; with FP32
-    values the relabelled path also rounds x to FP32 (1e-6).  The columns
-    kept all alpha-major give the same sums bit for bit, for one vector and
-    for the interleaved SpMM of two."""
+    """Every sigma layout gives sigma over H in determinant order ("external")
+    to 1e-12 relative with FP64 values (1e-6 with FP32 values, where the
+    relabelled paths also round x to FP32), for one vector and for the SpMM
+    of two: the row-ordered relabelled copy ("rows", the default, with
+    same-beta columns gathered from a beta-major copy of x), the same with
+    every column alpha-major ("alpha"), and the tiled copy ("tiled": blocks
+    of 256 relabelled rows with entries grouped by column tile).  Each
+    layout repeats its own result bit for bit."""
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
-    out = []
-    for layout in ("relabelled", "external", "alpha"):
-        if layout != "relabelled":
+    out = {}
+    for layout in ("tiled", "rows", "external", "alpha"):
+        monkeypatch.delenv("SHCI_SIGMA_LAYOUT", raising=False)
+        if layout != "rows":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
         g.set_precision(prec)
         g.set_dets(dets)
         g.make_hamiltonian()
-        out.append((g.hmult(x), g.hmult(x[:, 0].copy())))
+        out[layout] = (g.hmult(x), g.hmult(x[:, 0].copy()))
+        # deterministic: the same call again gives the same bits
+        np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
-    assert np.max(np.abs(out[0][0] - out[1][0])) <= tol * np.max(np.abs(out[1][0]))
-    np.testing.assert_array_equal(out[0][0], out[2][0])
-    np.testing.assert_array_equal(out[0][1], out[2][1])
+    ref = out["external"]
+    for layout in ("tiled", "rows", "alpha"):
+        for k in (0, 1):
+            assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
+
+
+@pytest.mark.parametrize("name", ["c0", "w2"])
+def test_tiled_sigma_bitwise_across_handles_and_growth(name, systems, monkeypatch):
+    """The tiled sigma's sums do not depend on scheduling: a handle whose
+    tiled copy was first built for a smaller space and then rebuilt for the
+    full one gives the bits of a fresh handle on the full space."""
+    monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
+    path = systems[name][0]
+    dets = _space(name, systems, 3000, 1)
+    x = np.random.default_rng(9).normal(size=len(dets))
+    g = _gpu(path)
+    g.set_dets(dets[:1000])
+    g.make_hamiltonian()
+    g.hmult(x[:1000])
+    g.set_dets(dets)
+    g.make_hamiltonian()
+    y1 = g.hmult(x)
+    f = _gpu(path)
+    f.set_dets(dets)
+    f.make_hamiltonian()
+    np.testing.assert_array_equal(y1, f.hmult(x))
 
 
 @pytest.mark.parametrize("name,nroots", [("c0", 1), ("hub", 2)])
@@ -512,7 +540,7 @@ def test_learned_form_gain_over_untrained_network(tmp_path, monkeypatch):
     with open(inp, "a") as f:
         f.write("prefix %s\n" % tmp_path)
     plain = shci.do_variational(inp, str(tmp_path / "p.log")).energies()[0]
-    r = shci.do_variational(inp, str(tmp_path / "l.log"), learned=True)
+    r = shci.do_variational(inp, str(tmp_path / "l.log"), learned="forced")
     recs, train = r.learned()
     ctl = shci.do_variational(inp, str(tmp_path / "c.log"), learned="control")
     crecs, _ = ctl.learned()
@@ -533,6 +561,30 @@ def test_learned_form_gain_over_untrained_network(tmp_path, monkeypatch):
     assert abs(ctl.energies()[0] - plain) < 1e-10
     # fewer determinants can only raise the variational energy
     assert r.energies()[0] >= plain - 1e-9
+
+
+def test_learned_form_gate_turns_itself_off(tmp_path):
+    """The gated learned form (the default) on a workload whose exact
+    screen costs less per source determinant than ranking its candidate
+    records would: after the second logging iteration it turns itself off
+    without training, leaves one fallback record, and gives the plain run's
+    energy and space exactly."""
+    from dice_amd import synth, shci
+    path = str(tmp_path / "FCIDUMP")
+    c, n, na, nb, nchol = synth.MOLECULAR["c1"]
+    synth.write_molecular_fcidump(path, c, n, na, nb, nchol)
+    inp = str(tmp_path / "input.dat")
+    synth.write_input(inp, path, synth.hf_occupation(na, nb), [(0, 1e-2)], 4)
+    with open(inp, "a") as f:
+        f.write("prefix %s\n" % tmp_path)
+    p = shci.do_variational(inp, str(tmp_path / "p.log"))
+    r = shci.do_variational(inp, str(tmp_path / "g.log"), learned=True)
+    recs, train = r.learned()
+    assert len(recs) == 1 and recs[0]["fallback"] and recs[0]["evaluated"] == 0 and recs[0]["candidates"] > 0
+    assert recs[0]["iteration"] == 1
+    assert train["seconds"] == 0.0
+    np.testing.assert_array_equal(r.dets(), p.dets())
+    assert r.energies()[0] == p.energies()[0]
 
 
 # ---- direct mode (no stored Hamiltonian) ----
