@@ -11,11 +11,14 @@ merge, Hamiltonian build, Davidson) and the final tight Davidson, with the
 integrals resident on the GPU.  `value` is the mean wall time of one
 variational iteration over the timed steps.
 
-The roofline object is the sigma = H x kernel (the Davidson SpMV over the
-alpha-major relabelled CSR, kSigmaPerm, with its x permutation kGatherPerm),
-timed with HIP events on the library's stream over the timed steps; achieved
-bandwidth uses algorithmic bytes per launch: stored values + column indices,
-row offsets, diagonal, x read once and y written once.  `traffic` is the HBM
+The roofline object is the sigma = H x kernel of the Davidson, timed with
+HIP events on the library's stream over the timed steps (with FP64 values
+kSigmaTiled over 256-row blocks of the alpha-major relabelled Hamiltonian
+whose entries are grouped by 1024-column tiles; with FP32 values kSigmaPerm
+over the row-ordered relabelled CSR; each with its x permutation,
+kGatherPerm); achieved bandwidth uses algorithmic bytes per launch: stored
+values + column indices, row offsets, diagonal, x read once and y written
+once.  `traffic` is the HBM
 bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
@@ -156,6 +159,114 @@ def batching_form(shci, args):
     res["workload"] = ("6x6 periodic Hubbard, t=1, U=4, 18a+18b, eps1 2e-4, 4 roots, 3 iterations "
                        "(the fourth would select 12.6M more determinants)")
     res["davidson_speedup"] = round(res["reference_davidson"]["davidson_s"] / res["block_davidson"]["davidson_s"], 3)
+    return res
+
+
+C3_EPS1 = 1e-2   # the 40-orbital system's eps1 (see c3_form)
+C3_NITER = 2
+C4_EPS1 = 2e-2   # the learned form on the 40-orbital system (see c4_form)
+C4_NITER = 3
+
+
+def c3_form(shci, args):
+    """The 40-orbital system (20 alpha + 20 beta electrons, 160 Cholesky
+    vectors, random stream 1003, two-word determinants) with two roots: one
+    variational step of C3_NITER iterations at eps1 C3_EPS1 from HF, in FP64
+    and in the FP32 form (the reference's single-vector Davidson).  The
+    configured schedule (down to 2e-5) selects far more than one GPU holds
+    on this synthetic construction; C3_EPS1 is the largest-space schedule
+    measured to fit and finish within the benchmark's time."""
+    from dice_amd import synth
+    d = os.path.join(tempfile.gettempdir(), "shci_bench_c3")
+    os.makedirs(d, exist_ok=True)
+    fcidump = os.path.join(d, "FCIDUMP")
+    c, n, na, nb, nchol = synth.MOLECULAR["c3"]
+    if not os.path.exists(fcidump):
+        synth.write_molecular_fcidump(fcidump, c, n, na, nb, nchol)
+    inp = os.path.join(d, "input.dat")
+    synth.write_input(inp, fcidump, synth.hf_occupation(na, nb), [(0, C3_EPS1)], C3_NITER, nroots=2,
+                      extra=["prefix " + d])
+    log = os.path.join(d, "gpu.log")
+    res = {"workload": "40-orbital synthetic FCIDUMP (20a+20b, 160 Cholesky vectors, stream 1003), 2 roots, "
+                       "heat-bath eps1 %.0e x %d iterations from HF" % (C3_EPS1, C3_NITER)}
+    for tag, prec in (("fp64", shci.PREC_FP64), ("fp32", shci.PREC_FP32)):
+        print("# C3 %s" % tag, file=sys.stderr, flush=True)
+        t0 = time.time()
+        r = shci.do_variational(inp, log, precision=prec, profile_sigma=True)
+        wall = time.time() - t0
+        ph, its = r.phases(), r.iterations()
+        calls, ms, nbytes = r.sigma_profile()
+        res[tag] = {
+            "wall_s": round(wall, 3),
+            "step_s": round(sum(a + b + c for a, b, c, _ in ph) + r.final_davidson_seconds(), 3),
+            "energies": [float(e) for e in r.energies()],
+            "final_ndets": its[-1]["ndets"], "nnz_lower": ph[-1][3],
+            "sigma_launches": calls, "sigma_avg_ms": round(ms / calls, 4) if calls else None,
+            "sigma_GBps": round(nbytes / (ms * 1e-3) / 1e9, 1) if ms else None,
+            "sigma_frac_of_hbm_peak": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if ms else None,
+            "iterations": [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"],
+                            "screen_s": round(p[0], 4), "hamiltonian_s": round(p[1], 4),
+                            "davidson_s": round(p[2], 4)} for it, p in zip(its, ph)],
+        }
+        del r
+        print("# C3 %s: %.1f s, %d dets, E %s" % (tag, wall, res[tag]["final_ndets"], res[tag]["energies"]),
+              file=sys.stderr, flush=True)
+    res["fp32"]["energy_error_vs_fp64_Ha"] = [a - b for a, b in zip(res["fp32"]["energies"], res["fp64"]["energies"])]
+    res["fp32"]["speedup_vs_fp64"] = round(res["fp64"]["step_s"] / res["fp32"]["step_s"], 3)
+    return res
+
+
+def c4_form(shci, args):
+    """The learned form on the 40-orbital system (two roots, FP32 values):
+    C4_NITER iterations at eps1 C4_EPS1 from HF (the learned form logs two
+    iterations and ranks from the third on), gated (the default) and forced
+    on, beside the plain FP32 run of the same schedule.  Reported: the
+    energy gap of each to the plain run, each ranked iteration's record,
+    and the screen's share of the plain step, which caps what any faster
+    screen can save."""
+    from dice_amd import synth
+    d = os.path.join(tempfile.gettempdir(), "shci_bench_c3")
+    os.makedirs(d, exist_ok=True)
+    fcidump = os.path.join(d, "FCIDUMP")
+    c, n, na, nb, nchol = synth.MOLECULAR["c3"]
+    if not os.path.exists(fcidump):
+        synth.write_molecular_fcidump(fcidump, c, n, na, nb, nchol)
+    inp = os.path.join(d, "input_c4.dat")
+    synth.write_input(inp, fcidump, synth.hf_occupation(na, nb), [(0, C4_EPS1)], C4_NITER, nroots=2,
+                      extra=["prefix " + d])
+    log = os.path.join(d, "gpu_c4.log")
+    res = {"workload": "40-orbital synthetic FCIDUMP (as c3_large), 2 roots, FP32 values, heat-bath eps1 %.0e x %d "
+                       "iterations from HF; learned form gated and forced" % (C4_EPS1, C4_NITER)}
+    runs = {}
+    for tag, learned in (("plain", False), ("gated", True), ("forced", "forced")):
+        print("# C4 %s" % tag, file=sys.stderr, flush=True)
+        t0 = time.time()
+        r = shci.do_variational(inp, log, precision=shci.PREC_FP32, learned=learned)
+        wall = time.time() - t0
+        ph, its = r.phases(), r.iterations()
+        runs[tag] = {
+            "wall_s": round(wall, 3),
+            "step_s": round(sum(a + b + c for a, b, c, _ in ph) + r.final_davidson_seconds(), 3),
+            "screen_s": round(sum(p[0] for p in ph), 4),
+            "energies": [float(e) for e in r.energies()],
+            "final_ndets": its[-1]["ndets"],
+            "iterations": [{"ndets": it["ndets"], "davidson_iters": it["davidson"], "screen_s": round(p[0], 4),
+                            "hamiltonian_s": round(p[1], 4), "davidson_s": round(p[2], 4)}
+                           for it, p in zip(its, ph)],
+        }
+        if learned:
+            recs, train = r.learned()
+            runs[tag]["ranked_iterations"] = recs
+            runs[tag]["training"] = train
+        del r
+        print("# C4 %s: %.1f s, %d dets, E %s" % (tag, wall, runs[tag]["final_ndets"], runs[tag]["energies"]),
+              file=sys.stderr, flush=True)
+    plain = runs["plain"]
+    for tag in ("gated", "forced"):
+        runs[tag]["energy_gap_vs_plain_Ha"] = [a - b for a, b in zip(runs[tag]["energies"], plain["energies"])]
+        runs[tag]["speedup_vs_plain"] = round(plain["step_s"] / runs[tag]["step_s"], 3)
+    res.update(runs)
+    res["screen_share_of_plain_step"] = round(plain["screen_s"] / plain["step_s"], 4)
     return res
 
 
@@ -352,8 +463,12 @@ def main():
     results = {}
     forms = [primary] if args.no_forms else [primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]
     block = dict(davidson_block=True, hip_graphs=True) if args.davidson_block else {}
+    # the headline form over the requested steps, the other forms over at
+    # most 5 steps after one warm-up (they are reported beside it)
+    fsteps, fwarm = min(args.steps, 5), min(args.warmup, 1)
     for prec in forms:
-        results[prec] = run_form(shci, full, log, prec, args.steps, args.warmup, profile=not block,
+        results[prec] = run_form(shci, full, log, prec, args.steps if prec == primary else fsteps,
+                                 args.warmup if prec == primary else fwarm, profile=not block,
                                  **(block if prec == primary else {}))
     prec = primary
     R = results[prec]
@@ -383,7 +498,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "kSigmaPerm (sigma = H x over the alpha-major relabelled CSR)",
+            "kernel": ("kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
+                       "entries grouped by 1024-column tiles)" if prec == shci.PREC_FP64 else
+                       "kSigmaPerm (sigma = H x over the alpha-major relabelled CSR, FP32 values)"),
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
@@ -417,13 +534,14 @@ def main():
             }
         }
         # learned screening + mixed precision (the combined fast form)
-        L = run_form(shci, full, log, shci.PREC_FP32, args.steps, args.warmup, tag="learned+fp32", learned=True)
+        L = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="learned+fp32", learned=True)
         recs, train = L["learned"]
         out["forms"]["learned_mixed_fp32"] = {
-            "description": "learned candidate ranking (MLP trained on the first two iterations' screening values "
-                           "predicts log10(|c||v|); candidates predicted above log10(eps1) - 0.8 are tested "
-                           "exactly, full screening when a sample shows more than half would be) with the FP32 "
-                           "form",
+            "description": "learned candidate ranking (gated: after the second logging iteration the form turns "
+                           "itself off unless ranking the candidate records at 2 ns each would cost less than the "
+                           "exact screen; when on, an MLP trained on the first two iterations' screening values "
+                           "predicts log10(|c||v|) and candidates predicted above log10(eps1) - 0.8 are tested "
+                           "exactly) with the FP32 form",
             "value": round(statistics.mean(L["iter_times"]), 6),
             "ms_per_step": round(1e3 * statistics.mean(L["step_times"]), 3),
             "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(L["step_times"]), 3),
@@ -437,7 +555,7 @@ def main():
         # batching + mixed precision on the same workload: block Davidson
         # (one root: 17-vector subspace, thick restart), hipGraph-replayed
         # iterations, FP32 values
-        B = run_form(shci, full, log, shci.PREC_FP32, args.steps, args.warmup, tag="block+fp32", profile=False,
+        B = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="block+fp32", profile=False,
                      davidson_block=True, hip_graphs=True)
         out["forms"]["batched_mixed_fp32"] = {
             "description": "block Davidson (thick restart, hipGraph-replayed iterations) with the FP32 form; its "
@@ -451,10 +569,27 @@ def main():
             "final_ndets": B["per_iter"][-1]["ndets"],
             "iterations": B["per_iter"],
         }
+        # the three forms together: gated learned screening, block Davidson
+        # with hipGraph replay, FP32 values
+        LB = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="learned+block+fp32", profile=False,
+                      davidson_block=True, hip_graphs=True, learned=True)
+        out["forms"]["learned_batched_mixed_fp32"] = {
+            "description": "gated learned screening + block Davidson (hipGraph-replayed) + FP32 values",
+            "value": round(statistics.mean(LB["iter_times"]), 6),
+            "ms_per_step": round(1e3 * statistics.mean(LB["step_times"]), 3),
+            "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(LB["step_times"]), 3),
+            "final_energy": LB["energies"][-1],
+            "energy_error_vs_fp64_Ha": LB["energies"][-1] - fp64["energies"][-1],
+            "final_ndets": LB["per_iter"][-1]["ndets"],
+            "learned": LB["learned"][0],
+            "iterations": LB["per_iter"],
+        }
         out["forms"]["batched_block"] = batching_form(shci, args)
 
     if not args.no_forms:
         out["pt_deterministic"] = pt_measure(shci, full, sample, log, args)
+        out["c3_large"] = c3_form(shci, args)
+        out["c4_learned_large"] = c4_form(shci, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -190,9 +190,10 @@ void Engine::loadSystem(HostSystem&& sys) {
   // SHCI_SIGMA_LAYOUT=alpha keeps every relabelled column alpha-major (no
   // beta-major copy of x)
   split = !(layout && std::strcmp(layout, "alpha") == 0);
-  // SHCI_SIGMA_LAYOUT=tiled: the one-vector sigma on the tiled copy of the
-  // relabelled Hamiltonian instead of the row-ordered one
-  tiledOn = layout && std::strcmp(layout, "tiled") == 0;
+  // the one-vector sigma on the tiled copy of the relabelled Hamiltonian:
+  // with FP64 values by default, SHCI_SIGMA_LAYOUT=tiled also with FP32
+  // values, SHCI_SIGMA_LAYOUT=rows never (the row-ordered copy instead)
+  tiledOn = layout && std::strcmp(layout, "tiled") == 0 ? 1 : (layout && std::strcmp(layout, "rows") == 0 ? 0 : 2);
   // SHCI_SIGMA_PERSISTENT=0/1: static or persistent workgroups for the
   // one-vector sigma (the autotuner, when on, decides per Hamiltonian)
   if (const char* p = std::getenv("SHCI_SIGMA_PERSISTENT")) sigmaCfg[0].persistent = std::atoi(p) != 0;
@@ -804,15 +805,20 @@ void Engine::buildLower(int64_t nOld) {
 }
 
 // Alpha-major relabelled copies of H (FP64 values, or the FP32 hi part) for
-// the sigma kernels: the tiled copy (one vector) and the row-ordered copy
-// (SpMM of the block Davidson, or one vector when the tiled copy is off);
-// each is skipped when it does not fit beside H.
+// the sigma kernels: the row-ordered copy (SpMM of the block Davidson, or
+// one vector without the tiled copy), and the tiled copy (one vector; built
+// from the row-ordered copy, which is then released unless the SpMM needs
+// it).  A copy that does not fit beside H is skipped: with no tiled copy the
+// one-vector sigma runs on the row-ordered one, with neither on H itself.
 void Engine::buildPermuted() {
   permRows = -1;
   splitRows = -1;
   tiledRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
+  // the tiled copy: FP64 by default (SHCI_SIGMA_LAYOUT=tiled: also FP32,
+  // =rows: never)
+  const bool wantTiled = tiledOn == 1 || (tiledOn == 2 && !f32);
   // a copy (columns, values, row offsets, inverse permutation) with 64
   // bytes per determinant of slack, and the permuted x of the sigma calls:
   // up to 8 interleaved vectors in the block Davidson, in alpha-major order
@@ -824,25 +830,20 @@ void Engine::buildPermuted() {
   const size_t xPlain = (size_t)n * kmax * xt, xSplit = 2 * (size_t)n * kmax * xt + (size_t)n * 4;
   size_t freeB = 0, totalB = 0;
   freeB = deviceFreeBytes(&totalB);
-  size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes + tiled.idx.bytes +
-                tiled.vals.bytes + tiled.inst.bytes;
-  const bool wantTiled = tiledOn, wantRows = !tiledOn || davidsonMode == 1;
-  const size_t copies = (wantTiled ? tiledCopy : 0) + (wantRows ? rowsCopy : 0);
-  if (base + copies + xPlain > have) {
-    // not both: the tiled copy alone, else none
+  const size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes + tiled.idx.bytes +
+                      tiled.vals.bytes + tiled.inst.bytes;
+  if (base + rowsCopy + xPlain > have) {
     permCols.release();
     permVals.release();
-    if (!wantTiled || base + tiledCopy + xPlain > have) {
-      tiled.release();
-      return;
-    }
+    tiled.release();
+    return;
   }
-  const bool buildRows = wantRows && base + copies + xPlain <= have;
+  const bool buildTiled = wantTiled && base + rowsCopy + tiledCopy + xPlain <= have;
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
   permPos.reserve((size_t)n * 4);
   launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
   // split columns only when their x copy fits too
-  const size_t used = base + (wantTiled ? tiledCopy : 0) + (buildRows ? rowsCopy : 0);
+  const size_t used = base + rowsCopy + (buildTiled ? tiledCopy : 0);
   const bool doSplit = split && 2 * n < 0x7fffffffLL && used + xSplit <= have;
   if (doSplit) {
     bPos.reserve((size_t)n * 4);
@@ -850,36 +851,40 @@ void Engine::buildPermuted() {
   }
   const uint32_t* bIdx = doSplit ? T.bIdx.as<uint32_t>() : nullptr;
   const uint32_t* bp = doSplit ? bPos.as<uint32_t>() : nullptr;
-  if (wantTiled) {
-    if (buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
-                      f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(), bIdx, bp, n, nnzOff, ws,
-                      stream))
-      tiledRows = n;
-    else
-      tiled.release();
-  }
-  if (buildRows || (wantTiled && tiledRows != n)) {
-    permRp.reserve((size_t)(n + 1) * 8);
-    int64_t* rp = permRp.as<int64_t>();
-    HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
-    launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
-    exclusiveScanI64(rp, n + 1, ws, stream);
-    permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-    if (f32) {
-      permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
-      launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(),
-                        bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
-    } else {
-      permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
-      launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(),
-                        bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
-    }
-    permRows = n;
+  permRp.reserve((size_t)(n + 1) * 8);
+  int64_t* rp = permRp.as<int64_t>();
+  HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
+  launchPermRowLengths(rowPtr.as<int64_t>(), perm, n, rp, stream);
+  exclusiveScanI64(rp, n + 1, ws, stream);
+  permCols.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+  if (f32) {
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 4);
+    launchPermRowsF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), perm, permPos.as<uint32_t>(),
+                      bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<float>(), stream);
   } else {
-    permCols.release();
-    permVals.release();
+    permVals.reserve((size_t)std::max<int64_t>(nnzOff, 1) * 8);
+    launchPermRowsF64(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals.as<double>(), perm, permPos.as<uint32_t>(),
+                      bIdx, bp, n, rp, permCols.as<int32_t>(), permVals.as<double>(), stream);
   }
+  permRows = n;
   if (doSplit) splitRows = n;
+  if (buildTiled) {
+    if (buildTiledHam(tiled, rp, permCols.as<int32_t>(), f32 ? nullptr : permVals.as<double>(),
+                      f32 ? permVals.as<float>() : nullptr, nullptr, nullptr, nullptr, nullptr, n, nnzOff, ws,
+                      stream, doSplit)) {
+      tiledRows = n;
+      if (davidsonMode != 1) {  // the row-ordered copy served only the build
+        HIP_CHECK(hipStreamSynchronize(stream));
+        permCols.release();
+        permVals.release();
+        permRows = -1;
+      }
+    } else {
+      tiled.release();
+    }
+  } else {
+    tiled.release();
+  }
   // re-tune when the space has doubled since the last tuning (small spaces
   // keep the default shape: their sigma is too short to time reliably)
   if (autotune && n >= 20000 && n >= 2 * tunedRows) tuneSigma();

@@ -74,7 +74,7 @@ struct Engine {
   // tiled copy for the one-vector sigma (blocks of 256 relabelled rows,
   // entries grouped by column tile); valid when tiledRows == n.  With it the
   // row-ordered relabelled copy is kept only for the block Davidson's SpMM.
-  bool tiledOn = false;
+  int tiledOn = 2;  // 0: never, 1: always, 2: with FP64 values
   int tiledStep = 16;           // instructions per wave per step (8 or 16; 5.5 against 6.0 ms)
   bool tiledPersistent = false;  // one workgroup per block (6.4 ms) beat persistent ones (7.5 ms)
   TiledHam tiled;

@@ -268,12 +268,15 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32);
 // Builds T from H in determinant order (rowPtr, cols, values: vals64 or, for
 // the FP32 form, vals32) through the alpha-major relabelling (perm: position
 // -> determinant, pos: determinant -> position) and, with bIdx / bPos given,
-// the split columns (the beta-major copy of x after the alpha-major one).
-// Returns false, with nothing built, when the permuted x has more tiles than
-// the build's bitmap holds.
+// the split columns (the beta-major copy of x after the alpha-major one);
+// or, with perm, pos, bIdx and bPos null, from the relabelled row copy
+// itself (rows and columns already permuted; splitCols says whether its
+// columns reach into the beta-major copy).  Returns false, with nothing
+// built, when the permuted x has more tiles than the build's bitmap holds.
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
-                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s);
+                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s,
+                   bool splitCols = false);
 // y = H x through the tiled copy (one vector, x and y in determinant order;
 // bPerm = the beta-major determinant order when the copy has split columns)
 // (step: instructions per wave per step, 8 or 16)

@@ -1762,10 +1762,17 @@ constexpr int kTileCols = 1 << kTileLog;
 constexpr int kTileMaxWords = 4096;  // bitmap of up to 131,072 tiles: x of up to 134M values
 constexpr int kTileInstPad = 16;  // zero descriptors after the last one (the largest step of kSigmaTiled)
 constexpr int kTileLdsSegs = 1024;  // blocks with at most this many segments count and fill in LDS
+// an entry's 32-bit word: column in tile (bits 0-9), distance to the first
+// entry of its row run within the instruction (bits 10-15), row in block
+// (bits 16-23), last entry of its run within the instruction (bit 31)
+constexpr int kTileDistShift = 10;
 
+// the permuted-x column of H's column J in a row with beta string index bi
+// (pos null: the input is the relabelled copy, whose columns are mapped)
 __device__ __forceinline__ uint32_t tileMapCol(int32_t J, uint32_t bi, const uint32_t* __restrict__ pos,
                                                const uint32_t* __restrict__ bIdx,
                                                const uint32_t* __restrict__ bPos, int64_t n) {
+  if (!pos) return (uint32_t)J;
   return (bIdx && bIdx[J] == bi) ? (uint32_t)(n + bPos[J]) : pos[J];
 }
 
@@ -1806,7 +1813,7 @@ __device__ int tileBitmap(const int64_t* __restrict__ rowPtr, const int32_t* __r
   const int64_t m0 = b * kTileRows;
   const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
   for (int r = wave; r < rows; r += kWaves) {
-    const int64_t I = perm[m0 + r];
+    const int64_t I = perm ? (int64_t)perm[m0 + r] : m0 + r;
     const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
     const uint32_t bi = bIdx ? bIdx[I] : 0u;
     for (int64_t p = e0 + lane; p < e1; p += 64) {
@@ -1865,7 +1872,7 @@ __global__ void __launch_bounds__(kBlock) kTileCount(const int64_t* rowPtr, cons
   const int64_t m0 = b * kTileRows;
   const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
   for (int r = wave; r < rows; r += kWaves) {
-    const int64_t I = perm[m0 + r];
+    const int64_t I = perm ? (int64_t)perm[m0 + r] : m0 + r;
     const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
     const uint32_t bi = bIdx ? bIdx[I] : 0u;
     for (int64_t p = e0 + lane; p < e1; p += 64) {
@@ -1950,7 +1957,7 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
   const int64_t m0 = b * kTileRows;
   const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
   for (int r = wave; r < rows; r += kWaves) {
-    const int64_t I = perm[m0 + r];
+    const int64_t I = perm ? (int64_t)perm[m0 + r] : m0 + r;
     const int64_t e0 = rowPtr[I], e1 = rowPtr[I + 1];
     const uint32_t bi = bIdx ? bIdx[I] : 0u;
     for (int64_t p0 = e0; p0 < e1; p0 += 64) {
@@ -1992,31 +1999,73 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
   }
 }
 
-// One step of a keyed inclusive scan across the wave by DPP: lane i adds
-// the value of the lane the DPP control CTRL names (within the rows of 16
-// lanes ROWMASK enables) when that lane has the same key.  Keys form
-// contiguous runs, so the steps row_shr 1, 2, 4, 8, then row_bcast 15 and
-// row_bcast 31 give every lane the sum of its run up to itself.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void keyedScanStep(double& s, int k) {
-  const int ok = __builtin_amdgcn_update_dpp(-1, k, CTRL, ROWMASK, 0xf, false);
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(s), CTRL, ROWMASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(s), CTRL, ROWMASK, 0xf, false);
-  if (ok == k) s += __hiloint2double(hi, lo);
+// pass 4: per entry, its distance to the first entry of its row run within
+// its instruction and whether it is the run's last entry there (one wave
+// per instruction)
+__global__ void __launch_bounds__(kBlock) kTileRunBits(const uint2* __restrict__ inst,
+                                                       const int64_t* __restrict__ blockI,
+                                                       const int64_t* __restrict__ blockE, uint32_t* tIdx) {
+  const int64_t b = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const int64_t i1 = blockI[b + 1], e0 = blockE[b];
+  for (int64_t i = blockI[b] + wave; i < i1; i += kWaves) {
+    const uint2 d = inst[i];
+    const int cnt = (int)(d.y & 127u);
+    const bool valid = lane < cnt;
+    const int64_t p = e0 + d.x + lane;
+    const uint32_t ix = valid ? tIdx[p] : 0u;
+    const int key = valid ? (int)((ix >> 16) & (kTileRows - 1)) : -1 - lane;
+    const int prev = __shfl_up(key, 1, 64), next = __shfl_down(key, 1, 64);
+    const bool start = lane == 0 || prev != key;
+    const bool last = lane == 63 || next != key;
+    const unsigned long long starts = __ballot(start);
+    // the highest run start at or below this lane
+    const unsigned long long upTo = starts & (~0ull >> (63 - lane));
+    const int first = 63 - __clzll(upTo);
+    const uint32_t dist = (uint32_t)(lane - first);
+    if (valid)
+      tIdx[p] = (ix & ~((63u << kTileDistShift) | 0x80000000u)) | (dist << kTileDistShift) | (last ? 0x80000000u : 0u);
+  }
 }
 
-__device__ __forceinline__ double keyedInclusiveScan(double s, int k) {
-  keyedScanStep<0x111, 0xf>(s, k);  // row_shr:1
-  keyedScanStep<0x112, 0xf>(s, k);  // row_shr:2
-  keyedScanStep<0x114, 0xf>(s, k);  // row_shr:4
-  keyedScanStep<0x118, 0xf>(s, k);  // row_shr:8
-  keyedScanStep<0x142, 0xa>(s, k);  // row_bcast:15 into rows 1 and 3
-  keyedScanStep<0x143, 0xc>(s, k);  // row_bcast:31 into rows 2 and 3
+// One step of a segmented inclusive scan across the wave by DPP: lane i
+// adds the value of the lane the DPP control CTRL names (within the rows of
+// 16 lanes ROWMASK enables) when that lane lies in its run, i.e. when the
+// lane's distance to the first lane of its run is at least `need` (need is
+// larger than any distance where CTRL brings no value into the lane, so the
+// value a disabled or out-of-row lane receives is never used).  Runs are
+// contiguous, so the steps row_shr 1, 2, 4, 8 (distance counted within the
+// row of 16), then row_bcast 15 and row_bcast 31 give every lane the sum of
+// its run up to itself.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void segScanStep(double& s, int dist, int need) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(s), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(s), CTRL, ROWMASK, 0xf, false);
+  if (dist >= need) s += __hiloint2double(hi, lo);
+}
+
+// dist: the lane's distance to the first lane of its run (runs contiguous);
+// c15, c31: per-lane thresholds of the two broadcast steps (tileScanConsts)
+__device__ __forceinline__ double segInclusiveScan(double s, int dist, int c15, int c31) {
+  const int inRow = dist < (int)(__lane_id() & 15u) ? dist : (int)(__lane_id() & 15u);
+  segScanStep<0x111, 0xf>(s, inRow, 1);  // row_shr:1
+  segScanStep<0x112, 0xf>(s, inRow, 2);  // row_shr:2
+  segScanStep<0x114, 0xf>(s, inRow, 4);  // row_shr:4
+  segScanStep<0x118, 0xf>(s, inRow, 8);  // row_shr:8
+  segScanStep<0x142, 0xa>(s, dist, c15);  // row_bcast:15 (lane 15 into row 1, lane 47 into row 3)
+  segScanStep<0x143, 0xc>(s, dist, c31);  // row_bcast:31 (lane 31 into rows 2 and 3)
   return s;
 }
 
-// key of the next lane (-1 past lane 63), by DPP wave_shl:1
-__device__ __forceinline__ int nextLaneKey(int k) { return __builtin_amdgcn_update_dpp(-1, k, 0x130, 0xf, 0xf, false); }
+// the broadcast steps' thresholds: a lane of row 1 or 3 takes the previous
+// row's last lane when that lane is in its run (distance > its index in the
+// row); a lane of row 2 or 3 takes lane 31 when the distance reaches back to
+// it; other lanes never take the broadcast value
+__device__ __forceinline__ void tileScanConsts(int& c15, int& c31) {
+  const int lane = (int)__lane_id();
+  c15 = ((lane >> 4) & 1) ? (lane & 15) + 1 : 64;
+  c31 = lane >= 32 ? lane - 31 : 64;
+}
 
 // y[perm[m]] = diag .* x + (H x) over the tiled copy, one vector; xP holds x
 // in the permuted order of the copy's columns
@@ -2031,6 +2080,8 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
   __shared__ double yw[kWaves][kTileRows];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
   static_assert(U <= kTileInstPad, "step larger than the descriptor padding");
+  int c15, c31;
+  tileScanConsts(c15, c31);
   ChunkSched<1, PERSIST> sched{nblocks, counter, -1};
   while (sched.next()) {
     const int64_t b = sched.cur;
@@ -2068,15 +2119,15 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
         const size_t tb = (size_t)(dy[u] >> 7) << kTileLog;
         xv[u] = (double)xP[tb + (valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u)];
       }
-      // products summed per row run by the keyed scan; the run's last lane
-      // adds the sum into the wave's partial row (lanes past the count carry
-      // keys of their own and are dropped)
+      // products summed per row run by the segmented scan; the run's last
+      // lane adds the sum into the wave's partial row (lanes past the count
+      // sit above every valid lane, so no valid lane takes their values, and
+      // are dropped)
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const bool valid = (uint32_t)lane < (dy[u] & 127u);
-        const int k = valid ? (int)(ix[u] >> 16) : kTileRows + lane;
-        const double sum = keyedInclusiveScan(valid ? v[u] * xv[u] : 0.0, k);
-        if (k < kTileRows && nextLaneKey(k) != k) yw[wave][k] += sum;
+        const double sum = segInclusiveScan(v[u] * xv[u], (int)((ix[u] >> kTileDistShift) & 63u), c15, c31);
+        if (valid && (int)ix[u] < 0) yw[wave][(ix[u] >> 16) & (kTileRows - 1)] += sum;
       }
     }
     __syncthreads();
@@ -3492,10 +3543,11 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32) {
 
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
-                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s) {
+                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s, bool splitCols) {
   T.n = -1;
   if (n <= 0) return false;
-  const int64_t nx = bIdx ? 2 * n : n;
+  const bool split = pos ? bIdx != nullptr : splitCols;
+  const int64_t nx = split ? 2 * n : n;
   const int64_t tiles = (nx + kTileCols - 1) / kTileCols;
   const int nwords = (int)((tiles + 31) / 32);
   if (nwords > kTileMaxWords) return false;
@@ -3557,13 +3609,15 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
                                               T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
                                               T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>());
   HIP_CHECK(hipGetLastError());
+  kTileRunBits<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), blockI, blockE, T.idx.as<uint32_t>());
+  HIP_CHECK(hipGetLastError());
   T.n = n;
   T.blocks = blocks;
   T.segments = S;
   T.nnz = E;
   T.instructions = I;
   T.f32 = f32;
-  T.split = bIdx != nullptr;
+  T.split = split;
   return true;
 }
 

@@ -192,19 +192,20 @@ def test_incremental_hamiltonian(name, prec, nsrc, eps, systems):
 def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     """Every sigma layout gives sigma over H in determinant order ("external")
     to 1e-12 relative with FP64 values (1e-6 with FP32 values, where the
-    relabelled paths also round x to FP32), for one vector and for the SpMM
-    of two: the row-ordered relabelled copy ("rows", the default, with
-    same-beta columns gathered from a beta-major copy of x), the same with
-    every column alpha-major ("alpha"), and the tiled copy ("tiled": blocks
-    of 256 relabelled rows with entries grouped by column tile).  Each
-    layout repeats its own result bit for bit."""
+    relabelled paths also round x to FP32), for one vector and for two: the
+    tiled copy ("tiled": blocks of 256 relabelled rows with entries grouped
+    by column tile, one launch per vector), the row-ordered relabelled copy
+    ("rows": one vector, or the SpMM of two), both with same-beta columns
+    gathered from a beta-major copy of x, every column alpha-major
+    ("alpha"), and the default (tiled with FP64 values, rows with FP32).
+    Each layout repeats its own result bit for bit."""
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = {}
-    for layout in ("tiled", "rows", "external", "alpha"):
+    for layout in ("default", "tiled", "rows", "external", "alpha"):
         monkeypatch.delenv("SHCI_SIGMA_LAYOUT", raising=False)
-        if layout != "rows":
+        if layout != "default":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
         g.set_precision(prec)
@@ -215,9 +216,11 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
     ref = out["external"]
-    for layout in ("tiled", "rows", "alpha"):
+    for layout in ("default", "tiled", "rows", "alpha"):
         for k in (0, 1):
             assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
+    # the default is the tiled copy with FP64 values and the row copy with FP32
+    np.testing.assert_array_equal(out["default"][1], out["tiled" if prec == 0 else "rows"][1])
 
 
 @pytest.mark.parametrize("name", ["c0", "w2"])
