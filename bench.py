@@ -12,13 +12,11 @@ integrals resident on the GPU.  `value` is the mean wall time of one
 variational iteration over the timed steps.
 
 The roofline object is the sigma = H x kernel of the Davidson, timed with
-HIP events on the library's stream over the timed steps (with FP64 values
-kSigmaTiled over 256-row blocks of the alpha-major relabelled Hamiltonian
-whose entries are grouped by 1024-column tiles; with FP32 values kSigmaPerm
-over the row-ordered relabelled CSR; each with its x permutation,
-kGatherPerm); achieved bandwidth uses algorithmic bytes per launch: stored
-values + column indices, row offsets, diagonal, x read once and y written
-once.  `traffic` is the HBM
+HIP events on the library's stream over the timed steps (kSigmaTiled over
+256-row blocks of the alpha-major relabelled Hamiltonian whose entries are
+grouped by 1024-column tiles, with its x permutation, kGatherPerm);
+achieved bandwidth uses algorithmic bytes per launch: stored values +
+column indices, row offsets, diagonal, x read once and y written once.  `traffic` is the HBM
 bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
@@ -131,8 +129,9 @@ def batching_form(shci, args):
     synth.write_input(inp, p, synth.neel_occupation(6, 6), [(0, 2e-4)], 3, nroots=4, extra=["prefix " + d])
     log = os.path.join(d, "gpu.log")
     res = {}
+    graphs = not args.no_graphs
     for name, method in (("reference_davidson", {}),
-                         ("block_davidson", dict(davidson_block=True, autotune=True, hip_graphs=True))):
+                         ("block_davidson", dict(davidson_block=True, autotune=True, hip_graphs=graphs))):
         # (per-launch sigma events would keep the block Davidson from replaying graphs)
         R = run_form(shci, inp, log, shci.PREC_FP64, 1, 1, tag="hubbard " + name, profile=not method, **method)
         res[name] = {
@@ -147,7 +146,7 @@ def batching_form(shci, args):
     # over the relabelled copy of the final Hamiltonian (wall time of
     # shci_sigma_device, x permutation included); algorithmic bytes as for
     # the one-vector sigma with 2 k N vector values
-    r = shci.do_variational(inp, log, davidson_block=True, autotune=True, hip_graphs=True)
+    r = shci.do_variational(inp, log, davidson_block=True, autotune=True, hip_graphs=graphs)
     n, nnz_lower = r.iterations()[-1]["ndets"], r.phases()[-1][3]
     t = time_sigma_device(shci, r._h, n, 4, 20)
     del r
@@ -213,6 +212,57 @@ def c3_form(shci, args):
               file=sys.stderr, flush=True)
     res["fp32"]["energy_error_vs_fp64_Ha"] = [a - b for a, b in zip(res["fp32"]["energies"], res["fp64"]["energies"])]
     res["fp32"]["speedup_vs_fp64"] = round(res["fp64"]["step_s"] / res["fp32"]["step_s"], 3)
+    return res
+
+
+C0_EPS1 = 1e-3
+C0_NITER = 4
+
+
+def c0_form(shci, args):
+    """The 16-orbital system (8 alpha + 8 beta electrons, 64 Cholesky
+    vectors, random stream 1000) at eps1 C0_EPS1 for C0_NITER iterations
+    from HF in FP64 (its configured second eps1, 1e-4, selects more than
+    one GPU holds on this synthetic construction), and its first iteration
+    (5,695 determinants, plus the final tight Davidson) on the GPU and on
+    the CPU oracle with the host threads of the cpu_baseline leg."""
+    from dice_amd import synth
+    d = os.path.join(tempfile.gettempdir(), "shci_bench_c0")
+    os.makedirs(d, exist_ok=True)
+    fcidump = os.path.join(d, "FCIDUMP")
+    c, n, na, nb, nchol = synth.MOLECULAR["c0"]
+    if not os.path.exists(fcidump):
+        synth.write_molecular_fcidump(fcidump, c, n, na, nb, nchol)
+    inp = os.path.join(d, "input.dat")
+    synth.write_input(inp, fcidump, synth.hf_occupation(na, nb), [(0, C0_EPS1)], C0_NITER, extra=["prefix " + d])
+    first = os.path.join(d, "first.dat")
+    synth.write_input(first, fcidump, synth.hf_occupation(na, nb), [(0, C0_EPS1)], 1, extra=["prefix " + d])
+    log = os.path.join(d, "gpu.log")
+    shci.do_variational(inp, log)  # warm-up
+    t0 = time.time()
+    r = shci.do_variational(inp, log)
+    wall = time.time() - t0
+    ph, its = r.phases(), r.iterations()
+    res = {"workload": "16-orbital synthetic FCIDUMP (8a+8b, 64 Cholesky vectors, stream 1000), 1 root, heat-bath "
+                       "eps1 %.0e x %d iterations from HF, FP64" % (C0_EPS1, C0_NITER),
+           "wall_s": round(wall, 3),
+           "iter_s": [round(a + b + c, 4) for a, b, c, _ in ph],
+           "final_ndets": its[-1]["ndets"], "energy": float(r.energies()[0]),
+           "iterations": [{"ndets": it["ndets"], "nnz_lower": p[3], "davidson_iters": it["davidson"]}
+                          for it, p in zip(its, ph)]}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import OracleRun
+        threads = host_threads()
+        os.environ["ORACLE_THREADS"] = str(threads)
+        t0 = time.time()
+        OracleRun(first, os.path.join(d, "oracle.log"))
+        cpu = time.time() - t0
+        t0 = time.time()
+        r1 = shci.do_variational(first, log)
+        gpu = time.time() - t0
+        res["first_iteration"] = {"ndets": r1.iterations()[-1]["ndets"], "cpu_oracle_s": round(cpu, 3),
+                                  "cpu_threads": threads, "gpu_s": round(gpu, 4)}
     return res
 
 
@@ -446,6 +496,12 @@ def main():
     ap.add_argument("--post-variational", action="store_true",
                     help="only the steps after the variational one (PT, RDMs) on the final space, printed as "
                          "their own JSON object (for profiling them)")
+    ap.add_argument("--form", choices=["c0", "c2", "c3", "c4"],
+                    help="only that configuration's line (c0: 16 orbitals; c2: the Hubbard batching form; c3: 40 "
+                         "orbitals; c4: the learned form on them), printed as its own JSON object (for profiling)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="block Davidson forms without hipGraph replay (rocprofv3's kernel tracer does not "
+                         "survive graph launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the variational step on a single MI355X "
@@ -460,9 +516,13 @@ def main():
     if args.post_variational:
         print(json.dumps({"post_variational": pt_measure(shci, full, sample, log, args)}))
         return
+    if args.form:
+        fn = {"c0": c0_form, "c2": batching_form, "c3": c3_form, "c4": c4_form}[args.form]
+        print(json.dumps({args.form: fn(shci, args)}))
+        return
     results = {}
     forms = [primary] if args.no_forms else [primary] + [p for p in (shci.PREC_FP64, shci.PREC_FP32) if p != primary]
-    block = dict(davidson_block=True, hip_graphs=True) if args.davidson_block else {}
+    block = dict(davidson_block=True, hip_graphs=not args.no_graphs) if args.davidson_block else {}
     # the headline form over the requested steps, the other forms over at
     # most 5 steps after one warm-up (they are reported beside it)
     fsteps, fwarm = min(args.steps, 5), min(args.warmup, 1)
@@ -498,9 +558,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
-                       "entries grouped by 1024-column tiles)" if prec == shci.PREC_FP64 else
-                       "kSigmaPerm (sigma = H x over the alpha-major relabelled CSR, FP32 values)"),
+            "kernel": "kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
+                      "entries grouped by 1024-column tiles)",
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
@@ -537,11 +596,11 @@ def main():
         L = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="learned+fp32", learned=True)
         recs, train = L["learned"]
         out["forms"]["learned_mixed_fp32"] = {
-            "description": "learned candidate ranking (gated: after the second logging iteration the form turns "
-                           "itself off unless ranking the candidate records at 2 ns each would cost less than the "
-                           "exact screen; when on, an MLP trained on the first two iterations' screening values "
-                           "predicts log10(|c||v|) and candidates predicted above log10(eps1) - 0.8 are tested "
-                           "exactly) with the FP32 form",
+            "description": "learned candidate ranking (gated: after the first exact screen, and again after the "
+                           "second logging iteration, the form turns itself off unless ranking the candidate "
+                           "records at 2 ns each would cost less than the exact screen; when on, an MLP trained on "
+                           "the first two iterations' screening values predicts log10(|c||v|) and candidates "
+                           "predicted above log10(eps1) - 0.8 are tested exactly) with the FP32 form",
             "value": round(statistics.mean(L["iter_times"]), 6),
             "ms_per_step": round(1e3 * statistics.mean(L["step_times"]), 3),
             "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(L["step_times"]), 3),
@@ -556,7 +615,7 @@ def main():
         # (one root: 17-vector subspace, thick restart), hipGraph-replayed
         # iterations, FP32 values
         B = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="block+fp32", profile=False,
-                     davidson_block=True, hip_graphs=True)
+                     davidson_block=True, hip_graphs=not args.no_graphs)
         out["forms"]["batched_mixed_fp32"] = {
             "description": "block Davidson (thick restart, hipGraph-replayed iterations) with the FP32 form; its "
                            "Davidson vectors differ from the reference Davidson's within the 5e-5 residual "
@@ -572,7 +631,7 @@ def main():
         # the three forms together: gated learned screening, block Davidson
         # with hipGraph replay, FP32 values
         LB = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="learned+block+fp32", profile=False,
-                      davidson_block=True, hip_graphs=True, learned=True)
+                      davidson_block=True, hip_graphs=not args.no_graphs, learned=True)
         out["forms"]["learned_batched_mixed_fp32"] = {
             "description": "gated learned screening + block Davidson (hipGraph-replayed) + FP32 values",
             "value": round(statistics.mean(LB["iter_times"]), 6),
@@ -588,6 +647,7 @@ def main():
 
     if not args.no_forms:
         out["pt_deterministic"] = pt_measure(shci, full, sample, log, args)
+        out["c0_small"] = c0_form(shci, args)
         out["c3_large"] = c3_form(shci, args)
         out["c4_learned_large"] = c4_form(shci, args)
 

@@ -389,17 +389,17 @@ shci_status runIterations(shci_handle* h, HostSchedule& schd, FILE* log, double 
     const double tScreenStart = wallTime();
     if (nnew < 0) nnew = E.screen(E.cmax.as<double>(), eps1);
     const double tExactScreen = wallTime() - tScreenStart;
+    // gated form: before the last logging iteration logs anything (the
+    // first starts from the reference determinant alone), rank only when
+    // ranking this iteration's candidate records (counted over a few
+    // windows of sources) is predicted to cost less than its exact screen
+    if (E.learned && E.learnedGate && iter == E.learnedLogIters - 1) {
+      E.learnedProbe(E.cmax.as<double>(), eps1);
+      if (!E.learnedWorthIt(tExactScreen, nold, iter)) E.learned = false;
+    }
     if (E.learned && iter < E.learnedLogIters) {
       E.learnedLog(E.cmax.as<double>(), eps1);
-      if (iter == E.learnedLogIters - 1) {
-        // gated form: train and rank only when ranking is predicted to cost
-        // less than the exact screen it replaces
-        if (E.learnedGate && !E.learnedWorthIt(tExactScreen, nold, iter)) {
-          E.learned = false;
-        } else {
-          E.learnedTrain();
-        }
-      }
+      if (iter == E.learnedLogIters - 1) E.learnedTrain();
     }
     if (std::getenv("SHCI_TIMING"))
       fprintf(stderr, "# iter %d: screening selected %lld new determinants\n", iter, (long long)nnew);

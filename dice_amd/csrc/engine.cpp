@@ -41,6 +41,7 @@ void Engine::initShared(const Engine& parent) {
   relabel = parent.relabel;
   split = parent.split;
   tiledOn = parent.tiledOn;
+  tiledFromH = parent.tiledFromH;
   tiledStep = parent.tiledStep;
   tiledPersistent = parent.tiledPersistent;
   lowerOnly = parent.lowerOnly;
@@ -190,15 +191,17 @@ void Engine::loadSystem(HostSystem&& sys) {
   // SHCI_SIGMA_LAYOUT=alpha keeps every relabelled column alpha-major (no
   // beta-major copy of x)
   split = !(layout && std::strcmp(layout, "alpha") == 0);
-  // the one-vector sigma on the tiled copy of the relabelled Hamiltonian:
-  // with FP64 values by default, SHCI_SIGMA_LAYOUT=tiled also with FP32
-  // values, SHCI_SIGMA_LAYOUT=rows never (the row-ordered copy instead)
-  tiledOn = layout && std::strcmp(layout, "tiled") == 0 ? 1 : (layout && std::strcmp(layout, "rows") == 0 ? 0 : 2);
+  // the one-vector sigma on the tiled copy of the relabelled Hamiltonian
+  // (SHCI_SIGMA_LAYOUT=rows: on the row-ordered copy instead)
+  tiledOn = !(layout && std::strcmp(layout, "rows") == 0);
   // SHCI_SIGMA_PERSISTENT=0/1: static or persistent workgroups for the
   // one-vector sigma (the autotuner, when on, decides per Hamiltonian)
   if (const char* p = std::getenv("SHCI_SIGMA_PERSISTENT")) sigmaCfg[0].persistent = std::atoi(p) != 0;
   // SHCI_SIGMA_TILE_STEP=8/16: instructions per wave per step of the tiled sigma
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_STEP")) tiledStep = std::atoi(p) >= 16 ? 16 : 8;
+  // SHCI_SIGMA_TILE_BUILD=direct: the tiled copy always built from H through
+  // the permutation (as when the row copy does not fit beside it)
+  if (const char* p = std::getenv("SHCI_SIGMA_TILE_BUILD")) tiledFromH = std::strcmp(p, "direct") == 0;
   // SHCI_SIGMA_TILE_PERSISTENT=1: persistent workgroups for the tiled sigma
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_PERSISTENT")) tiledPersistent = std::atoi(p) != 0;
   // SHCI_STORAGE=half: half storage from the first build
@@ -816,9 +819,10 @@ void Engine::buildPermuted() {
   tiledRows = -1;
   if (!relabel || n == 0) return;
   const bool f32 = precision == 1;
-  // the tiled copy: FP64 by default (SHCI_SIGMA_LAYOUT=tiled: also FP32,
-  // =rows: never)
-  const bool wantTiled = tiledOn == 1 || (tiledOn == 2 && !f32);
+  // (the block Davidson keeps the row copy: its SpMM needs it, and its
+  // solves, about half as many sigma calls as the reference Davidson's, do
+  // not repay building the tiled copy as well)
+  const bool wantTiled = tiledOn && davidsonMode != 1;
   // a copy (columns, values, row offsets, inverse permutation) with 64
   // bytes per determinant of slack, and the permuted x of the sigma calls:
   // up to 8 interleaved vectors in the block Davidson, in alpha-major order
@@ -832,18 +836,28 @@ void Engine::buildPermuted() {
   freeB = deviceFreeBytes(&totalB);
   const size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes + tiled.idx.bytes +
                       tiled.vals.bytes + tiled.inst.bytes;
-  if (base + rowsCopy + xPlain > have) {
+  // the tiled copy is built from the row-ordered one when both fit beside H;
+  // when only the tiled copy fits (and the SpMM does not need the row copy)
+  // it is built from H directly through the permutation (a slower build)
+  const bool viaRows = wantTiled && !tiledFromH && base + rowsCopy + tiledCopy + xPlain <= have;
+  const bool fromH =
+      wantTiled && !viaRows && base + tiledCopy + xPlain <= have;
+  if (!fromH && base + rowsCopy + xPlain > have) {
     permCols.release();
     permVals.release();
     tiled.release();
     return;
   }
-  const bool buildTiled = wantTiled && base + rowsCopy + tiledCopy + xPlain <= have;
+  const bool buildTiled = viaRows || fromH;
+  if (fromH) {  // no row copy this time: its memory goes to the tiled copy
+    permCols.release();
+    permVals.release();
+  }
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
   permPos.reserve((size_t)n * 4);
   launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
   // split columns only when their x copy fits too
-  const size_t used = base + rowsCopy + (buildTiled ? tiledCopy : 0);
+  const size_t used = base + (fromH ? 0 : rowsCopy) + (buildTiled ? tiledCopy : 0);
   const bool doSplit = split && 2 * n < 0x7fffffffLL && used + xSplit <= have;
   if (doSplit) {
     bPos.reserve((size_t)n * 4);
@@ -851,6 +865,17 @@ void Engine::buildPermuted() {
   }
   const uint32_t* bIdx = doSplit ? T.bIdx.as<uint32_t>() : nullptr;
   const uint32_t* bp = doSplit ? bPos.as<uint32_t>() : nullptr;
+  if (fromH) {
+    if (buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
+                      f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(), bIdx, bp, n, nnzOff, ws,
+                      stream)) {
+      tiledRows = n;
+      if (doSplit) splitRows = n;
+    } else {
+      tiled.release();
+    }
+    return;
+  }
   permRp.reserve((size_t)(n + 1) * 8);
   int64_t* rp = permRp.as<int64_t>();
   HIP_CHECK(hipMemsetAsync(rp + n, 0, 8, stream));
@@ -873,12 +898,11 @@ void Engine::buildPermuted() {
                       f32 ? permVals.as<float>() : nullptr, nullptr, nullptr, nullptr, nullptr, n, nnzOff, ws,
                       stream, doSplit)) {
       tiledRows = n;
-      if (davidsonMode != 1) {  // the row-ordered copy served only the build
-        HIP_CHECK(hipStreamSynchronize(stream));
-        permCols.release();
-        permVals.release();
-        permRows = -1;
-      }
+      // the row-ordered copy served only the build
+      HIP_CHECK(hipStreamSynchronize(stream));
+      permCols.release();
+      permVals.release();
+      permRows = -1;
     } else {
       tiled.release();
     }
@@ -906,6 +930,9 @@ void Engine::tuneSigma() {
   tuneLog.clear();
   char line[160];
   for (int which = 0; which < (davidsonMode == 1 ? 2 : 1); which++) {
+    // one vector goes to the tiled copy when it exists, several to the row
+    // copy's SpMM only when that copy was kept: nothing to tune otherwise
+    if ((which == 0 && tiledRows == n) || permRows != n) continue;
     const int k = ks[which];
     SigmaConfig best = sigmaCfg[which];
     float bestMs = 1e30f;

@@ -74,7 +74,8 @@ struct Engine {
   // tiled copy for the one-vector sigma (blocks of 256 relabelled rows,
   // entries grouped by column tile); valid when tiledRows == n.  With it the
   // row-ordered relabelled copy is kept only for the block Davidson's SpMM.
-  int tiledOn = 2;  // 0: never, 1: always, 2: with FP64 values
+  bool tiledOn = true;
+  bool tiledFromH = false;  // build the tiled copy from H even when the row copy fits
   int tiledStep = 16;           // instructions per wave per step (8 or 16; 5.5 against 6.0 ms)
   bool tiledPersistent = false;  // one workgroup per block (6.4 ms) beat persistent ones (7.5 ms)
   TiledHam tiled;
@@ -135,16 +136,20 @@ struct Engine {
   int learnedLogIters = 2;
   double learnedLoosen = 10.0, learnedMargin = 0.8, learnedMaxFraction = 0.5;
   bool learnedControl = false;  // keep the untrained network (control runs)
-  // gate: after the last logging iteration, rank only when the candidate
-  // records per source of that iteration's windows times the ranking cost
-  // per record (learnedNsPerRecord, an optimistic 2 ns) stay below the exact
-  // screen's cost per source in the same iteration; otherwise the form
-  // turns itself off (no training, full screening), recorded as one
-  // fallback record with evaluated = 0
+  // gate: rank only when the candidate records per source times the
+  // ranking cost per record (learnedNsPerRecord, an optimistic 2 ns) stay
+  // below the exact screen's cost in the same iteration.  Checked before the
+  // last logging iteration logs anything (records per source counted over a
+  // few windows of sources; the first iteration starts from one
+  // determinant); otherwise the form turns itself off (no more logging, no
+  // training, full screening), recorded as one fallback record with
+  // evaluated = 0
   bool learnedGate = false;
   double learnedNsPerRecord = 2.0;
   int64_t lastLogRecords = 0, lastLogSources = 0;
   bool learnedWorthIt(double exactScreenSeconds, int64_t sources, int iter);
+  // records per source over a few windows of sources (counted, not kept)
+  void learnedProbe(const double* dCmax, double eps1);
   int learnedHidden = 256;
   int64_t learnedMaxSamples = (int64_t)1 << 18;
   MlpModel mlp;

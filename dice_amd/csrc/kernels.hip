@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 

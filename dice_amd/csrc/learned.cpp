@@ -29,15 +29,16 @@ constexpr int kSample = 1 << 16;                   // records sampled for tau an
 }  // namespace
 
 // Logs training samples from the current (plain) iteration: the candidate
-// records of 16 windows of 256 source determinants at random offsets; from
+// records of 16 windows of 256 source determinants at random offsets (one
+// window holding the whole space when it is no larger); from
 // each window a random subset of records (features and targets
 // log10(|c| |v|)) so that the iterations that log fill learnedMaxSamples.
 void Engine::learnedLog(const double* dCmax, double eps1) {
   const int din = mlpDin();
-  recBuf.reserve((size_t)kRecordCap * sizeof(CandRecord));
   counter.reserve(64);
   const int64_t window = std::min<int64_t>(n, 256);
-  const int windows = 16;
+  // (a space no larger than one window is logged once)
+  const int windows = n > window ? 16 : 1;
   const int64_t perWindow = std::max<int64_t>(1, learnedMaxSamples / windows / std::max(1, learnedLogIters));
   DevMem picked;
   picked.reserve((size_t)perWindow * sizeof(CandRecord));
@@ -50,13 +51,21 @@ void Engine::learnedLog(const double* dCmax, double eps1) {
   lastLogSources = 0;
   for (int w = 0; w < windows; w++) {
     const int64_t begin = n > window ? (int64_t)(splitmix(learnedRng) % (uint64_t)(n - window + 1)) : 0;
+    // count the window's records, then hold them (the buffer grows to the
+    // largest window seen, at most kRecordCap records)
+    unsigned long long found = 0;
     HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
     launchScreenRecords(S, dets.as<uint64_t>(), begin, begin + window, dCmax, eps1, learnedLoosen, ncore, nact,
-                        recBuf.as<CandRecord>(), counter.as<unsigned long long>(), kRecordCap, stream);
-    unsigned long long found = 0;
+                        nullptr, counter.as<unsigned long long>(), 0, stream);
     HIP_CHECK(hipMemcpyAsync(&found, counter.p, 8, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     const int64_t total = std::min<int64_t>((int64_t)found, kRecordCap);
+    if (total > 0) {
+      recBuf.reserve((size_t)total * sizeof(CandRecord));
+      HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
+      launchScreenRecords(S, dets.as<uint64_t>(), begin, begin + window, dCmax, eps1, learnedLoosen, ncore, nact,
+                          recBuf.as<CandRecord>(), counter.as<unsigned long long>(), total, stream);
+    }
     lastLogRecords += (int64_t)found;
     lastLogSources += std::min<int64_t>(window, n - begin);
     if (total == 0) continue;
@@ -75,6 +84,25 @@ void Engine::learnedLog(const double* dCmax, double eps1) {
     HIP_CHECK(hipMemcpyAsync(sampT.data() + old, lVals.p, (size_t)m * sizeof(float), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     trainSamples += total;
+  }
+}
+
+void Engine::learnedProbe(const double* dCmax, double eps1) {
+  counter.reserve(64);
+  const int64_t window = std::min<int64_t>(n, 256);
+  lastLogRecords = 0;
+  lastLogSources = 0;
+  for (int w = 0; w < 4; w++) {
+    const int64_t begin = n > window ? (int64_t)(splitmix(learnedRng) % (uint64_t)(n - window + 1)) : 0;
+    HIP_CHECK(hipMemsetAsync(counter.p, 0, 8, stream));
+    // capacity 0: the kernel counts the records and writes none
+    launchScreenRecords(S, dets.as<uint64_t>(), begin, begin + window, dCmax, eps1, learnedLoosen, ncore, nact,
+                        nullptr, counter.as<unsigned long long>(), 0, stream);
+    unsigned long long found = 0;
+    HIP_CHECK(hipMemcpyAsync(&found, counter.p, 8, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    lastLogRecords += (int64_t)found;
+    lastLogSources += std::min<int64_t>(window, n - begin);
   }
 }
 

@@ -14,6 +14,14 @@
         HBM bytes per dispatch of NAME from a FETCH_SIZE run and a WRITE_SIZE
         run, each scaled by bytes / counter of the copy kernel measured the
         same way; writes a JSON summary.
+
+    python -m dice_amd.profiling requests --kernel NAME --read DIR --write DIR
+        the same bytes counted from the L2's memory-side requests by size:
+        read = 128 TCC_EA0_RDREQ_128B + 32 TCC_EA0_RDREQ_32B + 64 (the rest)
+        from a run of those counters (and TCC_EA0_RDREQ); write = 64
+        TCC_EA0_WRREQ_64B + 32 (the rest).  An independent check of the
+        FETCH_SIZE scale: on gfx950 FETCH_SIZE counts 128-byte reads through
+        TCC_BUBBLE, which stays 0, so each is counted as 64 bytes.
 """
 import argparse
 import csv
@@ -51,7 +59,8 @@ def counter_per_dispatch(directory, kernel_substr, counter):
 def copy_main(mib):
     import torch
     n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    # filled on the host and copied: the only kernels are the five products
+    a = torch.ones(n, dtype=torch.float64).to("cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         torch.mul(a, 2.0, out=b)
@@ -98,6 +107,20 @@ def traffic_main(a):
     print(text)
 
 
+def requests_main(a):
+    r128, n = counter_per_dispatch(a.read, a.kernel, "TCC_EA0_RDREQ_128B_sum")
+    r32, _ = counter_per_dispatch(a.read, a.kernel, "TCC_EA0_RDREQ_32B_sum")
+    rall, _ = counter_per_dispatch(a.read, a.kernel, "TCC_EA0_RDREQ_sum")
+    w64, nw = counter_per_dispatch(a.write, a.kernel, "TCC_EA0_WRREQ_64B_sum")
+    wall, _ = counter_per_dispatch(a.write, a.kernel, "TCC_EA0_WRREQ_sum")
+    read = 128.0 * r128 + 32.0 * r32 + 64.0 * (rall - r128 - r32)
+    write = 64.0 * w64 + 32.0 * (wall - w64)
+    out = {"kernel": a.kernel, "dispatches": {"read": n, "write": nw}, "read_bytes_per_dispatch": read,
+           "write_bytes_per_dispatch": write, "traffic_bytes_per_dispatch": read + write,
+           "rdreq": rall, "rdreq_128b": r128, "rdreq_32b": r32, "wrreq": wall, "wrreq_64b": w64}
+    print(json.dumps(out, indent=1))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m dice_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -112,14 +135,20 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--copy-fetch", required=True)
     t.add_argument("--copy-write", required=True)
-    t.add_argument("--copy-kernel", default="elementwise")
+    t.add_argument("--copy-kernel", default="MulFunctor")
     t.add_argument("--copy-mib", type=int, default=1024)
     t.add_argument("--out")
+    q = sub.add_parser("requests")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--read", required=True)
+    q.add_argument("--write", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_main(a.mib)
     elif a.cmd == "stats":
         stats_main(a.csv, a.top)
+    elif a.cmd == "requests":
+        requests_main(a)
     else:
         traffic_main(a)
 
