@@ -42,6 +42,8 @@ void Engine::initShared(const Engine& parent) {
   split = parent.split;
   tiledOn = parent.tiledOn;
   tiledFromH = parent.tiledFromH;
+  tiledStage = parent.tiledStage;
+  tiledStageMin = parent.tiledStageMin;
   tiledStep = parent.tiledStep;
   tiledPersistent = parent.tiledPersistent;
   lowerOnly = parent.lowerOnly;
@@ -202,6 +204,11 @@ void Engine::loadSystem(HostSystem&& sys) {
   // SHCI_SIGMA_TILE_BUILD=direct: the tiled copy always built from H through
   // the permutation (as when the row copy does not fit beside it)
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_BUILD")) tiledFromH = std::strcmp(p, "direct") == 0;
+  // SHCI_SIGMA_TILE_STAGE=0/1: the tiled sigma gathers a large segment's x
+  // from a copy of its tile in LDS (1) or from global memory (0)
+  if (const char* p = std::getenv("SHCI_SIGMA_TILE_STAGE")) tiledStage = std::atoi(p) != 0;
+  // SHCI_SIGMA_TILE_STAGE_MIN=k: segments of at least k instructions are staged
+  if (const char* p = std::getenv("SHCI_SIGMA_TILE_STAGE_MIN")) tiledStageMin = std::max(1, std::atoi(p));
   // SHCI_SIGMA_TILE_PERSISTENT=1: persistent workgroups for the tiled sigma
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_PERSISTENT")) tiledPersistent = std::atoi(p) != 0;
   // SHCI_STORAGE=half: half storage from the first build
@@ -868,7 +875,7 @@ void Engine::buildPermuted() {
   if (fromH) {
     if (buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
                       f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(), bIdx, bp, n, nnzOff, ws,
-                      stream)) {
+                      stream, false, tiledStageMin)) {
       tiledRows = n;
       if (doSplit) splitRows = n;
     } else {
@@ -896,7 +903,7 @@ void Engine::buildPermuted() {
   if (buildTiled) {
     if (buildTiledHam(tiled, rp, permCols.as<int32_t>(), f32 ? nullptr : permVals.as<double>(),
                       f32 ? permVals.as<float>() : nullptr, nullptr, nullptr, nullptr, nullptr, n, nnzOff, ws,
-                      stream, doSplit)) {
+                      stream, doSplit, tiledStageMin)) {
       tiledRows = n;
       // the row-ordered copy served only the build
       HIP_CHECK(hipStreamSynchronize(stream));
@@ -1225,7 +1232,7 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   if (tiledSigma)
     for (int r = 0; r < k; r++)
       launchSigmaTiled(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, dx + r * ld, dy + r * ld, xs,
-                       tiledPersistent, sigmaCounter.as<unsigned>(), stream, tiledStep);
+                       tiledPersistent, sigmaCounter.as<unsigned>(), stream, tiledStep, tiledStage);
   else if (perm && f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, sigmaCounter.as<unsigned>(),
@@ -1283,7 +1290,7 @@ void Engine::sigmaFixed(const double* dx, double* dy, int k, int64_t ld) {
   if (tiledSigma)
     for (int r = 0; r < k; r++)
       launchSigmaTiled(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(), bPerm, dx + r * ld, dy + r * ld, xs,
-                       tiledPersistent, ctr, stream, tiledStep);
+                       tiledPersistent, ctr, stream, tiledStep, tiledStage);
   else if (f32)
     launchSigmaPermF32(permRp.as<int64_t>(), permCols.as<int32_t>(), permVals.as<float>(), diag.as<double>(),
                        T.aMajorDet.as<uint32_t>(), bPerm, n, dx, dy, k, ld, xs, cfg, ctr, stream);

@@ -76,6 +76,8 @@ struct Engine {
   // row-ordered relabelled copy is kept only for the block Davidson's SpMM.
   bool tiledOn = true;
   bool tiledFromH = false;  // build the tiled copy from H even when the row copy fits
+  bool tiledStage = true;  // large segments gather x from an LDS copy of their tile
+  int tiledStageMin = 16;   // instructions that make a segment large
   int tiledStep = 16;           // instructions per wave per step (8 or 16; 5.5 against 6.0 ms)
   bool tiledPersistent = false;  // one workgroup per block (6.4 ms) beat persistent ones (7.5 ms)
   TiledHam tiled;

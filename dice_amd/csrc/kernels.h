@@ -271,18 +271,20 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32);
 // the split columns (the beta-major copy of x after the alpha-major one);
 // or, with perm, pos, bIdx and bPos null, from the relabelled row copy
 // itself (rows and columns already permuted; splitCols says whether its
-// columns reach into the beta-major copy).  Returns false, with nothing
+// columns reach into the beta-major copy).  Segments of at least stageMin
+// instructions are marked for the staged sigma (their x tile goes to LDS).  Returns false, with nothing
 // built, when the permuted x has more tiles than the build's bitmap holds.
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
                    const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s,
-                   bool splitCols = false);
+                   bool splitCols = false, int stageMin = 8);
 // y = H x through the tiled copy (one vector, x and y in determinant order;
 // bPerm = the beta-major determinant order when the copy has split columns)
-// (step: instructions per wave per step, 8 or 16)
+// (step: instructions per wave per step, 8 or 16; stage: each wave copies
+// the x tile of a large segment into LDS and gathers from there)
 void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                       const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
-                      hipStream_t s, int step = 8);
+                      hipStream_t s, int step = 8, bool stage = false);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -1767,6 +1767,10 @@ constexpr int kTileLdsSegs = 1024;  // blocks with at most this many segments co
 // entry of its row run within the instruction (bits 10-15), row in block
 // (bits 16-23), last entry of its run within the instruction (bit 31)
 constexpr int kTileDistShift = 10;
+// an instruction descriptor's second word: entry count (bits 0-6), tile
+// (bits 7-30), bit 31 set when its segment has at least stageMin
+// instructions (the staged kernel copies such a segment's x tile to LDS)
+constexpr uint32_t kTileDescMask = 0xffffffu;
 
 // the permuted-x column of H's column J in a row with beta string index bi
 // (pos null: the input is the relabelled copy, whose columns are mapped)
@@ -1943,7 +1947,7 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
                                                     const uint32_t* segCnt, const uint32_t* segStart,
                                                     const uint32_t* segInst, const uint32_t* segTotal, uint32_t* fill,
                                                     const int64_t* blockE, const int64_t* blockI, uint32_t* tIdx,
-                                                    VT* tVal, uint2* inst) {
+                                                    VT* tVal, uint2* inst, int stageMin) {
   __shared__ uint32_t bits[kTileMaxWords];
   __shared__ uint32_t pre[kTileMaxWords];
   __shared__ uint32_t fillL[kWaves][kTileLdsSegs];
@@ -1993,9 +1997,10 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
   for (int64_t s = threadIdx.x; s < D; s += kBlock) {
     const uint32_t tot = segTotal[base + s], st = segStart[base + s], tile = segTile[base + s];
     const int64_t i0 = ib + segInst[base + s];
+    const uint32_t big = (tot + 63u) / 64u >= (uint32_t)stageMin ? 0x80000000u : 0u;
     for (uint32_t j = 0; 64u * j < tot; j++) {
       const uint32_t cnt = tot - 64u * j < 64u ? tot - 64u * j : 64u;
-      inst[i0 + j] = make_uint2(st + 64u * j, (tile << 7) | cnt);
+      inst[i0 + j] = make_uint2(st + 64u * j, big | (tile << 7) | cnt);
     }
   }
 }
@@ -2070,39 +2075,71 @@ __device__ __forceinline__ void tileScanConsts(int& c15, int& c31) {
 
 // y[perm[m]] = diag .* x + (H x) over the tiled copy, one vector; xP holds x
 // in the permuted order of the copy's columns
-template <typename VT, typename XT, bool PERSIST, int U>
+// One x tile (kTileCols values from xP at tile << kTileLog, nx values in all)
+// into LDS by one wave, in 16-byte pieces; the last tile's values past nx
+// read as 0 (no entry points at them)
+template <typename XT>
+__device__ __forceinline__ void stageTile(const XT* __restrict__ xP, int64_t nx, uint32_t tile, XT* xs) {
+  constexpr int per = 16 / (int)sizeof(XT);  // values per 16-byte piece
+  const int lane = __lane_id();
+  const int64_t base = (int64_t)tile << kTileLog;
+  if (base + kTileCols <= nx) {
+    typedef XT V __attribute__((ext_vector_type(per)));
+#pragma unroll
+    for (int k = 0; k < kTileCols / (64 * per); k++) {
+      const int j = (k * 64 + lane) * per;
+      *(V*)(xs + j) = *(const V*)(xP + base + j);
+    }
+  } else {
+    for (int j = lane; j < kTileCols; j += 64) xs[j] = base + j < nx ? xP[base + j] : (XT)0;
+  }
+}
+
+// y[perm[m]] = diag .* x + (H x) over the tiled copy, one vector; xP holds x
+// in the permuted order of the copy's columns (nx values).  STAGE: each wave
+// takes a contiguous quarter of the block's instructions, and before an
+// instruction of a large segment (descriptor bit 31) gathers, the wave
+// copies that segment's x tile into its own LDS (once per tile) and gathers
+// from there; otherwise the waves take turns of U instructions and gather
+// from global memory.
+template <typename VT, typename XT, bool PERSIST, int U, bool STAGE>
 __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ inst, const int64_t* __restrict__ blockI,
                                                       const int64_t* __restrict__ blockE,
                                                       const uint32_t* __restrict__ tIdx, const VT* __restrict__ tVal,
                                                       const double* __restrict__ diag, const uint32_t* __restrict__ perm,
-                                                      int64_t n, int64_t nblocks, const XT* __restrict__ xP,
+                                                      int64_t n, int64_t nblocks, const XT* __restrict__ xP, int64_t nx,
                                                       const double* __restrict__ x, double* __restrict__ y,
                                                       unsigned* counter) {
   __shared__ double yw[kWaves][kTileRows];
+  __shared__ __align__(16) XT xs[STAGE ? kWaves : 1][STAGE ? kTileCols : 1];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
   static_assert(U <= kTileInstPad, "step larger than the descriptor padding");
   int c15, c31;
   tileScanConsts(c15, c31);
+  uint32_t staged = 0xffffffffu;  // the tile in this wave's LDS (x does not change within a launch)
   ChunkSched<1, PERSIST> sched{nblocks, counter, -1};
   while (sched.next()) {
     const int64_t b = sched.cur;
     for (int j = lane; j < kTileRows; j += 64) yw[wave][j] = 0.0;
     const int64_t i0 = blockI[b], i1 = blockI[b + 1], e0 = blockE[b];
+    const int64_t len = i1 - i0;
+    const int64_t w0 = STAGE ? i0 + len * wave / kWaves : i0 + (int64_t)wave * U;
+    const int64_t wEnd = STAGE ? i0 + len * (wave + 1) / kWaves : i1;
     // the U instructions' loads are issued without branches, all matrix loads
     // before all gathers (vector loads return in order, so a gather issued
     // between them would make every later wait include it); absent
     // instructions and lanes past an instruction's count read the block's
     // first entry and their tile's first x value, and are dropped
-    // each wave takes U consecutive instructions at a time (one scalar load
-    // of their descriptors; the array has U zero descriptors of padding)
-    for (int64_t i = i0 + (int64_t)wave * U; i < i1; i += (int64_t)kWaves * U) {
+    // each step takes U consecutive instructions (one scalar load of their
+    // descriptors; the array has U zero descriptors of padding)
+    for (int64_t i = w0; i < wEnd; i += STAGE ? (int64_t)U : (int64_t)kWaves * U) {
       uint32_t dx[U], dy[U];
       const uint2* dp = inst + i;
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const uint2 d = dp[u];
         dx[u] = d.x;
-        dy[u] = i + u < i1 ? d.y : (d.y & ~127u);  // count 0 past the block
+        dy[u] = i + u < wEnd ? d.y : (d.y & ~127u);  // count 0 past the range
       }
       uint32_t ix[U];
       double v[U];
@@ -2114,11 +2151,33 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
         v[u] = (double)__builtin_nontemporal_load(tVal + p);
       }
       double xv[U];
+      if (STAGE) {
+        // the tile of the step's first instruction from a large segment
+        uint32_t want = 0xffffffffu;
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const bool valid = (uint32_t)lane < (dy[u] & 127u);
-        const size_t tb = (size_t)(dy[u] >> 7) << kTileLog;
-        xv[u] = (double)xP[tb + (valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u)];
+        for (int u = 0; u < U; u++)
+          if (want == 0xffffffffu && (dy[u] & 127u) && (dy[u] >> 31)) want = (dy[u] >> 7) & kTileDescMask;
+        if (want != 0xffffffffu && want != staged) {
+          stageTile<XT>(xP, nx, want, &xs[wave][0]);
+          staged = want;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const bool valid = (uint32_t)lane < (dy[u] & 127u);
+          const uint32_t col = valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u;
+          const uint32_t tile = (dy[u] >> 7) & kTileDescMask;
+          if (tile == staged)
+            xv[u] = (double)xs[wave][col];
+          else
+            xv[u] = (double)xP[((size_t)tile << kTileLog) + col];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const bool valid = (uint32_t)lane < (dy[u] & 127u);
+          const size_t tb = (size_t)((dy[u] >> 7) & kTileDescMask) << kTileLog;
+          xv[u] = (double)xP[tb + (valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u)];
+        }
       }
       // products summed per row run by the segmented scan; the run's last
       // lane adds the sum into the wave's partial row (lanes past the count
@@ -2132,11 +2191,12 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
       }
     }
     __syncthreads();
-    const int t = threadIdx.x;
-    const int64_t m = b * kTileRows + t;
-    if (m < n) {
-      const int64_t I = perm[m];
-      y[I] = fma(diag[I], x[I], ((yw[0][t] + yw[1][t]) + yw[2][t]) + yw[3][t]);
+    for (int t = threadIdx.x; t < kTileRows; t += kBlock) {
+      const int64_t m = b * kTileRows + t;
+      if (m < n) {
+        const int64_t I = perm[m];
+        y[I] = fma(diag[I], x[I], ((yw[0][t] + yw[1][t]) + yw[2][t]) + yw[3][t]);
+      }
     }
     if (!PERSIST) break;
   }
@@ -3544,7 +3604,8 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32) {
 
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
-                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s, bool splitCols) {
+                   const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s, bool splitCols,
+                   int stageMin) {
   T.n = -1;
   if (n <= 0) return false;
   const bool split = pos ? bIdx != nullptr : splitCols;
@@ -3602,13 +3663,15 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
                                              T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
                                              T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
                                              T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
-                                             T.idx.as<uint32_t>(), T.vals.as<float>(), T.inst.as<uint2>());
+                                             T.idx.as<uint32_t>(), T.vals.as<float>(), T.inst.as<uint2>(),
+                                             stageMin);
   else
     kTileFill<double><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals64, perm, pos, bIdx, bPos, n, nwords, segBase,
                                               T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
                                               T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
                                               T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
-                                              T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>());
+                                              T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>(),
+                                              stageMin);
   HIP_CHECK(hipGetLastError());
   kTileRunBits<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), blockI, blockE, T.idx.as<uint32_t>());
   HIP_CHECK(hipGetLastError());
@@ -3622,24 +3685,33 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
   return true;
 }
 
-template <typename VT, typename XT, bool PERSIST>
-static void sigmaTiledU(int step, unsigned grid, const TiledHam& T, const double* diag, const uint32_t* perm,
-                        int64_t n, const XT* xs, const double* x, double* y, unsigned* counter, hipStream_t s) {
+template <typename VT, typename XT, bool PERSIST, bool STAGE>
+static void sigmaTiledS(int step, unsigned grid, const TiledHam& T, const double* diag, const uint32_t* perm,
+                        int64_t n, const XT* xs, int64_t nx, const double* x, double* y, unsigned* counter,
+                        hipStream_t s) {
   if (step >= 16)
-    kSigmaTiled<VT, XT, PERSIST, 16><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
-                                                             T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                             T.vals.as<VT>(), diag, perm, n, T.blocks, xs, x, y,
-                                                             counter);
+    kSigmaTiled<VT, XT, PERSIST, 16, STAGE><<<grid, kBlock, 0, s>>>(
+        T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(), T.idx.as<uint32_t>(), T.vals.as<VT>(),
+        diag, perm, n, T.blocks, xs, nx, x, y, counter);
   else
-    kSigmaTiled<VT, XT, PERSIST, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
-                                                            T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                            T.vals.as<VT>(), diag, perm, n, T.blocks, xs, x, y,
-                                                            counter);
+    kSigmaTiled<VT, XT, PERSIST, 8, STAGE><<<grid, kBlock, 0, s>>>(
+        T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(), T.idx.as<uint32_t>(), T.vals.as<VT>(),
+        diag, perm, n, T.blocks, xs, nx, x, y, counter);
+}
+
+template <typename VT, typename XT, bool PERSIST>
+static void sigmaTiledU(int step, bool stage, unsigned grid, const TiledHam& T, const double* diag,
+                        const uint32_t* perm, int64_t n, const XT* xs, int64_t nx, const double* x, double* y,
+                        unsigned* counter, hipStream_t s) {
+  if (stage)
+    sigmaTiledS<VT, XT, PERSIST, true>(step, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
+  else
+    sigmaTiledS<VT, XT, PERSIST, false>(step, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
 }
 
 void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                       const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
-                      hipStream_t s, int step) {
+                      hipStream_t s, int step, bool stage) {
   const int64_t n = T.n;
   if (n <= 0) return;
   if (T.split != (bPerm != nullptr)) throw HipError("tiled sigma: split columns need the beta-major order");
@@ -3651,16 +3723,16 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
     float* xs = (float*)xP.get((size_t)nx * 4);
     kGatherPerm<float, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, counter);
     if (persistent)
-      sigmaTiledU<float, float, true>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+      sigmaTiledU<float, float, true>(step, stage, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
     else
-      sigmaTiledU<float, float, false>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+      sigmaTiledU<float, float, false>(step, stage, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
   } else {
     double* xs = (double*)xP.get((size_t)nx * 8);
     kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, counter);
     if (persistent)
-      sigmaTiledU<double, double, true>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+      sigmaTiledU<double, double, true>(step, stage, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
     else
-      sigmaTiledU<double, double, false>(step, grid, T, diag, perm, n, xs, x, y, counter, s);
+      sigmaTiledU<double, double, false>(step, stage, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
   }
   HIP_CHECK(hipGetLastError());
 }
