@@ -905,11 +905,16 @@ void Engine::buildPermuted() {
                       f32 ? permVals.as<float>() : nullptr, nullptr, nullptr, nullptr, nullptr, n, nnzOff, ws,
                       stream, doSplit, tiledStageMin)) {
       tiledRows = n;
-      // the row-ordered copy served only the build
-      HIP_CHECK(hipStreamSynchronize(stream));
-      permCols.release();
-      permVals.release();
+      // the row-ordered copy served only the build: its buffers are kept
+      // for the next build (mapping them again costs more than the copy)
+      // unless memory is short
       permRows = -1;
+      size_t totalB = 0;
+      if (deviceFreeBytes(&totalB) < totalB / 4) {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        permCols.release();
+        permVals.release();
+      }
     } else {
       tiled.release();
     }

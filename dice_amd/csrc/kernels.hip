@@ -1856,13 +1856,15 @@ __global__ void __launch_bounds__(kBlock) kTileMark(const int64_t* rowPtr, const
 }
 
 // pass 2: per (segment, wave) entry counts; segment tiles; per segment its
-// entry and instruction offsets within the block and the exclusive prefix
-// of its wave counts; per block its entries and instructions
+// entry and instruction offsets within the block; per (segment, wave) the
+// fill cursor: the block-relative position of the wave's first entry in
+// the segment; per block its entries and instructions
 __global__ void __launch_bounds__(kBlock) kTileCount(const int64_t* rowPtr, const int32_t* cols, const uint32_t* perm,
                                                      const uint32_t* pos, const uint32_t* bIdx, const uint32_t* bPos,
                                                      int64_t n, int nwords, const int64_t* segBase, uint32_t* segTile,
                                                      uint32_t* segCnt, uint32_t* segStart, uint32_t* segInst,
-                                                     uint32_t* segTotal, int64_t* blockE, int64_t* blockI) {
+                                                     uint32_t* segTotal, uint32_t* fill, int64_t* blockE,
+                                                     int64_t* blockI) {
   __shared__ uint32_t bits[kTileMaxWords];
   __shared__ uint32_t pre[kTileMaxWords];
   __shared__ uint32_t cntL[kWaves][kTileLdsSegs];
@@ -1910,16 +1912,12 @@ __global__ void __launch_bounds__(kBlock) kTileCount(const int64_t* rowPtr, cons
   for (int64_t s0 = 0; s0 < D; s0 += kBlock) {
     const int64_t s = s0 + threadIdx.x;
     uint64_t mine = 0;
+    uint32_t c[4] = {0u, 0u, 0u, 0u};
     if (s < D) {
-      uint32_t c[4];
 #pragma unroll
       for (int w = 0; w < 4; w++)
         c[w] = __hip_atomic_load(&segCnt[(base + s) * 4 + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t tot = c[0] + c[1] + c[2] + c[3];
-      segCnt[(base + s) * 4 + 0] = 0u;
-      segCnt[(base + s) * 4 + 1] = c[0];
-      segCnt[(base + s) * 4 + 2] = c[0] + c[1];
-      segCnt[(base + s) * 4 + 3] = c[0] + c[1] + c[2];
       segTotal[base + s] = tot;
       mine = (uint64_t)tot | ((uint64_t)((tot + 63u) / 64u) << 32);
     }
@@ -1927,8 +1925,14 @@ __global__ void __launch_bounds__(kBlock) kTileCount(const int64_t* rowPtr, cons
     const uint64_t incl = blockInclusiveScan(mine, &chunk);
     if (s < D) {
       const uint64_t ex = carry + incl - mine;
-      segStart[base + s] = (uint32_t)ex;
+      const uint32_t st = (uint32_t)ex;
+      segStart[base + s] = st;
       segInst[base + s] = (uint32_t)(ex >> 32);
+      // the segment's entries are ordered by the wave that places them
+      fill[(base + s) * 4 + 0] = st;
+      fill[(base + s) * 4 + 1] = st + c[0];
+      fill[(base + s) * 4 + 2] = st + c[0] + c[1];
+      fill[(base + s) * 4 + 3] = st + c[0] + c[1] + c[2];
     }
     carry += chunk;
   }
@@ -1944,8 +1948,8 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
                                                     const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
                                                     const uint32_t* bPos, int64_t n, int nwords,
                                                     const int64_t* segBase, const uint32_t* segTile,
-                                                    const uint32_t* segCnt, const uint32_t* segStart,
-                                                    const uint32_t* segInst, const uint32_t* segTotal, uint32_t* fill,
+                                                    const uint32_t* segStart, const uint32_t* segInst,
+                                                    const uint32_t* segTotal, uint32_t* fill,
                                                     const int64_t* blockE, const int64_t* blockI, uint32_t* tIdx,
                                                     VT* tVal, uint2* inst, int stageMin) {
   __shared__ uint32_t bits[kTileMaxWords];
@@ -1957,7 +1961,7 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
   const int64_t base = segBase[b], eb = blockE[b];
   const bool inLds = segBase[b + 1] - base <= kTileLdsSegs;
   if (inLds)
-    for (int k = threadIdx.x; k < kWaves * kTileLdsSegs; k += kBlock) (&fillL[0][0])[k] = 0u;
+    for (int64_t k = threadIdx.x; k < (segBase[b + 1] - base) * 4; k += kBlock) fillL[k & 3][k >> 2] = fill[base * 4 + k];
   __syncthreads();
   const int64_t m0 = b * kTileRows;
   const int rows = (int)(n - m0 < kTileRows ? n - m0 : kTileRows);
@@ -1975,10 +1979,10 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
         s = tileRank(bits, pre, c >> kTileLog);
         v = vals[p];
       }
-      // the entry's place among the wave's entries of its segment: the
-      // wave's fill counter of the segment, taken by an atomic add with
-      // return (lanes of one instruction that share a counter receive
-      // consecutive values in lane order)
+      // the entry's place in the block: the wave's fill cursor of the
+      // segment, taken by an atomic add with return (lanes of one
+      // instruction that share a cursor receive consecutive values in lane
+      // order)
       uint32_t got = 0;
       if (valid) {
         if (inLds)
@@ -1987,7 +1991,7 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
           got = atomicAdd(&fill[(base + s) * 4 + wave], 1u);
       }
       if (valid) {
-        const int64_t q = eb + segStart[base + s] + segCnt[(base + s) * 4 + wave] + got;
+        const int64_t q = eb + got;
         tIdx[q] = ((uint32_t)r << 16) | (c & (uint32_t)(kTileCols - 1));
         tVal[q] = v;
       }
@@ -3635,14 +3639,14 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
   T.blockE.reserve((size_t)(blocks + 1) * 8);
   T.blockI.reserve((size_t)(blocks + 1) * 8);
   HIP_CHECK(hipMemsetAsync(T.segCnt.p, 0, sb * 16, s));
-  HIP_CHECK(hipMemsetAsync(T.fill.p, 0, sb * 16, s));
   int64_t* blockE = T.blockE.as<int64_t>();
   int64_t* blockI = T.blockI.as<int64_t>();
   HIP_CHECK(hipMemsetAsync(blockE + blocks, 0, 8, s));
   HIP_CHECK(hipMemsetAsync(blockI + blocks, 0, 8, s));
   kTileCount<<<grid, kBlock, 0, s>>>(rowPtr, cols, perm, pos, bIdx, bPos, n, nwords, segBase,
                                      T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(), T.segStart.as<uint32_t>(),
-                                     T.segInst.as<uint32_t>(), T.segTotal.as<uint32_t>(), blockE, blockI);
+                                     T.segInst.as<uint32_t>(), T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(),
+                                     blockE, blockI);
   HIP_CHECK(hipGetLastError());
   exclusiveScanI64(blockE, blocks + 1, ws, s);
   exclusiveScanI64(blockI, blocks + 1, ws, s);
@@ -3660,15 +3664,15 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
   HIP_CHECK(hipMemsetAsync(T.inst.as<uint2>() + I, 0, (size_t)kTileInstPad * 8, s));
   if (f32)
     kTileFill<float><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals32, perm, pos, bIdx, bPos, n, nwords, segBase,
-                                             T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
-                                             T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
+                                             T.segTile.as<uint32_t>(), T.segStart.as<uint32_t>(),
+                                             T.segInst.as<uint32_t>(),
                                              T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
                                              T.idx.as<uint32_t>(), T.vals.as<float>(), T.inst.as<uint2>(),
                                              stageMin);
   else
     kTileFill<double><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals64, perm, pos, bIdx, bPos, n, nwords, segBase,
-                                              T.segTile.as<uint32_t>(), T.segCnt.as<uint32_t>(),
-                                              T.segStart.as<uint32_t>(), T.segInst.as<uint32_t>(),
+                                              T.segTile.as<uint32_t>(), T.segStart.as<uint32_t>(),
+                                              T.segInst.as<uint32_t>(),
                                               T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
                                               T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>(),
                                               stageMin);

@@ -197,19 +197,27 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     by column tile, one launch per vector), the row-ordered relabelled copy
     ("rows": one vector, or the SpMM of two), both with same-beta columns
     gathered from a beta-major copy of x, every column alpha-major
-    ("alpha"), and the default (the tiled copy).
-    Each layout repeats its own result bit for bit, and the tiled copy built
-    from H directly equals the one built from the row copy bit for bit."""
+    ("alpha"), and the default (the tiled copy, large segments gathering
+    from an LDS copy of their x tile; "tiled-global": every gather from
+    global memory).  Each layout repeats its own result bit for bit, staging
+    every segment gives the default's bits, and the tiled copy built from H
+    directly equals the one built from the row copy bit for bit."""
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = {}
-    for layout in ("default", "tiled", "tiled-from-H", "rows", "external", "alpha"):
-        monkeypatch.delenv("SHCI_SIGMA_LAYOUT", raising=False)
-        monkeypatch.delenv("SHCI_SIGMA_TILE_BUILD", raising=False)
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "rows", "external", "alpha"):
+        for var in ("SHCI_SIGMA_LAYOUT", "SHCI_SIGMA_TILE_BUILD", "SHCI_SIGMA_TILE_STAGE", "SHCI_SIGMA_TILE_STAGE_MIN"):
+            monkeypatch.delenv(var, raising=False)
         if layout == "tiled-from-H":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_BUILD", "direct")
+        elif layout == "tiled-global":
+            monkeypatch.setenv("SHCI_SIGMA_TILE_STAGE", "0")
+        elif layout == "tiled":
+            # every segment staged, down to one instruction
+            monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
+            monkeypatch.setenv("SHCI_SIGMA_TILE_STAGE_MIN", "1")
         elif layout != "default":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
@@ -221,15 +229,16 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
     ref = out["external"]
-    for layout in ("default", "tiled", "tiled-from-H", "rows", "alpha"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "rows", "alpha"):
         for k in (0, 1):
             assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
-    # the default is the tiled copy
-    np.testing.assert_array_equal(out["default"][1], out["tiled"][1])
-    # the tiled copy built from H through the permutation (when the row copy
-    # does not fit beside it) holds the same entries in the same order
+    # the staged kernel's sums do not depend on which segments it stages
+    # (the default stages large ones, "tiled" all of them); the tiled copy
+    # built from H through the permutation (when the row copy does not fit
+    # beside it) holds the same entries in the same order
     for k in (0, 1):
-        np.testing.assert_array_equal(out["tiled-from-H"][k], out["tiled"][k])
+        np.testing.assert_array_equal(out["default"][k], out["tiled"][k])
+        np.testing.assert_array_equal(out["tiled-from-H"][k], out["default"][k])
 
 
 @pytest.mark.parametrize("name", ["c0", "w2"])
