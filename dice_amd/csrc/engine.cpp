@@ -844,11 +844,10 @@ void Engine::buildPermuted() {
   const size_t have = freeB + permCols.bytes + permVals.bytes + permX.bytes + bPos.bytes + tiled.idx.bytes +
                       tiled.vals.bytes + tiled.inst.bytes;
   // the tiled copy is built from the row-ordered one when both fit beside H;
-  // when only the tiled copy fits (and the SpMM does not need the row copy)
-  // it is built from H directly through the permutation (a slower build)
+  // when only the tiled copy fits it is built from H directly through the
+  // permutation (a slower build)
   const bool viaRows = wantTiled && !tiledFromH && base + rowsCopy + tiledCopy + xPlain <= have;
-  const bool fromH =
-      wantTiled && !viaRows && base + tiledCopy + xPlain <= have;
+  const bool fromH = wantTiled && !viaRows && base + tiledCopy + xPlain <= have;
   if (!fromH && base + rowsCopy + xPlain > have) {
     permCols.release();
     permVals.release();
@@ -909,8 +908,7 @@ void Engine::buildPermuted() {
       // for the next build (mapping them again costs more than the copy)
       // unless memory is short
       permRows = -1;
-      size_t totalB = 0;
-      if (deviceFreeBytes(&totalB) < totalB / 4) {
+      if (deviceFreeBytes() < totalB / 4) {
         HIP_CHECK(hipStreamSynchronize(stream));
         permCols.release();
         permVals.release();

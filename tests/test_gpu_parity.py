@@ -206,14 +206,17 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = {}
-    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "rows", "external", "alpha"):
-        for var in ("SHCI_SIGMA_LAYOUT", "SHCI_SIGMA_TILE_BUILD", "SHCI_SIGMA_TILE_STAGE", "SHCI_SIGMA_TILE_STAGE_MIN"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "external", "alpha"):
+        for var in ("SHCI_SIGMA_LAYOUT", "SHCI_SIGMA_TILE_BUILD", "SHCI_SIGMA_TILE_STAGE", "SHCI_SIGMA_TILE_STAGE_MIN",
+                    "SHCI_SIGMA_TILE_PERSISTENT"):
             monkeypatch.delenv(var, raising=False)
         if layout == "tiled-from-H":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_BUILD", "direct")
         elif layout == "tiled-global":
             monkeypatch.setenv("SHCI_SIGMA_TILE_STAGE", "0")
+        elif layout == "tiled-persistent":
+            monkeypatch.setenv("SHCI_SIGMA_TILE_PERSISTENT", "1")
         elif layout == "tiled":
             # every segment staged, down to one instruction
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
@@ -229,7 +232,7 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
     ref = out["external"]
-    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "rows", "alpha"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "alpha"):
         for k in (0, 1):
             assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
     # the staged kernel's sums do not depend on which segments it stages
@@ -239,6 +242,9 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     for k in (0, 1):
         np.testing.assert_array_equal(out["default"][k], out["tiled"][k])
         np.testing.assert_array_equal(out["tiled-from-H"][k], out["default"][k])
+        # persistent workgroups take the blocks in another order, each block
+        # summed the same way
+        np.testing.assert_array_equal(out["tiled-persistent"][k], out["default"][k])
 
 
 @pytest.mark.parametrize("name", ["c0", "w2"])
