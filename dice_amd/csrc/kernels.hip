@@ -1753,10 +1753,13 @@ __global__ void __launch_bounds__(kBlock) kSigmaLower(const int64_t* __restrict_
 // 128-byte x lines per entry, a 256-row block's 0.038.  Within a segment
 // the entries are ordered by the wave that placed them (row mod 4), then by
 // row, then in the row's own order, so each row is one contiguous run; an
-// instruction's products are summed per row by a keyed segmented scan
-// across lanes and added into that wave's partial row sums in LDS, and the
-// four partials are added in a fixed order at the end of the block: the
-// result does not depend on scheduling.
+// instruction's products are summed per row by a segmented scan across
+// lanes (each entry carries its distance to its run's first entry) and
+// added into the executing wave's partial row sums in LDS, and the four
+// partials are added in a fixed order at the end of the block.  Which wave
+// takes which instructions is fixed by the layout, so the result does not
+// depend on scheduling.  Segments of many instructions can gather from a
+// copy of their x tile in LDS instead of from L1 (kSigmaTiled, STAGE).
 constexpr int kTileRows = 256;
 constexpr int kTileLog = 10;
 constexpr int kTileCols = 1 << kTileLog;
@@ -1767,8 +1770,9 @@ constexpr int kTileLdsSegs = 1024;  // blocks with at most this many segments co
 // entry of its row run within the instruction (bits 10-15), row in block
 // (bits 16-23), last entry of its run within the instruction (bit 31)
 constexpr int kTileDistShift = 10;
-// an instruction descriptor's second word: entry count (bits 0-6), tile
-// (bits 7-30), bit 31 set when its segment has at least stageMin
+// an instruction descriptor: its first entry (offset within the block),
+// and a second word with the entry count (bits 0-6), the tile (bits 7-30),
+// and bit 31 set when its segment has at least stageMin
 // instructions (the staged kernel copies such a segment's x tile to LDS)
 constexpr uint32_t kTileDescMask = 0xffffffu;
 
