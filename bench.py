@@ -412,20 +412,23 @@ def cpu_final_space(shci, full, args):
     nnz_lower = G.make_hamiltonian()
     gpu_build = time.time() - t0
     gpu_sigma = time_sigma_device(shci, G._h, n, 1, 20)
+    x = np.random.default_rng(2).normal(size=n)
+    y_gpu = G.hmult(x)
     G.close()
     del r
     o = OracleSystem(fcidump)
     t0 = time.time()
     o.csr_append(dets)
     cpu_build = time.time() - t0
-    x = np.random.default_rng(2).normal(size=n)
-    o.sigma(x)
+    y_cpu = o.sigma(x)
     t0 = time.time()
     reps = 3
     for _ in range(reps):
         o.sigma(x)
     cpu_sigma = (time.time() - t0) / reps
     del o
+    # the two products on the full space: a parity check at the workload's size
+    sigma_rel_diff = float(np.max(np.abs(y_gpu - y_cpu)) / np.max(np.abs(y_cpu)))
     print("# final space %d dets: build CPU %.2f s / GPU %.3f s; sigma CPU %.3f s / GPU %.2f ms (%d threads)"
           % (n, cpu_build, gpu_build, cpu_sigma, 1e3 * gpu_sigma, threads), file=sys.stderr, flush=True)
     return {"what": "the reference's Hamiltonian build (MakeHfromSMHelpers2, rows round-robin) and sigma "
@@ -435,7 +438,8 @@ def cpu_final_space(shci, full, args):
             "ndets": n, "nnz_lower": nnz_lower, "cores": threads, "host_nproc": os.cpu_count(),
             "cpu_build_s": round(cpu_build, 3), "gpu_build_s": round(gpu_build, 4),
             "cpu_sigma_s": round(cpu_sigma, 4), "gpu_sigma_s": round(gpu_sigma, 6),
-            "sigma_speedup": round(cpu_sigma / gpu_sigma, 1), "build_speedup": round(cpu_build / gpu_build, 1)}
+            "sigma_speedup": round(cpu_sigma / gpu_sigma, 1), "build_speedup": round(cpu_build / gpu_build, 1),
+            "sigma_max_rel_diff_gpu_vs_cpu": sigma_rel_diff}
 
 
 def pt_measure(shci, full, sample, log, args):
