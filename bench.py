@@ -200,14 +200,15 @@ C4_EPS1 = 2e-2   # the learned form on the 40-orbital system (see c4_form)
 C4_NITER = 3
 
 
-def c3_form(shci, args):
+def c3_form(shci, args, half_only=False):
     """The 40-orbital system (20 alpha + 20 beta electrons, 160 Cholesky
     vectors, random stream 1003, two-word determinants) with two roots: one
     variational step of C3_NITER iterations at eps1 C3_EPS1 from HF, in FP64
     and in the FP32 form (the reference's single-vector Davidson).  The
     configured schedule (down to 2e-5) selects far more than one GPU holds
     on this synthetic construction; C3_EPS1 is the largest-space schedule
-    measured to fit and finish within the benchmark's time."""
+    measured to fit and finish within the benchmark's time.  half_only: the
+    FP64 run with half storage (the lower triangle alone) instead."""
     from dice_amd import synth
     d = os.path.join(tempfile.gettempdir(), "shci_bench_c3")
     os.makedirs(d, exist_ok=True)
@@ -221,10 +222,12 @@ def c3_form(shci, args):
     log = os.path.join(d, "gpu.log")
     res = {"workload": "40-orbital synthetic FCIDUMP (20a+20b, 160 Cholesky vectors, stream 1003), 2 roots, "
                        "heat-bath eps1 %.0e x %d iterations from HF" % (C3_EPS1, C3_NITER)}
-    for tag, prec in (("fp64", shci.PREC_FP64), ("fp32", shci.PREC_FP32)):
+    runs = ((("fp64_half_storage", shci.PREC_FP64, True),) if half_only else
+            (("fp64", shci.PREC_FP64, False), ("fp32", shci.PREC_FP32, False)))
+    for tag, prec, half in runs:
         print("# C3 %s" % tag, file=sys.stderr, flush=True)
         t0 = time.time()
-        r = shci.do_variational(inp, log, precision=prec, profile_sigma=True)
+        r = shci.do_variational(inp, log, precision=prec, profile_sigma=True, half_storage=half)
         wall = time.time() - t0
         ph, its = r.phases(), r.iterations()
         calls, ms, nbytes = r.sigma_profile()
@@ -243,6 +246,11 @@ def c3_form(shci, args):
         del r
         print("# C3 %s: %.1f s, %d dets, E %s" % (tag, wall, res[tag]["final_ndets"], res[tag]["energies"]),
               file=sys.stderr, flush=True)
+    if half_only:
+        res["fp64_half_storage"]["description"] = (
+            "the lower triangle alone (the reference's layout, half the matrix bytes), sigma in determinant order "
+            "with FP64 atomics for the transposed half; sigma bytes count the stored lower triangle")
+        return res
     res["fp32"]["energy_error_vs_fp64_Ha"] = [a - b for a, b in zip(res["fp32"]["energies"], res["fp64"]["energies"])]
     res["fp32"]["speedup_vs_fp64"] = round(res["fp64"]["step_s"] / res["fp32"]["step_s"], 3)
     return res
@@ -533,10 +541,10 @@ def main():
     ap.add_argument("--post-variational", action="store_true",
                     help="only the steps after the variational one (PT, RDMs) on the final space, printed as "
                          "their own JSON object (for profiling them)")
-    ap.add_argument("--form", choices=["c0", "c2", "c2big", "c3", "c4"],
+    ap.add_argument("--form", choices=["c0", "c2", "c2big", "c3", "c3half", "c4"],
                     help="only that configuration's line (c0: 16 orbitals; c2: the Hubbard batching form; c2big: its "
-                         "fourth iteration; c3: 40 orbitals; c4: the learned form on them), printed as its own JSON "
-                         "object (for profiling)")
+                         "fourth iteration; c3: 40 orbitals; c3half: the same in FP64 with half storage; c4: the "
+                         "learned form on them), printed as its own JSON object (for profiling)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="block Davidson forms without hipGraph replay (rocprofv3's kernel tracer does not "
                          "survive graph launches)")
@@ -556,7 +564,7 @@ def main():
         return
     if args.form:
         fn = {"c0": c0_form, "c2": batching_form, "c2big": batching_fourth_iteration, "c3": c3_form,
-              "c4": c4_form}[args.form]
+              "c3half": lambda shci, args: c3_form(shci, args, half_only=True), "c4": c4_form}[args.form]
         print(json.dumps({args.form: fn(shci, args)}))
         return
     results = {}
