@@ -47,6 +47,9 @@ void Engine::initShared(const Engine& parent) {
   tiledStep = parent.tiledStep;
   tiledPersistent = parent.tiledPersistent;
   lowerOnly = parent.lowerOnly;
+  lowerTiled = parent.lowerTiled;
+  lowerHybrid = parent.lowerHybrid;
+  lowerStageMin = parent.lowerStageMin;
   autotune = parent.autotune;
   davidsonMode = parent.davidsonMode;
   sigmaCfg[0] = parent.sigmaCfg[0];
@@ -214,6 +217,14 @@ void Engine::loadSystem(HostSystem&& sys) {
   // SHCI_STORAGE=half: half storage from the first build
   const char* storage = std::getenv("SHCI_STORAGE");
   if (storage && std::strcmp(storage, "half") == 0) lowerOnly = true;
+  // SHCI_HALF_SIGMA=rows: half storage's sigma on the stored rows (no tiled
+  // copy); =lds: every segment sums its transposed products in LDS first;
+  // SHCI_HALF_STAGE_MIN=k: segments of at least k instructions do
+  if (const char* p = std::getenv("SHCI_HALF_SIGMA")) {
+    lowerTiled = std::strcmp(p, "rows") != 0;
+    lowerHybrid = std::strcmp(p, "lds") != 0;
+  }
+  if (const char* p = std::getenv("SHCI_HALF_STAGE_MIN")) lowerStageMin = std::max(1, std::atoi(p));
   uploadRestrictions();
   counter.reserve(64);
   // the allocation-free sigma's fixed scratch: its address on this device
@@ -767,6 +778,10 @@ void Engine::buildHamiltonian() {
 // determinants have larger indices).
 void Engine::buildLower(int64_t nOld) {
   const bool f32 = precision == 1;
+  // the tiled copy goes first: the stored rows may have to move to grow
+  tiled.release();
+  tiledRows = -1;
+  splitRows = -1;
   buildStringTables(S, dets.as<uint64_t>(), n, T, ws, stream);
   rowPtrNew.reserve((size_t)(n + 1) * 8);
   int64_t* cnt = rowPtrNew.as<int64_t>();
@@ -809,9 +824,42 @@ void Engine::buildLower(int64_t nOld) {
   nnzOff = total;
   hamRows = n;
   permRows = -1;
-  splitRows = -1;
-  tiledRows = -1;
+  buildLowerTiled();
   HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// The tiled copy of the stored half, built from it through the alpha-major
+// relabelling (and the split columns when their x copy fits), when it fits
+// beside it with the permuted x and the transposed sums.
+void Engine::buildLowerTiled() {
+  tiledRows = -1;
+  splitRows = -1;
+  if (!relabel || !tiledOn || !lowerTiled || n == 0) return;
+  const bool f32 = precision == 1;
+  const size_t xt = f32 ? 4 : 8;
+  const size_t tiledCopy = tiledHamBytes(n, nnzOff, f32);
+  const size_t base = (size_t)n * 4 + (size_t)n * 64;
+  const size_t xPlain = (size_t)n * (xt + 8), xSplit = 2 * (size_t)n * (xt + 8) + (size_t)n * 4;
+  const size_t have = deviceFreeBytes() + permX.bytes + lowerYT.bytes + bPos.bytes;
+  if (base + tiledCopy + xPlain > have) return;
+  const uint32_t* perm = T.aMajorDet.as<uint32_t>();
+  permPos.reserve((size_t)n * 4);
+  launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
+  const bool doSplit = split && 2 * n < 0x7fffffffLL && base + tiledCopy + xSplit <= have;
+  if (doSplit) {
+    bPos.reserve((size_t)n * 4);
+    launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);
+  }
+  if (!buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
+                     f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(),
+                     doSplit ? T.bIdx.as<uint32_t>() : nullptr, doSplit ? bPos.as<uint32_t>() : nullptr, n, nnzOff,
+                     ws, stream, false, lowerStageMin)) {
+    tiled.release();
+    return;
+  }
+  lowerYT.reserve((size_t)(doSplit ? 2 * n : n) * 8);
+  tiledRows = n;
+  if (doSplit) splitRows = n;
 }
 
 // Alpha-major relabelled copies of H (FP64 values, or the FP32 hi part) for
@@ -1211,6 +1259,17 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   }
   if (direct) {
     launchSigmaDirect(S, dets.as<uint64_t>(), T, diag.as<double>(), dx, dy, k, ld, stream);
+    if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
+    return;
+  }
+  if (lowerOnly && tiledRows == n && !(f32 && exact)) {
+    const bool sp = splitRows == n;
+    const XScratch xs{&permX};
+    for (int r = 0; r < k; r++)
+      launchSigmaTiledSym(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(),
+                          sp ? T.bMajorDet.as<uint32_t>() : nullptr, permPos.as<uint32_t>(),
+                          sp ? bPos.as<uint32_t>() : nullptr, dx + r * ld, dy + r * ld, xs, lowerYT.as<double>(),
+                          lowerHybrid, stream);
     if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
     return;
   }

@@ -90,6 +90,15 @@ struct Engine {
   // half with atomics.
   bool lowerOnly = false;
   void buildLower(int64_t nOld);
+  // with half storage, the tiled copy holds the stored half (built from it
+  // when both fit) and sigma runs kSigmaTiledSym on it, its transposed sums
+  // gathered in lowerYT (permuted order); the allocation-free sigma and the
+  // FP32 form's exact sigma keep the row kernel
+  bool lowerTiled = true;
+  bool lowerHybrid = true;  // small segments add their transposed products directly
+  int lowerStageMin = 4;    // instructions that make a segment large (2: 3.60, 4: 3.29, 8: 3.66 ms)
+  DevMem lowerYT;
+  void buildLowerTiled();
 
   // Davidson scratch
   DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;

@@ -285,6 +285,14 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
 void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                       const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
                       hipStream_t s, int step = 8, bool stage = false);
+// y = H x through a tiled copy of half-stored H (each pair once): the copy's
+// entries applied as stored and transposed; yT: scratch of nx doubles (the
+// permuted x's length), pos / bPos: determinant -> alpha-major / beta-major
+// position.  The transposed sums are added with atomics (no fixed order);
+// hybrid: only large segments sum them in LDS first.
+void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
+                         const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
+                         double* yT, bool hybrid, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

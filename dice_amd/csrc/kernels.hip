@@ -2210,6 +2210,122 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
   }
 }
 
+// Half storage over the tiled copy: the copy holds each stored pair (I, J)
+// once, in row I's block and J's column tile.  y[perm[m]] = diag .* x +
+// (lower rows) x as kSigmaTiled computes it, and the transposed half adds
+// v x[row] into yT[col] (yT in the permuted order of x, cleared before):
+// each wave sums its current tile's transposed contributions in LDS with
+// LDS atomics and adds the nonzero ones into yT with global FP64 atomics
+// when its tile changes (HYB: only for segments with the large-segment bit;
+// the others add each product into yT directly).  Each wave takes a
+// contiguous quarter of its block's instructions.  The transposed sums
+// arrive in no fixed order.
+template <typename VT, typename XT, int U, bool HYB>
+__global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict__ inst,
+                                                         const int64_t* __restrict__ blockI,
+                                                         const int64_t* __restrict__ blockE,
+                                                         const uint32_t* __restrict__ tIdx,
+                                                         const VT* __restrict__ tVal, const double* __restrict__ diag,
+                                                         const uint32_t* __restrict__ perm, int64_t n,
+                                                         const XT* __restrict__ xP, const double* __restrict__ x,
+                                                         double* __restrict__ y, double* yT) {
+  __shared__ double yw[kWaves][kTileRows];
+  __shared__ double ya[kWaves][kTileCols];  // the wave's current tile of transposed sums
+  __shared__ double xr[kTileRows];          // x of the block's rows
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
+  static_assert(U <= kTileInstPad, "step larger than the descriptor padding");
+  int c15, c31;
+  tileScanConsts(c15, c31);
+  const int64_t b = blockIdx.x;
+  const int64_t m0 = b * kTileRows;
+  for (int j = lane; j < kTileRows; j += 64) yw[wave][j] = 0.0;
+  for (int j = lane; j < kTileCols; j += 64) ya[wave][j] = 0.0;
+  for (int t = threadIdx.x; t < kTileRows; t += kBlock) xr[t] = m0 + t < n ? (double)xP[m0 + t] : 0.0;
+  __syncthreads();
+  uint32_t cur = 0xffffffffu;
+  auto flush = [&](uint32_t tile) {
+    for (int j = lane; j < kTileCols; j += 64) {
+      const double a = ya[wave][j];
+      if (a != 0.0) {
+        unsafeAtomicAdd(&yT[((size_t)tile << kTileLog) + j], a);
+        ya[wave][j] = 0.0;
+      }
+    }
+  };
+  const int64_t i0 = blockI[b], i1 = blockI[b + 1], e0 = blockE[b];
+  const int64_t len = i1 - i0;
+  const int64_t w0 = i0 + len * wave / kWaves, wEnd = i0 + len * (wave + 1) / kWaves;
+  for (int64_t i = w0; i < wEnd; i += U) {
+    uint32_t dx[U], dy[U];
+    const uint2* dp = inst + i;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint2 d = dp[u];
+      dx[u] = d.x;
+      dy[u] = i + u < wEnd ? d.y : (d.y & ~127u);  // count 0 past the range
+    }
+    uint32_t ix[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool valid = (uint32_t)lane < (dy[u] & 127u);
+      const int64_t p = e0 + (valid ? (int64_t)dx[u] + lane : 0);
+      ix[u] = __builtin_nontemporal_load(tIdx + p);
+      v[u] = (double)__builtin_nontemporal_load(tVal + p);
+    }
+    double xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool valid = (uint32_t)lane < (dy[u] & 127u);
+      const size_t tb = (size_t)((dy[u] >> 7) & kTileDescMask) << kTileLog;
+      xv[u] = (double)xP[tb + (valid ? (ix[u] & (uint32_t)(kTileCols - 1)) : 0u)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if ((dy[u] & 127u) == 0) continue;
+      const bool valid = (uint32_t)lane < (dy[u] & 127u);
+      const uint32_t row = (ix[u] >> 16) & (kTileRows - 1);
+      const double sum = segInclusiveScan(v[u] * xv[u], (int)((ix[u] >> kTileDistShift) & 63u), c15, c31);
+      if (valid && (int)ix[u] < 0) yw[wave][row] += sum;
+      const uint32_t tile = (dy[u] >> 7) & kTileDescMask;
+      const uint32_t col = ix[u] & (uint32_t)(kTileCols - 1);
+      const double t = v[u] * xr[row];
+      if (HYB && !(dy[u] >> 31)) {
+        if (valid && t != 0.0) unsafeAtomicAdd(&yT[((size_t)tile << kTileLog) + col], t);
+        continue;
+      }
+      if (tile != cur) {
+        if (cur != 0xffffffffu) flush(cur);
+        cur = tile;
+      }
+      if (valid) unsafeAtomicAdd(&ya[wave][col], t);
+    }
+  }
+  if (cur != 0xffffffffu) flush(cur);
+  __syncthreads();
+  for (int t = threadIdx.x; t < kTileRows; t += kBlock) {
+    const int64_t m = m0 + t;
+    if (m < n) {
+      const int64_t I = perm[m];
+      y[I] = fma(diag[I], x[I], ((yw[0][t] + yw[1][t]) + yw[2][t]) + yw[3][t]);
+    }
+  }
+}
+
+// y[I] += yT[pos[I]] (+ yT[n + bPos[I]] with split columns)
+__global__ void kAddTransposed(double* y, const double* yT, const uint32_t* pos, const uint32_t* bPos, int64_t n) {
+  for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
+    double t = yT[pos[I]];
+    if (bPos) t += yT[n + bPos[I]];
+    y[I] += t;
+  }
+}
+
+__global__ void kFillZero(double* v, int64_t m) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0;
+}
+
 // SpMM: K interleaved vectors per gather (one K*sizeof(XT)-byte load per
 // stored element serves all K products); rows read once.
 template <typename XT, int K>
@@ -3742,6 +3858,33 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
     else
       sigmaTiledU<double, double, false>(step, stage, grid, T, diag, perm, n, xs, nx, x, y, counter, s);
   }
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
+                         const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
+                         double* yT, bool hybrid, hipStream_t s) {
+  const int64_t n = T.n;
+  if (n <= 0) return;
+  if (T.split != (bPerm != nullptr) || T.split != (bPos != nullptr))
+    throw HipError("tiled sigma: split columns need the beta-major order");
+  const int64_t nx = bPerm ? 2 * n : n;
+  const unsigned grid = (unsigned)T.blocks;
+  kFillZero<<<gridFor(nx), kBlock, 0, s>>>(yT, nx);
+  if (T.f32) {
+    float* xs = (float*)xP.get((size_t)nx * 4);
+    kGatherPerm<float, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, nullptr);
+    auto k = hybrid ? kSigmaTiledSym<float, float, 8, true> : kSigmaTiledSym<float, float, 8, false>;
+    k<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(),
+                              T.idx.as<uint32_t>(), T.vals.as<float>(), diag, perm, n, xs, x, y, yT);
+  } else {
+    double* xs = (double*)xP.get((size_t)nx * 8);
+    kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, nullptr);
+    auto k = hybrid ? kSigmaTiledSym<double, double, 8, true> : kSigmaTiledSym<double, double, 8, false>;
+    k<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(),
+                              T.idx.as<uint32_t>(), T.vals.as<double>(), diag, perm, n, xs, x, y, yT);
+  }
+  kAddTransposed<<<gridFor(n), kBlock, 0, s>>>(y, yT, pos, bPos, n);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -2,7 +2,9 @@
 lower-triangular rows without the diagonal, SHCI/SHCImakeHamiltonian.cpp:549-711),
 the layout for spaces whose two-triangle Hamiltonian does not fit.  Sigma
 applies every stored element to both rows as Hmult2 does
-(SHCI/Hmult.h:110-121), the transposed half by atomic adds.  Bar: the rows
+(SHCI/Hmult.h:110-121), the transposed half by atomic adds (on the tiled
+copy of the stored half by default, summed per column tile in LDS for large
+segments or everywhere, or on the stored rows).  Bar: the rows
 equal the two-triangle build's lower part bit for bit; sigma equals the
 oracle's to 1e-12 relative (its sums are in no fixed order); runs give the
 oracle's determinant sets and energies to 1e-10 Ha."""
@@ -31,8 +33,9 @@ def _space(path, occ, n, W):
     return dets[:n]
 
 
+@pytest.mark.parametrize("mode", ["tiled", "lds", "lds-all", "rows"])
 @pytest.mark.parametrize("case", ["c0", "w2"])
-def test_half_storage_rows_and_sigma(case, tmp_path, monkeypatch):
+def test_half_storage_rows_and_sigma(case, mode, tmp_path, monkeypatch):
     from dice_amd import shci, synth
     path = str(tmp_path / "FCIDUMP")
     if case == "c0":
@@ -46,6 +49,10 @@ def test_half_storage_rows_and_sigma(case, tmp_path, monkeypatch):
     full.set_dets(dets)
     rp, cols, vals = full.hamiltonian_rows()
     monkeypatch.setenv("SHCI_STORAGE", "half")
+    if mode != "tiled":
+        monkeypatch.setenv("SHCI_HALF_SIGMA", "rows" if mode == "rows" else "lds")
+    if mode == "lds-all":  # every segment sums in LDS, also split over waves
+        monkeypatch.setenv("SHCI_HALF_STAGE_MIN", "1")
     half = shci.System.from_fcidump(path)
     half.set_dets(dets)
     hrp, hcols, hvals = half.hamiltonian_rows()
@@ -61,8 +68,9 @@ def test_half_storage_rows_and_sigma(case, tmp_path, monkeypatch):
         assert np.max(np.abs(y[:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
+@pytest.mark.parametrize("mode", ["tiled", "rows"])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_half_storage_run_matches_oracle(prec, tmp_path, monkeypatch):
+def test_half_storage_run_matches_oracle(prec, mode, tmp_path, monkeypatch):
     """A whole schedule (incremental builds append new rows in place) on a
     12-orbital system: the oracle's determinant set after every iteration and
     its energies (FP64); the FP32 form within 1e-4 Ha with the same space."""
@@ -74,6 +82,8 @@ def test_half_storage_run_matches_oracle(prec, tmp_path, monkeypatch):
                       extra=["prefix %s" % tmp_path])
     o = OracleRun(inp, str(tmp_path / "oracle.log"), record=True)
     monkeypatch.setenv("SHCI_STORAGE", "half")
+    if mode == "rows":
+        monkeypatch.setenv("SHCI_HALF_SIGMA", "rows")
     r = shci.do_variational(inp, str(tmp_path / "gpu.log"), precision=prec)
     assert [x["ndets"] for x in r.iterations()] == [x["ndets"] for x in o.iterations()]
     np.testing.assert_array_equal(r.dets(), o.dets(1))
