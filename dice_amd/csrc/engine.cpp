@@ -48,7 +48,6 @@ void Engine::initShared(const Engine& parent) {
   tiledPersistent = parent.tiledPersistent;
   lowerOnly = parent.lowerOnly;
   lowerTiled = parent.lowerTiled;
-  lowerHybrid = parent.lowerHybrid;
   lowerStageMin = parent.lowerStageMin;
   autotune = parent.autotune;
   davidsonMode = parent.davidsonMode;
@@ -218,12 +217,9 @@ void Engine::loadSystem(HostSystem&& sys) {
   const char* storage = std::getenv("SHCI_STORAGE");
   if (storage && std::strcmp(storage, "half") == 0) lowerOnly = true;
   // SHCI_HALF_SIGMA=rows: half storage's sigma on the stored rows (no tiled
-  // copy); =lds: every segment sums its transposed products in LDS first;
-  // SHCI_HALF_STAGE_MIN=k: segments of at least k instructions do
-  if (const char* p = std::getenv("SHCI_HALF_SIGMA")) {
-    lowerTiled = std::strcmp(p, "rows") != 0;
-    lowerHybrid = std::strcmp(p, "lds") != 0;
-  }
+  // copy); SHCI_HALF_STAGE_MIN=k: segments of at least k instructions sum
+  // their transposed products in LDS first (1: every segment)
+  if (const char* p = std::getenv("SHCI_HALF_SIGMA")) lowerTiled = std::strcmp(p, "rows") != 0;
   if (const char* p = std::getenv("SHCI_HALF_STAGE_MIN")) lowerStageMin = std::max(1, std::atoi(p));
   uploadRestrictions();
   counter.reserve(64);
@@ -797,7 +793,8 @@ void Engine::buildLower(int64_t nOld) {
   if (std::getenv("SHCI_TIMING"))
     std::fprintf(stderr, "# half-storage build n=%lld: %lld stored off-diagonal elements\n", (long long)n,
                  (long long)total);
-  const int64_t added = total - (nOld ? nnzOff : 0);
+  const int64_t nnzOld = nOld ? nnzOff : 0;
+  const int64_t added = total - nnzOld;
   {
     size_t totalB = 0;
     const size_t freeB = deviceFreeBytes(&totalB);
@@ -824,14 +821,15 @@ void Engine::buildLower(int64_t nOld) {
   nnzOff = total;
   hamRows = n;
   permRows = -1;
-  buildLowerTiled();
+  buildLowerTiled(nOld, nnzOld);
   HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // The tiled copy of the stored half, built from it through the alpha-major
 // relabelling (and the split columns when their x copy fits), when it fits
-// beside it with the permuted x and the transposed sums.
-void Engine::buildLowerTiled() {
+// beside it with the permuted x and the transposed sums (nOld, nnzOld: the
+// rows and entries kept from the previous build).
+void Engine::buildLowerTiled(int64_t nOld, int64_t nnzOld) {
   tiledRows = -1;
   splitRows = -1;
   if (!relabel || !tiledOn || !lowerTiled || n == 0) return;
@@ -839,7 +837,7 @@ void Engine::buildLowerTiled() {
   const size_t xt = f32 ? 4 : 8;
   const size_t tiledCopy = tiledHamBytes(n, nnzOff, f32);
   const size_t base = (size_t)n * 4 + (size_t)n * 64;
-  const size_t xPlain = (size_t)n * (xt + 8), xSplit = 2 * (size_t)n * (xt + 8) + (size_t)n * 4;
+  const size_t xPlain = (size_t)n * (xt + 16) + 16, xSplit = 2 * (size_t)n * (xt + 8) + (size_t)n * 12 + 16;
   const size_t have = deviceFreeBytes() + permX.bytes + lowerYT.bytes + bPos.bytes;
   if (base + tiledCopy + xPlain > have) return;
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
@@ -857,7 +855,20 @@ void Engine::buildLowerTiled() {
     tiled.release();
     return;
   }
-  lowerYT.reserve((size_t)(doSplit ? 2 * n : n) * 8);
+  const int64_t nx = doSplit ? 2 * n : n;
+  lowerYT.reserve((size_t)(nx + 1) * 8);
+  // the column sums grow with the half: when they cover exactly the rows
+  // kept from the previous build, only the added rows are summed
+  const bool grow = nOld > 0 && lowerAbsRows == nOld && lowerAbsNnz == nnzOld && lowerAbsF32 == f32;
+  const int64_t r0 = grow ? lowerAbsRows : 0, e0 = grow ? lowerAbsNnz : 0;
+  lowerColAbs.reserve((size_t)n * 8, stream, grow);
+  lowerBound.reserve(8);
+  launchSymColumnBound(cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(), f32 ? vals32.as<float>() : nullptr, e0,
+                       nnzOff, r0, n, lowerColAbs.as<unsigned long long>(), lowerBound.as<unsigned long long>(),
+                       stream);
+  lowerAbsRows = n;
+  lowerAbsNnz = nnzOff;
+  lowerAbsF32 = f32;
   tiledRows = n;
   if (doSplit) splitRows = n;
 }
@@ -1268,8 +1279,8 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     for (int r = 0; r < k; r++)
       launchSigmaTiledSym(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(),
                           sp ? T.bMajorDet.as<uint32_t>() : nullptr, permPos.as<uint32_t>(),
-                          sp ? bPos.as<uint32_t>() : nullptr, dx + r * ld, dy + r * ld, xs, lowerYT.as<double>(),
-                          lowerHybrid, stream);
+                          sp ? bPos.as<uint32_t>() : nullptr, dx + r * ld, dy + r * ld, xs,
+                          lowerYT.as<unsigned long long>(), lowerBound.as<unsigned long long>(), stream);
     if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
     return;
   }
@@ -1312,8 +1323,16 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
   if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
 }
 
+// the allocation-free half-storage sigma on the tiled copy: the permuted x
+// (rounded up to 256 bytes), then the transposed sums and max|x|
+static size_t symXBytes(int64_t n, bool split, bool f32) {
+  return ((size_t)(split ? 2 * n : n) * (f32 ? 4 : 8) + 255) & ~(size_t)255;
+}
+
 size_t Engine::sigmaFixedScratchNeed(int k) const {
-  if (lowerOnly) return 0;
+  if (lowerOnly)
+    return tiledRows == n ? symXBytes(n, splitRows == n, precision == 1) + (size_t)((splitRows == n ? 2 : 1) * n + 1) * 8
+                          : 0;
   const bool tiledSigma = (k == 1 || permRows != n) && tiledRows == n;
   if (permRows != n && !tiledSigma) return 0;  // the unrelabelled sigma needs no scratch
   return sigmaPermScratchBytes(n, splitRows == n, tiledSigma ? 1 : k, precision == 1 ? 4 : 8);
@@ -1324,6 +1343,20 @@ size_t Engine::sigmaFixedScratchNeed(int k) const {
 void Engine::sigmaFixed(const double* dx, double* dy, int k, int64_t ld) {
   if (trev || direct) throw std::invalid_argument("the allocation-free sigma does not run in time-reversal or direct mode");
   const bool f32 = precision == 1;
+  if (lowerOnly && tiledRows == n) {
+    size_t cap = 0;
+    unsigned char* base = (unsigned char*)fixedSigmaScratch(&cap);
+    const bool sp = splitRows == n;
+    const size_t xb = symXBytes(n, sp, f32);
+    const XScratch xs{nullptr, base + 256, xb};
+    auto* yT = (unsigned long long*)(base + 256 + xb);
+    for (int r = 0; r < k; r++)
+      launchSigmaTiledSym(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(),
+                          sp ? T.bMajorDet.as<uint32_t>() : nullptr, permPos.as<uint32_t>(),
+                          sp ? bPos.as<uint32_t>() : nullptr, dx + r * ld, dy + r * ld, xs, yT,
+                          lowerBound.as<unsigned long long>(), stream);
+    return;
+  }
   if (lowerOnly) {
     if (f32)
       launchSigmaLowerF32(rowPtr.as<int64_t>(), cols.as<int32_t>(), vals32.as<float>(), nullptr, diag.as<double>(), n,

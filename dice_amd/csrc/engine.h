@@ -92,13 +92,17 @@ struct Engine {
   void buildLower(int64_t nOld);
   // with half storage, the tiled copy holds the stored half (built from it
   // when both fit) and sigma runs kSigmaTiledSym on it, its transposed sums
-  // gathered in lowerYT (permuted order); the allocation-free sigma and the
-  // FP32 form's exact sigma keep the row kernel
+  // gathered in lowerYT (fixed point, permuted order; then max|x|); the FP32
+  // form's exact sigma keeps the row kernel
   bool lowerTiled = true;
-  bool lowerHybrid = true;  // small segments add their transposed products directly
   int lowerStageMin = 4;    // instructions that make a segment large (2: 3.60, 4: 3.29, 8: 3.66 ms)
   DevMem lowerYT;
-  void buildLowerTiled();
+  // the column sums of |v| (fixed point) over the first lowerAbsRows rows'
+  // entries, grown with the half, and their largest (kSigmaTiledSym's bound)
+  DevMem lowerColAbs, lowerBound;
+  int64_t lowerAbsRows = -1, lowerAbsNnz = 0;
+  bool lowerAbsF32 = false;
+  void buildLowerTiled(int64_t nOld, int64_t nnzOld);
 
   // Davidson scratch
   DevMem dvB, dvS, dvR, dvScal, dvPart, dvG, dvW, dvV, dvX;

@@ -286,13 +286,22 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
                       const double* x, double* y, const XScratch& xP, bool persistent, unsigned* counter,
                       hipStream_t s, int step = 8, bool stage = false);
 // y = H x through a tiled copy of half-stored H (each pair once): the copy's
-// entries applied as stored and transposed; yT: scratch of nx doubles (the
-// permuted x's length), pos / bPos: determinant -> alpha-major / beta-major
-// position.  The transposed sums are added with atomics (no fixed order);
-// hybrid: only large segments sum them in LDS first.
+// entries applied as stored and transposed; pos / bPos: determinant ->
+// alpha-major / beta-major position; yT: nx + 1 words of scratch (nx: the
+// permuted x's length); bound: the column bound set by launchSymColumnBound.
+// The transposed sums are 64-bit fixed point, summed with integer atomics:
+// the result does not depend on their order.
 void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                          const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
-                         double* yT, bool hybrid, hipStream_t s);
+                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s);
+// kSigmaTiledSym's column bound (the largest column sum of |v| over the
+// stored half) into *bound, from the column sums in acc (n words): the
+// entries [nnzOld, nnz) of a half grown from nOld rows are added to the
+// sums of the first nOld columns (entries, columns and values: vals64 or
+// vals32, as stored; nOld = nnzOld = 0 for a new half).
+void launchSymColumnBound(const int32_t* cols, const double* vals64, const float* vals32, int64_t nnzOld,
+                          int64_t nnz, int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound,
+                          hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -2213,42 +2213,54 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
 // Half storage over the tiled copy: the copy holds each stored pair (I, J)
 // once, in row I's block and J's column tile.  y[perm[m]] = diag .* x +
 // (lower rows) x as kSigmaTiled computes it, and the transposed half adds
-// v x[row] into yT[col] (yT in the permuted order of x, cleared before):
-// each wave sums its current tile's transposed contributions in LDS with
-// LDS atomics and adds the nonzero ones into yT with global FP64 atomics
-// when its tile changes (HYB: only for segments with the large-segment bit;
-// the others add each product into yT directly).  Each wave takes a
-// contiguous quarter of its block's instructions.  The transposed sums
-// arrive in no fixed order.
-template <typename VT, typename XT, int U, bool HYB>
+// v x[row] into column col's sum.  The transposed sums are kept in 64-bit
+// fixed point, so that their order does not matter and the result is the
+// same on every run: a product t becomes the integer nearest t 2^-e, with
+// 2^(e+61) above B max|x| (B: the largest column sum of |H_ij| over the
+// stored half), so no column's sum of integers reaches 2^62 in magnitude,
+// and each integer is off by at most 2^(e-1) (below 2^-61 B max|x|).  Each
+// wave sums its current tile's integers in LDS (segments with the
+// large-segment bit; the others add theirs into yT directly) and adds the
+// nonzero sums into yT with 64-bit integer atomics when its tile changes.
+// Each wave takes a contiguous quarter of its block's instructions.
+// yT: nx sums (permuted order of x, cleared first), then the bits of
+// max|x|; bound: the bits of B.
+__device__ __forceinline__ int symFixExp(const unsigned long long* yT, int64_t nx, const unsigned long long* bound) {
+  const double S = __longlong_as_double((long long)yT[nx]) * __longlong_as_double((long long)*bound);
+  return S > 0.0 ? ilogb(S) + 1 - 61 : 0;
+}
+
+template <typename VT, typename XT, int U>
 __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict__ inst,
                                                          const int64_t* __restrict__ blockI,
                                                          const int64_t* __restrict__ blockE,
                                                          const uint32_t* __restrict__ tIdx,
                                                          const VT* __restrict__ tVal, const double* __restrict__ diag,
-                                                         const uint32_t* __restrict__ perm, int64_t n,
+                                                         const uint32_t* __restrict__ perm, int64_t n, int64_t nx,
                                                          const XT* __restrict__ xP, const double* __restrict__ x,
-                                                         double* __restrict__ y, double* yT) {
+                                                         double* __restrict__ y, unsigned long long* yT,
+                                                         const unsigned long long* __restrict__ bound) {
   __shared__ double yw[kWaves][kTileRows];
-  __shared__ double ya[kWaves][kTileCols];  // the wave's current tile of transposed sums
-  __shared__ double xr[kTileRows];          // x of the block's rows
+  __shared__ unsigned long long ya[kWaves][kTileCols];  // the wave's current tile of transposed sums
+  __shared__ double xr[kTileRows];                      // x of the block's rows, times 2^-e
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
   static_assert(U <= kTileInstPad, "step larger than the descriptor padding");
   int c15, c31;
   tileScanConsts(c15, c31);
+  const int e = symFixExp(yT, nx, bound);
   const int64_t b = blockIdx.x;
   const int64_t m0 = b * kTileRows;
   for (int j = lane; j < kTileRows; j += 64) yw[wave][j] = 0.0;
-  for (int j = lane; j < kTileCols; j += 64) ya[wave][j] = 0.0;
-  for (int t = threadIdx.x; t < kTileRows; t += kBlock) xr[t] = m0 + t < n ? (double)xP[m0 + t] : 0.0;
+  for (int j = lane; j < kTileCols; j += 64) ya[wave][j] = 0ull;
+  for (int t = threadIdx.x; t < kTileRows; t += kBlock) xr[t] = m0 + t < n ? ldexp((double)xP[m0 + t], -e) : 0.0;
   __syncthreads();
   uint32_t cur = 0xffffffffu;
   auto flush = [&](uint32_t tile) {
     for (int j = lane; j < kTileCols; j += 64) {
-      const double a = ya[wave][j];
-      if (a != 0.0) {
-        unsafeAtomicAdd(&yT[((size_t)tile << kTileLog) + j], a);
-        ya[wave][j] = 0.0;
+      const unsigned long long a = ya[wave][j];
+      if (a != 0ull) {
+        atomicAdd(&yT[((size_t)tile << kTileLog) + j], a);
+        ya[wave][j] = 0ull;
       }
     }
   };
@@ -2289,16 +2301,16 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
       if (valid && (int)ix[u] < 0) yw[wave][row] += sum;
       const uint32_t tile = (dy[u] >> 7) & kTileDescMask;
       const uint32_t col = ix[u] & (uint32_t)(kTileCols - 1);
-      const double t = v[u] * xr[row];
-      if (HYB && !(dy[u] >> 31)) {
-        if (valid && t != 0.0) unsafeAtomicAdd(&yT[((size_t)tile << kTileLog) + col], t);
+      const unsigned long long q = (unsigned long long)__double2ll_rn(v[u] * xr[row]);
+      if (!(dy[u] >> 31)) {
+        if (valid && q != 0ull) atomicAdd(&yT[((size_t)tile << kTileLog) + col], q);
         continue;
       }
       if (tile != cur) {
         if (cur != 0xffffffffu) flush(cur);
         cur = tile;
       }
-      if (valid) unsafeAtomicAdd(&ya[wave][col], t);
+      if (valid) atomicAdd(&ya[wave][col], q);
     }
   }
   if (cur != 0xffffffffu) flush(cur);
@@ -2312,18 +2324,65 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
   }
 }
 
-// y[I] += yT[pos[I]] (+ yT[n + bPos[I]] with split columns)
-__global__ void kAddTransposed(double* y, const double* yT, const uint32_t* pos, const uint32_t* bPos, int64_t n) {
+// y[I] += 2^e (yT[pos[I]] (+ yT[n + bPos[I]] with split columns))
+__global__ void kAddTransposed(double* y, const unsigned long long* yT, const unsigned long long* bound,
+                               const uint32_t* pos, const uint32_t* bPos, int64_t n, int64_t nx) {
+  const int e = symFixExp(yT, nx, bound);
   for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
-    double t = yT[pos[I]];
-    if (bPos) t += yT[n + bPos[I]];
-    y[I] += t;
+    long long t = (long long)yT[pos[I]];
+    if (bPos) t += (long long)yT[n + bPos[I]];
+    y[I] += ldexp((double)t, e);
   }
 }
 
-__global__ void kFillZero(double* v, int64_t m) {
+__global__ void kFillZeroU64(unsigned long long* v, int64_t m) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = 0.0;
+    v[i] = 0ull;
+}
+
+// xP = x in the permuted order (as kGatherPerm for one vector), and the
+// bits of max|x| into *mx (cleared first; the bits of non-negative doubles
+// order as their values)
+template <typename XT>
+__global__ void kGatherPermMax(const double* __restrict__ x, const uint32_t* __restrict__ perm,
+                               const uint32_t* __restrict__ bPerm, int64_t n, XT* __restrict__ xP,
+                               unsigned long long* mx) {
+  const int64_t nx = bPerm ? 2 * n : n;
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nx; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[i < n ? perm[i] : bPerm[i - n]];
+    xP[i] = (XT)v;
+    m = fmax(m, fabs(v));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (__lane_id() == 0 && m > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
+}
+
+// column sums of |v| over stored entries (columns J in determinant order),
+// each |v| rounded up to a multiple of 2^-32 and summed as integers
+// (exactly, so in any order), added into acc
+template <typename VT>
+__global__ void kColAbsSum(const int32_t* __restrict__ cols, const VT* __restrict__ vals, int64_t nnz,
+                           unsigned long long* acc) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&acc[cols[p]], (unsigned long long)ceil(ldexp(fabs((double)vals[p]), 32)));
+}
+
+__global__ void kMaxU64(const unsigned long long* __restrict__ a, int64_t n, unsigned long long* out) {
+  unsigned long long m = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = a[i] > m ? a[i] : m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = __shfl_xor(m, o, 64);
+    m = t > m ? t : m;
+  }
+  if (__lane_id() == 0 && m) atomicMax(out, m);
+}
+
+__global__ void kU64ScaledToDouble(unsigned long long* v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *v = (unsigned long long)__double_as_longlong(ldexp((double)*v, -32));
 }
 
 // SpMM: K interleaved vectors per gather (one K*sizeof(XT)-byte load per
@@ -3863,28 +3922,47 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
 
 void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                          const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
-                         double* yT, bool hybrid, hipStream_t s) {
+                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s) {
   const int64_t n = T.n;
   if (n <= 0) return;
   if (T.split != (bPerm != nullptr) || T.split != (bPos != nullptr))
     throw HipError("tiled sigma: split columns need the beta-major order");
   const int64_t nx = bPerm ? 2 * n : n;
   const unsigned grid = (unsigned)T.blocks;
-  kFillZero<<<gridFor(nx), kBlock, 0, s>>>(yT, nx);
+  kFillZeroU64<<<gridFor(nx + 1), kBlock, 0, s>>>(yT, nx + 1);
   if (T.f32) {
     float* xs = (float*)xP.get((size_t)nx * 4);
-    kGatherPerm<float, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, nullptr);
-    auto k = hybrid ? kSigmaTiledSym<float, float, 8, true> : kSigmaTiledSym<float, float, 8, false>;
-    k<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(),
-                              T.idx.as<uint32_t>(), T.vals.as<float>(), diag, perm, n, xs, x, y, yT);
+    kGatherPermMax<float><<<gridFor(nx), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
+    kSigmaTiledSym<float, float, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
+                                                            T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+                                                            T.vals.as<float>(), diag, perm, n, nx, xs, x, y, yT,
+                                                            bound);
   } else {
     double* xs = (double*)xP.get((size_t)nx * 8);
-    kGatherPerm<double, 1><<<gridFor(nx), kBlock, 0, s>>>(x, 1, n, perm, bPerm, n, xs, nullptr);
-    auto k = hybrid ? kSigmaTiledSym<double, double, 8, true> : kSigmaTiledSym<double, double, 8, false>;
-    k<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(),
-                              T.idx.as<uint32_t>(), T.vals.as<double>(), diag, perm, n, xs, x, y, yT);
+    kGatherPermMax<double><<<gridFor(nx), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
+    kSigmaTiledSym<double, double, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
+                                                              T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+                                                              T.vals.as<double>(), diag, perm, n, nx, xs, x, y, yT,
+                                                              bound);
   }
-  kAddTransposed<<<gridFor(n), kBlock, 0, s>>>(y, yT, pos, bPos, n);
+  kAddTransposed<<<gridFor(n), kBlock, 0, s>>>(y, yT, bound, pos, bPos, n, nx);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launchSymColumnBound(const int32_t* cols, const double* vals64, const float* vals32, int64_t nnzOld,
+                          int64_t nnz, int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound,
+                          hipStream_t s) {
+  if (n > nOld) kFillZeroU64<<<gridFor(n - nOld), kBlock, 0, s>>>(acc + nOld, n - nOld);
+  kFillZeroU64<<<1, 64, 0, s>>>(bound, 1);
+  if (nnz > nnzOld) {
+    if (vals32)
+      kColAbsSum<float><<<gridFor(nnz - nnzOld), kBlock, 0, s>>>(cols + nnzOld, vals32 + nnzOld, nnz - nnzOld, acc);
+    else
+      kColAbsSum<double><<<gridFor(nnz - nnzOld), kBlock, 0, s>>>(cols + nnzOld, vals64 + nnzOld, nnz - nnzOld,
+                                                                  acc);
+  }
+  if (n > 0) kMaxU64<<<gridFor(n), kBlock, 0, s>>>(acc, n, bound);
+  kU64ScaledToDouble<<<1, 64, 0, s>>>(bound);
   HIP_CHECK(hipGetLastError());
 }
 

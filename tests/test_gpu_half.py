@@ -2,9 +2,9 @@
 lower-triangular rows without the diagonal, SHCI/SHCImakeHamiltonian.cpp:549-711),
 the layout for spaces whose two-triangle Hamiltonian does not fit.  Sigma
 applies every stored element to both rows as Hmult2 does
-(SHCI/Hmult.h:110-121), the transposed half by atomic adds (on the tiled
-copy of the stored half by default, summed per column tile in LDS for large
-segments or everywhere, or on the stored rows).  Bar: the rows
+(SHCI/Hmult.h:110-121), the transposed half by atomic adds (by default on a
+tiled copy of the stored half, in 64-bit fixed point, so in a fixed result;
+or in FP64 on the stored rows).  Bar: the rows
 equal the two-triangle build's lower part bit for bit; sigma equals the
 oracle's to 1e-12 relative (its sums are in no fixed order); runs give the
 oracle's determinant sets and energies to 1e-10 Ha."""
@@ -33,7 +33,7 @@ def _space(path, occ, n, W):
     return dets[:n]
 
 
-@pytest.mark.parametrize("mode", ["tiled", "lds", "lds-all", "rows"])
+@pytest.mark.parametrize("mode", ["tiled", "rows"])
 @pytest.mark.parametrize("case", ["c0", "w2"])
 def test_half_storage_rows_and_sigma(case, mode, tmp_path, monkeypatch):
     from dice_amd import shci, synth
@@ -49,10 +49,8 @@ def test_half_storage_rows_and_sigma(case, mode, tmp_path, monkeypatch):
     full.set_dets(dets)
     rp, cols, vals = full.hamiltonian_rows()
     monkeypatch.setenv("SHCI_STORAGE", "half")
-    if mode != "tiled":
-        monkeypatch.setenv("SHCI_HALF_SIGMA", "rows" if mode == "rows" else "lds")
-    if mode == "lds-all":  # every segment sums in LDS, also split over waves
-        monkeypatch.setenv("SHCI_HALF_STAGE_MIN", "1")
+    if mode == "rows":
+        monkeypatch.setenv("SHCI_HALF_SIGMA", "rows")
     half = shci.System.from_fcidump(path)
     half.set_dets(dets)
     hrp, hcols, hvals = half.hamiltonian_rows()
@@ -66,6 +64,41 @@ def test_half_storage_rows_and_sigma(case, mode, tmp_path, monkeypatch):
     for r in range(3):
         yo = o.sigma(np.ascontiguousarray(x[:, r]))
         assert np.max(np.abs(y[:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("case", ["c0", "w2"])
+def test_half_storage_tiled_sigma_is_deterministic(case, tmp_path, monkeypatch):
+    """The tiled half-storage sigma sums the transposed half in 64-bit fixed
+    point: repeated calls, and a copy whose every segment sums in LDS first
+    (a different split between LDS and direct integer atomics), give the
+    same bits, within 1e-12 of the oracle."""
+    from dice_amd import shci, synth
+    path = str(tmp_path / "FCIDUMP")
+    if case == "c0":
+        synth.write_molecular_fcidump(path, 0, 16, 8, 8, 64)
+        occ, W = synth.hf_occupation(8, 8), 1
+    else:
+        synth.write_molecular_fcidump(path, 5, 34, 6, 6, 12)
+        occ, W = synth.hf_occupation(6, 6), 2
+    dets = _space(path, occ, 3000, W)
+    x = np.random.default_rng(5).normal(size=(len(dets), 2))
+    x[7, 0] = 50.0  # one large value sets the scale of the fixed point
+    monkeypatch.setenv("SHCI_STORAGE", "half")
+    ys = []
+    for stage_min in ("4", "1"):
+        monkeypatch.setenv("SHCI_HALF_STAGE_MIN", stage_min)
+        h = shci.System.from_fcidump(path)
+        h.set_dets(dets)
+        h.make_hamiltonian()
+        ys += [h.hmult(x), h.hmult(x)]
+        del h
+    for y in ys[1:]:
+        np.testing.assert_array_equal(y, ys[0])
+    o = OracleSystem(path)
+    o.csr_append(dets)
+    for r in range(2):
+        yo = o.sigma(np.ascontiguousarray(x[:, r]))
+        assert np.max(np.abs(ys[0][:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
 @pytest.mark.parametrize("mode", ["tiled", "rows"])

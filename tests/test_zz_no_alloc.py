@@ -78,16 +78,20 @@ def _sigma_pair(S, x):
             H.hipFree(d)
 
 
+@pytest.mark.parametrize("storage", ["full", "half"])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_noalloc_sigma_matches_allocating_sigma(prec, tmp_path):
+def test_noalloc_sigma_matches_allocating_sigma(prec, storage, tmp_path, monkeypatch):
     from dice_amd import shci, synth
+    if storage == "half":  # the tiled copy of the stored half (fixed-point transposed sums)
+        monkeypatch.setenv("SHCI_STORAGE", "half")
     big = str(tmp_path / "c0")
     synth.write_molecular_fcidump(big, 0, 16, 8, 8, 64)
     small = str(tmp_path / "w2")  # 34 orbitals: two-word determinants
     synth.write_molecular_fcidump(small, 5, 34, 6, 6, 12)
     cases = [(big, synth.hf_occupation(8, 8), 3000, 1, 3), (small, synth.hf_occupation(6, 6), 1200, 2, 1)]
     for path, occ, n, W, k in cases:  # the second call is on the smaller space
-        print("noalloc sigma: %s n=%d k=%d precision=%d" % (os.path.basename(path), n, k, prec), flush=True)
+        print("noalloc sigma: %s n=%d k=%d precision=%d storage=%s" % (os.path.basename(path), n, k, prec, storage),
+              flush=True)
         dets = _space(path, occ, n, W)
         S = shci.System.from_fcidump(path)
         S.set_precision(prec)
