@@ -12,11 +12,16 @@ integrals resident on the GPU.  `value` is the mean wall time of one
 variational iteration over the timed steps.
 
 The roofline object is the sigma = H x kernel of the Davidson, timed with
-HIP events on the library's stream over the timed steps (kSigmaTiled over
-256-row blocks of the alpha-major relabelled Hamiltonian whose entries are
-grouped by 1024-column tiles, with its x permutation, kGatherPerm);
-achieved bandwidth uses algorithmic bytes per launch: stored values +
-column indices, row offsets, diagonal, x read once and y written once.  `traffic` is the HBM
+HIP events on the library's stream over the timed steps.  In FP64 the
+Hamiltonian's lower triangle alone is stored (the library's default for the
+single-vector Davidson in FP64): kSigmaTiledSym over a tiled copy of it
+(256-row blocks of the alpha-major relabelled rows, entries grouped by
+1024-column tiles), each stored element applied to its row and, in 64-bit
+fixed point, to its column, with its x permutation and the transposed sums'
+addition; with FP32 values both triangles are stored and the kernel is
+kSigmaTiled.  Achieved bandwidth uses algorithmic bytes per launch: stored
+values + column indices, row offsets, diagonal, x read once and y written
+once.  `traffic` is the HBM
 bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
@@ -200,7 +205,7 @@ C4_EPS1 = 2e-2   # the learned form on the 40-orbital system (see c4_form)
 C4_NITER = 3
 
 
-def c3_form(shci, args, half_only=False):
+def c3_form(shci, args):
     """The 40-orbital system (20 alpha + 20 beta electrons, 160 Cholesky
     vectors, random stream 1003, two-word determinants) with two roots: one
     variational step of C3_NITER iterations at eps1 C3_EPS1 from HF, in FP64
@@ -208,8 +213,8 @@ def c3_form(shci, args, half_only=False):
     configured schedule (down to 2e-5) selects far more than one GPU holds
     on this synthetic construction; C3_EPS1 is the largest-space schedule
     measured to fit and finish within the benchmark's time.  The FP64 run
-    with half storage (the lower triangle alone) follows (half_only: it
-    alone)."""
+    stores the lower triangle alone (the default); an FP64 run with both
+    triangles stored follows."""
     from dice_amd import synth
     d = os.path.join(tempfile.gettempdir(), "shci_bench_c3")
     os.makedirs(d, exist_ok=True)
@@ -223,8 +228,8 @@ def c3_form(shci, args, half_only=False):
     log = os.path.join(d, "gpu.log")
     res = {"workload": "40-orbital synthetic FCIDUMP (20a+20b, 160 Cholesky vectors, stream 1003), 2 roots, "
                        "heat-bath eps1 %.0e x %d iterations from HF" % (C3_EPS1, C3_NITER)}
-    runs = ((() if half_only else (("fp64", shci.PREC_FP64, False), ("fp32", shci.PREC_FP32, False))) +
-            (("fp64_half_storage", shci.PREC_FP64, True),))
+    runs = (("fp64", shci.PREC_FP64, None), ("fp32", shci.PREC_FP32, None),
+            ("fp64_full_storage", shci.PREC_FP64, False))
     for tag, prec, half in runs:
         print("# C3 %s" % tag, file=sys.stderr, flush=True)
         t0 = time.time()
@@ -247,16 +252,15 @@ def c3_form(shci, args, half_only=False):
         del r
         print("# C3 %s: %.1f s, %d dets, E %s" % (tag, wall, res[tag]["final_ndets"], res[tag]["energies"]),
               file=sys.stderr, flush=True)
-    res["fp64_half_storage"]["description"] = (
+    res["fp64"]["description"] = (
         "the lower triangle alone (the reference's layout, half the matrix bytes); sigma on its tiled copy "
-        "(kSigmaTiledSym, transposed half by per-tile LDS sums and FP64 atomics); sigma bytes count the stored "
-        "lower triangle")
-    if half_only:
-        return res
-    res["fp64_half_storage"]["speedup_vs_fp64"] = round(res["fp64"]["step_s"] / res["fp64_half_storage"]["step_s"],
-                                                        3)
-    res["fp64_half_storage"]["energy_error_vs_fp64_Ha"] = [
-        a - b for a, b in zip(res["fp64_half_storage"]["energies"], res["fp64"]["energies"])]
+        "(kSigmaTiledSym, the transposed half in 64-bit fixed point); sigma bytes count the stored lower triangle")
+    res["fp64_full_storage"]["description"] = (
+        "both triangles stored; sigma on their tiled copy (kSigmaTiled); sigma bytes count both triangles")
+    res["fp64_full_storage"]["speedup_vs_fp64"] = round(
+        res["fp64"]["step_s"] / res["fp64_full_storage"]["step_s"], 3)
+    res["fp64_full_storage"]["energy_error_vs_fp64_Ha"] = [
+        a - b for a, b in zip(res["fp64_full_storage"]["energies"], res["fp64"]["energies"])]
     res["fp32"]["energy_error_vs_fp64_Ha"] = [a - b for a, b in zip(res["fp32"]["energies"], res["fp64"]["energies"])]
     res["fp32"]["speedup_vs_fp64"] = round(res["fp64"]["step_s"] / res["fp32"]["step_s"], 3)
     return res
@@ -547,9 +551,9 @@ def main():
     ap.add_argument("--post-variational", action="store_true",
                     help="only the steps after the variational one (PT, RDMs) on the final space, printed as "
                          "their own JSON object (for profiling them)")
-    ap.add_argument("--form", choices=["c0", "c2", "c2big", "c3", "c3half", "c4"],
+    ap.add_argument("--form", choices=["c0", "c2", "c2big", "c3", "c4"],
                     help="only that configuration's line (c0: 16 orbitals; c2: the Hubbard batching form; c2big: its "
-                         "fourth iteration; c3: 40 orbitals; c3half: the same in FP64 with half storage; c4: the "
+                         "fourth iteration; c3: 40 orbitals; c4: the "
                          "learned form on them), printed as its own JSON object (for profiling)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="block Davidson forms without hipGraph replay (rocprofv3's kernel tracer does not "
@@ -570,7 +574,7 @@ def main():
         return
     if args.form:
         fn = {"c0": c0_form, "c2": batching_form, "c2big": batching_fourth_iteration, "c3": c3_form,
-              "c3half": lambda shci, args: c3_form(shci, args, half_only=True), "c4": c4_form}[args.form]
+              "c4": c4_form}[args.form]
         print(json.dumps({args.form: fn(shci, args)}))
         return
     results = {}
@@ -611,8 +615,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
-                      "entries grouped by 1024-column tiles)",
+            "kernel": ("kSigmaTiledSym (sigma = H x over the stored lower triangle: 256-row blocks of the "
+                       "alpha-major relabelled rows, entries grouped by 1024-column tiles, each entry applied to its "
+                       "row and, in 64-bit fixed point, to its column)" if prec == shci.PREC_FP64 else
+                       "kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
+                       "entries grouped by 1024-column tiles)"),
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
@@ -645,25 +652,26 @@ def main():
                 "iterations": mixed["per_iter"],
             }
         }
-        # half storage: the lower triangle alone, sigma on its tiled copy
-        # (stored entries applied as stored and transposed)
-        H = run_form(shci, full, log, shci.PREC_FP64, fsteps, fwarm, tag="half", half_storage=True)
-        hms = H["ms"] / H["calls"] if H["calls"] else None
-        out["forms"]["half_storage_fp64"] = {
-            "description": "the lower triangle alone (the reference's layout, half the matrix bytes); sigma on a "
-                           "tiled copy of it (kSigmaTiledSym: each stored entry applied to its row and, through "
-                           "per-tile LDS sums or direct FP64 atomics, to its column), so its sums are in no fixed "
-                           "order; sigma bytes count the stored half",
-            "value": round(statistics.mean(H["iter_times"]), 6),
-            "ms_per_step": round(1e3 * statistics.mean(H["step_times"]), 3),
-            "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(H["step_times"]), 3),
-            "final_energy": H["energies"][-1],
-            "energy_error_vs_fp64_Ha": H["energies"][-1] - fp64["energies"][-1],
-            "final_ndets": H["per_iter"][-1]["ndets"],
-            "sigma_avg_ms": round(hms, 4) if hms else None,
-            "sigma_avg_ms_full_storage": round(fp64["ms"] / fp64["calls"], 4) if fp64["calls"] else None,
-            "sigma_GBps": round(H["nbytes"] / (H["ms"] * 1e-3) / 1e9, 1) if H["ms"] else None,
-            "iterations": H["per_iter"],
+        # both triangles stored (the FP64 default stores the lower one alone):
+        # sigma on their tiled copy, kSigmaTiled
+        F = run_form(shci, full, log, shci.PREC_FP64, fsteps, fwarm, tag="fp64 full storage", half_storage=False)
+        fms = F["ms"] / F["calls"] if F["calls"] else None
+        fgbs = F["nbytes"] / (F["ms"] * 1e-3) / 1e9 if F["ms"] else None
+        out["forms"]["full_storage_fp64"] = {
+            "description": "both triangles of H stored; sigma on their tiled copy (kSigmaTiled: the segmented "
+                           "per-row scan, large segments gathering x from an LDS copy of their tile); sigma bytes "
+                           "count both triangles",
+            "value": round(statistics.mean(F["iter_times"]), 6),
+            "ms_per_step": round(1e3 * statistics.mean(F["step_times"]), 3),
+            "speedup_vs_fp64": round(statistics.mean(fp64["step_times"]) / statistics.mean(F["step_times"]), 3),
+            "final_energy": F["energies"][-1],
+            "energy_error_vs_fp64_Ha": F["energies"][-1] - fp64["energies"][-1],
+            "final_ndets": F["per_iter"][-1]["ndets"],
+            "sigma_avg_ms": round(fms, 4) if fms else None,
+            "sigma_GBps": round(fgbs, 1) if fgbs else None,
+            "sigma_frac_of_hbm_peak": round(fgbs / PEAK_HBM_GBS, 4) if fgbs else None,
+            "sigma_traffic_bytes": traffic_bytes("fp64_full_storage"),
+            "iterations": F["per_iter"],
         }
         # learned screening + mixed precision (the combined fast form)
         L = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="learned+fp32", learned=True)

@@ -89,10 +89,10 @@ void applyMethodOptions(Engine& E, const shci_run_options& opt) {
   E.davidsonMode = opt.davidson_block ? 1 : 0;
   E.autotune = opt.autotune != 0;
   E.useGraphs = opt.hip_graphs != 0;
-  if (opt.half_storage && !E.lowerOnly) {  // from the next build on, rebuilt in half storage
-    E.lowerOnly = true;
-    E.hamRows = -1;
-  }
+  // half_storage > 0 / < 0: half storage always / never, 0: the engine's
+  // default (SHCI_STORAGE, else chosen per form); a change of storage takes
+  // effect at the next build
+  if (opt.half_storage) E.storageMode = opt.half_storage > 0 ? 1 : -1;
   E.learned = opt.learned != 0;
   // 1: gated (the ranking runs only where it is predicted to pay); 2: the
   // untrained control and 3: the trained network, both always ranking

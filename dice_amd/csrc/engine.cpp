@@ -49,6 +49,8 @@ void Engine::initShared(const Engine& parent) {
   lowerOnly = parent.lowerOnly;
   lowerTiled = parent.lowerTiled;
   lowerStageMin = parent.lowerStageMin;
+  storageMode = parent.storageMode;
+  lowerByMemory = parent.lowerByMemory;
   autotune = parent.autotune;
   davidsonMode = parent.davidsonMode;
   sigmaCfg[0] = parent.sigmaCfg[0];
@@ -213,9 +215,11 @@ void Engine::loadSystem(HostSystem&& sys) {
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_STAGE_MIN")) tiledStageMin = std::max(1, std::atoi(p));
   // SHCI_SIGMA_TILE_PERSISTENT=1: persistent workgroups for the tiled sigma
   if (const char* p = std::getenv("SHCI_SIGMA_TILE_PERSISTENT")) tiledPersistent = std::atoi(p) != 0;
-  // SHCI_STORAGE=half: half storage from the first build
+  // SHCI_STORAGE=half / full: half storage always / never (the default
+  // chooses per form)
   const char* storage = std::getenv("SHCI_STORAGE");
-  if (storage && std::strcmp(storage, "half") == 0) lowerOnly = true;
+  if (storage && std::strcmp(storage, "half") == 0) storageMode = 1;
+  if (storage && std::strcmp(storage, "full") == 0) storageMode = -1;
   // SHCI_HALF_SIGMA=rows: half storage's sigma on the stored rows (no tiled
   // copy); SHCI_HALF_STAGE_MIN=k: segments of at least k instructions sum
   // their transposed products in LDS first (1: every segment)
@@ -631,6 +635,18 @@ void Engine::buildHamiltonian() {
     HIP_CHECK(hipStreamSynchronize(stream));
     return;
   }
+  if (wantLower() != lowerOnly) {  // the storage changes: rebuilt from scratch in the other one
+    for (DevMem* m : {&cols, &vals, &vals32, &valsLo, &colsNew, &valsNew, &vals32New, &valsLoNew, &permCols,
+                      &permVals, &rowPtrNew})
+      m->release();
+    tiled.release();
+    permRows = -1;
+    tiledRows = -1;
+    splitRows = -1;
+    nnzOff = 0;
+    hamRows = -1;
+    lowerOnly = !lowerOnly;
+  }
   const int64_t nOld = (incremental && hamRows > 0 && hamRows < n) ? hamRows : 0;
   if (lowerOnly) {
     buildLower(nOld);
@@ -692,6 +708,7 @@ void Engine::buildHamiltonian() {
       permRows = -1;
       tiledRows = -1;
       lowerOnly = true;
+      lowerByMemory = true;
       hamRows = -1;
       buildLower(0);
       return;
@@ -766,6 +783,12 @@ void Engine::buildHamiltonian() {
     std::fprintf(stderr, "%s%s\n", line.c_str(), buf);
     for (Mark& m : marks) (void)hipEventDestroy(m.ev);
   }
+}
+
+bool Engine::wantLower() const {
+  if (lowerByMemory || storageMode > 0) return true;
+  if (storageMode < 0) return false;
+  return precision == 0 && davidsonMode != 1 && relabel && tiledOn && lowerTiled;
 }
 
 // Half storage: the rows of the determinants added since the last build

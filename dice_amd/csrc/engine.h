@@ -85,17 +85,26 @@ struct Engine {
   int64_t nnzOff = 0, hamRows = -1;
   // half storage: each pair once, in row max(I, J) (the reference's
   // lower-triangular rows without the diagonal), grown in place by new rows
-  // only; chosen by SHCI_STORAGE=half, the half_storage option, or when the
-  // two-triangle Hamiltonian does not fit.  Sigma then adds the transposed
-  // half with atomics.
+  // only.  storageMode 1 (SHCI_STORAGE=half, half_storage > 0) always, -1
+  // (SHCI_STORAGE=full, half_storage < 0) never, 0: when the FP64
+  // single-vector Davidson runs on the relabelled tiled layout (its fastest
+  // form; the block Davidson's SpMM and the FP32 form's hi + lo refinement
+  // run on the two triangles); and from the build on which the two-triangle
+  // Hamiltonian does not fit (lowerByMemory).  Sigma then adds the
+  // transposed half.
   bool lowerOnly = false;
+  int storageMode = 0;
+  bool lowerByMemory = false;
+  bool wantLower() const;
   void buildLower(int64_t nOld);
   // with half storage, the tiled copy holds the stored half (built from it
   // when both fit) and sigma runs kSigmaTiledSym on it, its transposed sums
   // gathered in lowerYT (fixed point, permuted order; then max|x|); the FP32
   // form's exact sigma keeps the row kernel
   bool lowerTiled = true;
-  int lowerStageMin = 4;    // instructions that make a segment large (2: 3.60, 4: 3.29, 8: 3.66 ms)
+  // instructions that make a segment large (24-orbital workload: 2: 3.60,
+  // 4: 3.29, 8: 3.66 ms; 8 instructions per wave per step: 3.28 ms, 16: 3.65)
+  int lowerStageMin = 4;
   DevMem lowerYT;
   // the column sums of |v| (fixed point) over the first lowerAbsRows rows'
   // entries, grown with the half, and their largest (kSigmaTiledSym's bound)

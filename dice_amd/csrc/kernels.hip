@@ -2356,7 +2356,15 @@ __global__ void kGatherPermMax(const double* __restrict__ x, const uint32_t* __r
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-  if (__lane_id() == 0 && m > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
+  // one atomic per workgroup (the launch is capped at a few per CU):
+  // atomics on one address are serialised
+  __shared__ double wm[kWaves];
+  if (__lane_id() == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; w++) m = fmax(m, wm[w]);
+    if (m > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // column sums of |v| over stored entries (columns J in determinant order),
@@ -3932,14 +3940,14 @@ void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* 
   kFillZeroU64<<<gridFor(nx + 1), kBlock, 0, s>>>(yT, nx + 1);
   if (T.f32) {
     float* xs = (float*)xP.get((size_t)nx * 4);
-    kGatherPermMax<float><<<gridFor(nx), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
+    kGatherPermMax<float><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
     kSigmaTiledSym<float, float, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
                                                             T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
                                                             T.vals.as<float>(), diag, perm, n, nx, xs, x, y, yT,
                                                             bound);
   } else {
     double* xs = (double*)xP.get((size_t)nx * 8);
-    kGatherPermMax<double><<<gridFor(nx), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
+    kGatherPermMax<double><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
     kSigmaTiledSym<double, double, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
                                                               T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
                                                               T.vals.as<double>(), diag, perm, n, nx, xs, x, y, yT,
@@ -3961,7 +3969,7 @@ void launchSymColumnBound(const int32_t* cols, const double* vals64, const float
       kColAbsSum<double><<<gridFor(nnz - nnzOld), kBlock, 0, s>>>(cols + nnzOld, vals64 + nnzOld, nnz - nnzOld,
                                                                   acc);
   }
-  if (n > 0) kMaxU64<<<gridFor(n), kBlock, 0, s>>>(acc, n, bound);
+  if (n > 0) kMaxU64<<<gridFor(n, kBlock, 1024), kBlock, 0, s>>>(acc, n, bound);
   kU64ScaledToDouble<<<1, 64, 0, s>>>(bound);
   HIP_CHECK(hipGetLastError());
 }

@@ -473,13 +473,14 @@ class VariationalResult:
 
 
 def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma=False, davidson_block=False,
-                   autotune=False, hip_graphs=False, learned=False, half_storage=False):
+                   autotune=False, hip_graphs=False, learned=False, half_storage=None):
     """Run the variational step of ``Dice input.dat``; returns a VariationalResult.
 
     precision: PREC_FP64 (the drop-in) or PREC_FP32 (FP32 values, FP64
     accumulation and refinement); davidson_block / autotune / hip_graphs
-    select the batching form, learned the learned screening form (see
-    shci_run_options)."""
+    select the batching form, learned the learned screening form;
+    half_storage True / False: the Hamiltonian's lower triangle alone / both
+    triangles (None: the library's choice per form; see shci_run_options)."""
     L = library()
     h = ctypes.c_void_p()
     opt = RunOptions()
@@ -488,7 +489,7 @@ def do_variational(input_path, log_path=None, precision=PREC_FP64, profile_sigma
     opt.davidson_block = 1 if davidson_block else 0
     opt.autotune = 1 if autotune else 0
     opt.hip_graphs = 1 if hip_graphs else 0
-    opt.half_storage = 1 if half_storage else 0
+    opt.half_storage = 0 if half_storage is None else (1 if half_storage else -1)
     opt.learned = {"control": 2, "forced": 3}.get(learned, 1 if learned else 0) if isinstance(learned, str) \
         else (1 if learned else 0)
     st = L.shci_run_ex(os.fsencode(input_path), os.fsencode(log_path) if log_path else None, ctypes.byref(opt),

@@ -248,11 +248,14 @@ typedef struct {
                          tables (HmultDirect, SHCI/Hmult.h:153-382).  shci_run_ex also turns it on, with
                          davidsonDirect's rules (SHCI/Davidson.cpp:353-590), for the `directdavidson`
                          keyword */
-  int half_storage;   /* nonzero: each connected pair stored once, in row max(I, J) (the reference's
+  int half_storage;   /* > 0: each connected pair stored once, in row max(I, J) (the reference's
                          lower-triangular rows, SHCI/SHCImakeHamiltonian.cpp:549-711), sigma applying
-                         the transposed half by FP64 atomic adds (SHCI/Hmult.h:110-121), for spaces whose
-                         two-triangle Hamiltonian does not fit; a build switches to it by itself when the
-                         two-triangle one does not fit */
+                         each stored element to both rows (SHCI/Hmult.h:110-121; on a tiled copy, the
+                         transposed half summed in 64-bit fixed point, so in a fixed result); < 0: both
+                         triangles stored; 0: the library's choice (environment SHCI_STORAGE=half/full,
+                         else half storage for the FP64 single-vector Davidson, its fastest form, and
+                         both triangles for the block Davidson and the FP32 form).  A build switches to
+                         half storage by itself when the two-triangle Hamiltonian does not fit */
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,

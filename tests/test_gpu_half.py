@@ -45,6 +45,7 @@ def test_half_storage_rows_and_sigma(case, mode, tmp_path, monkeypatch):
         synth.write_molecular_fcidump(path, 5, 34, 6, 6, 12)
         occ, W = synth.hf_occupation(6, 6), 2
     dets = _space(path, occ, 3000, W)
+    monkeypatch.setenv("SHCI_STORAGE", "full")
     full = shci.System.from_fcidump(path)
     full.set_dets(dets)
     rp, cols, vals = full.hamiltonian_rows()

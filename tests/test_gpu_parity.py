@@ -197,19 +197,23 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     by column tile, one launch per vector), the row-ordered relabelled copy
     ("rows": one vector, or the SpMM of two), both with same-beta columns
     gathered from a beta-major copy of x, every column alpha-major
-    ("alpha"), and the default (the tiled copy, large segments gathering
-    from an LDS copy of their x tile; "tiled-global": every gather from
-    global memory).  Each layout repeats its own result bit for bit, staging
-    every segment gives the default's bits, and the tiled copy built from H
-    directly equals the one built from the row copy bit for bit."""
+    ("alpha"), and the two-triangle default (the tiled copy, large segments
+    gathering from an LDS copy of their x tile; "tiled-global": every gather
+    from global memory); and the tiled copy of the lower triangle alone
+    ("half", the transposed half in fixed point).  Each layout repeats its
+    own result bit for bit, staging every segment gives the default's bits,
+    and the tiled copy built from H directly equals the one built from the
+    row copy bit for bit."""
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = {}
-    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "external", "alpha"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "external", "alpha",
+                   "half"):
         for var in ("SHCI_SIGMA_LAYOUT", "SHCI_SIGMA_TILE_BUILD", "SHCI_SIGMA_TILE_STAGE", "SHCI_SIGMA_TILE_STAGE_MIN",
                     "SHCI_SIGMA_TILE_PERSISTENT"):
             monkeypatch.delenv(var, raising=False)
+        monkeypatch.setenv("SHCI_STORAGE", "half" if layout == "half" else "full")
         if layout == "tiled-from-H":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_BUILD", "direct")
@@ -221,7 +225,7 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
             # every segment staged, down to one instruction
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_STAGE_MIN", "1")
-        elif layout != "default":
+        elif layout not in ("default", "half"):
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
         g.set_precision(prec)
@@ -232,7 +236,7 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
     ref = out["external"]
-    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "alpha"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "alpha", "half"):
         for k in (0, 1):
             assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
     # the staged kernel's sums do not depend on which segments it stages
@@ -247,12 +251,16 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out["tiled-persistent"][k], out["default"][k])
 
 
+@pytest.mark.parametrize("storage", ["full", "half"])
 @pytest.mark.parametrize("name", ["c0", "w2"])
-def test_tiled_sigma_bitwise_across_handles_and_growth(name, systems, monkeypatch):
+def test_tiled_sigma_bitwise_across_handles_and_growth(name, storage, systems, monkeypatch):
     """The tiled sigma's sums do not depend on scheduling: a handle whose
     tiled copy was first built for a smaller space and then rebuilt for the
-    full one gives the bits of a fresh handle on the full space."""
+    full one gives the bits of a fresh handle on the full space (both
+    triangles stored, or the lower one with the transposed half in fixed
+    point)."""
     monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
+    monkeypatch.setenv("SHCI_STORAGE", storage)
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     x = np.random.default_rng(9).normal(size=len(dets))
@@ -422,9 +430,11 @@ def test_run_mixed_precision_energy_gap(tmp_path):
 # ---- batching form: SpMM, block Davidson, hipGraph replay, autotuned launch shapes ----
 
 @pytest.mark.parametrize("name,prec", [("c0", 0), ("hub", 0), ("w2", 0), ("c0", 1)])
-def test_spmm_matches_column_sigma(name, prec, systems):
-    """sigma of k columns at once (interleaved SpMM, k = 2..8 and 11 = 8 + 3)
-    equals k single-vector sigmas to 1e-12 relative."""
+def test_spmm_matches_column_sigma(name, prec, systems, monkeypatch):
+    """sigma of k columns at once (interleaved SpMM over both stored
+    triangles, k = 2..8 and 11 = 8 + 3) equals k single-vector sigmas to
+    1e-12 relative."""
+    monkeypatch.setenv("SHCI_STORAGE", "full")
     path = systems[name][0]
     dets = _space(name, systems, 3000, 1)
     g = _gpu(path)

@@ -82,8 +82,9 @@ def _sigma_pair(S, x):
 @pytest.mark.parametrize("prec", [0, 1])
 def test_noalloc_sigma_matches_allocating_sigma(prec, storage, tmp_path, monkeypatch):
     from dice_amd import shci, synth
-    if storage == "half":  # the tiled copy of the stored half (fixed-point transposed sums)
-        monkeypatch.setenv("SHCI_STORAGE", "half")
+    # both triangles, or the tiled copy of the stored half (fixed-point
+    # transposed sums)
+    monkeypatch.setenv("SHCI_STORAGE", storage)
     big = str(tmp_path / "c0")
     synth.write_molecular_fcidump(big, 0, 16, 8, 8, 64)
     small = str(tmp_path / "w2")  # 34 orbitals: two-word determinants
