@@ -880,15 +880,15 @@ void Engine::buildLowerTiled(int64_t nOld, int64_t nnzOld) {
   }
   const int64_t nx = doSplit ? 2 * n : n;
   lowerYT.reserve((size_t)(nx + 1) * 8);
-  // the column sums grow with the half: when they cover exactly the rows
-  // kept from the previous build, only the added rows are summed
+  // the row sums grow with the half: when they cover exactly the rows kept
+  // from the previous build, only the added rows are summed
   const bool grow = nOld > 0 && lowerAbsRows == nOld && lowerAbsNnz == nnzOld && lowerAbsF32 == f32;
-  const int64_t r0 = grow ? lowerAbsRows : 0, e0 = grow ? lowerAbsNnz : 0;
-  lowerColAbs.reserve((size_t)n * 8, stream, grow);
+  const int64_t r0 = grow ? lowerAbsRows : 0;
+  lowerRowAbs.reserve((size_t)n * 8, stream, grow);
   lowerBound.reserve(8);
-  launchSymColumnBound(cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(), f32 ? vals32.as<float>() : nullptr, e0,
-                       nnzOff, r0, n, lowerColAbs.as<unsigned long long>(), lowerBound.as<unsigned long long>(),
-                       stream);
+  launchSymRowBound(rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
+                    f32 ? vals32.as<float>() : nullptr, r0, n, lowerRowAbs.as<unsigned long long>(),
+                    lowerBound.as<unsigned long long>(), stream);
   lowerAbsRows = n;
   lowerAbsNnz = nnzOff;
   lowerAbsF32 = f32;

@@ -106,9 +106,10 @@ struct Engine {
   // 4: 3.29, 8: 3.66 ms; 8 instructions per wave per step: 3.28 ms, 16: 3.65)
   int lowerStageMin = 4;
   DevMem lowerYT;
-  // the column sums of |v| (fixed point) over the first lowerAbsRows rows'
-  // entries, grown with the half, and their largest (kSigmaTiledSym's bound)
-  DevMem lowerColAbs, lowerBound;
+  // the off-diagonal row sums of |H| (fixed point) over the first
+  // lowerAbsRows rows' stored entries, grown with the half, and their
+  // largest (kSigmaTiledSym's bound)
+  DevMem lowerRowAbs, lowerBound;
   int64_t lowerAbsRows = -1, lowerAbsNnz = 0;
   bool lowerAbsF32 = false;
   void buildLowerTiled(int64_t nOld, int64_t nnzOld);

@@ -288,20 +288,19 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
 // y = H x through a tiled copy of half-stored H (each pair once): the copy's
 // entries applied as stored and transposed; pos / bPos: determinant ->
 // alpha-major / beta-major position; yT: nx + 1 words of scratch (nx: the
-// permuted x's length); bound: the column bound set by launchSymColumnBound.
-// The transposed sums are 64-bit fixed point, summed with integer atomics:
+// permuted x's length); bound: the row bound set by launchSymRowBound.  The
+// off-diagonal sums are 64-bit fixed point, summed with integer atomics:
 // the result does not depend on their order.
 void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                          const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
                          unsigned long long* yT, const unsigned long long* bound, hipStream_t s);
-// kSigmaTiledSym's column bound (the largest column sum of |v| over the
-// stored half) into *bound, from the column sums in acc (n words): the
-// entries [nnzOld, nnz) of a half grown from nOld rows are added to the
-// sums of the first nOld columns (entries, columns and values: vals64 or
-// vals32, as stored; nOld = nnzOld = 0 for a new half).
-void launchSymColumnBound(const int32_t* cols, const double* vals64, const float* vals32, int64_t nnzOld,
-                          int64_t nnz, int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound,
-                          hipStream_t s);
+// kSigmaTiledSym's bound (the largest off-diagonal row sum of |H|) into
+// *bound, from the row sums in acc (n words): the rows [nOld, n) of a half
+// grown from nOld rows (CSR rowPtr, cols, values: vals64 or vals32, as
+// stored) are added to the sums of the first nOld rows (nOld = 0 for a new
+// half).
+void launchSymRowBound(const int64_t* rowPtr, const int32_t* cols, const double* vals64, const float* vals32,
+                       int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound, hipStream_t s);
 
 // --- Davidson vector kernels ----------------------------------------------------------
 // G[i*mb + j] = A[:,i] . B[:,j] for i < ma, j < mb (columns of length n, ld)

@@ -2211,19 +2211,23 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
 }
 
 // Half storage over the tiled copy: the copy holds each stored pair (I, J)
-// once, in row I's block and J's column tile.  y[perm[m]] = diag .* x +
-// (lower rows) x as kSigmaTiled computes it, and the transposed half adds
-// v x[row] into column col's sum.  The transposed sums are kept in 64-bit
-// fixed point, so that their order does not matter and the result is the
-// same on every run: a product t becomes the integer nearest t 2^-e, with
-// 2^(e+61) above B max|x| (B: the largest column sum of |H_ij| over the
-// stored half), so no column's sum of integers reaches 2^62 in magnitude,
-// and each integer is off by at most 2^(e-1) (below 2^-61 B max|x|).  Each
-// wave sums its current tile's integers in LDS (segments with the
-// large-segment bit; the others add theirs into yT directly) and adds the
-// nonzero sums into yT with 64-bit integer atomics when its tile changes.
-// Each wave takes a contiguous quarter of its block's instructions.
-// yT: nx sums (permuted order of x, cleared first), then the bits of
+// once, in row I's block and J's column tile, and sigma needs each entry
+// twice: v x[J] into row I and v x[I] into row J.  Both go into one
+// accumulator per row, yT (the permuted order of x: row I's sum is split
+// between its alpha-major and, with split columns, beta-major positions),
+// kept in 64-bit fixed point so that the order of the sums does not matter
+// and the result is the same on every run: a value t becomes the integer
+// nearest t 2^-e, with 2^(e+61) above B max|x| (B: the largest off-diagonal
+// row sum of |H_IJ|), so no row's sum of integers reaches 2^62 in
+// magnitude, and each integer is off by at most 2^(e-1) (below
+// 2^-61 B max|x|).  Row parts: each instruction's products are summed per
+// row run in FP64 by the segmented scan, and each run's sum is added as an
+// integer into the block's row sums in LDS, which go to yT at the end.
+// Transposed parts: each wave sums its current tile's integers in LDS
+// (segments with the large-segment bit; the others add theirs into yT
+// directly) and adds the nonzero sums into yT with 64-bit integer atomics
+// when its tile changes.  Each wave takes a contiguous quarter of its
+// block's instructions.  yT: nx sums (cleared first), then the bits of
 // max|x|; bound: the bits of B.
 __device__ __forceinline__ int symFixExp(const unsigned long long* yT, int64_t nx, const unsigned long long* bound) {
   const double S = __longlong_as_double((long long)yT[nx]) * __longlong_as_double((long long)*bound);
@@ -2235,12 +2239,10 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
                                                          const int64_t* __restrict__ blockI,
                                                          const int64_t* __restrict__ blockE,
                                                          const uint32_t* __restrict__ tIdx,
-                                                         const VT* __restrict__ tVal, const double* __restrict__ diag,
-                                                         const uint32_t* __restrict__ perm, int64_t n, int64_t nx,
-                                                         const XT* __restrict__ xP, const double* __restrict__ x,
-                                                         double* __restrict__ y, unsigned long long* yT,
+                                                         const VT* __restrict__ tVal, int64_t n, int64_t nx,
+                                                         const XT* __restrict__ xP, unsigned long long* yT,
                                                          const unsigned long long* __restrict__ bound) {
-  __shared__ double yw[kWaves][kTileRows];
+  __shared__ unsigned long long yr[kTileRows];          // the block's row sums
   __shared__ unsigned long long ya[kWaves][kTileCols];  // the wave's current tile of transposed sums
   __shared__ double xr[kTileRows];                      // x of the block's rows, times 2^-e
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = __lane_id();
@@ -2250,9 +2252,11 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
   const int e = symFixExp(yT, nx, bound);
   const int64_t b = blockIdx.x;
   const int64_t m0 = b * kTileRows;
-  for (int j = lane; j < kTileRows; j += 64) yw[wave][j] = 0.0;
   for (int j = lane; j < kTileCols; j += 64) ya[wave][j] = 0ull;
-  for (int t = threadIdx.x; t < kTileRows; t += kBlock) xr[t] = m0 + t < n ? ldexp((double)xP[m0 + t], -e) : 0.0;
+  for (int t = threadIdx.x; t < kTileRows; t += kBlock) {
+    yr[t] = 0ull;
+    xr[t] = m0 + t < n ? ldexp((double)xP[m0 + t], -e) : 0.0;
+  }
   __syncthreads();
   uint32_t cur = 0xffffffffu;
   auto flush = [&](uint32_t tile) {
@@ -2298,7 +2302,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
       const bool valid = (uint32_t)lane < (dy[u] & 127u);
       const uint32_t row = (ix[u] >> 16) & (kTileRows - 1);
       const double sum = segInclusiveScan(v[u] * xv[u], (int)((ix[u] >> kTileDistShift) & 63u), c15, c31);
-      if (valid && (int)ix[u] < 0) yw[wave][row] += sum;
+      if (valid && (int)ix[u] < 0) atomicAdd(&yr[row], (unsigned long long)__double2ll_rn(ldexp(sum, -e)));
       const uint32_t tile = (dy[u] >> 7) & kTileDescMask;
       const uint32_t col = ix[u] & (uint32_t)(kTileCols - 1);
       const unsigned long long q = (unsigned long long)__double2ll_rn(v[u] * xr[row]);
@@ -2315,23 +2319,21 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
   }
   if (cur != 0xffffffffu) flush(cur);
   __syncthreads();
-  for (int t = threadIdx.x; t < kTileRows; t += kBlock) {
-    const int64_t m = m0 + t;
-    if (m < n) {
-      const int64_t I = perm[m];
-      y[I] = fma(diag[I], x[I], ((yw[0][t] + yw[1][t]) + yw[2][t]) + yw[3][t]);
-    }
-  }
+  for (int t = threadIdx.x; t < kTileRows; t += kBlock)
+    if (m0 + t < n && yr[t] != 0ull) atomicAdd(&yT[m0 + t], yr[t]);
 }
 
-// y[I] += 2^e (yT[pos[I]] (+ yT[n + bPos[I]] with split columns))
-__global__ void kAddTransposed(double* y, const unsigned long long* yT, const unsigned long long* bound,
-                               const uint32_t* pos, const uint32_t* bPos, int64_t n, int64_t nx) {
+// y[I] = diag[I] x[I] + 2^e (yT[pos[I]] (+ yT[n + bPos[I]] with split
+// columns))
+__global__ void kSymFinish(const double* __restrict__ diag, const double* __restrict__ x, double* __restrict__ y,
+                           const unsigned long long* __restrict__ yT, const unsigned long long* __restrict__ bound,
+                           const uint32_t* __restrict__ pos, const uint32_t* __restrict__ bPos, int64_t n,
+                           int64_t nx) {
   const int e = symFixExp(yT, nx, bound);
   for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
     long long t = (long long)yT[pos[I]];
     if (bPos) t += (long long)yT[n + bPos[I]];
-    y[I] += ldexp((double)t, e);
+    y[I] = fma(diag[I], x[I], ldexp((double)t, e));
   }
 }
 
@@ -2367,14 +2369,26 @@ __global__ void kGatherPermMax(const double* __restrict__ x, const uint32_t* __r
   }
 }
 
-// column sums of |v| over stored entries (columns J in determinant order),
-// each |v| rounded up to a multiple of 2^-32 and summed as integers
-// (exactly, so in any order), added into acc
+// off-diagonal row sums of |H| over a stored half grown by rows
+// [nOld, n): row I's stored entries add into acc[I] and each into its
+// column's acc[J]; every |v| is rounded up to a multiple of 2^-32 and
+// summed as an integer (exactly, so in any order).  One wave per row.
 template <typename VT>
-__global__ void kColAbsSum(const int32_t* __restrict__ cols, const VT* __restrict__ vals, int64_t nnz,
-                           unsigned long long* acc) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&acc[cols[p]], (unsigned long long)ceil(ldexp(fabs((double)vals[p]), 32)));
+__global__ void kRowAbsSum(const int64_t* __restrict__ rowPtr, const int32_t* __restrict__ cols,
+                           const VT* __restrict__ vals, int64_t nOld, int64_t n, unsigned long long* acc) {
+  const int lane = __lane_id();
+  for (int64_t I = nOld + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; I < n;
+       I += ((int64_t)gridDim.x * blockDim.x) / 64) {
+    unsigned long long r = 0ull;
+    for (int64_t p = rowPtr[I] + lane; p < rowPtr[I + 1]; p += 64) {
+      const unsigned long long q = (unsigned long long)ceil(ldexp(fabs((double)vals[p]), 32));
+      atomicAdd(&acc[cols[p]], q);
+      r += q;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (lane == 0 && r) atomicAdd(&acc[I], r);
+  }
 }
 
 __global__ void kMaxU64(const unsigned long long* __restrict__ a, int64_t n, unsigned long long* out) {
@@ -3943,31 +3957,27 @@ void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* 
     kGatherPermMax<float><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
     kSigmaTiledSym<float, float, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
                                                             T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                            T.vals.as<float>(), diag, perm, n, nx, xs, x, y, yT,
-                                                            bound);
+                                                            T.vals.as<float>(), n, nx, xs, yT, bound);
   } else {
     double* xs = (double*)xP.get((size_t)nx * 8);
     kGatherPermMax<double><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
     kSigmaTiledSym<double, double, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
                                                               T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                              T.vals.as<double>(), diag, perm, n, nx, xs, x, y, yT,
-                                                              bound);
+                                                              T.vals.as<double>(), n, nx, xs, yT, bound);
   }
-  kAddTransposed<<<gridFor(n), kBlock, 0, s>>>(y, yT, bound, pos, bPos, n, nx);
+  kSymFinish<<<gridFor(n), kBlock, 0, s>>>(diag, x, y, yT, bound, pos, bPos, n, nx);
   HIP_CHECK(hipGetLastError());
 }
 
-void launchSymColumnBound(const int32_t* cols, const double* vals64, const float* vals32, int64_t nnzOld,
-                          int64_t nnz, int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound,
-                          hipStream_t s) {
+void launchSymRowBound(const int64_t* rowPtr, const int32_t* cols, const double* vals64, const float* vals32,
+                       int64_t nOld, int64_t n, unsigned long long* acc, unsigned long long* bound, hipStream_t s) {
   if (n > nOld) kFillZeroU64<<<gridFor(n - nOld), kBlock, 0, s>>>(acc + nOld, n - nOld);
   kFillZeroU64<<<1, 64, 0, s>>>(bound, 1);
-  if (nnz > nnzOld) {
+  if (n > nOld) {
     if (vals32)
-      kColAbsSum<float><<<gridFor(nnz - nnzOld), kBlock, 0, s>>>(cols + nnzOld, vals32 + nnzOld, nnz - nnzOld, acc);
+      kRowAbsSum<float><<<gridFor(n - nOld, kWaves), kBlock, 0, s>>>(rowPtr, cols, vals32, nOld, n, acc);
     else
-      kColAbsSum<double><<<gridFor(nnz - nnzOld), kBlock, 0, s>>>(cols + nnzOld, vals64 + nnzOld, nnz - nnzOld,
-                                                                  acc);
+      kRowAbsSum<double><<<gridFor(n - nOld, kWaves), kBlock, 0, s>>>(rowPtr, cols, vals64, nOld, n, acc);
   }
   if (n > 0) kMaxU64<<<gridFor(n, kBlock, 1024), kBlock, 0, s>>>(acc, n, bound);
   kU64ScaledToDouble<<<1, 64, 0, s>>>(bound);
