@@ -102,8 +102,9 @@ struct Engine {
   // gathered in lowerYT (fixed point, permuted order; then max|x|); the FP32
   // form's exact sigma keeps the row kernel
   bool lowerTiled = true;
-  // instructions that make a segment large (24-orbital workload: 2: 3.60,
-  // 4: 3.29, 8: 3.66 ms; 8 instructions per wave per step: 3.28 ms, 16: 3.65)
+  // instructions that make a segment large (sigma on the 24-orbital
+  // workload, 1: 3.42, 2: 3.22, 3: 3.10, 4: 3.09, 6: 3.28, 8: 3.59 ms; 16
+  // instructions per wave per step instead of 8: 12% slower)
   int lowerStageMin = 4;
   DevMem lowerYT;
   // the off-diagonal row sums of |H| (fixed point) over the first
