@@ -12,16 +12,15 @@ integrals resident on the GPU.  `value` is the mean wall time of one
 variational iteration over the timed steps.
 
 The roofline object is the sigma = H x kernel of the Davidson, timed with
-HIP events on the library's stream over the timed steps.  In FP64 the
-Hamiltonian's lower triangle alone is stored (the library's default for the
-single-vector Davidson in FP64): kSigmaTiledSym over a tiled copy of it
-(256-row blocks of the alpha-major relabelled rows, entries grouped by
-1024-column tiles), each stored element applied to its row and, in 64-bit
-fixed point, to its column, with its x permutation and the transposed sums'
-addition; with FP32 values both triangles are stored and the kernel is
-kSigmaTiled.  Achieved bandwidth uses algorithmic bytes per launch: stored
-values + column indices, row offsets, diagonal, x read once and y written
-once.  `traffic` is the HBM
+HIP events on the library's stream over the timed steps.  The Hamiltonian's
+lower triangle alone is stored (the library's default for the single-vector
+Davidson): kSigmaTiledSym over a tiled copy of it (256-row blocks of the
+alpha-major relabelled rows, entries grouped by 1024-column tiles), each
+stored element applied to its row and, in 64-bit fixed point, to its
+column, with its x permutation and the final addition into y.  Achieved
+bandwidth uses algorithmic bytes per launch: stored values + column
+indices, row offsets, diagonal, x read once and y written once.  `traffic`
+is the HBM
 bytes per sigma launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 this benchmark (scaled by a copy kernel of known size), read from
 profiles/sigma_traffic_c1_<precision>.json (see profiles/sigma_traffic_c1.md) when that summary exists.
@@ -617,9 +616,9 @@ def main():
             "bound": "hbm",
             "kernel": ("kSigmaTiledSym (sigma = H x over the stored lower triangle: 256-row blocks of the "
                        "alpha-major relabelled rows, entries grouped by 1024-column tiles, each entry applied to its "
-                       "row and, in 64-bit fixed point, to its column)" if prec == shci.PREC_FP64 else
-                       "kSigmaTiled (sigma = H x over 256-row blocks of the alpha-major relabelled Hamiltonian, "
-                       "entries grouped by 1024-column tiles)"),
+                       "row and, in 64-bit fixed point, to its column)" if not block else
+                       "kSigmaPerm / kSpmmPerm (sigma over the row-ordered alpha-major relabelled Hamiltonian, both "
+                       "triangles)"),
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

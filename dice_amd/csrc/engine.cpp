@@ -788,7 +788,7 @@ void Engine::buildHamiltonian() {
 bool Engine::wantLower() const {
   if (lowerByMemory || storageMode > 0) return true;
   if (storageMode < 0) return false;
-  return precision == 0 && davidsonMode != 1 && relabel && tiledOn && lowerTiled;
+  return davidsonMode != 1 && relabel && tiledOn && lowerTiled;
 }
 
 // Half storage: the rows of the determinants added since the last build
@@ -857,10 +857,12 @@ void Engine::buildLowerTiled(int64_t nOld, int64_t nnzOld) {
   splitRows = -1;
   if (!relabel || !tiledOn || !lowerTiled || n == 0) return;
   const bool f32 = precision == 1;
-  const size_t xt = f32 ? 4 : 8;
-  const size_t tiledCopy = tiledHamBytes(n, nnzOff, f32);
+  // (the FP32 form's copy holds the lo values too, for its exact sigma)
+  const size_t tiledCopy = tiledHamBytes(n, nnzOff, f32) + (f32 ? (size_t)nnzOff * 4 : 0);
   const size_t base = (size_t)n * 4 + (size_t)n * 64;
-  const size_t xPlain = (size_t)n * (xt + 16) + 16, xSplit = 2 * (size_t)n * (xt + 8) + (size_t)n * 12 + 16;
+  // permuted x (FP64 also in the FP32 form's exact sigma) and the
+  // accumulators, plus bPos with split columns
+  const size_t xPlain = (size_t)n * 16 + 16, xSplit = 2 * (size_t)n * 16 + (size_t)n * 4 + 16;
   const size_t have = deviceFreeBytes() + permX.bytes + lowerYT.bytes + bPos.bytes;
   if (base + tiledCopy + xPlain > have) return;
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
@@ -874,7 +876,7 @@ void Engine::buildLowerTiled(int64_t nOld, int64_t nnzOld) {
   if (!buildTiledHam(tiled, rowPtr.as<int64_t>(), cols.as<int32_t>(), f32 ? nullptr : vals.as<double>(),
                      f32 ? vals32.as<float>() : nullptr, perm, permPos.as<uint32_t>(),
                      doSplit ? T.bIdx.as<uint32_t>() : nullptr, doSplit ? bPos.as<uint32_t>() : nullptr, n, nnzOff,
-                     ws, stream, false, lowerStageMin)) {
+                     ws, stream, false, lowerStageMin, f32 ? valsLo.as<float>() : nullptr)) {
     tiled.release();
     return;
   }
@@ -1296,14 +1298,15 @@ void Engine::sigma(const double* dx, double* dy, int k, int64_t ld, bool exact) 
     if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
     return;
   }
-  if (lowerOnly && tiledRows == n && !(f32 && exact)) {
+  if (lowerOnly && tiledRows == n && (!(f32 && exact) || tiled.hasLo)) {
     const bool sp = splitRows == n;
     const XScratch xs{&permX};
     for (int r = 0; r < k; r++)
       launchSigmaTiledSym(tiled, diag.as<double>(), T.aMajorDet.as<uint32_t>(),
                           sp ? T.bMajorDet.as<uint32_t>() : nullptr, permPos.as<uint32_t>(),
                           sp ? bPos.as<uint32_t>() : nullptr, dx + r * ld, dy + r * ld, xs,
-                          lowerYT.as<unsigned long long>(), lowerBound.as<unsigned long long>(), stream);
+                          lowerYT.as<unsigned long long>(), lowerBound.as<unsigned long long>(), stream,
+                          f32 && exact);
     if (ev) HIP_CHECK(hipEventRecord(ev->stop, stream));
     return;
   }

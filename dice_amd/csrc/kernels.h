@@ -258,8 +258,9 @@ void* fixedSigmaScratch(size_t* bytes);
 // tile), and the value (FP64, or the FP32 hi part).
 struct TiledHam {
   int64_t n = -1, blocks = 0, segments = 0, nnz = 0, instructions = 0;
-  bool f32 = false, split = false;
-  DevMem segBase, segTile, segCnt, segStart, segInst, segTotal, fill, blockE, blockI, idx, vals, inst;
+  bool f32 = false, split = false, hasLo = false;
+  // lo: the FP32 values' remainders (lo = fp32(v - hi)), when built with them
+  DevMem segBase, segTile, segCnt, segStart, segInst, segTotal, fill, blockE, blockI, idx, vals, lo, inst;
   void release();
 };
 // device bytes buildTiledHam needs for nnz entries over n rows (upper bound
@@ -272,12 +273,15 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32);
 // or, with perm, pos, bIdx and bPos null, from the relabelled row copy
 // itself (rows and columns already permuted; splitCols says whether its
 // columns reach into the beta-major copy).  Segments of at least stageMin
-// instructions are marked for the staged sigma (their x tile goes to LDS).  Returns false, with nothing
-// built, when the permuted x has more tiles than the build's bitmap holds.
+// instructions are marked as large (the staged sigma copies their x tile to
+// LDS; the half-storage sigma sums their transposed products there).  With
+// vals32, valsLo (the FP32 remainders) is copied beside them when given.
+// Returns false, with nothing built, when the permuted x has more tiles than
+// the build's bitmap holds.
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
                    const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s,
-                   bool splitCols = false, int stageMin = 8);
+                   bool splitCols = false, int stageMin = 8, const float* valsLo = nullptr);
 // y = H x through the tiled copy (one vector, x and y in determinant order;
 // bPerm = the beta-major determinant order when the copy has split columns)
 // (step: instructions per wave per step, 8 or 16; stage: each wave copies
@@ -290,10 +294,11 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
 // alpha-major / beta-major position; yT: nx + 1 words of scratch (nx: the
 // permuted x's length); bound: the row bound set by launchSymRowBound.  The
 // off-diagonal sums are 64-bit fixed point, summed with integer atomics:
-// the result does not depend on their order.
+// the result does not depend on their order.  exact (FP32 values): hi + lo
+// values and x in FP64 (the copy must hold lo).
 void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                          const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
-                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s);
+                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s, bool exact = false);
 // kSigmaTiledSym's bound (the largest off-diagonal row sum of |H|) into
 // *bound, from the row sums in acc (n words): the rows [nOld, n) of a half
 // grown from nOld rows (CSR rowPtr, cols, values: vals64 or vals32, as

@@ -1955,7 +1955,8 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
                                                     const uint32_t* segStart, const uint32_t* segInst,
                                                     const uint32_t* segTotal, uint32_t* fill,
                                                     const int64_t* blockE, const int64_t* blockI, uint32_t* tIdx,
-                                                    VT* tVal, uint2* inst, int stageMin) {
+                                                    VT* tVal, uint2* inst, int stageMin, const float* lo,
+                                                    float* tLo) {
   __shared__ uint32_t bits[kTileMaxWords];
   __shared__ uint32_t pre[kTileMaxWords];
   __shared__ uint32_t fillL[kWaves][kTileLdsSegs];
@@ -1998,6 +1999,7 @@ __global__ void __launch_bounds__(kBlock) kTileFill(const int64_t* rowPtr, const
         const int64_t q = eb + got;
         tIdx[q] = ((uint32_t)r << 16) | (c & (uint32_t)(kTileCols - 1));
         tVal[q] = v;
+        if (lo) tLo[q] = lo[p];
       }
     }
   }
@@ -2234,12 +2236,13 @@ __device__ __forceinline__ int symFixExp(const unsigned long long* yT, int64_t n
   return S > 0.0 ? ilogb(S) + 1 - 61 : 0;
 }
 
-template <typename VT, typename XT, int U>
+template <typename VT, typename XT, int U, bool LO>
 __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict__ inst,
                                                          const int64_t* __restrict__ blockI,
                                                          const int64_t* __restrict__ blockE,
                                                          const uint32_t* __restrict__ tIdx,
-                                                         const VT* __restrict__ tVal, int64_t n, int64_t nx,
+                                                         const VT* __restrict__ tVal,
+                                                         const float* __restrict__ tLo, int64_t n, int64_t nx,
                                                          const XT* __restrict__ xP, unsigned long long* yT,
                                                          const unsigned long long* __restrict__ bound) {
   __shared__ unsigned long long yr[kTileRows];          // the block's row sums
@@ -2288,6 +2291,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiledSym(const uint2* __restrict
       const int64_t p = e0 + (valid ? (int64_t)dx[u] + lane : 0);
       ix[u] = __builtin_nontemporal_load(tIdx + p);
       v[u] = (double)__builtin_nontemporal_load(tVal + p);
+      if (LO) v[u] += (double)__builtin_nontemporal_load(tLo + p);
     }
     double xv[U];
 #pragma unroll
@@ -3792,9 +3796,10 @@ void launchSigmaPermF32(const int64_t* rp, const int32_t* cols, const float* val
 
 void TiledHam::release() {
   for (DevMem* m : {&segBase, &segTile, &segCnt, &segStart, &segInst, &segTotal, &fill, &blockE, &blockI, &idx, &vals,
-                    &inst})
+                    &lo, &inst})
     m->release();
   n = -1;
+  hasLo = false;
 }
 
 size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32) {
@@ -3810,8 +3815,9 @@ size_t tiledHamBytes(int64_t n, int64_t nnz, bool f32) {
 bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, const double* vals64,
                    const float* vals32, const uint32_t* perm, const uint32_t* pos, const uint32_t* bIdx,
                    const uint32_t* bPos, int64_t n, int64_t nnz, Workspace& ws, hipStream_t s, bool splitCols,
-                   int stageMin) {
+                   int stageMin, const float* valsLo) {
   T.n = -1;
+  T.hasLo = false;
   if (n <= 0) return false;
   const bool split = pos ? bIdx != nullptr : splitCols;
   const int64_t nx = split ? 2 * n : n;
@@ -3859,6 +3865,8 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
   const bool f32 = vals64 == nullptr;
   T.idx.reserve((size_t)std::max<int64_t>(E, 1) * 4);
   T.vals.reserve((size_t)std::max<int64_t>(E, 1) * (f32 ? 4 : 8));
+  const bool withLo = f32 && valsLo;
+  if (withLo) T.lo.reserve((size_t)std::max<int64_t>(E, 1) * 4);
   // kSigmaTiled reads up to kTileInstPad descriptors past a block's last
   // one: zero padding after the last block (count 0, tile 0)
   T.inst.reserve((size_t)(I + kTileInstPad) * 8);
@@ -3869,14 +3877,15 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
                                              T.segInst.as<uint32_t>(),
                                              T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
                                              T.idx.as<uint32_t>(), T.vals.as<float>(), T.inst.as<uint2>(),
-                                             stageMin);
+                                             stageMin, withLo ? valsLo : nullptr,
+                                             withLo ? T.lo.as<float>() : nullptr);
   else
     kTileFill<double><<<grid, kBlock, 0, s>>>(rowPtr, cols, vals64, perm, pos, bIdx, bPos, n, nwords, segBase,
                                               T.segTile.as<uint32_t>(), T.segStart.as<uint32_t>(),
                                               T.segInst.as<uint32_t>(),
                                               T.segTotal.as<uint32_t>(), T.fill.as<uint32_t>(), blockE, blockI,
                                               T.idx.as<uint32_t>(), T.vals.as<double>(), T.inst.as<uint2>(),
-                                              stageMin);
+                                              stageMin, nullptr, nullptr);
   HIP_CHECK(hipGetLastError());
   kTileRunBits<<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), blockI, blockE, T.idx.as<uint32_t>());
   HIP_CHECK(hipGetLastError());
@@ -3887,6 +3896,7 @@ bool buildTiledHam(TiledHam& T, const int64_t* rowPtr, const int32_t* cols, cons
   T.instructions = I;
   T.f32 = f32;
   T.split = split;
+  T.hasLo = withLo;
   return true;
 }
 
@@ -3944,26 +3954,32 @@ void launchSigmaTiled(const TiledHam& T, const double* diag, const uint32_t* per
 
 void launchSigmaTiledSym(const TiledHam& T, const double* diag, const uint32_t* perm, const uint32_t* bPerm,
                          const uint32_t* pos, const uint32_t* bPos, const double* x, double* y, const XScratch& xP,
-                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s) {
+                         unsigned long long* yT, const unsigned long long* bound, hipStream_t s, bool exact) {
   const int64_t n = T.n;
   if (n <= 0) return;
   if (T.split != (bPerm != nullptr) || T.split != (bPos != nullptr))
     throw HipError("tiled sigma: split columns need the beta-major order");
+  if (exact && T.f32 && !T.hasLo) throw HipError("tiled sigma: the exact FP32 form needs the lo values");
   const int64_t nx = bPerm ? 2 * n : n;
   const unsigned grid = (unsigned)T.blocks;
   kFillZeroU64<<<gridFor(nx + 1), kBlock, 0, s>>>(yT, nx + 1);
-  if (T.f32) {
+  if (T.f32 && !exact) {
     float* xs = (float*)xP.get((size_t)nx * 4);
     kGatherPermMax<float><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
-    kSigmaTiledSym<float, float, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
-                                                            T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                            T.vals.as<float>(), n, nx, xs, yT, bound);
+    kSigmaTiledSym<float, float, 8, false><<<grid, kBlock, 0, s>>>(
+        T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(), T.idx.as<uint32_t>(), T.vals.as<float>(),
+        nullptr, n, nx, xs, yT, bound);
   } else {
     double* xs = (double*)xP.get((size_t)nx * 8);
     kGatherPermMax<double><<<gridFor(nx, kBlock, 1024), kBlock, 0, s>>>(x, perm, bPerm, n, xs, yT + nx);
-    kSigmaTiledSym<double, double, 8><<<grid, kBlock, 0, s>>>(T.inst.as<uint2>(), T.blockI.as<int64_t>(),
-                                                              T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
-                                                              T.vals.as<double>(), n, nx, xs, yT, bound);
+    if (T.f32)  // the exact FP32 form: hi + lo values, x in FP64
+      kSigmaTiledSym<float, double, 8, true><<<grid, kBlock, 0, s>>>(
+          T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+          T.vals.as<float>(), T.lo.as<float>(), n, nx, xs, yT, bound);
+    else
+      kSigmaTiledSym<double, double, 8, false><<<grid, kBlock, 0, s>>>(
+          T.inst.as<uint2>(), T.blockI.as<int64_t>(), T.blockE.as<int64_t>(), T.idx.as<uint32_t>(),
+          T.vals.as<double>(), nullptr, n, nx, xs, yT, bound);
   }
   kSymFinish<<<gridFor(n), kBlock, 0, s>>>(diag, x, y, yT, bound, pos, bPos, n, nx);
   HIP_CHECK(hipGetLastError());
