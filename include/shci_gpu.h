@@ -131,7 +131,9 @@ shci_status shci_sigma_device(shci_handle* h, const double* d_x, double* d_y, in
  * Hamiltonian must already be built (shci_build_hamiltonian).  The permuted
  * x and work counters live in one fixed scratch region per GPU (64 MiB of x:
  * with split columns n * vectors-per-pass up to 4,194,304 in FP64, where a
- * pass takes 1, 2, 4 or 8 of the k vectors), held until the process ends;
+ * pass takes 1, 2, 4 or 8 of the k vectors; with half storage the permuted x
+ * and the 8-byte fixed-point sums, n up to about 2 million in FP64 with
+ * split columns), held until the process ends;
  * a larger request returns SHCI_UNSUPPORTED before anything is enqueued, as
  * do time-reversal and direct mode.  Every call on one GPU shares that
  * region: calls that overlap on one GPU (other handles or streams) race.
