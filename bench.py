@@ -121,8 +121,10 @@ def batching_form(shci, args):
     """Batching form on the 6x6 Hubbard lattice (18+18 electrons, t = 1,
     U = 4, site basis, Neel reference), eps1 2e-4, four roots, three
     variational iterations (428,641 determinants): the reference's Davidson
-    vs the block Davidson (one SpMM per block, hipGraph-replayed iterations,
-    autotuned sigma launch shape)."""
+    vs the block Davidson (one sigma per block of corrections,
+    hipGraph-replayed iterations), both on the default half storage; and
+    the block's sigma of four vectors alone, on half storage (one
+    kSigmaTiledSym per vector) and on both triangles (one SpMM)."""
     from dice_amd import synth
     d = os.path.join(tempfile.gettempdir(), "shci_bench_hubbard6x6")
     os.makedirs(d, exist_ok=True)
@@ -146,19 +148,27 @@ def batching_form(shci, args):
             "sigma_GBps": round(R["nbytes"] / (R["ms"] * 1e-3) / 1e9, 1) if R["ms"] else None,
             "iterations": R["per_iter"],
         }
-    # the block Davidson's sigma alone: k = 4 interleaved vectors as one SpMM
-    # over the relabelled copy of the final Hamiltonian (wall time of
-    # shci_sigma_device, x permutation included); algorithmic bytes as for
-    # the one-vector sigma with 2 k N vector values
-    r = shci.do_variational(inp, log, davidson_block=True, autotune=True, hip_graphs=graphs)
-    n, nnz_lower = r.iterations()[-1]["ndets"], r.phases()[-1][3]
-    t = time_sigma_device(shci, r._h, n, 4, 20)
-    del r
-    nbytes = 2 * (nnz_lower - n) * 12 + (n + 1) * 8 + n * 8 + 2 * 4 * n * 8
-    res["block_davidson"]["spmm_k4"] = {"ndets": n, "nnz_stored": 2 * (nnz_lower - n), "ms": round(1e3 * t, 4),
-                                        "bytes": nbytes, "GBps": round(nbytes / t / 1e9, 1),
-                                        "frac_of_hbm_peak": round(nbytes / t / 1e9 / PEAK_HBM_GBS, 4)}
-    print("# Hubbard SpMM k=4: %.3f ms, %.1f GB/s" % (1e3 * t, nbytes / t / 1e9), file=sys.stderr, flush=True)
+    # the block Davidson's sigma alone, k = 4 vectors on the final
+    # Hamiltonian (wall time of shci_sigma_device, x permutation included):
+    # on half storage four launches over the stored half, on both triangles
+    # one SpMM of 4 interleaved vectors over the row-ordered relabelled
+    # copy; algorithmic bytes per pass over the matrix as for the
+    # one-vector sigma, with 2 k N vector values
+    for key, half in (("sigma_k4_half_storage", None), ("spmm_k4_full_storage", False)):
+        r = shci.do_variational(inp, log, davidson_block=True, autotune=True, hip_graphs=graphs, half_storage=half)
+        n, nnz_lower = r.iterations()[-1]["ndets"], r.phases()[-1][3]
+        t = time_sigma_device(shci, r._h, n, 4, 20)
+        del r
+        if half is None:
+            stored = nnz_lower - n
+            nbytes = 4 * (stored * 12 + (n + 1) * 8 + n * 8 + 2 * n * 8)
+        else:
+            stored = 2 * (nnz_lower - n)
+            nbytes = stored * 12 + (n + 1) * 8 + n * 8 + 2 * 4 * n * 8
+        res["block_davidson"][key] = {"ndets": n, "nnz_stored": stored, "ms": round(1e3 * t, 4), "bytes": nbytes,
+                                      "GBps": round(nbytes / t / 1e9, 1),
+                                      "frac_of_hbm_peak": round(nbytes / t / 1e9 / PEAK_HBM_GBS, 4)}
+        print("# Hubbard %s: %.3f ms, %.1f GB/s" % (key, 1e3 * t, nbytes / t / 1e9), file=sys.stderr, flush=True)
     res["workload"] = ("6x6 periodic Hubbard, t=1, U=4, 18a+18b, eps1 2e-4, 4 roots, 3 iterations "
                        "(the fourth would select 12.6M more determinants)")
     res["davidson_speedup"] = round(res["reference_davidson"]["davidson_s"] / res["block_davidson"]["davidson_s"], 3)
@@ -614,11 +624,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("kSigmaTiledSym (sigma = H x over the stored lower triangle: 256-row blocks of the "
-                       "alpha-major relabelled rows, entries grouped by 1024-column tiles, each entry applied to its "
-                       "row and, in 64-bit fixed point, to its column)" if not block else
-                       "kSigmaPerm / kSpmmPerm (sigma over the row-ordered alpha-major relabelled Hamiltonian, both "
-                       "triangles)"),
+            "kernel": "kSigmaTiledSym (sigma = H x over the stored lower triangle: 256-row blocks of the "
+                      "alpha-major relabelled rows, entries grouped by 1024-column tiles, each entry applied to its "
+                      "row and, in 64-bit fixed point, to its column)",
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
@@ -693,7 +701,7 @@ def main():
         }
         # batching + mixed precision on the same workload: block Davidson
         # (one root: 17-vector subspace, thick restart), hipGraph-replayed
-        # iterations, FP32 values
+        # iterations, FP32 values (half storage, as the FP64 path)
         B = run_form(shci, full, log, shci.PREC_FP32, fsteps, fwarm, tag="block+fp32", profile=False,
                      davidson_block=True, hip_graphs=not args.no_graphs)
         out["forms"]["batched_mixed_fp32"] = {

@@ -788,7 +788,7 @@ void Engine::buildHamiltonian() {
 bool Engine::wantLower() const {
   if (lowerByMemory || storageMode > 0) return true;
   if (storageMode < 0) return false;
-  return davidsonMode != 1 && relabel && tiledOn && lowerTiled;
+  return relabel && tiledOn && lowerTiled;
 }
 
 // Half storage: the rows of the determinants added since the last build

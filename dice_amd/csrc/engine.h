@@ -86,12 +86,11 @@ struct Engine {
   // half storage: each pair once, in row max(I, J) (the reference's
   // lower-triangular rows without the diagonal), grown in place by new rows
   // only.  storageMode 1 (SHCI_STORAGE=half, half_storage > 0) always, -1
-  // (SHCI_STORAGE=full, half_storage < 0) never, 0: when the FP64
-  // single-vector Davidson runs on the relabelled tiled layout (its fastest
-  // form; the block Davidson's SpMM and the FP32 form's hi + lo refinement
-  // run on the two triangles); and from the build on which the two-triangle
-  // Hamiltonian does not fit (lowerByMemory).  Sigma then adds the
-  // transposed half.
+  // (SHCI_STORAGE=full, half_storage < 0) never, 0: whenever sigma runs on
+  // the relabelled tiled layout (the fastest form measured, for the
+  // reference and the block Davidson, FP64 and FP32 values); and from the
+  // build on which the two-triangle Hamiltonian does not fit
+  // (lowerByMemory).  Sigma then adds the transposed half.
   bool lowerOnly = false;
   int storageMode = 0;
   bool lowerByMemory = false;

@@ -255,9 +255,9 @@ typedef struct {
                          each stored element to both rows (SHCI/Hmult.h:110-121; on a tiled copy, the
                          transposed half summed in 64-bit fixed point, so in a fixed result); < 0: both
                          triangles stored; 0: the library's choice (environment SHCI_STORAGE=half/full,
-                         else half storage for the FP64 single-vector Davidson, its fastest form, and
-                         both triangles for the block Davidson and the FP32 form).  A build switches to
-                         half storage by itself when the two-triangle Hamiltonian does not fit */
+                         else half storage, the fastest layout measured for every form).  A build
+                         switches to half storage by itself when the two-triangle Hamiltonian does not
+                         fit */
 } shci_run_options;
 /* shci_run with options; same files, log and result handle. */
 shci_status shci_run_ex(const char* input_path, const char* log_path, const shci_run_options* options,

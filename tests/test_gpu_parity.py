@@ -503,9 +503,11 @@ def test_block_davidson_graph_replays_match_eager(systems):
         np.testing.assert_array_equal(x0, x1)
 
 
-def test_autotuned_sigma_matches_default(systems):
-    """With autotune on, the Hamiltonian build times every launch shape and
-    keeps the fastest; sigma results do not depend on the shape (1e-12)."""
+def test_autotuned_sigma_matches_default(systems, monkeypatch):
+    """With autotune on (both triangles stored: the row copy's launch shapes),
+    the Hamiltonian build times every launch shape and keeps the fastest;
+    sigma results do not depend on the shape (1e-12)."""
+    monkeypatch.setenv("SHCI_STORAGE", "full")
     path = systems["c0"][0]
     dets = _space("c0", systems, 24000, 1)
     x = np.random.default_rng(7).normal(size=(len(dets), 3))
