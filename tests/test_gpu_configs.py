@@ -137,8 +137,9 @@ def test_c1_final_space_sigma_matches_oracle(tmp_path):
     iterations at eps1 5e-3 (1,177,638 determinants, 0.91e9 lower-triangle
     elements), then the oracle's Hamiltonian build and sigma (the
     reference's MakeHfromSMHelpers2 and Hmult2, rank-parallel loops on the
-    host threads) on the same final space: the GPU sigma over the tiled copy
-    equals it to 1e-12 relative, and the number of stored elements agrees."""
+    host threads) on the same final space: the GPU sigma (the default: the
+    tiled copy of the stored lower triangle, sums in fixed point) equals it
+    to 1e-12 relative, and the number of stored elements agrees."""
     from dice_amd import synth, shci
     from oracle_bind import OracleSystem
     c, n, na, nb, nchol = synth.MOLECULAR["c1"]
