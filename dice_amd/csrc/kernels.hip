@@ -7,6 +7,7 @@
 //   Davidson vector algebra   SHCI/Davidson.cpp:85-348
 #include <algorithm>
 #include <type_traits>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
@@ -2233,7 +2234,7 @@ __global__ void __launch_bounds__(kBlock) kSigmaTiled(const uint2* __restrict__ 
 // max|x|; bound: the bits of B.
 __device__ __forceinline__ int symFixExp(const unsigned long long* yT, int64_t nx, const unsigned long long* bound) {
   const double S = __longlong_as_double((long long)yT[nx]) * __longlong_as_double((long long)*bound);
-  return S > 0.0 ? ilogb(S) + 1 - 61 : 0;
+  return S > 0.0 && S <= DBL_MAX ? ilogb(S) + 1 - 61 : 0;  // (x holding inf: no scale is right)
 }
 
 template <typename VT, typename XT, int U, bool LO>

@@ -102,6 +102,32 @@ def test_half_storage_tiled_sigma_is_deterministic(case, tmp_path, monkeypatch):
         assert np.max(np.abs(ys[0][:, r] - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
+def test_half_storage_tiled_sigma_scales(tmp_path, monkeypatch):
+    """The fixed point follows the scale of x: sigma of x scaled by 2^-600,
+    1e-150, 1e150 and 2^600 equals the oracle's to 1e-12 relative (a power
+    of two scales the result exactly), and x = 0 gives y = 0."""
+    from dice_amd import shci, synth
+    path = str(tmp_path / "FCIDUMP")
+    synth.write_molecular_fcidump(path, 0, 16, 8, 8, 64)
+    dets = _space(path, synth.hf_occupation(8, 8), 3000, 1)
+    monkeypatch.setenv("SHCI_STORAGE", "half")
+    h = shci.System.from_fcidump(path)
+    h.set_dets(dets)
+    h.make_hamiltonian()
+    o = OracleSystem(path)
+    o.csr_append(dets)
+    x = np.random.default_rng(11).normal(size=len(dets))
+    y1 = h.hmult(x)
+    for scale in (2.0 ** -600, 1e-150, 1e150, 2.0 ** 600):
+        xs = x * scale
+        y = h.hmult(xs)
+        yo = o.sigma(np.ascontiguousarray(xs))
+        assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo)), scale
+        if scale in (2.0 ** -600, 2.0 ** 600):
+            np.testing.assert_array_equal(y, y1 * scale)
+    np.testing.assert_array_equal(h.hmult(np.zeros(len(dets))), np.zeros(len(dets)))
+
+
 @pytest.mark.parametrize("mode", ["tiled", "rows"])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_half_storage_run_matches_oracle(prec, mode, tmp_path, monkeypatch):
