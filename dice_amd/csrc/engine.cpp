@@ -49,6 +49,7 @@ void Engine::initShared(const Engine& parent) {
   lowerOnly = parent.lowerOnly;
   lowerTiled = parent.lowerTiled;
   lowerStageMin = parent.lowerStageMin;
+  lowerSplit = parent.lowerSplit;
   storageMode = parent.storageMode;
   lowerByMemory = parent.lowerByMemory;
   autotune = parent.autotune;
@@ -225,6 +226,8 @@ void Engine::loadSystem(HostSystem&& sys) {
   // their transposed products in LDS first (1: every segment)
   if (const char* p = std::getenv("SHCI_HALF_SIGMA")) lowerTiled = std::strcmp(p, "rows") != 0;
   if (const char* p = std::getenv("SHCI_HALF_STAGE_MIN")) lowerStageMin = std::max(1, std::atoi(p));
+  // SHCI_HALF_SPLIT=1: split columns in the half's tiled copy too
+  if (const char* p = std::getenv("SHCI_HALF_SPLIT")) lowerSplit = std::atoi(p) != 0;
   uploadRestrictions();
   counter.reserve(64);
   // the allocation-free sigma's fixed scratch: its address on this device
@@ -868,7 +871,7 @@ void Engine::buildLowerTiled(int64_t nOld, int64_t nnzOld) {
   const uint32_t* perm = T.aMajorDet.as<uint32_t>();
   permPos.reserve((size_t)n * 4);
   launchInversePerm(perm, n, permPos.as<uint32_t>(), stream);
-  const bool doSplit = split && 2 * n < 0x7fffffffLL && base + tiledCopy + xSplit <= have;
+  const bool doSplit = split && lowerSplit && 2 * n < 0x7fffffffLL && base + tiledCopy + xSplit <= have;
   if (doSplit) {
     bPos.reserve((size_t)n * 4);
     launchInversePerm(T.bMajorDet.as<uint32_t>(), n, bPos.as<uint32_t>(), stream);

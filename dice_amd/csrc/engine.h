@@ -105,6 +105,9 @@ struct Engine {
   // workload, 1: 3.42, 2: 3.22, 3: 3.10, 4: 3.09, 6: 3.28, 8: 3.59 ms; 16
   // instructions per wave per step instead of 8: 12% slower)
   int lowerStageMin = 4;
+  // split columns in the half's tiled copy (sigma on the 24-orbital
+  // workload: 3.08 ms with, 2.96 ms without, and a shorter build)
+  bool lowerSplit = false;
   DevMem lowerYT;
   // the off-diagonal row sums of |H| (fixed point) over the first
   // lowerAbsRows rows' stored entries, grown with the half, and their

@@ -200,7 +200,8 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     ("alpha"), and the two-triangle default (the tiled copy, large segments
     gathering from an LDS copy of their x tile; "tiled-global": every gather
     from global memory); and the tiled copy of the lower triangle alone
-    ("half", the transposed half in fixed point).  Each layout repeats its
+    ("half", the transposed half in fixed point; "half-split" with split
+    columns).  Each layout repeats its
     own result bit for bit, staging every segment gives the default's bits,
     and the tiled copy built from H directly equals the one built from the
     row copy bit for bit."""
@@ -209,11 +210,13 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
     x = np.random.default_rng(5).normal(size=(len(dets), 2))
     out = {}
     for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "external", "alpha",
-                   "half"):
+                   "half", "half-split"):
         for var in ("SHCI_SIGMA_LAYOUT", "SHCI_SIGMA_TILE_BUILD", "SHCI_SIGMA_TILE_STAGE", "SHCI_SIGMA_TILE_STAGE_MIN",
-                    "SHCI_SIGMA_TILE_PERSISTENT"):
+                    "SHCI_SIGMA_TILE_PERSISTENT", "SHCI_HALF_SPLIT"):
             monkeypatch.delenv(var, raising=False)
-        monkeypatch.setenv("SHCI_STORAGE", "half" if layout == "half" else "full")
+        monkeypatch.setenv("SHCI_STORAGE", "half" if layout.startswith("half") else "full")
+        if layout == "half-split":
+            monkeypatch.setenv("SHCI_HALF_SPLIT", "1")
         if layout == "tiled-from-H":
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_BUILD", "direct")
@@ -225,7 +228,7 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
             # every segment staged, down to one instruction
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", "tiled")
             monkeypatch.setenv("SHCI_SIGMA_TILE_STAGE_MIN", "1")
-        elif layout not in ("default", "half"):
+        elif layout not in ("default", "half", "half-split"):
             monkeypatch.setenv("SHCI_SIGMA_LAYOUT", layout)
         g = _gpu(path)
         g.set_precision(prec)
@@ -236,7 +239,8 @@ def test_sigma_layouts_agree(name, prec, systems, monkeypatch):
         np.testing.assert_array_equal(out[layout][1], g.hmult(x[:, 0].copy()))
     tol = 1e-12 if prec == 0 else 1e-6
     ref = out["external"]
-    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "alpha", "half"):
+    for layout in ("default", "tiled", "tiled-from-H", "tiled-global", "tiled-persistent", "rows", "alpha", "half",
+                   "half-split"):
         for k in (0, 1):
             assert np.max(np.abs(out[layout][k] - ref[k])) <= tol * np.max(np.abs(ref[k])), (layout, k)
     # the staged kernel's sums do not depend on which segments it stages
