@@ -162,8 +162,9 @@ void launchCsrFillSplit(const DevSystem& S, const uint64_t* dets, const StringTa
                         bool lower = false);
 // half storage (each pair once, in row max(I, J)): row offsets of rows
 // [nOld, n] from the exclusive scan cnt of the new rows' lengths, and sigma
-// applying every stored element to both rows (atomic adds for the
-// transposed half; y must not overlap x)
+// on the stored rows applying every stored element to both rows (FP64
+// atomic adds for the transposed half, so in no fixed order; the tiled
+// copy's launchSigmaTiledSym is the default; y must not overlap x)
 void launchLowerRowPtr(const int64_t* cnt, int64_t nOld, int64_t n, int64_t* rowPtr, hipStream_t s);
 void launchSigmaLowerF64(const int64_t* rowPtr, const int32_t* cols, const double* vals, const double* diag, int64_t n,
                          const double* x, double* y, int k, int64_t ld, hipStream_t s);
